@@ -1,0 +1,234 @@
+/*
+ * yasps_amd - C ABI of the MI355X (gfx950) Newton linear-solve path.
+ *
+ * Every entry point takes plain device pointers and host scalars, never torch types.
+ * Entry points are asynchronous on the given HIP stream (NULL = default stream)
+ * unless they return a host value (a count or an iteration status), in which case
+ * they synchronise the stream before returning. Return value: 0 (or a non-negative
+ * count) on success, a negative YSP_ERR_* code on failure; ysp_last_error() gives the
+ * message. The library never frees caller memory.
+ *
+ * Each declaration names the reference interface it replaces (paths relative to the
+ * reference repository root).
+ */
+#ifndef YASPS_AMD_H
+#define YASPS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define YSP_OK 0
+#define YSP_ERR_HIP (-1)          /* device/runtime failure (reference launchers return -1) */
+#define YSP_ERR_ARG (-2)          /* invalid argument or size */
+#define YSP_ERR_UNSUPPORTED (-6)  /* shape outside what the kernels implement */
+
+#define YSP_MAX_UNION_CHILDREN 8
+#define YSP_MAX_PROGRAM_CONNS 8
+#define YSP_MAX_PROGRAM_UNIONS 8
+
+/* ---------------------------------------------------------------- runtime */
+const char* ysp_last_error(void);
+int ysp_version(void);
+int ysp_sync(void* stream);
+/* Frees the library's grow-on-demand scratch buffers. */
+void ysp_release_scratch(void);
+
+/* A union of per-child FP64 attribute arrays, e.g. [fixed_vertices.position,
+ * moving_vertices.position] stacked in one union primitive
+ * (yasps/yasps/primitive/primitiveUnion.pyx:96-105,115-210). counts is a device
+ * array of the children's instance counts (children_primitive_counts_gpu).
+ * Union instance v belongs to the first child c with sum(counts[0..c]) > v and reads
+ * data[c][(v - sum(counts[0..c-1])) * width + k]. */
+typedef struct ysp_union_f64 {
+  uint32_t num_children;
+  uint32_t width;                          /* doubles per instance (3 for a position) */
+  const uint32_t* counts;                  /* device [num_children] */
+  const double* data[YSP_MAX_UNION_CHILDREN];
+} ysp_union_f64;
+
+/* ------------------------------------------------------- index pipeline */
+/* Index program bytecode (int32, device memory). Every node starts with
+ * [op, k (index slots), g (gradient size), len (ints in this node incl. children)]:
+ *   LEAF   [0, 1, size, 6, wrt_pos (-1 = not a target), per_instance (0/1)]
+ *   JOIN   [1, k, g, len, conn_slot, dim, n_children, child...]
+ *   UNION  [2, k, g, len, counts_slot, n_union_children,
+ *           for each union child: n_members, member...]
+ *   CONCAT [3, k, g, len, n_children, child...]
+ * conns[conn_slot] is a row-major uint32 connectivity [instances x dim];
+ * union_counts[counts_slot] the device child-count array of that union.
+ * Replaces the generated get_indices(<join conns>, <union counts>, wrtStartIndices,
+ * outputIndices, outputSizes, NUM_INSTANCES)
+ * (yasps/yasps/kernel/Coordinate/gradientIndicesKernel.pyx:904-926): same slot
+ * encoding (scalar start + 2; 1 = not a target; 0 = union padding, ``:699-713,809-811``).
+ * out_idx / out_sizes must be zeroed by the caller (``:969-972``). */
+int ysp_get_indices(const int32_t* program, int32_t program_len,
+                    const uint32_t* const* conns, int32_t num_conns,
+                    const uint32_t* const* union_counts, int32_t num_unions,
+                    const uint32_t* wrt_start_indices,
+                    uint32_t* out_indices, uint16_t* out_sizes,
+                    uint32_t num_instances, uint32_t num_indices_for_each_instance,
+                    void* stream);
+
+/* Replaces compress_indices (gradientIndicesKernel.pyx:132-214), same arguments:
+ * local permutations, compressed gradient sizes, coordinate-count prefix
+ * (coordinate_counts_outer[0] = 0 is written here), stable grouping of instances by
+ * compressed size (unique_sizes, grouped_indices, offsets[U+1]) and the device
+ * count of unique sizes. */
+int ysp_compress_indices(const uint32_t* indices, const uint16_t* index_sizes,
+                         int16_t* permutations, uint16_t* total_gradient_sizes,
+                         uint32_t* coordinate_counts_outer, uint16_t* unique_sizes,
+                         uint32_t* grouped_indices, uint32_t* offsets,
+                         uint16_t* num_unique,
+                         uint32_t num_instances, uint32_t num_indices_for_each_instance,
+                         uint32_t compute_coordinate_metadata, void* stream);
+
+/* Replaces computeCoordinates (gradientIndicesKernel.pyx:234-318). */
+int ysp_compute_coordinates(const uint32_t* indices, const int16_t* permutations,
+                            const uint16_t* index_sizes, uint32_t* coordinates,
+                            uint16_t* dimensions, const uint32_t* coordinate_counts_outer,
+                            uint32_t num_indices_for_each_instance, uint32_t num_instances,
+                            void* stream);
+
+/* 16-byte record layout of coord_dim_tmp (coordinateCompressionKernel.pyx:14-20). */
+typedef struct ysp_coord_dim {
+  uint32_t row, col;
+  uint16_t h, w;
+  uint32_t placeholder;
+} ysp_coord_dim;
+
+/* Replaces get_unique_coords (coordinateCompressionKernel.pyx:93-144): sorts the
+ * coordinates by (h, w, row, col), writes each occurrence's unique ordinal to lookup
+ * and the unique records to coord_dim_tmp[0..num_unique). Synchronous; the count is
+ * returned through *num_unique_coords (host). */
+int ysp_get_unique_coords(const uint32_t* coords, const uint16_t* dims,
+                          uint32_t num_coordinates, ysp_coord_dim* coord_dim_tmp,
+                          uint32_t* lookup, uint32_t* num_unique_coords, void* stream);
+
+/* Replaces compress_unique_coords (coordinateCompressionKernel.pyx:264-419):
+ * unique (row, col) pairs, per-(h,w) category dims / value starts (doubles) / counts,
+ * and lookup -> value offset in doubles. Synchronous (returns the category count
+ * through *num_unique_dims). */
+int ysp_compress_unique_coords(const ysp_coord_dim* coord_dim_tmp, uint32_t num_unique_coords,
+                               uint32_t num_coordinates,
+                               uint32_t* unique_coords_out, uint16_t* unique_dims_out,
+                               uint32_t* unique_dims_outer_indices,
+                               uint32_t* unique_dims_block_counts,
+                               uint32_t* lookup, uint32_t* num_unique_dims, void* stream);
+
+/* ----------------------------------------------- element gradient/Hessian */
+/* Shared index/output block of every element entry point; the same arguments the
+ * generated compute_hessian_and_gradient_with_compression receives
+ * (yasps/yasps/kernel/Hessian/hessianKernelHost.pyx:27-50, consumed as in
+ * hessianKernelNoProject.pyx:16-264). gradient / hessian_blocks / diagonal_blocks are
+ * accumulated into (FP64 atomics); the caller zeroes them. */
+typedef struct ysp_scatter_args {
+  const uint32_t* segment_indices;       /* [N x K] */
+  const uint16_t* segment_sizes;         /* [N x K] */
+  const int16_t* local_permutations;     /* [N x K] */
+  const uint32_t* lookups;               /* value offset per coordinate */
+  const uint32_t* coordinates_outer;     /* [N + 1] */
+  uint32_t max_num_indices;              /* K */
+  uint32_t num_attributes;               /* number of minimisation targets */
+  const uint32_t* diagonal_blocks_start; /* device [num_attributes + 1] */
+  const uint32_t* gradient_segments_start; /* device [num_attributes + 1] */
+  double* gradient;
+  double* hessian_blocks;
+  double* diagonal_blocks;
+} ysp_scatter_args;
+
+/* Rest-shape data of the Stable Neo-Hookean tet (examples/one_bunny/helpers.py:42-54):
+ * dm_inv = (TB^T)^-1 row-major [T x 9] and the signed vol = det(TB^T)/6 [T]. */
+int ysp_snh_rest_data(const ysp_union_f64* rest_positions, const uint32_t* tet2v,
+                      uint32_t num_tets, double* dm_inv, double* vol, void* stream);
+
+/* Stable Neo-Hookean tet term (examples/one_bunny/helpers.py:42-67), PSD projection
+ * of the 12x12 local Hessian (hessianKernelHeader.pyx:79-252, method 0/1/2) and the
+ * NoProject scatter. mu/lam are device arrays read at index tet*stride (stride 0 =
+ * one value for every tet). Replaces the generated kernel of the energy
+ * ``stable_neo_hookean`` with projection_method = 2 (one_bunny.py:149). */
+int ysp_eval_snh_tet(const ysp_union_f64* positions, const uint32_t* tet2v,
+                     uint32_t num_tets, const double* dm_inv, const double* vol,
+                     const double* mu, uint32_t mu_stride,
+                     const double* lam, uint32_t lam_stride, double dt,
+                     uint32_t projection_method, const ysp_scatter_args* out,
+                     void* stream);
+
+/* Backward-Euler inertia on the moving vertices (examples/one_bunny/helpers.py:69-71):
+ * x~ = x_prev + v dt - (0, 9.8 dt^2, 0), E = 1/2 m |x - x~|^2. */
+int ysp_eval_inertia(const double* position, const double* last_position,
+                     const double* velocity, const double* mass, double dt,
+                     uint32_t num_vertices, uint32_t projection_method,
+                     const ysp_scatter_args* out, void* stream);
+
+/* Per-element energies (value kernels of kernel/Compute/globalKernel.pyx:35-434, used
+ * by minimizer.computeTotalEnergy, minimizer.pyx:396-407). */
+int ysp_energy_snh_tet(const ysp_union_f64* positions, const uint32_t* tet2v,
+                       uint32_t num_tets, const double* dm_inv, const double* vol,
+                       const double* mu, uint32_t mu_stride, const double* lam,
+                       uint32_t lam_stride, double dt, double* energy, void* stream);
+int ysp_energy_inertia(const double* position, const double* last_position,
+                       const double* velocity, const double* mass, double dt,
+                       uint32_t num_vertices, double* energy, void* stream);
+
+/* --------------------------------------------------------- block storage */
+/* Symmetric upper block storage of one Hessian part (hessian.pyx:783-808,
+ * matrix.pyx:138-282). Host arrays describe the (h, w) categories. */
+typedef struct ysp_block_matrix {
+  const double* values;              /* blocks_flattened */
+  const uint32_t* positions;         /* device [2 x nb] (row, col) scalar starts */
+  uint32_t num_categories;
+  const uint32_t* starts;            /* host [num_categories] value offsets */
+  const uint32_t* counts;            /* host [num_categories] */
+  const uint32_t* dims;              /* host [2 x num_categories] (h, w) */
+} ysp_block_matrix;
+
+/* y (+)= H x for symmetric upper storage: y_r += B x_c and, r != c, y_c += B^T x_r
+ * (matrix.pyx:21-86 / solverKernel.pyx:157-274). transpose applies H^T (identical for
+ * square symmetric storage). accumulate=0 overwrites y. */
+int ysp_block_spmv(const ysp_block_matrix* part, const ysp_block_matrix* dynamic_part,
+                   const double* x, double* y, uint32_t n, int transpose, int accumulate,
+                   void* stream);
+
+/* ------------------------------------------------------------ solver */
+/* Replaces invert_diagonal_blocks (kernel/Solver/diagonalBlockInverseKernel.pyx:
+ * 107-136): per-attribute blocks [count x size x size]; eigenvalues |l| < 1e-6 are
+ * kept as |l|, others become |1/l| (``:69,91``). starts/counts/sizes are host arrays
+ * (starts in doubles). */
+int ysp_invert_diagonal_blocks(const double* diagonal_blocks, double* inverse,
+                               const uint32_t* starts, const uint32_t* counts,
+                               const uint32_t* sizes, int32_t num_attributes, void* stream);
+
+typedef struct ysp_pcg_stats {
+  int32_t iterations;
+  int32_t status;              /* return code, see ysp_pcg_solve */
+  double delta0;               /* r0^T M^-1 r0 */
+  double delta;                /* final r^T M^-1 r */
+  double rr0;                  /* ||b||^2 */
+  double rr;                   /* final recurrence ||r||^2 */
+} ysp_pcg_stats;
+
+#define YSP_PCG_REFERENCE_RATIO 0   /* stop when r^T M^-1 r <= tol * r0^T M^-1 r0 */
+#define YSP_PCG_RELATIVE_RESIDUAL 1 /* stop when ||r|| <= tol * ||b|| (recurrence r) */
+
+/* Replaces computeSolution (kernel/Solver/solverKernel.pyx:477-687): block-Jacobi PCG
+ * with device-resident recurrence scalars. threshold_mode selects the stopping rule.
+ * work must hold 5 * n doubles, cg_scalars 64 doubles. Synchronous. Returns the
+ * iteration count (>= 0), -4 when max_iterations is reached, -5 for an invalid
+ * initial residual, -(it + 4) for a non-positive curvature at iteration it. */
+int ysp_pcg_solve(uint32_t max_iterations, double threshold, int32_t threshold_mode,
+                  const ysp_block_matrix* part, const ysp_block_matrix* dynamic_part,
+                  const double* diagonal_blocks_inverse,
+                  const uint32_t* diagonal_blocks_start, const uint32_t* diagonal_blocks_count,
+                  const uint32_t* diagonal_blocks_size, const uint32_t* gradient_segments_start,
+                  int32_t num_attributes,
+                  const double* rhs, uint32_t n,
+                  double* solution, const double* initial_guess, int32_t zero_initial_guess,
+                  double* work, double* cg_scalars, ysp_pcg_stats* stats, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* YASPS_AMD_H */

@@ -1,0 +1,115 @@
+"""ctypes binding of ``libyasps_amd.so`` (the C ABI declared in ``include/yasps_amd.h``).
+
+The library is required: importing the kernels without it raises, there is no CPU
+fallback for any device entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libyasps_amd.so"
+
+MAX_UNION_CHILDREN = 8
+MAX_PROGRAM_CONNS = 8
+MAX_PROGRAM_UNIONS = 8
+
+
+class UnionF64(C.Structure):
+    _fields_ = [("num_children", C.c_uint32), ("width", C.c_uint32),
+                ("counts", C.c_void_p), ("data", C.c_void_p * MAX_UNION_CHILDREN)]
+
+
+class ScatterArgs(C.Structure):
+    _fields_ = [("segment_indices", C.c_void_p), ("segment_sizes", C.c_void_p),
+                ("local_permutations", C.c_void_p), ("lookups", C.c_void_p),
+                ("coordinates_outer", C.c_void_p), ("max_num_indices", C.c_uint32),
+                ("num_attributes", C.c_uint32), ("diagonal_blocks_start", C.c_void_p),
+                ("gradient_segments_start", C.c_void_p), ("gradient", C.c_void_p),
+                ("hessian_blocks", C.c_void_p), ("diagonal_blocks", C.c_void_p)]
+
+
+class BlockMatrix(C.Structure):
+    _fields_ = [("values", C.c_void_p), ("positions", C.c_void_p),
+                ("num_categories", C.c_uint32), ("starts", C.POINTER(C.c_uint32)),
+                ("counts", C.POINTER(C.c_uint32)), ("dims", C.POINTER(C.c_uint32))]
+
+
+class CoordDim(C.Structure):
+    _fields_ = [("row", C.c_uint32), ("col", C.c_uint32), ("h", C.c_uint16),
+                ("w", C.c_uint16), ("placeholder", C.c_uint32)]
+
+
+class PcgStats(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("status", C.c_int32), ("delta0", C.c_double),
+                ("delta", C.c_double), ("rr0", C.c_double), ("rr", C.c_double)]
+
+
+P = C.c_void_p
+U32 = C.c_uint32
+I32 = C.c_int32
+D = C.c_double
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "ysp_last_error": (C.c_char_p, []),
+    "ysp_version": (C.c_int, []),
+    "ysp_sync": (C.c_int, [P]),
+    "ysp_release_scratch": (None, []),
+    "ysp_get_indices": (C.c_int, [P, I32, P, I32, P, I32, P, P, P, U32, U32, P]),
+    "ysp_compress_indices": (C.c_int, [P, P, P, P, P, P, P, P, P, U32, U32, U32, P]),
+    "ysp_compute_coordinates": (C.c_int, [P, P, P, P, P, P, U32, U32, P]),
+    "ysp_get_unique_coords": (C.c_int, [P, P, U32, P, P, C.POINTER(U32), P]),
+    "ysp_compress_unique_coords": (C.c_int, [P, U32, U32, P, P, P, P, P, C.POINTER(U32), P]),
+    "ysp_snh_rest_data": (C.c_int, [C.POINTER(UnionF64), P, U32, P, P, P]),
+    "ysp_eval_snh_tet": (C.c_int, [C.POINTER(UnionF64), P, U32, P, P, P, U32, P, U32, D, U32,
+                                   C.POINTER(ScatterArgs), P]),
+    "ysp_eval_inertia": (C.c_int, [P, P, P, P, D, U32, U32, C.POINTER(ScatterArgs), P]),
+    "ysp_energy_snh_tet": (C.c_int, [C.POINTER(UnionF64), P, U32, P, P, P, U32, P, U32, D, P, P]),
+    "ysp_energy_inertia": (C.c_int, [P, P, P, P, D, U32, P, P]),
+    "ysp_block_spmv": (C.c_int, [C.POINTER(BlockMatrix), C.POINTER(BlockMatrix), P, P, U32,
+                                 C.c_int, C.c_int, P]),
+    "ysp_invert_diagonal_blocks": (C.c_int, [P, P, C.POINTER(U32), C.POINTER(U32),
+                                             C.POINTER(U32), I32, P]),
+    "ysp_pcg_solve": (C.c_int, [U32, D, I32, C.POINTER(BlockMatrix), C.POINTER(BlockMatrix), P,
+                                C.POINTER(U32), C.POINTER(U32), C.POINTER(U32), C.POINTER(U32),
+                                I32, P, U32, P, P, I32, P, P, C.POINTER(PcgStats), P]),
+}
+
+_lib = None
+
+
+def load(path: os.PathLike | None = None):
+    """Load the shared library once and attach the signatures. Raises if missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise ImportError(
+            f"yasps_amd: {p} is missing. Build it with `python -m yasps_amd.build` "
+            "(hipcc, gfx950); there is no CPU fallback.")
+    lib = C.CDLL(str(p))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class YspError(RuntimeError):
+    pass
+
+
+def check(rc: int, what: str) -> int:
+    if rc < 0 and rc in (-1, -2, -6):
+        msg = load().ysp_last_error()
+        raise YspError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+    return rc
+
+
+def u32_array(values):
+    arr = (C.c_uint32 * max(1, len(values)))(*[int(v) for v in values])
+    return arr

@@ -1,0 +1,94 @@
+// Small dense FP64 linear algebra held in registers: the packed-Schur PSD test and a
+// cyclic Jacobi symmetric eigensolver (Golub & Van Loan, "Matrix Computations",
+// section 8.5: symmetric Schur 2x2 rotations swept row by row).
+#pragma once
+#include "common.hpp"
+
+namespace ysp {
+
+// Packed-Schur PSD test (hessianKernelHeader.pyx:79-141) on a packed lower triangle
+// L[r*(r+1)/2 + c] (c <= r) that it overwrites.
+template <int N>
+__device__ __forceinline__ bool psd_test_packed(double* L) {
+  double scale = 1.0;
+#pragma unroll
+  for (int r = 0; r < N; ++r) scale = fmax(scale, fabs(L[r * (r + 1) / 2 + r]));
+  const double tol = scale * 1.0e-10;
+  bool ok = true;
+#pragma unroll
+  for (int c = 0; c < N; ++c) {
+    const double pivot = L[c * (c + 1) / 2 + c];
+    if (pivot < -tol) ok = false;
+    const bool small = fabs(pivot) <= tol;
+    if (small) {
+#pragma unroll
+      for (int r = c + 1; r < N; ++r)
+        if (fabs(L[r * (r + 1) / 2 + c]) > tol) ok = false;
+    } else {
+      const double inv = 1.0 / pivot;
+#pragma unroll
+      for (int r = c + 1; r < N; ++r) {
+        const double factor = L[r * (r + 1) / 2 + c] * inv;
+#pragma unroll
+        for (int t = c + 1; t <= r; ++t)
+          L[r * (r + 1) / 2 + t] = fma(-factor, L[t * (t + 1) / 2 + c], L[r * (r + 1) / 2 + t]);
+      }
+    }
+  }
+  return ok;
+}
+
+// Cyclic Jacobi eigen-decomposition of a symmetric N x N matrix held in registers.
+// On return A is diagonal (eigenvalues) and V holds the eigenvectors as columns.
+template <int N>
+__device__ __forceinline__ void jacobi_eigen(double (&A)[N][N], double (&V)[N][N]) {
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) V[i][j] = (i == j) ? 1.0 : 0.0;
+  double fro = 0.0;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) fro += A[i][j] * A[i][j];
+  const double stop = fro * 1.0e-32;
+  for (int sweep = 0; sweep < 24; ++sweep) {
+    double off = 0.0;
+#pragma unroll
+    for (int p = 0; p < N; ++p)
+#pragma unroll
+      for (int q = p + 1; q < N; ++q) off += A[p][q] * A[p][q];
+    if (off <= stop) break;
+#pragma unroll
+    for (int p = 0; p < N; ++p)
+#pragma unroll
+      for (int q = p + 1; q < N; ++q) {
+        const double apq = A[p][q];
+        if (apq == 0.0) continue;
+        const double theta = (A[q][q] - A[p][p]) / (2.0 * apq);
+        const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(1.0 + theta * theta));
+        const double c = 1.0 / sqrt(1.0 + t * t);
+        const double s = t * c;
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          const double akp = A[k][p], akq = A[k][q];
+          A[k][p] = c * akp - s * akq;
+          A[k][q] = s * akp + c * akq;
+        }
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          const double apk = A[p][k], aqk = A[q][k];
+          A[p][k] = c * apk - s * aqk;
+          A[q][k] = s * apk + c * aqk;
+        }
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          const double vkp = V[k][p], vkq = V[k][q];
+          V[k][p] = c * vkp - s * vkq;
+          V[k][q] = s * vkp + c * vkq;
+        }
+      }
+  }
+}
+
+}  // namespace ysp

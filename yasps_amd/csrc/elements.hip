@@ -1,0 +1,664 @@
+// Element gradient/Hessian kernels: Stable Neo-Hookean tet and inertia, PSD projection
+// and the scatter into upper block storage.
+//
+// SNH closed form (derived by hand from examples/one_bunny/helpers.py:42-67):
+//   F = Ds Dm^-1, I_C = |F|^2, J = det F, cof F = [f1 x f2, f2 x f0, f0 x f1] (columns),
+//   a1 = mu (1 - 1/(I_C+1)), a2 = 2 mu/(I_C+1)^2, a3 = lam (J - alpha), alpha = 1 + 3mu/(4lam),
+//   shape vectors g_a (a = 0..3): g_0 = -sum_k Dm^-1[k,:], g_{k+1} = Dm^-1[k,:],
+//   u_a = F g_a, w_a = cof(F) g_a, s = dt^2 vol,
+//   grad_a = s (a1 u_a + a3 w_a),
+//   H_ab   = s (a1 (g_a.g_b) I + a2 u_a u_b^T + lam w_a w_b^T - a3 [F (g_a x g_b)]x).
+// The Hessian annihilates translations, so with an orthonormal basis U (4x3) of the
+// complement of (1,1,1,1), H = (U x I3) M (U x I3)^T where M is the same formula on the
+// three "virtual vertices" c_k = sum_a U[a,k] g_a. The PSD projection of H is therefore
+// (U x I3) proj(M) (U x I3)^T: a 9x9 eigenproblem instead of 12x12.
+//
+// Projection decision: the packed Schur PSD test of hessianKernelHeader.pyx:79-141 runs on
+// the 12x12 H; only elements that fail it are projected (second kernel, work queue).
+#include "common.hpp"
+#include "dense.hpp"
+
+namespace ysp {
+namespace {
+
+constexpr unsigned kBlock = 128;
+
+struct UnionView {
+  uint32_t nc;
+  uint32_t width;
+  const uint32_t* counts;
+  const double* data[YSP_MAX_UNION_CHILDREN];
+};
+
+UnionView make_view(const ysp_union_f64* u) {
+  UnionView v{};
+  v.nc = u->num_children;
+  v.width = u->width;
+  v.counts = u->counts;
+  for (uint32_t i = 0; i < YSP_MAX_UNION_CHILDREN; ++i) v.data[i] = u->data[i];
+  return v;
+}
+
+__device__ __forceinline__ const double* union_row(const UnionView& U, uint32_t v) {
+  uint32_t total = 0;
+  for (uint32_t c = 0; c < U.nc; ++c) {
+    const uint32_t cnt = U.counts[c];
+    if (v < total + cnt) return U.data[c] + static_cast<size_t>(v - total) * U.width;
+    total += cnt;
+  }
+  return nullptr;
+}
+
+struct ScatterArgs {
+  const uint32_t* seg_idx;
+  const uint16_t* seg_size;
+  const int16_t* perm;
+  const uint32_t* lookups;
+  const uint32_t* coord_outer;
+  uint32_t K;
+  uint32_t num_attr;
+  const uint32_t* diag_start;
+  const uint32_t* grad_start;
+  double* gradient;
+  double* blocks;
+  double* diag_blocks;
+};
+
+ScatterArgs make_scatter(const ysp_scatter_args* a) {
+  return ScatterArgs{a->segment_indices, a->segment_sizes, a->local_permutations, a->lookups,
+                     a->coordinates_outer, a->max_num_indices, a->num_attributes,
+                     a->diagonal_blocks_start, a->gradient_segments_start, a->gradient,
+                     a->hessian_blocks, a->diagonal_blocks};
+}
+
+constexpr int kMaxSlots = 16;
+
+// Slot bookkeeping of one element (hessianKernelNoProject.pyx:93-163): groups of slots
+// with the same |perm| in first-occurrence order, and the local 3-vector of every slot.
+struct SlotInfo {
+  uint32_t idx[kMaxSlots];
+  int8_t vtx[kMaxSlots];      // local vertex of the slot, -1 for zero-size padding
+  int8_t group_first[kMaxSlots];
+  uint32_t group_mask[kMaxSlots];
+  int ng;
+  bool ok;
+};
+
+__device__ __forceinline__ void load_slots(const ScatterArgs& S, uint32_t e, SlotInfo& si) {
+  const uint32_t K = S.K;
+  si.ng = 0;
+  si.ok = K <= kMaxSlots;
+  if (!si.ok) return;
+  int placement[kMaxSlots];
+  uint32_t col = 0;
+  for (uint32_t i = 0; i < K; ++i) {
+    const size_t o = static_cast<size_t>(e) * K + i;
+    const uint32_t ii = S.seg_idx[o];
+    const uint32_t sz = S.seg_size[o];
+    const int pm = S.perm[o];
+    si.idx[i] = ii;
+    if (sz == 3) {
+      si.vtx[i] = static_cast<int8_t>(col / 3);
+    } else {
+      si.vtx[i] = -1;
+      if (sz != 0) si.ok = false;
+    }
+    col += sz;
+    if (pm == 0) continue;
+    const int p = (pm < 0 ? -pm : pm) - 1;
+    int g = -1;
+    for (int k = 0; k < si.ng; ++k)
+      if (placement[k] == p) g = k;
+    if (g < 0) {
+      g = si.ng++;
+      placement[g] = p;
+      si.group_first[g] = static_cast<int8_t>(i);
+      si.group_mask[g] = 0;
+    }
+    si.group_mask[g] |= 1u << i;
+  }
+}
+
+__device__ __forceinline__ uint32_t diag_offset(const ScatterArgs& S, uint32_t seg, uint32_t size) {
+  uint32_t attr = 0;
+  while (attr < S.num_attr && seg >= S.grad_start[attr + 1]) ++attr;
+  const uint32_t inst = (seg - S.grad_start[attr]) / size;
+  return S.diag_start[attr] + inst * size * size;
+}
+
+// Add gradient segments (hessianKernelNoProject.pyx:63-88).
+template <typename GradFn>
+__device__ __forceinline__ void scatter_gradient(const ScatterArgs& S, const SlotInfo& si,
+                                                 GradFn grad) {
+  for (uint32_t i = 0; i < S.K; ++i) {
+    if (si.idx[i] < 2 || si.vtx[i] < 0) continue;
+    const uint32_t p = si.idx[i] - 2;
+    double g3[3];
+    grad(si.vtx[i], g3);
+    atomic_add_f64(S.gradient + p, g3[0]);
+    atomic_add_f64(S.gradient + p + 1, g3[1]);
+    atomic_add_f64(S.gradient + p + 2, g3[2]);
+  }
+}
+
+// Add Hessian blocks (hessianKernelNoProject.pyx:165-264). blockFn(va, vb, out[3][3]) adds
+// the local 3x3 block between local vertices va and vb into out.
+template <typename BlockFn>
+__device__ __forceinline__ void scatter_hessian(const ScatterArgs& S, uint32_t e,
+                                                const SlotInfo& si, BlockFn blockfn) {
+  uint32_t count = 0;
+  const uint32_t cstart = S.coord_outer[e];
+  for (int gi = 0; gi < si.ng; ++gi) {
+    const uint32_t ii = si.idx[si.group_first[gi]];
+    if (ii < 2) continue;
+    for (int gj = gi; gj < si.ng; ++gj) {
+      const uint32_t jj = si.idx[si.group_first[gj]];
+      if (jj < 2) continue;
+      double blk[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      for (uint32_t a = 0; a < S.K; ++a) {
+        if (!((si.group_mask[gi] >> a) & 1u) || si.vtx[a] < 0) continue;
+        for (uint32_t b = 0; b < S.K; ++b) {
+          if (!((si.group_mask[gj] >> b) & 1u) || si.vtx[b] < 0) continue;
+          blockfn(si.vtx[a], si.vtx[b], blk);
+        }
+      }
+      const uint32_t place = S.lookups[cstart + count];
+      double* dst = S.blocks + place;
+      if (ii < jj) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+          for (int l = 0; l < 3; ++l) atomic_add_f64(dst + 3 * k + l, blk[k][l]);
+      } else {
+        double* dd = nullptr;
+        if (gi == gj && S.diag_blocks) dd = S.diag_blocks + diag_offset(S, ii - 2, 3);
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+          for (int l = 0; l < 3; ++l) {
+            atomic_add_f64(dst + 3 * k + l, blk[l][k]);
+            if (dd) atomic_add_f64(dd + 3 * k + l, blk[l][k]);
+          }
+      }
+      ++count;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ SNH element
+struct Snh {
+  double F[3][3];    // F[i][j]
+  double cof[3][3];  // cof[i][j] = column j of cof F
+  double g[4][3];    // shape vectors
+  double u[4][3];
+  double w[4][3];
+  double s, a1, a2, a3, lam, ic, J;
+};
+
+__device__ __forceinline__ void cross3(const double* a, const double* b, double* c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+__device__ __forceinline__ void snh_setup(const double x[4][3], const double* Di, double vol,
+                                          double mu, double lam, double dt, Snh& S) {
+  double d[3][3];  // d[k] = x_{k+1} - x_0
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) d[k][i] = x[k + 1][i] - x[0][i];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      S.F[i][j] = d[0][i] * Di[0 * 3 + j] + d[1][i] * Di[1 * 3 + j] + d[2][i] * Di[2 * 3 + j];
+  double f[3][3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) f[j][i] = S.F[i][j];
+  double c0[3], c1[3], c2[3];
+  cross3(f[1], f[2], c0);
+  cross3(f[2], f[0], c1);
+  cross3(f[0], f[1], c2);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    S.cof[i][0] = c0[i];
+    S.cof[i][1] = c1[i];
+    S.cof[i][2] = c2[i];
+  }
+  double ic = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) ic += S.F[i][j] * S.F[i][j];
+  const double J = f[0][0] * c0[0] + f[0][1] * c0[1] + f[0][2] * c0[2];
+  const double alpha = 1.0 + 0.75 * mu / lam;
+  const double ip1 = 1.0 / (ic + 1.0);
+  S.ic = ic;
+  S.J = J;
+  S.a1 = mu * (1.0 - ip1);
+  S.a2 = 2.0 * mu * ip1 * ip1;
+  S.a3 = lam * (J - alpha);
+  S.lam = lam;
+  S.s = dt * dt * vol;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    S.g[0][j] = -(Di[0 * 3 + j] + Di[1 * 3 + j] + Di[2 * 3 + j]);
+    S.g[1][j] = Di[0 * 3 + j];
+    S.g[2][j] = Di[1 * 3 + j];
+    S.g[3][j] = Di[2 * 3 + j];
+  }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      S.u[a][i] = S.F[i][0] * S.g[a][0] + S.F[i][1] * S.g[a][1] + S.F[i][2] * S.g[a][2];
+      S.w[a][i] = S.cof[i][0] * S.g[a][0] + S.cof[i][1] * S.g[a][1] + S.cof[i][2] * S.g[a][2];
+    }
+}
+
+// 3x3 block of the SNH Hessian for shape vectors ga, gb with u/w images.
+__device__ __forceinline__ void snh_block(const Snh& S, const double* ga, const double* gb,
+                                          const double* ua, const double* ub, const double* wa,
+                                          const double* wb, double out[3][3]) {
+  const double gg = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+  double c[3];
+  cross3(ga, gb, c);
+  double t[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) t[i] = S.F[i][0] * c[0] + S.F[i][1] * c[1] + S.F[i][2] * c[2];
+  // [t]x = [[0,-t2,t1],[t2,0,-t0],[-t1,t0,0]]
+  const double tx[3][3] = {{0.0, -t[2], t[1]}, {t[2], 0.0, -t[0]}, {-t[1], t[0], 0.0}};
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      double v = S.a2 * ua[i] * ub[k] + S.lam * wa[i] * wb[k] - S.a3 * tx[i][k];
+      if (i == k) v += S.a1 * gg;
+      out[i][k] = S.s * v;
+    }
+}
+
+__device__ __forceinline__ double snh_energy_value(const Snh& S, double mu) {
+  const double alpha_gap = S.a3 / S.lam;  // J - alpha
+  return S.s * (0.5 * mu * (S.ic - 3.0) - 0.5 * mu * log(S.ic + 1.0) +
+                0.5 * S.lam * alpha_gap * alpha_gap);
+}
+
+__device__ __forceinline__ void load_tet(const UnionView& P, const uint32_t* tet2v, uint32_t e,
+                                         double x[4][3], uint32_t vid[4]) {
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    vid[a] = tet2v[4 * static_cast<size_t>(e) + a];
+    const double* p = union_row(P, vid[a]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) x[a][i] = p[i];
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) snh_rest_kernel(UnionView R, const uint32_t* tet2v,
+                                                          uint32_t n, double* dm_inv, double* vol) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  double x[4][3];
+  uint32_t vid[4];
+  load_tet(R, tet2v, e, x, vid);
+  double m[3][3];  // columns = rest edges
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) m[i][k] = x[k + 1][i] - x[0][i];
+  const double c00 = m[1][1] * m[2][2] - m[1][2] * m[2][1];
+  const double c01 = m[1][2] * m[2][0] - m[1][0] * m[2][2];
+  const double c02 = m[1][0] * m[2][1] - m[1][1] * m[2][0];
+  const double det = m[0][0] * c00 + m[0][1] * c01 + m[0][2] * c02;
+  const double id = 1.0 / det;
+  double* o = dm_inv + 9 * static_cast<size_t>(e);
+  o[0] = c00 * id;
+  o[1] = (m[0][2] * m[2][1] - m[0][1] * m[2][2]) * id;
+  o[2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) * id;
+  o[3] = c01 * id;
+  o[4] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) * id;
+  o[5] = (m[0][2] * m[1][0] - m[0][0] * m[1][2]) * id;
+  o[6] = c02 * id;
+  o[7] = (m[0][1] * m[2][0] - m[0][0] * m[2][1]) * id;
+  o[8] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * id;
+  vol[e] = det / 6.0;
+}
+
+struct SnhParams {
+  const double* dm_inv;
+  const double* vol;
+  const double* mu;
+  uint32_t mu_stride;
+  const double* lam;
+  uint32_t lam_stride;
+  double dt;
+  uint32_t method;
+};
+
+__device__ __forceinline__ void snh_load(const UnionView& P, const uint32_t* tet2v,
+                                         const SnhParams& prm, uint32_t e, Snh& S, double& mu) {
+  double x[4][3];
+  uint32_t vid[4];
+  load_tet(P, tet2v, e, x, vid);
+  double Di[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) Di[k] = prm.dm_inv[9 * static_cast<size_t>(e) + k];
+  mu = prm.mu[static_cast<size_t>(e) * prm.mu_stride];
+  const double lam = prm.lam[static_cast<size_t>(e) * prm.lam_stride];
+  snh_setup(x, Di, prm.vol[e], mu, lam, prm.dt, S);
+}
+
+// Pass 1: gradient, PSD test, scatter of PSD (or unprojected) Hessians, queue the rest.
+__global__ void __launch_bounds__(kBlock)
+    snh_eval_kernel(UnionView P, const uint32_t* __restrict__ tet2v, uint32_t n, SnhParams prm,
+                    ScatterArgs SA, uint32_t* __restrict__ queue, uint32_t* __restrict__ qcount,
+                    int* __restrict__ error_flag) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  Snh S;
+  double mu;
+  snh_load(P, tet2v, prm, e, S, mu);
+  SlotInfo si;
+  load_slots(SA, e, si);
+  if (!si.ok) {
+    atomicOr(error_flag, 1);
+    return;
+  }
+  scatter_gradient(SA, si, [&](int a, double* g3) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) g3[i] = S.s * (S.a1 * S.u[a][i] + S.a3 * S.w[a][i]);
+  });
+  bool psd = true;
+  if (prm.method != 0) {
+    double L[78];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b <= a; ++b) {
+        double blk[3][3];
+        snh_block(S, S.g[a], S.g[b], S.u[a], S.u[b], S.w[a], S.w[b], blk);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            const int r = 3 * a + i, c = 3 * b + k;
+            if (c <= r) L[r * (r + 1) / 2 + c] = blk[i][k];
+          }
+      }
+    psd = psd_test_packed<12>(L);
+  }
+  if (psd) {
+    scatter_hessian(SA, e, si, [&](int a, int b, double out[3][3]) {
+      double blk[3][3];
+      snh_block(S, S.g[a], S.g[b], S.u[a], S.u[b], S.w[a], S.w[b], blk);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) out[i][k] += blk[i][k];
+    });
+  } else {
+    const uint32_t slot = atomicAdd(qcount, 1u);
+    queue[slot] = e;
+  }
+}
+
+// Orthonormal basis of the complement of (1,1,1,1): columns of U (4x3).
+__constant__ double kU[4][3] = {
+    {0.70710678118654752440, 0.40824829046386301637, 0.28867513459481288225},
+    {-0.70710678118654752440, 0.40824829046386301637, 0.28867513459481288225},
+    {0.0, -0.81649658092772603273, 0.28867513459481288225},
+    {0.0, 0.0, -0.86602540378443864676}};
+
+// Pass 2: project the queued (non-PSD) SNH Hessians through the 9x9 reduction.
+__global__ void __launch_bounds__(64)
+    snh_project_kernel(UnionView P, const uint32_t* __restrict__ tet2v, SnhParams prm,
+                       ScatterArgs SA, const uint32_t* __restrict__ queue,
+                       const uint32_t* __restrict__ qcount) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= *qcount) return;
+  const uint32_t e = queue[q];
+  Snh S;
+  double mu;
+  snh_load(P, tet2v, prm, e, S, mu);
+  // virtual vertices
+  double cg[3][3], cu[3][3], cw[3][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      cg[k][j] = kU[0][k] * S.g[0][j] + kU[1][k] * S.g[1][j] + kU[2][k] * S.g[2][j] +
+                 kU[3][k] * S.g[3][j];
+      cu[k][j] = kU[0][k] * S.u[0][j] + kU[1][k] * S.u[1][j] + kU[2][k] * S.u[2][j] +
+                 kU[3][k] * S.u[3][j];
+      cw[k][j] = kU[0][k] * S.w[0][j] + kU[1][k] * S.w[1][j] + kU[2][k] * S.w[2][j] +
+                 kU[3][k] * S.w[3][j];
+    }
+  double M[9][9];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int l = k; l < 3; ++l) {
+      double blk[3][3];
+      snh_block(S, cg[k], cg[l], cu[k], cu[l], cw[k], cw[l], blk);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          M[3 * k + i][3 * l + j] = blk[i][j];
+          M[3 * l + j][3 * k + i] = blk[i][j];
+        }
+    }
+  double V[9][9];
+  jacobi_eigen<9>(M, V);
+  double lam_c[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    const double l = M[i][i];
+    lam_c[i] = (l < 0.0) ? (prm.method == 1 ? -l : 0.0) : l;
+  }
+  // projected M' = V diag(lam_c) V^T
+  double Mp[9][9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i)
+#pragma unroll
+    for (int j = i; j < 9; ++j) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) acc += V[i][k] * lam_c[k] * V[j][k];
+      Mp[i][j] = acc;
+      Mp[j][i] = acc;
+    }
+  SlotInfo si;
+  load_slots(SA, e, si);
+  scatter_hessian(SA, e, si, [&](int a, int b, double out[3][3]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int l = 0; l < 3; ++l) {
+        const double w = kU[a][k] * kU[b][l];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) out[i][j] += w * Mp[3 * k + i][3 * l + j];
+      }
+  });
+}
+
+__global__ void __launch_bounds__(kBlock)
+    snh_energy_kernel(UnionView P, const uint32_t* __restrict__ tet2v, uint32_t n, SnhParams prm,
+                      double* __restrict__ energy) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  Snh S;
+  double mu;
+  snh_load(P, tet2v, prm, e, S, mu);
+  energy[e] = snh_energy_value(S, mu);
+}
+
+// ------------------------------------------------------------------ inertia
+struct InertiaArgs {
+  const double* x;
+  const double* xp;
+  const double* v;
+  const double* m;
+  double dt;
+};
+
+__device__ __forceinline__ void inertia_diff(const InertiaArgs& I, uint32_t e, double d[3]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    double target = I.xp[3 * static_cast<size_t>(e) + i] + I.v[3 * static_cast<size_t>(e) + i] * I.dt;
+    if (i == 1) target -= 9.8 * I.dt * I.dt;
+    d[i] = I.x[3 * static_cast<size_t>(e) + i] - target;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+    inertia_eval_kernel(InertiaArgs I, uint32_t n, uint32_t method, ScatterArgs SA,
+                        int* __restrict__ error_flag) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  double d[3];
+  inertia_diff(I, e, d);
+  double m = I.m[e];
+  // Hessian m I: PSD for m >= -tol; otherwise the clamp rule on the eigenvalue m.
+  double mh = m;
+  if (method != 0 && m < -1.0e-10 * fmax(1.0, fabs(m))) mh = (method == 1) ? -m : 0.0;
+  SlotInfo si;
+  load_slots(SA, e, si);
+  if (!si.ok) {
+    atomicOr(error_flag, 1);
+    return;
+  }
+  scatter_gradient(SA, si, [&](int, double* g3) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) g3[i] = m * d[i];
+  });
+  scatter_hessian(SA, e, si, [&](int, int, double out[3][3]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) out[i][i] += mh;
+  });
+}
+
+__global__ void __launch_bounds__(kBlock)
+    inertia_energy_kernel(InertiaArgs I, uint32_t n, double* __restrict__ energy) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  double d[3];
+  inertia_diff(I, e, d);
+  energy[e] = 0.5 * I.m[e] * (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+}
+
+int check_flag(int* flag, hipStream_t s, const char* what) {
+  int h = 0;
+  YSP_CHECK(hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+  YSP_CHECK(hipStreamSynchronize(s));
+  if (h) {
+    ysp_set_error(__FILE__, __LINE__, what);
+    return YSP_ERR_UNSUPPORTED;
+  }
+  return YSP_OK;
+}
+
+}  // namespace
+}  // namespace ysp
+
+using namespace ysp;
+
+extern "C" int ysp_snh_rest_data(const ysp_union_f64* rest_positions, const uint32_t* tet2v,
+                                  uint32_t num_tets, double* dm_inv, double* vol, void* stream) {
+  if (num_tets == 0) return YSP_OK;
+  if (!rest_positions || rest_positions->num_children == 0 ||
+      rest_positions->num_children > YSP_MAX_UNION_CHILDREN || rest_positions->width != 3) {
+    ysp_set_error(__FILE__, __LINE__, "ysp_snh_rest_data: bad union view");
+    return YSP_ERR_ARG;
+  }
+  hipLaunchKernelGGL(snh_rest_kernel, dim3(grid_for(num_tets, kBlock)), dim3(kBlock), 0,
+                     as_stream(stream), make_view(rest_positions), tet2v, num_tets, dm_inv, vol);
+  YSP_CHECK_LAUNCH();
+  return YSP_OK;
+}
+
+extern "C" int ysp_eval_snh_tet(const ysp_union_f64* positions, const uint32_t* tet2v,
+                                uint32_t num_tets, const double* dm_inv, const double* vol,
+                                const double* mu, uint32_t mu_stride, const double* lam,
+                                uint32_t lam_stride, double dt, uint32_t projection_method,
+                                const ysp_scatter_args* out, void* stream) {
+  if (num_tets == 0) return YSP_OK;
+  if (!positions || positions->num_children == 0 ||
+      positions->num_children > YSP_MAX_UNION_CHILDREN || positions->width != 3 || !out ||
+      out->max_num_indices > kMaxSlots || projection_method > 2) {
+    ysp_set_error(__FILE__, __LINE__, "ysp_eval_snh_tet: bad arguments");
+    return YSP_ERR_ARG;
+  }
+  hipStream_t s = as_stream(stream);
+  uint32_t* queue = static_cast<uint32_t*>(scratch(30, sizeof(uint32_t) * num_tets));
+  uint32_t* qcount = static_cast<uint32_t*>(scratch(31, 2 * sizeof(uint32_t) + sizeof(int)));
+  if (!queue || !qcount) return YSP_ERR_HIP;
+  int* flag = reinterpret_cast<int*>(qcount + 1);
+  YSP_CHECK(hipMemsetAsync(qcount, 0, 2 * sizeof(uint32_t), s));
+  SnhParams prm{dm_inv, vol, mu, mu_stride, lam, lam_stride, dt, projection_method};
+  const UnionView P = make_view(positions);
+  const ScatterArgs SA = make_scatter(out);
+  hipLaunchKernelGGL(snh_eval_kernel, dim3(grid_for(num_tets, kBlock)), dim3(kBlock), 0, s, P,
+                     tet2v, num_tets, prm, SA, queue, qcount, flag);
+  YSP_CHECK_LAUNCH();
+  if (projection_method != 0) {
+    hipLaunchKernelGGL(snh_project_kernel, dim3(grid_for(num_tets, 64)), dim3(64), 0, s, P, tet2v,
+                       prm, SA, queue, qcount);
+    YSP_CHECK_LAUNCH();
+  }
+  return check_flag(flag, s, "ysp_eval_snh_tet: slot layout not supported (K>16 or size!=3)");
+}
+
+extern "C" int ysp_energy_snh_tet(const ysp_union_f64* positions, const uint32_t* tet2v,
+                                  uint32_t num_tets, const double* dm_inv, const double* vol,
+                                  const double* mu, uint32_t mu_stride, const double* lam,
+                                  uint32_t lam_stride, double dt, double* energy, void* stream) {
+  if (num_tets == 0) return YSP_OK;
+  if (!positions || positions->num_children == 0 ||
+      positions->num_children > YSP_MAX_UNION_CHILDREN || positions->width != 3) {
+    ysp_set_error(__FILE__, __LINE__, "ysp_energy_snh_tet: bad union view");
+    return YSP_ERR_ARG;
+  }
+  SnhParams prm{dm_inv, vol, mu, mu_stride, lam, lam_stride, dt, 0};
+  hipLaunchKernelGGL(snh_energy_kernel, dim3(grid_for(num_tets, kBlock)), dim3(kBlock), 0,
+                     as_stream(stream), make_view(positions), tet2v, num_tets, prm, energy);
+  YSP_CHECK_LAUNCH();
+  return YSP_OK;
+}
+
+extern "C" int ysp_eval_inertia(const double* position, const double* last_position,
+                                const double* velocity, const double* mass, double dt,
+                                uint32_t num_vertices, uint32_t projection_method,
+                                const ysp_scatter_args* out, void* stream) {
+  if (num_vertices == 0) return YSP_OK;
+  if (!out || out->max_num_indices > kMaxSlots || projection_method > 2) {
+    ysp_set_error(__FILE__, __LINE__, "ysp_eval_inertia: bad arguments");
+    return YSP_ERR_ARG;
+  }
+  hipStream_t s = as_stream(stream);
+  int* flag = static_cast<int*>(scratch(32, sizeof(int)));
+  if (!flag) return YSP_ERR_HIP;
+  YSP_CHECK(hipMemsetAsync(flag, 0, sizeof(int), s));
+  InertiaArgs I{position, last_position, velocity, mass, dt};
+  hipLaunchKernelGGL(inertia_eval_kernel, dim3(grid_for(num_vertices, kBlock)), dim3(kBlock), 0, s,
+                     I, num_vertices, projection_method, make_scatter(out), flag);
+  YSP_CHECK_LAUNCH();
+  return check_flag(flag, s, "ysp_eval_inertia: slot layout not supported");
+}
+
+extern "C" int ysp_energy_inertia(const double* position, const double* last_position,
+                                  const double* velocity, const double* mass, double dt,
+                                  uint32_t num_vertices, double* energy, void* stream) {
+  if (num_vertices == 0) return YSP_OK;
+  InertiaArgs I{position, last_position, velocity, mass, dt};
+  hipLaunchKernelGGL(inertia_energy_kernel, dim3(grid_for(num_vertices, kBlock)), dim3(kBlock), 0,
+                     as_stream(stream), I, num_vertices, energy);
+  YSP_CHECK_LAUNCH();
+  return YSP_OK;
+}

@@ -1,0 +1,416 @@
+// Block-Jacobi PCG with device-resident recurrence scalars.
+//
+// One iteration k is three kernels on one stream:
+//   (1) q = A p, plus p^T A p accumulated from the blocks' quadratic forms;
+//   (2) alpha = rz / pAp, x += alpha p, r -= alpha q, q = 0, z = M^-1 r and the partial
+//       dots r.z, r.r (one pass per diagonal block);
+//   (3) beta = rz_new / rz_old, p = z + beta p and the convergence test.
+// Scalars live in rotating slots (k mod 3) so that every kernel of iteration k reads
+// complete sums and no kernel overwrites a slot another kernel of the same launch reads.
+// Every kernel returns immediately once a status was recorded, so the host can enqueue
+// several iterations ahead and read the status once per chunk (the reference copies two
+// doubles to the host after every iteration, solverKernel.pyx:667-668).
+#include "common.hpp"
+#include "dense.hpp"
+#include "spmv.hpp"
+
+#include <algorithm>
+
+namespace ysp {
+namespace {
+
+enum : int {
+  S_RZ0 = 0,      // delta0 (reference: r0^T M^-1 r0, or b^T M^-1 b with an initial guess)
+  S_BB = 1,       // ||b||^2
+  S_STATUS = 2,   // 0 running, 1 converged, -1 non-positive curvature, -2 invalid start
+  S_ITER = 3,     // iteration at which the status was set
+  S_THR = 4,      // absolute threshold on rz (mode 0) or rr (mode 1)
+  S_MODE = 5,
+  S_PAP = 8,      // [3]
+  S_RZ = 11,      // [3]
+  S_RR = 14,      // [3]
+  S_FINAL_RZ = 17,
+  S_FINAL_RR = 18,
+  S_COUNT = 64
+};
+
+constexpr int kMaxAttr = 8;
+
+struct DiagLayout {
+  int num_attr;
+  uint32_t cum[kMaxAttr + 1];     // prefix of block counts
+  uint32_t grad_start[kMaxAttr];  // first DoF
+  uint32_t diag_start[kMaxAttr];  // first value of the inverse blocks
+  uint32_t size[kMaxAttr];
+};
+
+__device__ __forceinline__ bool locate(const DiagLayout& L, uint32_t t, uint32_t& dof0,
+                                       uint32_t& inv0, uint32_t& s) {
+  if (t >= L.cum[L.num_attr]) return false;
+  int a = 0;
+  while (t >= L.cum[a + 1]) ++a;
+  const uint32_t i = t - L.cum[a];
+  s = L.size[a];
+  dof0 = L.grad_start[a] + i * s;
+  inv0 = L.diag_start[a] + i * s * s;
+  return true;
+}
+
+constexpr int kMaxBlock = 12;
+
+// z = Minv r for one diagonal block; returns (r.z, r.r).
+__device__ __forceinline__ void apply_block(const double* __restrict__ minv, const double* r,
+                                            double* z, uint32_t s, double& rz, double& rr) {
+  for (uint32_t i = 0; i < s; ++i) {
+    double acc = 0.0;
+    for (uint32_t j = 0; j < s; ++j) acc += minv[i * s + j] * r[j];
+    z[i] = acc;
+    rz += r[i] * acc;
+    rr += r[i] * r[i];
+  }
+}
+
+constexpr int kVecBlock = 256;
+
+__global__ void __launch_bounds__(kVecBlock)
+    init_kernel(DiagLayout L, const double* __restrict__ minv, const double* __restrict__ b,
+                double* __restrict__ x, const double* __restrict__ x0, double* __restrict__ r,
+                double* __restrict__ p, double* __restrict__ z, double* __restrict__ q,
+                double* __restrict__ st, int have_guess) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  double rz = 0.0, rr = 0.0, bmb = 0.0, bb = 0.0;
+  uint32_t dof0, inv0, s;
+  if (locate(L, t, dof0, inv0, s)) {
+    double rl[kMaxBlock], zl[kMaxBlock];
+    for (uint32_t i = 0; i < s; ++i) {
+      const double bi = b[dof0 + i];
+      // with an initial guess q holds A x0 on entry
+      rl[i] = have_guess ? bi - q[dof0 + i] : bi;
+      x[dof0 + i] = have_guess ? x0[dof0 + i] : 0.0;
+      q[dof0 + i] = 0.0;
+    }
+    apply_block(minv + inv0, rl, zl, s, rz, rr);
+    if (have_guess) {
+      double bl[kMaxBlock], tmp[kMaxBlock];
+      for (uint32_t i = 0; i < s; ++i) bl[i] = b[dof0 + i];
+      apply_block(minv + inv0, bl, tmp, s, bmb, bb);
+    } else {
+      bmb = rz;
+      bb = rr;
+    }
+    for (uint32_t i = 0; i < s; ++i) {
+      r[dof0 + i] = rl[i];
+      z[dof0 + i] = zl[i];
+      p[dof0 + i] = zl[i];
+    }
+  }
+  __shared__ double red[kVecBlock / 64];
+  rz = block_sum<kVecBlock>(rz, red);
+  rr = block_sum<kVecBlock>(rr, red);
+  bmb = block_sum<kVecBlock>(bmb, red);
+  bb = block_sum<kVecBlock>(bb, red);
+  if (threadIdx.x == 0) {
+    atomic_add_f64(st + S_RZ + 0, rz);
+    atomic_add_f64(st + S_RR + 0, rr);
+    atomic_add_f64(st + S_RZ0, bmb);
+    atomic_add_f64(st + S_BB, bb);
+  }
+}
+
+__global__ void init_finish_kernel(double* st, double tol, int mode) {
+  const double rz0 = st[S_RZ0];
+  const double rz = st[S_RZ + 0];
+  const double bb = st[S_BB];
+  const double rr = st[S_RR + 0];
+  st[S_MODE] = mode;
+  if (!isfinite(rz0) || rz0 < 0.0 || !isfinite(rz) || rz < 0.0) {
+    st[S_STATUS] = -2.0;
+    st[S_ITER] = 0.0;
+    return;
+  }
+  const double thr = (mode == 0) ? tol * rz0 : tol * tol * bb;
+  st[S_THR] = thr;
+  st[S_FINAL_RZ] = rz;
+  st[S_FINAL_RR] = rr;
+  if (!isfinite(thr) || thr < 0.0) {
+    st[S_STATUS] = -2.0;
+    st[S_ITER] = 0.0;
+    return;
+  }
+  if ((mode == 0 ? rz : rr) <= thr) {
+    st[S_STATUS] = 1.0;
+    st[S_ITER] = 0.0;
+  }
+}
+
+__global__ void __launch_bounds__(kVecBlock)
+    update_kernel(DiagLayout L, const double* __restrict__ minv, double* __restrict__ x,
+                  double* __restrict__ r, const double* __restrict__ p, double* __restrict__ q,
+                  double* __restrict__ z, double* __restrict__ st, int k) {
+  if (st[S_STATUS] != 0.0) return;
+  const int cur = k % 3, prev = (k + 2) % 3, next = (k + 1) % 3;
+  const double pap = st[S_PAP + cur];
+  const double rzo = st[S_RZ + prev];
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (!isfinite(pap) || pap <= 0.0) {
+    if (t == 0) {
+      st[S_STATUS] = -1.0;
+      st[S_ITER] = k;
+    }
+    return;
+  }
+  const double alpha = rzo / pap;
+  if (t == 0) {
+    st[S_PAP + next] = 0.0;
+    st[S_RZ + next] = 0.0;
+    st[S_RR + next] = 0.0;
+  }
+  double rz = 0.0, rr = 0.0;
+  uint32_t dof0, inv0, s;
+  if (locate(L, t, dof0, inv0, s)) {
+    double rl[kMaxBlock], zl[kMaxBlock];
+    for (uint32_t i = 0; i < s; ++i) {
+      const uint32_t d = dof0 + i;
+      x[d] = x[d] + p[d] * alpha;
+      rl[i] = r[d] - q[d] * alpha;
+      r[d] = rl[i];
+      q[d] = 0.0;
+    }
+    apply_block(minv + inv0, rl, zl, s, rz, rr);
+    for (uint32_t i = 0; i < s; ++i) z[dof0 + i] = zl[i];
+  }
+  __shared__ double red[kVecBlock / 64];
+  rz = block_sum<kVecBlock>(rz, red);
+  rr = block_sum<kVecBlock>(rr, red);
+  if (threadIdx.x == 0) {
+    atomic_add_f64(st + S_RZ + cur, rz);
+    atomic_add_f64(st + S_RR + cur, rr);
+  }
+}
+
+__global__ void __launch_bounds__(kVecBlock)
+    direction_kernel(const double* __restrict__ z, double* __restrict__ p, double* __restrict__ st,
+                     uint32_t n, int k) {
+  if (st[S_STATUS] != 0.0) return;
+  const int cur = k % 3, prev = (k + 2) % 3;
+  const double rzn = st[S_RZ + cur];
+  const double rzo = st[S_RZ + prev];
+  const double rr = st[S_RR + cur];
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (!isfinite(rzn) || rzn < 0.0 || !isfinite(rzo) || rzo <= 0.0) {
+    if (t == 0) {
+      st[S_STATUS] = -1.0;
+      st[S_ITER] = k;
+    }
+    return;
+  }
+  const double beta = rzn / rzo;
+  const bool conv = (st[S_MODE] == 0.0 ? rzn : rr) <= st[S_THR];
+  for (uint32_t i = t; i < n; i += gridDim.x * blockDim.x) p[i] = z[i] + p[i] * beta;
+  if (t == 0) {
+    st[S_FINAL_RZ] = rzn;
+    st[S_FINAL_RR] = rr;
+    if (conv) {
+      st[S_STATUS] = 1.0;
+      st[S_ITER] = k;
+    }
+  }
+}
+
+// Pseudo-inverse of diagonal blocks (diagonalBlockInverseKernel.pyx:51-104).
+template <int N>
+__global__ void __launch_bounds__(64)
+    invert_blocks_kernel(const double* __restrict__ in, double* __restrict__ out, uint32_t count) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= count) return;
+  double A[N][N], V[N][N];
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) A[i][j] = in[static_cast<size_t>(b) * N * N + i * N + j];
+  jacobi_eigen<N>(A, V);
+  double inv[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const double l = A[i][i];
+    inv[i] = fabs(l) < 1e-6 ? fabs(l) : fabs(1.0 / l);
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < N; ++k) acc += V[i][k] * inv[k] * V[j][k];
+      out[static_cast<size_t>(b) * N * N + i * N + j] = acc;
+    }
+}
+
+int launch_invert(const double* in, double* out, uint32_t count, uint32_t size, hipStream_t s) {
+  const dim3 g(grid_for(count, 64)), bl(64);
+  switch (size) {
+    case 1: hipLaunchKernelGGL(invert_blocks_kernel<1>, g, bl, 0, s, in, out, count); break;
+    case 2: hipLaunchKernelGGL(invert_blocks_kernel<2>, g, bl, 0, s, in, out, count); break;
+    case 3: hipLaunchKernelGGL(invert_blocks_kernel<3>, g, bl, 0, s, in, out, count); break;
+    case 4: hipLaunchKernelGGL(invert_blocks_kernel<4>, g, bl, 0, s, in, out, count); break;
+    case 6: hipLaunchKernelGGL(invert_blocks_kernel<6>, g, bl, 0, s, in, out, count); break;
+    case 9: hipLaunchKernelGGL(invert_blocks_kernel<9>, g, bl, 0, s, in, out, count); break;
+    case 12: hipLaunchKernelGGL(invert_blocks_kernel<12>, g, bl, 0, s, in, out, count); break;
+    default:
+      ysp_set_error(__FILE__, __LINE__, "ysp_invert_diagonal_blocks: unsupported block size");
+      return YSP_ERR_UNSUPPORTED;
+  }
+  YSP_CHECK_LAUNCH();
+  return YSP_OK;
+}
+
+// Pinned status mailbox shared by the solves issued from this process.
+double* status_mailbox() {
+  static double* box = nullptr;
+  if (!box) {
+    if (hipHostMalloc(reinterpret_cast<void**>(&box), 8 * sizeof(double), hipHostMallocDefault) !=
+        hipSuccess)
+      box = nullptr;
+  }
+  return box;
+}
+
+}  // namespace
+}  // namespace ysp
+
+using namespace ysp;
+
+extern "C" int ysp_invert_diagonal_blocks(const double* diagonal_blocks, double* inverse,
+                                          const uint32_t* starts, const uint32_t* counts,
+                                          const uint32_t* sizes, int32_t num_attributes,
+                                          void* stream) {
+  hipStream_t s = as_stream(stream);
+  for (int a = 0; a < num_attributes; ++a) {
+    if (counts[a] == 0) continue;
+    int rc = launch_invert(diagonal_blocks + starts[a], inverse + starts[a], counts[a], sizes[a], s);
+    if (rc) return rc;
+  }
+  return YSP_OK;
+}
+
+extern "C" int ysp_pcg_solve(uint32_t max_iterations, double threshold, int32_t threshold_mode,
+                             const ysp_block_matrix* part, const ysp_block_matrix* dynamic_part,
+                             const double* diagonal_blocks_inverse,
+                             const uint32_t* diagonal_blocks_start,
+                             const uint32_t* diagonal_blocks_count,
+                             const uint32_t* diagonal_blocks_size,
+                             const uint32_t* gradient_segments_start, int32_t num_attributes,
+                             const double* rhs, uint32_t n, double* solution,
+                             const double* initial_guess, int32_t zero_initial_guess,
+                             double* work, double* cg_scalars, ysp_pcg_stats* stats,
+                             void* stream) {
+  if (stats) *stats = ysp_pcg_stats{0, 0, 0, 0, 0, 0};
+  if (n == 0) return 0;
+  if (num_attributes <= 0 || num_attributes > kMaxAttr || !work || !cg_scalars ||
+      (threshold_mode != 0 && threshold_mode != 1)) {
+    ysp_set_error(__FILE__, __LINE__, "ysp_pcg_solve: bad arguments");
+    return YSP_ERR_ARG;
+  }
+  hipStream_t s = as_stream(stream);
+  DiagLayout L{};
+  L.num_attr = num_attributes;
+  L.cum[0] = 0;
+  uint32_t total_dof = 0;
+  for (int a = 0; a < num_attributes; ++a) {
+    if (diagonal_blocks_size[a] > static_cast<uint32_t>(kMaxBlock)) {
+      ysp_set_error(__FILE__, __LINE__, "ysp_pcg_solve: diagonal block larger than 12");
+      return YSP_ERR_UNSUPPORTED;
+    }
+    L.cum[a + 1] = L.cum[a] + diagonal_blocks_count[a];
+    L.grad_start[a] = gradient_segments_start[a];
+    L.diag_start[a] = diagonal_blocks_start[a];
+    L.size[a] = diagonal_blocks_size[a];
+    total_dof += diagonal_blocks_count[a] * diagonal_blocks_size[a];
+  }
+  if (total_dof != n) {
+    ysp_set_error(__FILE__, __LINE__, "ysp_pcg_solve: diagonal layout does not cover n");
+    return YSP_ERR_ARG;
+  }
+  double* r = work;
+  double* p = work + n;
+  double* z = work + 2 * static_cast<size_t>(n);
+  double* q = work + 3 * static_cast<size_t>(n);
+  double* x = solution;
+  double* st = cg_scalars;
+  const uint32_t nblocks = L.cum[num_attributes];
+  const dim3 vgrid(grid_for(nblocks, kVecBlock)), vblock(kVecBlock);
+  const dim3 dgrid(std::min<unsigned>(grid_for(n, kVecBlock), 2048u));
+
+  YSP_CHECK(hipMemsetAsync(st, 0, S_COUNT * sizeof(double), s));
+  const int have_guess = (!zero_initial_guess && initial_guess) ? 1 : 0;
+  if (have_guess) {
+    YSP_CHECK(hipMemsetAsync(q, 0, sizeof(double) * n, s));
+    int rc = launch_part_spmv(part, initial_guess, q, 0, nullptr, nullptr, s);
+    if (rc) return rc;
+    rc = launch_part_spmv(dynamic_part, initial_guess, q, 0, nullptr, nullptr, s);
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(init_kernel, vgrid, vblock, 0, s, L, diagonal_blocks_inverse, rhs, x,
+                     initial_guess, r, p, z, q, st, have_guess);
+  hipLaunchKernelGGL(init_finish_kernel, dim3(1), dim3(1), 0, s, st, threshold, threshold_mode);
+  YSP_CHECK_LAUNCH();
+
+  double* box = status_mailbox();
+  if (!box) {
+    ysp_set_error(__FILE__, __LINE__, "ysp_pcg_solve: pinned status allocation failed");
+    return YSP_ERR_HIP;
+  }
+  auto read_status = [&](double* dst) -> int {
+    YSP_CHECK(hipMemcpyAsync(dst, st, 8 * sizeof(double), hipMemcpyDeviceToHost, s));
+    YSP_CHECK(hipStreamSynchronize(s));
+    return YSP_OK;
+  };
+  double h[8];
+  if (int rc = read_status(box)) return rc;
+  for (int i = 0; i < 8; ++i) h[i] = box[i];
+
+  uint32_t k = 1;
+  uint32_t chunk = 4;
+  while (h[S_STATUS] == 0.0 && k <= max_iterations) {
+    const uint32_t last = std::min<uint32_t>(max_iterations, k + chunk - 1);
+    for (uint32_t it = k; it <= last; ++it) {
+      const int cur = static_cast<int>(it % 3);
+      int rc = launch_part_spmv(part, p, q, 0, st + S_PAP + cur, st + S_STATUS, s);
+      if (rc) return rc;
+      rc = launch_part_spmv(dynamic_part, p, q, 0, st + S_PAP + cur, st + S_STATUS, s);
+      if (rc) return rc;
+      hipLaunchKernelGGL(update_kernel, vgrid, vblock, 0, s, L, diagonal_blocks_inverse, x, r, p,
+                         q, z, st, static_cast<int>(it));
+      hipLaunchKernelGGL(direction_kernel, dgrid, vblock, 0, s, z, p, st, n,
+                         static_cast<int>(it));
+    }
+    YSP_CHECK_LAUNCH();
+    if (int rc = read_status(box)) return rc;
+    for (int i = 0; i < 8; ++i) h[i] = box[i];
+    k = last + 1;
+    chunk = std::min<uint32_t>(chunk * 2, 64);
+  }
+  double fin[64];
+  YSP_CHECK(hipMemcpyAsync(fin, st, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, s));
+  YSP_CHECK(hipStreamSynchronize(s));
+  int code;
+  const int iters = static_cast<int>(fin[S_ITER]);
+  if (fin[S_STATUS] == 1.0) {
+    code = iters;
+  } else if (fin[S_STATUS] == -1.0) {
+    code = -iters - 4;
+  } else if (fin[S_STATUS] == -2.0) {
+    code = -5;
+  } else {
+    code = -4;
+  }
+  if (stats) {
+    stats->iterations = fin[S_STATUS] == 0.0 ? static_cast<int>(max_iterations) : iters;
+    stats->status = code;
+    stats->delta0 = fin[S_RZ0];
+    stats->delta = fin[S_FINAL_RZ];
+    stats->rr0 = fin[S_BB];
+    stats->rr = fin[S_FINAL_RR];
+  }
+  return code;
+}

@@ -1,0 +1,130 @@
+"""Synthetic tet-cube scenes used by the solve benchmarks and the parity tests.
+
+The cube scene follows the Newton step of ``examples/one_bunny/one_bunny.py:40-160``:
+vertices are split into a fixed and a moving primitive that are stacked in one
+union (``one_bunny.py:58-60``), tets join the union vertices, the Stable
+Neo-Hookean energy uses the re-parameterised Lame constants of
+``one_bunny.py:12-17`` and the inertia term uses a per-vertex lumped mass.
+
+Only numpy is used here: the scene is plain input data for both the HIP path and
+the CPU oracle.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+import numpy as np
+
+
+def lame_parameters(young: float, poisson: float):
+    """Return (mu, lam) of the Stable Neo-Hookean re-parameterisation.
+
+    ``examples/one_bunny/one_bunny.py:14-17``: mu = 4/3 mu_L and
+    lam = lam_L + 5/6 mu_L.
+    """
+    mu_l = young / (2.0 * (1.0 + poisson))
+    lam_l = young * poisson / ((1.0 + poisson) * (1.0 - 2.0 * poisson))
+    return 4.0 * mu_l / 3.0, lam_l + 5.0 * mu_l / 6.0
+
+
+# The six vertex paths from corner (0,0,0) to (1,1,1) of a unit hex, one per
+# permutation of the axes (Kuhn / Freudenthal split along the main diagonal).
+_AXIS_PERMUTATIONS = ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0))
+
+
+def cube_tet_mesh(n: int, size: float = 1.0):
+    """Split [0,size]^3 into n^3 hexes and each hex into 6 positively oriented tets.
+
+    Returns (positions [V,3] float64, tets [T,4] int64) with lexicographic vertex
+    ids ``i + (n+1)*(j + (n+1)*k)`` for grid coordinates (i, j, k).
+    """
+    if n < 1:
+        raise ValueError("cube_tet_mesh: n must be >= 1")
+    m = n + 1
+    g = np.arange(m, dtype=np.float64) * (size / n)
+    z, y, x = np.meshgrid(g, g, g, indexing="ij")
+    positions = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
+
+    def vid(i, j, k):
+        return i + m * (j + m * k)
+
+    ii, jj, kk = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    ii, jj, kk = ii.ravel(), jj.ravel(), kk.ravel()
+    # hex index order: k slowest, then j, then i
+    order = np.lexsort((ii, jj, kk))
+    ii, jj, kk = ii[order], jj[order], kk[order]
+    tets = []
+    for perm in _AXIS_PERMUTATIONS:
+        c = [ii.copy(), jj.copy(), kk.copy()]
+        path = [vid(*c)]
+        for axis in perm:
+            c[axis] = c[axis] + 1
+            path.append(vid(*c))
+        tets.append(np.stack(path, axis=1))
+    tets = np.stack(tets, axis=1).reshape(-1, 4)
+    # make every tet positively oriented: det(Dm) > 0
+    p = positions[tets]
+    dm = np.stack([p[:, 1] - p[:, 0], p[:, 2] - p[:, 0], p[:, 3] - p[:, 0]], axis=2)
+    neg = np.linalg.det(dm) < 0
+    tets[neg, 2], tets[neg, 3] = tets[neg, 3].copy(), tets[neg, 2].copy()
+    return positions, tets.astype(np.int64)
+
+
+@dataclass
+class CubeScene:
+    """A fixed/moving split tet cube ready for one Newton linear solve.
+
+    Vertex ids index the union ``[fixed_vertices, moving_vertices]``: the
+    ``n_fixed`` fixed vertices come first, as in ``one_bunny.py:58-60``.
+    """
+    n: int
+    rest: np.ndarray            # [V,3] union order
+    position: np.ndarray        # [V,3] union order (fixed rows equal rest)
+    last_position: np.ndarray   # [V,3]
+    velocity: np.ndarray        # [V,3]
+    tets: np.ndarray            # [T,4] union vertex ids
+    mass: np.ndarray            # [V] lumped rho*vol/4
+    n_fixed: int
+    mu: float
+    lam: float
+    dt: float
+    young: float
+    poisson: float
+    density: float
+    extras: dict = field(default_factory=dict)
+
+    @property
+    def n_moving(self) -> int:
+        return self.rest.shape[0] - self.n_fixed
+
+    @property
+    def n_dof(self) -> int:
+        return 3 * self.n_moving
+
+
+def cube_scene(n: int = 8, young: float = 1e5, poisson: float = 0.4, density: float = 1000.0,
+               dt: float = 1e-2, noise: float = 0.02, seed: int = 0) -> CubeScene:
+    """Config-1 style scene: [0,1]^3 cube, n^3 hexes x 6 tets, bottom face (z=0) fixed.
+
+    Moving vertices get positions rest + N(0, (noise*h)^2) from ``default_rng(seed)``;
+    fixed vertices stay at rest. last_position = rest, velocity = 0.
+    """
+    pos, tets = cube_tet_mesh(n)
+    h = 1.0 / n
+    fixed_mask = pos[:, 2] == 0.0
+    order = np.concatenate([np.nonzero(fixed_mask)[0], np.nonzero(~fixed_mask)[0]])
+    inv = np.empty_like(order)
+    inv[order] = np.arange(order.size)
+    rest = pos[order]
+    tets_u = inv[tets]
+    n_fixed = int(fixed_mask.sum())
+    rng = np.random.default_rng(seed)
+    position = rest.copy()
+    position[n_fixed:] += rng.normal(0.0, noise * h, size=(rest.shape[0] - n_fixed, 3))
+    p = rest[tets_u]
+    vol = np.linalg.det(np.stack([p[:, 1] - p[:, 0], p[:, 2] - p[:, 0], p[:, 3] - p[:, 0]], axis=2)) / 6.0
+    mass = np.zeros(rest.shape[0])
+    np.add.at(mass, tets_u.ravel(), np.repeat(density * vol / 4.0, 4))
+    mu, lam = lame_parameters(young, poisson)
+    return CubeScene(n=n, rest=rest, position=position, last_position=rest.copy(),
+                     velocity=np.zeros_like(rest), tets=tets_u, mass=mass, n_fixed=n_fixed,
+                     mu=mu, lam=lam, dt=dt, young=young, poisson=poisson, density=density)
