@@ -1,0 +1,318 @@
+"""Assembled Hessian in upper block storage, its gradient, and device vectors.
+
+Orchestration follows ``matrixAndVector/matrix/hessian.pyx``: the gradient layout is the
+minimisation-target order (``:110-127``); static terms get their sparsity layout once
+(``getSparseIndices`` ``:783-808``), dynamic terms are re-indexed on every ``compute``
+into an independent layout (``:846-880``); ``compute`` zeroes gradient, diagonal blocks
+and block storage, then runs every term's element kernel (``:1159-1207``). Every index
+stage runs on the device through the C ABI.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from . import _lib, device as dev
+from .program import encode
+
+
+class vector:
+    """A device FP64 vector (``matrixAndVector/vector``)."""
+
+    def __init__(self, size: int = 0, value=None):
+        self.value = dev.zeros(size, "float64") if value is None else value
+
+    @property
+    def size(self) -> int:
+        return int(self.value.numel())
+
+    def updateValue(self, values):
+        t = dev.torch()
+        if isinstance(values, t.Tensor):
+            self.value = values.reshape(-1).to(dev.device(), t.float64)
+        else:
+            self.value = dev.to_device(np.asarray(values, np.float64).reshape(-1), "float64")
+        return self
+
+    def get(self) -> np.ndarray:
+        return dev.to_host(self.value)
+
+
+@dataclass
+class TermIndex:
+    """Device index buffers of one term (gradientIndicesKernel outputs)."""
+    K: int
+    n: int
+    indices: object = None
+    sizes: object = None
+    permutations: object = None
+    total_sizes: object = None
+    coord_outer: object = None
+    unique_sizes: object = None
+    grouped: object = None
+    grouped_outer: object = None
+    num_unique: object = None
+    coords: object = None
+    dims: object = None
+    num_coords: int = 0
+    lookups: object = None
+
+
+@dataclass
+class Part:
+    """Block storage of one (static or dynamic) part."""
+    values: object = None
+    positions: object = None
+    starts: List[int] = field(default_factory=list)
+    counts: List[int] = field(default_factory=list)
+    dims: List[int] = field(default_factory=list)
+    total: int = 0
+    num_blocks: int = 0
+    _ct: tuple = ()
+
+    def ctype(self) -> Optional[_lib.BlockMatrix]:
+        if not self.counts:
+            return None
+        starts = _lib.u32_array(self.starts[:-1] if len(self.starts) > len(self.counts) else self.starts)
+        counts = _lib.u32_array(self.counts)
+        dims = _lib.u32_array(self.dims)
+        m = _lib.BlockMatrix(dev.ptr(self.values), dev.ptr(self.positions), len(self.counts),
+                             starts, counts, dims)
+        self._ct = (starts, counts, dims, m)
+        return m
+
+
+def build_term_index(program, n: int, wrt_start_dev, stream) -> TermIndex:
+    """get_indices -> compress_indices -> computeCoordinates for one term."""
+    lib = _lib.load()
+    conns, unions = [], []
+    code = encode(program, conns, unions)
+    K = code[1]
+    ti = TermIndex(K=K, n=n)
+    if n == 0:
+        return ti
+    code_dev = dev.to_device(np.asarray(code, np.int32), "int32")
+    conn_ptrs = (C.c_void_p * max(1, len(conns)))(*[dev.ptr(c.value) for c in conns])
+    union_ptrs = (C.c_void_p * max(1, len(unions)))(*[dev.ptr(u.counts_device) for u in unions])
+    ti.indices = dev.zeros(n * K, "uint32")
+    ti.sizes = dev.zeros(n * K, "uint16")
+    _lib.check(lib.ysp_get_indices(dev.ptr(code_dev), len(code), conn_ptrs, len(conns), union_ptrs,
+                                   len(unions), dev.ptr(wrt_start_dev), dev.ptr(ti.indices),
+                                   dev.ptr(ti.sizes), n, K, stream), "ysp_get_indices")
+    ti.permutations = dev.zeros(n * K, "int16")
+    ti.total_sizes = dev.empty(n, "uint16")
+    ti.coord_outer = dev.zeros(n + 1, "uint32")
+    ti.unique_sizes = dev.zeros(n + 1, "uint16")
+    ti.grouped = dev.empty(n, "uint32")
+    ti.grouped_outer = dev.zeros(n + 2, "uint32")
+    ti.num_unique = dev.zeros(1, "uint16")
+    _lib.check(lib.ysp_compress_indices(
+        dev.ptr(ti.indices), dev.ptr(ti.sizes), dev.ptr(ti.permutations), dev.ptr(ti.total_sizes),
+        dev.ptr(ti.coord_outer), dev.ptr(ti.unique_sizes), dev.ptr(ti.grouped),
+        dev.ptr(ti.grouped_outer), dev.ptr(ti.num_unique), n, K, 1, stream), "ysp_compress_indices")
+    ti.num_coords = int(dev.to_host(ti.coord_outer[n:n + 1], "uint32")[0])
+    ti.coords = dev.empty(max(1, 2 * ti.num_coords), "uint32")
+    ti.dims = dev.empty(max(1, 2 * ti.num_coords), "uint16")
+    if ti.num_coords:
+        _lib.check(lib.ysp_compute_coordinates(
+            dev.ptr(ti.indices), dev.ptr(ti.permutations), dev.ptr(ti.sizes), dev.ptr(ti.coords),
+            dev.ptr(ti.dims), dev.ptr(ti.coord_outer), K, n, stream), "ysp_compute_coordinates")
+    return ti
+
+
+def build_part(terms: List[TermIndex], stream) -> Part:
+    """get_unique_coords + compress_unique_coords over the concatenated coordinates
+    (``coordinateCompressionKernel.pyx:692-726``)."""
+    lib = _lib.load()
+    t = dev.torch()
+    total = sum(ti.num_coords for ti in terms)
+    part = Part()
+    if total == 0:
+        for ti in terms:
+            ti.lookups = dev.empty(0, "uint32")
+        return part
+    coords = t.cat([ti.coords[:2 * ti.num_coords] for ti in terms if ti.num_coords])
+    dims = t.cat([ti.dims[:2 * ti.num_coords] for ti in terms if ti.num_coords])
+    tmp = dev.empty(4 * total, "uint32")   # 16-byte records
+    lookup = dev.empty(total, "uint32")
+    nu = C.c_uint32(0)
+    _lib.check(lib.ysp_get_unique_coords(dev.ptr(coords), dev.ptr(dims), total, dev.ptr(tmp),
+                                         dev.ptr(lookup), C.byref(nu), stream), "ysp_get_unique_coords")
+    nu = nu.value
+    positions = dev.empty(2 * nu, "uint32")
+    max_cat = 64
+    udims = dev.zeros(2 * max_cat, "uint16")
+    outer = dev.zeros(max_cat + 1, "uint32")
+    counts = dev.zeros(max_cat, "uint32")
+    nd = C.c_uint32(0)
+    _lib.check(lib.ysp_compress_unique_coords(dev.ptr(tmp), nu, total, dev.ptr(positions),
+                                              dev.ptr(udims), dev.ptr(outer), dev.ptr(counts),
+                                              dev.ptr(lookup), C.byref(nd), stream),
+               "ysp_compress_unique_coords")
+    nd = nd.value
+    part.positions = positions
+    part.dims = [int(v) for v in dev.to_host(udims[:2 * nd], "uint16")]
+    part.starts = [int(v) for v in dev.to_host(outer[:nd + 1], "uint32")]
+    part.counts = [int(v) for v in dev.to_host(counts[:nd], "uint32")]
+    part.total = part.starts[-1]
+    part.num_blocks = nu
+    part.values = dev.zeros(part.total, "float64")
+    off = 0
+    for ti in terms:
+        ti.lookups = lookup[off:off + ti.num_coords]
+        off += ti.num_coords
+    return part
+
+
+class hessian:
+    """Sum of energy terms w.r.t. a list of targets, in symmetric upper block storage."""
+
+    def __init__(self, wrt, terms, dynamic_terms):
+        self.wrt = list(wrt)
+        self.terms = list(terms)                 # [(expr, projection_method)]
+        self.dynamic_terms = list(dynamic_terms)
+        sizes = [w.size * w.numInstances for w in self.wrt]
+        self.gradient_segments_start_cpu = [0]
+        self.diagonal_blocks_start_cpu = [0]
+        for w, s in zip(self.wrt, sizes):
+            self.gradient_segments_start_cpu.append(self.gradient_segments_start_cpu[-1] + s)
+            self.diagonal_blocks_start_cpu.append(self.diagonal_blocks_start_cpu[-1]
+                                                  + w.size * w.size * w.numInstances)
+        self.wrt_start_indices = self.gradient_segments_start_cpu[:-1]
+        self.rows = self.cols = self.gradient_segments_start_cpu[-1]
+        if self.rows <= 0:
+            raise ValueError("hessian: total size must be positive")
+        self._wrt_pos = {w: i for i, w in enumerate(self.wrt)}
+        self._setup = False
+        self.static_index: List[TermIndex] = []
+        self.dynamic_index: List[TermIndex] = []
+        self.static = Part()
+        self.dynamic = Part()
+
+    # --------------------------------------------------------------- layout
+    def _ensure_setup(self):
+        if self._setup:
+            return
+        s = dev.stream()
+        self._wrt_start_dev = dev.to_device(np.asarray(self.gradient_segments_start_cpu, np.uint32),
+                                            "uint32")
+        self._diag_start_dev = dev.to_device(np.asarray(self.diagonal_blocks_start_cpu, np.uint32),
+                                             "uint32")
+        self.gradient = vector(self.rows)
+        self.diagonal_blocks = dev.zeros(self.diagonal_blocks_start_cpu[-1], "float64")
+        self.diagonal_blocks_inverse = dev.zeros(self.diagonal_blocks_start_cpu[-1], "float64")
+        self.static_index = [build_term_index(e.program(self._wrt_pos), e.num_instances,
+                                              self._wrt_start_dev, s) for e, _ in self.terms]
+        self.static = build_part(self.static_index, s)
+        self._setup = True
+
+    def _reindex_dynamic(self):
+        if not self.dynamic_terms:
+            return
+        s = dev.stream()
+        self.dynamic_index = [build_term_index(e.program(self._wrt_pos), e.num_instances,
+                                               self._wrt_start_dev, s) for e, _ in self.dynamic_terms]
+        self.dynamic = build_part(self.dynamic_index, s)
+
+    # --------------------------------------------------------------- storage views
+    @property
+    def blocks_flattened(self):
+        return self.static.values
+
+    @property
+    def block_positions(self):
+        return self.static.positions
+
+    @property
+    def blocks_start_indices(self):
+        return self.static.starts
+
+    @property
+    def block_counts(self):
+        return self.static.counts
+
+    @property
+    def block_dimensions(self):
+        return self.static.dims
+
+    @property
+    def blocks_flattened_dynamic(self):
+        return self.dynamic.values
+
+    @property
+    def block_positions_dynamic(self):
+        return self.dynamic.positions
+
+    @property
+    def blocks_start_indices_dynamic(self):
+        return self.dynamic.starts
+
+    @property
+    def block_counts_dynamic(self):
+        return self.dynamic.counts
+
+    @property
+    def block_dimensions_dynamic(self):
+        return self.dynamic.dims
+
+    # --------------------------------------------------------------- numeric
+    def _scatter_args(self, ti: TermIndex, part: Part) -> _lib.ScatterArgs:
+        return _lib.ScatterArgs(dev.ptr(ti.indices), dev.ptr(ti.sizes), dev.ptr(ti.permutations),
+                                dev.ptr(ti.lookups), dev.ptr(ti.coord_outer), ti.K, len(self.wrt),
+                                dev.ptr(self._diag_start_dev), dev.ptr(self._wrt_start_dev),
+                                dev.ptr(self.gradient.value), dev.ptr(part.values),
+                                dev.ptr(self.diagonal_blocks))
+
+    def compute(self, local_gradient=None):
+        self._ensure_setup()
+        if local_gradient is not None and local_gradient is not self.gradient:
+            self.gradient = local_gradient
+        self._reindex_dynamic()
+        self.gradient.value.zero_()
+        self.diagonal_blocks.zero_()
+        if self.static.values is not None:
+            self.static.values.zero_()
+        if self.dynamic.values is not None:
+            self.dynamic.values.zero_()
+        for (expr, method), ti in zip(self.terms, self.static_index):
+            if ti.n:
+                expr.evaluate(self._scatter_args(ti, self.static), method)
+        for (expr, method), ti in zip(self.dynamic_terms, self.dynamic_index):
+            if ti.n:
+                expr.evaluate(self._scatter_args(ti, self.dynamic), method)
+        return self
+
+    def spmv(self, vec: vector, transpose: bool = False) -> vector:
+        self._ensure_setup()
+        out = vector(self.rows)
+        a, b = self.static.ctype(), self.dynamic.ctype()
+        _lib.check(_lib.load().ysp_block_spmv(C.byref(a) if a else None, C.byref(b) if b else None,
+                                              dev.ptr(vec.value), dev.ptr(out.value), self.rows,
+                                              int(transpose), 0, dev.stream()), "ysp_block_spmv")
+        return out
+
+    # --------------------------------------------------------------- host views
+    def dense(self) -> np.ndarray:
+        """Dense symmetric matrix (small problems; tests)."""
+        n = self.rows
+        out = np.zeros((n, n))
+        for part in (self.static, self.dynamic):
+            if not part.counts:
+                continue
+            pos = dev.to_host(part.positions, "uint32").reshape(-1, 2)
+            vals = dev.to_host(part.values)
+            k = 0
+            for c, cnt in enumerate(part.counts):
+                h, w = part.dims[2 * c], part.dims[2 * c + 1]
+                for b in range(cnt):
+                    r, cc = map(int, pos[k])
+                    o = part.starts[c] + b * h * w
+                    blk = vals[o:o + h * w].reshape(h, w)
+                    out[r:r + h, cc:cc + w] += blk
+                    if r != cc:
+                        out[cc:cc + w, r:r + h] += blk.T
+                    k += 1
+        return out

@@ -128,3 +128,46 @@ def cube_scene(n: int = 8, young: float = 1e5, poisson: float = 0.4, density: fl
     return CubeScene(n=n, rest=rest, position=position, last_position=rest.copy(),
                      velocity=np.zeros_like(rest), tets=tets_u, mass=mass, n_fixed=n_fixed,
                      mu=mu, lam=lam, dt=dt, young=young, poisson=poisson, density=density)
+
+
+def build_yasps_scene(cs: CubeScene, projection_method: int = 2, solver_name: str = "jacobian",
+                      **solver_options):
+    """The cube scene through the drop-in API, laid out as ``examples/one_bunny/one_bunny.py``
+    (fixed/moving vertex primitives, union ``vertices``, tets joining the union, SNH on tets,
+    inertia on moving vertices, target = moving positions). Returns (scene, handles)."""
+    from . import scene, stable_neo_hookean, inertia
+    s0 = scene("scene0")
+    dt = s0.addConstant("dt", rows=1, cols=1)
+    dt.updateValue([cs.dt])
+    m = s0.addMesh("cube")
+    fv = m.addPrimitive("fixed_vertices", numInstances=cs.n_fixed)
+    mv = m.addPrimitive("moving_vertices", numInstances=cs.n_moving)
+    m.addPrimitiveUnion("vertices", [fv, mv])
+    tets = m.addPrimitive("tets", numInstances=cs.tets.shape[0])
+    nf = cs.n_fixed
+    fv.addAttribute("position", rows=3, cols=1).updateValue(cs.position[:nf])
+    fv.addConstant("rest_position", rows=3, cols=1).updateValue(cs.rest[:nf])
+    mv.addConstant("rest_position", rows=3, cols=1).updateValue(cs.rest[nf:])
+    mv.addAttribute("position", rows=3, cols=1).updateValue(cs.position[nf:])
+    mv.addConstant("last_position", rows=3, cols=1).updateValue(cs.last_position[nf:])
+    mv.addConstant("velocity", rows=3, cols=1).updateValue(cs.velocity[nf:])
+    mv.addConstant("mass", rows=1, cols=1).updateValue(cs.mass[nf:])
+    m.vertices.addAttribute("position")
+    m.vertices.addAttribute("rest_position")
+    mu = m.addConstant("mu", rows=1, cols=1)
+    mu.updateValue([cs.mu])
+    lam = m.addConstant("lam", rows=1, cols=1)
+    lam.updateValue([cs.lam])
+    tet2v = tets.addConnectivity("tet2v", m.vertices, cs.tets, 4)
+    tp = tets.addAttribute("positions", through=tet2v, source=m.vertices["position"])
+    trp = tets.addAttribute("rest_positions", through=tet2v, source=m.vertices["rest_position"])
+    snh = tets.addAttribute("stable_neo_hookean",
+                            computed_attribute=stable_neo_hookean(trp, tp, mu, lam, dt))
+    ine = mv.addAttribute("inertia", computed_attribute=inertia(
+        mv["last_position"], mv["velocity"], dt, mv["position"], mv["mass"]))
+    s0.addEnergy(snh, projection_method=projection_method)
+    s0.addEnergy(ine, projection_method=projection_method)
+    s0.addMinimizeTarget([mv["position"]])
+    s0.minimizer.setSolver(solver_name, **solver_options)
+    return s0, dict(mesh=m, tets=tets, moving=mv, fixed=fv, snh=snh, inertia=ine, tet2v=tet2v,
+                    mu=mu, lam=lam, dt=dt)

@@ -1,0 +1,163 @@
+"""Linear solvers of H x = g (``solver/solver.py``, ``solver/jacobianPCGSolver.pyx``).
+
+``solver("jacobian")`` is the block-Jacobi PCG of ``kernel/Solver/solverKernel.pyx``:
+diagonal blocks pseudo-inverted with the |l| < 1e-6 rule
+(``diagonalBlockInverseKernel.pyx:69,91``), device-resident recurrence, the reference
+stopping rule r^T M^-1 r <= tol * r0^T M^-1 r0 (tolerance is a squared, preconditioned
+ratio) and the reference return codes mapped as in ``solverKernel.pyx:800-824``
+(iterations -> 0 on success).
+
+Extension: ``threshold_mode="relative_residual"`` stops on ||r|| <= tol ||b|| instead.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+from . import _lib, device as dev
+from .hessian import vector
+
+_MODES = {"reference": 0, "relative_residual": 1}
+
+
+class jacobianPCGSolver:
+    def __init__(self, threshold_mode: str = "reference"):
+        if threshold_mode not in _MODES:
+            raise ValueError(f"threshold_mode must be one of {sorted(_MODES)}")
+        self.threshold_mode = threshold_mode
+        self.__solution = None
+        self.__work = None
+        self.__scalars = None
+        self.__statistics = {}
+        self.iterations = 0
+
+    @property
+    def solution(self):
+        return self.__solution
+
+    @property
+    def statistics(self) -> dict:
+        return dict(self.__statistics)
+
+    def reset(self):
+        self.__solution = None
+        self.__work = None
+        self.__scalars = None
+        self.__statistics = {}
+
+    def _ensure(self, n):
+        if self.__solution is None or self.__solution.numel() != n:
+            self.__solution = dev.zeros(n, "float64")
+            self.__work = dev.zeros(5 * n, "float64")
+            self.__scalars = dev.zeros(64, "float64")
+
+    def computeSolution(self, active_hessian, right_hand_side, initial_guess=None, tolerance=1e-3,
+                        maxIterations=20000, zero_initial_guess=False):
+        t0 = time.perf_counter()
+        if not isinstance(right_hand_side, vector):
+            raise TypeError("jacobianPCGSolver.computeSolution: right_hand_side must be a yasps_amd vector")
+        n = right_hand_side.size
+        if n == 0:
+            self.__statistics = {"solver": "jacobian", "converged": True, "iterations": 0,
+                                 "solve_seconds": time.perf_counter() - t0}
+            return 0
+        self._ensure(n)
+        if active_hessian is None:
+            self.__solution.zero_()
+            self.__statistics = {"solver": "jacobian", "converged": True, "iterations": 0,
+                                 "solve_seconds": time.perf_counter() - t0}
+            return 0
+        if n != active_hessian.rows:
+            raise ValueError("jacobianPCGSolver.computeSolution: right_hand_side size must match the Hessian size.")
+        lib = _lib.load()
+        h = active_hessian
+        s = dev.stream()
+        counts = [w.numInstances for w in h.wrt]
+        sizes = [w.size for w in h.wrt]
+        starts = h.diagonal_blocks_start_cpu[:-1]
+        t1 = time.perf_counter()
+        _lib.check(lib.ysp_invert_diagonal_blocks(
+            dev.ptr(h.diagonal_blocks), dev.ptr(h.diagonal_blocks_inverse), _lib.u32_array(starts),
+            _lib.u32_array(counts), _lib.u32_array(sizes), len(h.wrt), s), "ysp_invert_diagonal_blocks")
+        t2 = time.perf_counter()
+        a, b = h.static.ctype(), h.dynamic.ctype()
+        stats = _lib.PcgStats()
+        guess = None
+        if not zero_initial_guess and initial_guess is not None:
+            guess = initial_guess.value if hasattr(initial_guess, "value") else initial_guess
+        rc = lib.ysp_pcg_solve(
+            int(maxIterations), float(tolerance), _MODES[self.threshold_mode],
+            C.byref(a) if a else None, C.byref(b) if b else None, dev.ptr(h.diagonal_blocks_inverse),
+            _lib.u32_array(starts), _lib.u32_array(counts), _lib.u32_array(sizes),
+            _lib.u32_array(h.gradient_segments_start_cpu[:-1]), len(h.wrt),
+            dev.ptr(right_hand_side.value), n, dev.ptr(self.__solution), dev.ptr(guess),
+            int(bool(zero_initial_guess) or guess is None), dev.ptr(self.__work),
+            dev.ptr(self.__scalars), C.byref(stats), s)
+        _lib.check(rc, "ysp_pcg_solve")
+        t3 = time.perf_counter()
+        self.iterations = max(0, stats.iterations)
+        self.__statistics = {
+            "solver": "jacobian", "converged": rc >= 0, "iterations": int(stats.iterations),
+            "result": int(rc), "delta0": stats.delta0, "delta": stats.delta,
+            "rr0": stats.rr0, "rr": stats.rr,
+            "diagonal_inverse_seconds": t2 - t1, "pcg_seconds": t3 - t2,
+            "solve_seconds": t3 - t0, "matrix_size": n, "tolerance": float(tolerance),
+            "dynamic_block_count": int(sum(h.block_counts_dynamic)),
+        }
+        if rc == -5:
+            self.__solution.copy_(0.5 * right_hand_side.value)
+            return rc
+        if rc < 0:
+            return rc
+        return 0
+
+
+class solver:
+    """Solver dispatch (``solver/solver.py:9-68``)."""
+
+    def __init__(self, solver="jacobian", **options):
+        self.__name = ""
+        self.__impl = None
+        self.setSolver(solver, **options)
+
+    @property
+    def solverName(self) -> str:
+        return self.__name
+
+    @property
+    def implementation(self):
+        return self.__impl
+
+    @property
+    def solution(self):
+        return self.__impl.solution
+
+    @property
+    def statistics(self) -> dict:
+        return self.__impl.statistics
+
+    def setSolver(self, solver="jacobian", **options):
+        if not isinstance(solver, str):
+            raise TypeError("solver must be either 'jacobian' or 'mas'")
+        name = solver.strip().lower()
+        if name == "jacobian":
+            unexpected = set(options) - {"threshold_mode"}
+            if unexpected:
+                raise TypeError(f"jacobianPCGSolver does not accept options: {', '.join(sorted(unexpected))}")
+            impl = jacobianPCGSolver(**options)
+        elif name == "mas":
+            from .mas import masSolver
+            impl = masSolver(**options)
+        else:
+            raise ValueError("solver must be either 'jacobian' or 'mas'")
+        if self.__impl is not None:
+            self.__impl.reset()
+        self.__name = name
+        self.__impl = impl
+        return self
+
+    def reset(self):
+        self.__impl.reset()
+
+    def computeSolution(self, *args, **kwargs):
+        return self.__impl.computeSolution(*args, **kwargs)
