@@ -1,0 +1,192 @@
+"""Newton linear-solve benchmark on one MI355X.
+
+One step = one Newton linear solve of the tet-cube scene (Stable Neo-Hookean + inertia,
+bottom face fixed): element gradients/Hessians with PSD projection, scatter into the
+3x3 block storage, block-Jacobi setup and PCG to a true relative residual of 1e-6.
+Inputs stay resident in HBM; the static sparsity layout is built once before timing.
+
+Prints one JSON line (see the keys below). ``--gpus N`` with N > 1 exits: this build
+runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=48, help="cube resolution (n^3 hexes x 6 tets)")
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--spmv-reps", type=int, default=50)
+    return ap.parse_args()
+
+
+def spmv_roofline(hs, reps):
+    """Average duration of the 3x3 block SpMV kernel, timed with HIP events on the
+    stream it is launched on, and its algorithmic bytes per launch."""
+    from yasps_amd import _lib, device as dev
+    torch = dev.torch()
+    lib = _lib.load()
+    part = hs.static.ctype()
+    x = dev.to_device(np.linspace(-1.0, 1.0, hs.rows), "float64")
+    y = dev.zeros(hs.rows, "float64")
+    stream = torch.cuda.current_stream()
+    for _ in range(3):
+        lib.ysp_block_spmv(C.byref(part), None, dev.ptr(x), dev.ptr(y), hs.rows, 0, 1,
+                           stream.cuda_stream)
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(reps):
+        lib.ysp_block_spmv(C.byref(part), None, dev.ptr(x), dev.ptr(y), hs.rows, 0, 1,
+                           stream.cuda_stream)
+    end.record(stream)
+    end.synchronize()
+    ms = start.elapsed_time(end) / reps
+    nb = hs.static.num_blocks
+    n = hs.rows
+    # symmetric upper storage, each block read once: 72 B values + 8 B (row, col),
+    # plus x read and y written once per launch
+    bytes_alg = nb * (9 * 8 + 8) + n * 16
+    gbs = bytes_alg / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "bsr3_spmv_kernel<double>", "avg_us": round(ms * 1e3, 2),
+            "bytes_per_launch": int(bytes_alg)}
+
+
+def cpu_baseline(cs, values_host, st, tol):
+    """The oracle on the host: SNH + inertia assembly timed on 1/8 of the tets (scaled x8)
+    and block-Jacobi CG (scipy, one thread) timed in full on the same matrix."""
+    from oracle import energies, newton, projection
+    import scipy.sparse.linalg as spla
+    import contextlib
+    try:
+        from threadpoolctl import threadpool_limits
+        limiter = threadpool_limits(limits=1)
+        threads = 1
+    except Exception:  # threadpoolctl absent: numpy keeps its default thread count
+        limiter = contextlib.nullcontext()
+        threads = os.cpu_count() or 1
+    with limiter:
+        T = cs.tets.shape[0]
+        frac = 8
+        sub = slice(0, T // frac)
+        t0 = time.perf_counter()
+        rd = energies.snh_rest_data(cs.rest[cs.tets[sub]])
+        g, H = energies.snh_gradient_hessian(cs.position[cs.tets[sub]], rd[0], rd[1], cs.mu,
+                                             cs.lam, cs.dt)
+        H = projection.project_batch(H, 2)
+        t_asm = (time.perf_counter() - t0) * frac
+        t0 = time.perf_counter()
+        A = newton.bsr_matrix(st, values_host)
+        diag = values_host[(st.vert_lookup[:, None] + np.arange(9)).reshape(-1)].reshape(-1, 3, 3)
+        inv = projection.jacobi_block_inverse(diag)
+        b = np.asarray(cs.extras["gradient"])
+        M = spla.LinearOperator(A.shape, matvec=lambda v: np.einsum(
+            "bij,bj->bi", inv, v.reshape(-1, 3)).reshape(-1), dtype=np.float64)
+        x, info = spla.cg(A, b, rtol=tol, atol=0.0, maxiter=20000, M=M)
+        t_cg = time.perf_counter() - t0
+        rel = float(np.linalg.norm(b - A @ x) / np.linalg.norm(b))
+    total = t_asm + t_cg
+    return {"value": round(1.0 / total, 4), "unit": "Newton solves/s", "cores": threads,
+            "kind": "port",
+            "sample": (f"n={cs.n} cube: SNH assembly+projection on 1/{frac} of the {T} tets "
+                       f"timed ({t_asm / frac:.2f}s) and scaled x{frac}; block-Jacobi CG "
+                       f"(scipy, tol {tol:g}) timed in full ({t_cg:.2f}s, info={info}, "
+                       f"rel.res {rel:.2e}) on the GPU-assembled matrix"),
+            "seconds_per_solve": round(total, 3)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; rerun with --gpus 1"}))
+        sys.exit(2)
+    from yasps_amd.scenes import cube_scene, build_yasps_scene
+    from yasps_amd import device as dev
+    from yasps_amd.hessian import vector
+    torch = dev.torch()
+
+    cs = cube_scene(args.n, seed=args.seed)
+    s0, hnd = build_yasps_scene(cs, threshold_mode="relative_residual")
+    # setup: static sparsity layout, rest data, first solve
+    s0.minimizeEnergy(tolerance=args.tol)
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        s0.minimizeEnergy(tolerance=args.tol)
+    torch.cuda.synchronize()
+
+    lat = []
+    iters = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        s0.minimizeEnergy(tolerance=args.tol)
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - ts)
+        iters.append(s0.minimizer.linearSolver.statistics["iterations"])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+
+    hs = s0.minimizer.activeHessian
+    sol = s0.minimizer.linearSolver.solution
+    g = hs.gradient.value
+    r = hs.spmv(vector(hs.rows, sol.clone())).value - g
+    true_rel = float(torch.linalg.norm(r) / torch.linalg.norm(g))
+
+    roof = spmv_roofline(hs, args.spmv_reps)
+    out = {
+        "metric": "Newton solves/s at 1e-6 rel. residual (plain FP64)",
+        "value": round(args.steps / elapsed, 3),
+        "unit": "Newton solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": (f"tet cube n={args.n} ({cs.tets.shape[0]} tets, {cs.n_dof} unknowns, "
+                                f"{hs.static.num_blocks} 3x3 blocks), SNH E=1e5 nu=0.4 + inertia "
+                                "dt=1e-2, bottom fixed, block-Jacobi PCG to ||g-Hx||/||g|| <= "
+                                f"{args.tol:g}"),
+                   "n": args.n, "solver": "jacobian", "precision": "fp64"},
+        "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
+        "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
+        "pcg_iterations": int(np.median(iters)),
+        "true_rel_residual": true_rel,
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        from oracle import newton
+        st = newton.fast_structure(cs)
+        vals = dev.to_host(hs.blocks_flattened)
+        cs.extras["gradient"] = dev.to_host(g)
+        out["cpu_baseline"] = cpu_baseline(cs, vals, st, args.tol)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -120,3 +120,65 @@ def inertia_gradient_hessian(position, last_position, velocity, mass, dt):
     grad = m[:, None] * d
     H = m[:, None, None] * np.eye(3)[None]
     return grad, H
+
+
+# ---------------------------------------------------------------------------
+# Closed-form block evaluation (used by the timed CPU baseline; pinned against the
+# K-tensor contraction of ``snh_gradient_hessian`` in the tests).
+# ---------------------------------------------------------------------------
+_U4 = np.array([[1.0, 1.0, 1.0], [-1.0, 1.0, 1.0], [0.0, -2.0, 1.0], [0.0, 0.0, -3.0]]) \
+    / np.sqrt([2.0, 6.0, 12.0])
+
+
+def _snh_block_terms(F, cof, ic, J, mu, lam, alpha, G, s):
+    a1 = mu * (1.0 - 1.0 / (ic + 1.0))
+    a2 = 2.0 * mu / (ic + 1.0) ** 2
+    a3 = lam * (J - alpha)
+    u = np.einsum("tij,taj->tai", F, G)
+    w = np.einsum("tij,taj->tai", cof, G)
+    return a1, a2, a3, u, w
+
+
+def _blocks_from_vectors(F, G, u, w, a1, a2, lam, a3, s):
+    """H_ab = s (a1 (g_a.g_b) I + a2 u_a u_b^T + lam w_a w_b^T - a3 [F (g_a x g_b)]x)."""
+    T, V = G.shape[:2]
+    gg = np.einsum("taj,tbj->tab", G, G)
+    H = (a2[:, None, None, None, None] * u[:, :, None, :, None] * u[:, None, :, None, :]
+         + lam * w[:, :, None, :, None] * w[:, None, :, None, :])
+    H += (a1[:, None, None] * gg)[:, :, :, None, None] * np.eye(3)
+    c = np.cross(G[:, :, None, :], G[:, None, :, :])          # [T,V,V,3]
+    t = np.einsum("tij,tabj->tabi", F, c)
+    z = np.zeros(t.shape[:-1])
+    tx = np.stack([np.stack([z, -t[..., 2], t[..., 1]], -1),
+                   np.stack([t[..., 2], z, -t[..., 0]], -1),
+                   np.stack([-t[..., 1], t[..., 0], z], -1)], -2)
+    H -= a3[:, None, None, None, None] * tx
+    return H * s[:, None, None, None, None]                   # [T,V,V,3,3]
+
+
+def snh_gradient_hessian_fast(tet_positions, dm_inv, vol, mu, lam, dt, method=2):
+    """Gradient [T,12] and PSD-projected Hessian [T,12,12] with the closed form and the
+    translation-free 9x9 reduction of the projection."""
+    from .projection import is_psd_batch
+    F, ic, J, cof, alpha = _snh_terms(tet_positions, dm_inv, mu, lam)
+    G = _shape_gradients(dm_inv)
+    s = dt * dt * vol
+    a1, a2, a3, u, w = _snh_block_terms(F, cof, ic, J, mu, lam, alpha, G, s)
+    grad = (a1[:, None, None] * u + a3[:, None, None] * w) * s[:, None, None]
+    T = F.shape[0]
+    Hb = _blocks_from_vectors(F, G, u, w, a1, a2, lam, a3, s)
+    H = Hb.transpose(0, 1, 3, 2, 4).reshape(T, 12, 12)
+    if method > 0:
+        bad = ~is_psd_batch(H)
+        if bad.any():
+            Gc = np.einsum("ak,taj->tkj", _U4, G[bad])
+            uc = np.einsum("ak,tai->tki", _U4, u[bad])
+            wc = np.einsum("ak,tai->tki", _U4, w[bad])
+            Mb = _blocks_from_vectors(F[bad], Gc, uc, wc, a1[bad], a2[bad], lam, a3[bad], s[bad])
+            M = Mb.transpose(0, 1, 3, 2, 4).reshape(-1, 9, 9)
+            lw, V = np.linalg.eigh(M)
+            lw = np.where(lw < 0, np.abs(lw) if method == 1 else 0.0, lw)
+            Mp = np.einsum("bik,bk,bjk->bij", V, lw, V).reshape(-1, 3, 3, 3, 3)
+            Hp = np.einsum("ak,bl,tkilj->taibj", _U4, _U4, Mp).reshape(-1, 12, 12)
+            H[bad] = Hp
+    return grad.reshape(T, 12), H

@@ -27,7 +27,7 @@ def test_header_declares_entry_points():
 def test_library_exports_every_declared_symbol():
     if not _lib.LIB_PATH.exists():
         pytest.skip("library not built (python -m yasps_amd.build)")
-    lib = ctypes.CDLL(str(_lib.LIB_PATH))
+    lib = _lib.load()
     missing = [n for n in declared_functions() if not hasattr(lib, n)]
     assert not missing, missing
 
@@ -41,3 +41,13 @@ def test_load_attaches_signatures():
         pytest.skip("library not built")
     lib = _lib.load()
     assert lib.ysp_version() >= 1
+
+
+def test_one_hip_runtime_in_process():
+    """The kernels and PyTorch must share one HIP runtime (one libamdhip64 mapped)."""
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("library not built")
+    _lib.load()
+    maps = Path("/proc/self/maps").read_text()
+    runtimes = sorted(set(re.findall(r"/\S*libamdhip64\S*", maps)))
+    assert len(runtimes) == 1, runtimes

@@ -86,6 +86,10 @@ def load(path: os.PathLike | None = None):
     if _lib is not None:
         return _lib
     p = Path(path) if path else LIB_PATH
+    # PyTorch-ROCm ships its own HIP runtime whose soname (libamdhip64.so.7) is the one
+    # the library needs: importing torch first makes the dynamic loader reuse that
+    # runtime instead of loading a second one from /opt/rocm.
+    import torch  # noqa: F401
     if not p.exists():
         raise ImportError(
             f"yasps_amd: {p} is missing. Build it with `python -m yasps_amd.build` "

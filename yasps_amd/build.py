@@ -43,6 +43,20 @@ def _compile(src: Path) -> Path:
     return obj
 
 
+def hip_runtime_dir() -> str:
+    """Directory of the HIP runtime the process uses. PyTorch-ROCm ships its own
+    libamdhip64.so; the library links that one so that torch and the kernels share one
+    runtime (streams, allocations and ordering are per runtime)."""
+    try:
+        import torch  # noqa: F401
+        d = Path(torch.__file__).resolve().parent / "lib"
+        if (d / "libamdhip64.so").exists():
+            return str(d)
+    except Exception:
+        pass
+    return "/opt/rocm/lib"
+
+
 def build(verbose: bool = True) -> Path:
     BUILD.mkdir(exist_ok=True)
     sources = sorted(CSRC.glob("*.hip"))
@@ -50,7 +64,10 @@ def build(verbose: bool = True) -> Path:
     with ThreadPoolExecutor(workers) as ex:
         objs = list(ex.map(_compile, sources))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", str(tmp), *map(str, objs)]
+    rt = hip_runtime_dir()
+    # plain host link: device code is already bundled in every object
+    cmd = ["g++", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs), f"-L{rt}",
+           "-l:libamdhip64.so", f"-Wl,-rpath,{rt}"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")

@@ -45,8 +45,10 @@ void* scratch(int slot, size_t bytes) {
       s.bytes = 0;
     }
     size_t want = bytes + bytes / 4;
-    if (hipMalloc(&s.ptr, want) != hipSuccess) {
+    const hipError_t e = hipMalloc(&s.ptr, want);
+    if (e != hipSuccess) {
       s.ptr = nullptr;
+      ysp_set_error(__FILE__, __LINE__, hipGetErrorString(e));
       return nullptr;
     }
     s.bytes = want;
