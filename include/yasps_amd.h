@@ -228,6 +228,44 @@ int ysp_pcg_solve(uint32_t max_iterations, double threshold, int32_t threshold_m
                   double* solution, const double* initial_guess, int32_t zero_initial_guess,
                   double* work, double* cg_scalars, ysp_pcg_stats* stats, void* stream);
 
+/* ---------------------------------------------- full 3x3 block-CSR (solve loop) */
+/* The Krylov loop runs on a mirrored full block-CSR built from the upper storage of
+ * the static and dynamic parts (all blocks 3x3): row_ptr [n/3 + 1], col [nnz] (block
+ * column), src [nnz] (bit 31 = stored transposed, bit 30 = dynamic part, low bits =
+ * upper block index). No reference counterpart: it is this build's layout of the
+ * operator spmvWithSystem applies (kernel/Solver/solverKernel.pyx:217-274). capacity
+ * bounds nnz (2 * (nb_static + nb_dynamic) always suffices). Synchronous. */
+int ysp_bsr3_build(const uint32_t* pos_static, uint32_t nb_static,
+                   const uint32_t* pos_dynamic, uint32_t nb_dynamic, uint32_t n,
+                   uint32_t* row_ptr, uint32_t* col, uint32_t* src, uint32_t capacity,
+                   uint32_t* nnz, void* stream);
+
+/* Component-major values of the full block-CSR: out[j * pad64(nnz) + k] is entry j
+ * (row-major) of block k; FP64 and/or FP32 copies (either pointer may be NULL). */
+int ysp_bsr3_gather_values(const double* static_values, const double* dynamic_values,
+                           const uint32_t* src, uint32_t nnz, double* out64, float* out32,
+                           void* stream);
+
+/* out[i] = (float) in[i] (FP32 copies of FP64 device arrays, e.g. inverse blocks). */
+int ysp_cast_f32(const double* in, float* out, uint64_t n, void* stream);
+
+/* y = H x over the full block-CSR (values FP64 or FP32, arithmetic FP64). */
+int ysp_bsr3_spmv(const uint32_t* row_ptr, const uint32_t* col, const void* values,
+                  int32_t values_f32, uint32_t nnz, uint32_t n, const double* x, double* y,
+                  void* stream);
+
+/* Block-Jacobi PCG on the full block-CSR with 3x3 diagonal blocks (the fast path of
+ * ysp_pcg_solve: two kernels per iteration, direction update fused into the SpMV
+ * gather). Same recurrence, stopping rules and return codes as ysp_pcg_solve; values
+ * and the inverse diagonal blocks may be FP32 (FP64 vectors, dots and scalars).
+ * work holds 5 * n doubles. Synchronous. */
+int ysp_pcg3_solve(uint32_t max_iterations, double threshold, int32_t threshold_mode,
+                   const uint32_t* row_ptr, const uint32_t* col, const void* values,
+                   int32_t values_f32, uint32_t nnz, const void* diagonal_inverse,
+                   int32_t inverse_f32, const double* rhs, uint32_t n, double* solution,
+                   const double* initial_guess, int32_t zero_initial_guess, double* work,
+                   double* cg_scalars, ysp_pcg_stats* stats, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

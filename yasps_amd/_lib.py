@@ -75,6 +75,12 @@ SIGNATURES = {
     "ysp_pcg_solve": (C.c_int, [U32, D, I32, C.POINTER(BlockMatrix), C.POINTER(BlockMatrix), P,
                                 C.POINTER(U32), C.POINTER(U32), C.POINTER(U32), C.POINTER(U32),
                                 I32, P, U32, P, P, I32, P, P, C.POINTER(PcgStats), P]),
+    "ysp_bsr3_build": (C.c_int, [P, U32, P, U32, U32, P, P, P, U32, C.POINTER(U32), P]),
+    "ysp_bsr3_gather_values": (C.c_int, [P, P, P, U32, P, P, P]),
+    "ysp_cast_f32": (C.c_int, [P, P, C.c_uint64, P]),
+    "ysp_bsr3_spmv": (C.c_int, [P, P, P, I32, U32, U32, P, P, P]),
+    "ysp_pcg3_solve": (C.c_int, [U32, D, I32, P, P, P, I32, U32, P, I32, P, U32, P, P, I32, P, P,
+                                 C.POINTER(PcgStats), P]),
 }
 
 _lib = None

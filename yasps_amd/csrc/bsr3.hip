@@ -1,0 +1,592 @@
+// Full (mirrored) 3x3 block-CSR used by the solve loop, its SpMV and the 3x3 PCG.
+//
+// The assembled Hessian keeps the reference's symmetric upper storage. For the Krylov
+// loop it is mirrored once per assembly into a full block-CSR whose rows own every block
+// they touch, so the SpMV is a pure row gather: no FP64 atomics (device-scope atomics
+// cross the eight XCDs' L2 caches on MI355X) and no zero-fill of the output.
+//
+// Layout: row_ptr[nbr + 1], col[nnz] (block column), src[nnz] (bit 31: stored
+// transposed, bit 30: dynamic part, low bits: upper block index) and component-major
+// values v[j * stride + k] (j = 0..8 row-major entry of block k, stride = nnz rounded up
+// to 64) so that 16 lanes reading 16 consecutive blocks issue fully coalesced loads.
+#include "common.hpp"
+#include "dense.hpp"
+
+#include <hipcub/hipcub.hpp>
+#include <algorithm>
+
+namespace ysp {
+namespace {
+
+constexpr unsigned kB = 256;
+constexpr uint32_t kTrans = 1u << 31;
+constexpr uint32_t kDyn = 1u << 30;
+
+__host__ __device__ __forceinline__ uint32_t pad64(uint32_t n) { return (n + 63u) & ~63u; }
+
+// Blocks per XCD-contiguous chunk: consecutive blockIdx values land on different XCDs
+// (round robin over 8), so remap them so that each XCD walks one contiguous range of
+// block rows and keeps its share of the gathered vector in its own L2.
+__device__ __forceinline__ uint32_t xcd_swizzle(uint32_t bid, uint32_t nblocks) {
+  constexpr uint32_t X = 8;
+  if (nblocks < X) return bid;
+  const uint32_t per = nblocks / X;
+  const uint32_t xcd = bid % X, slot = bid / X;
+  if (slot >= per) return bid;  // tail of a grid that is not a multiple of 8
+  return xcd * per + slot;
+}
+
+__global__ void emit_entries_kernel(const uint32_t* __restrict__ pos, uint32_t nb, uint32_t tag,
+                                    uint32_t nbr, uint64_t* __restrict__ keys,
+                                    uint32_t* __restrict__ vals, uint32_t* __restrict__ cursor) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const uint32_t r = pos[2 * b] / 3, c = pos[2 * b + 1] / 3;
+  const bool diag = r == c;
+  // wave-aggregated slot reservation
+  const uint32_t need = diag ? 1u : 2u;
+  const uint32_t slot = atomicAdd(cursor, need);
+  keys[slot] = static_cast<uint64_t>(r) * nbr + c;
+  vals[slot] = b | tag;
+  if (!diag) {
+    keys[slot + 1] = static_cast<uint64_t>(c) * nbr + r;
+    vals[slot + 1] = b | tag | kTrans;
+  }
+}
+
+__global__ void split_keys_kernel(const uint64_t* __restrict__ keys, uint32_t nnz, uint32_t nbr,
+                                  uint32_t* __restrict__ col, uint32_t* __restrict__ row_count) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  const uint64_t key = keys[k];
+  col[k] = static_cast<uint32_t>(key % nbr);
+  atomicAdd(row_count + static_cast<uint32_t>(key / nbr), 1u);
+}
+
+template <typename VT>
+__global__ void gather_values_kernel(const double* __restrict__ sv, const double* __restrict__ dv,
+                                     const uint32_t* __restrict__ src, uint32_t nnz,
+                                     VT* __restrict__ out) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  const uint32_t s = src[k];
+  const double* B = ((s & kDyn) ? dv : sv) + 9 * static_cast<size_t>(s & ~(kTrans | kDyn));
+  const uint32_t stride = pad64(nnz);
+  double v[9];
+#pragma unroll
+  for (int j = 0; j < 9; ++j) v[j] = B[j];
+  if (s & kTrans) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) out[(3 * i + j) * static_cast<size_t>(stride) + k] = static_cast<VT>(v[3 * j + i]);
+  } else {
+#pragma unroll
+    for (int j = 0; j < 9; ++j) out[j * static_cast<size_t>(stride) + k] = static_cast<VT>(v[j]);
+  }
+}
+
+constexpr int kG = 16;  // lanes per block row
+
+// One 16-lane group per block row. x_c is p_prev[c] * beta + z[c] when z != nullptr
+// (the CG direction update fused into the gather), x[c] otherwise.
+template <typename VT, bool kFused>
+__device__ __forceinline__ void row_product(const uint32_t* __restrict__ row_ptr,
+                                            const uint32_t* __restrict__ col,
+                                            const VT* __restrict__ v, uint32_t stride,
+                                            const double* __restrict__ x,
+                                            const double* __restrict__ z, double beta,
+                                            uint32_t r, int l, double& y0, double& y1,
+                                            double& y2) {
+  y0 = y1 = y2 = 0.0;
+  const uint32_t k0 = row_ptr[r], k1 = row_ptr[r + 1];
+  for (uint32_t k = k0 + l; k < k1; k += kG) {
+    const uint32_t c = 3 * col[k];
+    double x0, x1, x2;
+    if (kFused) {
+      x0 = fma(x[c], beta, z[c]);
+      x1 = fma(x[c + 1], beta, z[c + 1]);
+      x2 = fma(x[c + 2], beta, z[c + 2]);
+    } else {
+      x0 = x[c];
+      x1 = x[c + 1];
+      x2 = x[c + 2];
+    }
+    const double b0 = static_cast<double>(v[0 * static_cast<size_t>(stride) + k]);
+    const double b1 = static_cast<double>(v[1 * static_cast<size_t>(stride) + k]);
+    const double b2 = static_cast<double>(v[2 * static_cast<size_t>(stride) + k]);
+    const double b3 = static_cast<double>(v[3 * static_cast<size_t>(stride) + k]);
+    const double b4 = static_cast<double>(v[4 * static_cast<size_t>(stride) + k]);
+    const double b5 = static_cast<double>(v[5 * static_cast<size_t>(stride) + k]);
+    const double b6 = static_cast<double>(v[6 * static_cast<size_t>(stride) + k]);
+    const double b7 = static_cast<double>(v[7 * static_cast<size_t>(stride) + k]);
+    const double b8 = static_cast<double>(v[8 * static_cast<size_t>(stride) + k]);
+    y0 += b0 * x0 + b1 * x1 + b2 * x2;
+    y1 += b3 * x0 + b4 * x1 + b5 * x2;
+    y2 += b6 * x0 + b7 * x1 + b8 * x2;
+  }
+#pragma unroll
+  for (int off = kG / 2; off > 0; off >>= 1) {
+    y0 += __shfl_xor(y0, off, kG);
+    y1 += __shfl_xor(y1, off, kG);
+    y2 += __shfl_xor(y2, off, kG);
+  }
+}
+
+template <typename VT>
+__global__ void __launch_bounds__(kB)
+    bsr3_full_spmv_kernel(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
+                          const VT* __restrict__ v, uint32_t stride, uint32_t nbr,
+                          const double* __restrict__ x, double* __restrict__ y) {
+  const uint32_t blk = xcd_swizzle(blockIdx.x, gridDim.x);
+  const uint32_t r = blk * (kB / kG) + threadIdx.x / kG;
+  const int l = threadIdx.x % kG;
+  if (r >= nbr) return;
+  double y0, y1, y2;
+  row_product<VT, false>(row_ptr, col, v, stride, x, nullptr, 0.0, r, l, y0, y1, y2);
+  if (l == 0) {
+    y[3 * r] = y0;
+    y[3 * r + 1] = y1;
+    y[3 * r + 2] = y2;
+  }
+}
+
+// ------------------------------------------------------------------ PCG (3x3 blocks)
+enum : int {
+  S_RZ0 = 0, S_BB = 1, S_STATUS = 2, S_ITER = 3, S_THR = 4, S_MODE = 5,
+  S_PAP = 8, S_RZ = 11, S_RR = 14, S_FINAL_RZ = 17, S_FINAL_RR = 18, S_COUNT = 64
+};
+
+__device__ __forceinline__ void wg_add(double v, double* dst, double* lds) {
+  v = block_sum<kB>(v, lds);
+  if (threadIdx.x == 0) atomic_add_f64(dst, v);
+}
+
+template <typename MT>
+__device__ __forceinline__ void apply3(const MT* __restrict__ minv, uint32_t v, const double r[3],
+                                       double z[3]) {
+  const MT* M = minv + 9 * static_cast<size_t>(v);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    z[i] = static_cast<double>(M[3 * i]) * r[0] + static_cast<double>(M[3 * i + 1]) * r[1] +
+           static_cast<double>(M[3 * i + 2]) * r[2];
+}
+
+// r = b (or b - A x0 held in q), x = x0 or 0, z = M^-1 r, p buffers = 0.
+template <typename MT>
+__global__ void __launch_bounds__(kB)
+    pcg3_init_kernel(const MT* __restrict__ minv, const double* __restrict__ b,
+                     const double* __restrict__ x0, const double* __restrict__ ax0,
+                     double* __restrict__ x, double* __restrict__ r, double* __restrict__ z,
+                     double* __restrict__ pa, double* __restrict__ pb, uint32_t nv,
+                     double* __restrict__ st) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  double rz = 0, rr = 0, bmb = 0, bb = 0;
+  if (v < nv) {
+    double rl[3], bl[3], zl[3], tmp[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const uint32_t d = 3 * v + i;
+      bl[i] = b[d];
+      rl[i] = ax0 ? bl[i] - ax0[d] : bl[i];
+      x[d] = x0 ? x0[d] : 0.0;
+      pa[d] = 0.0;
+      pb[d] = 0.0;
+      r[d] = rl[i];
+    }
+    apply3(minv, v, rl, zl);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      z[3 * v + i] = zl[i];
+      rz += rl[i] * zl[i];
+      rr += rl[i] * rl[i];
+    }
+    if (x0) {
+      apply3(minv, v, bl, tmp);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        bmb += bl[i] * tmp[i];
+        bb += bl[i] * bl[i];
+      }
+    } else {
+      bmb = rz;
+      bb = rr;
+    }
+  }
+  __shared__ double lds[kB / 64];
+  wg_add(rz, st + S_RZ + 0, lds);
+  wg_add(rr, st + S_RR + 0, lds);
+  wg_add(bmb, st + S_RZ0, lds);
+  wg_add(bb, st + S_BB, lds);
+}
+
+__global__ void pcg3_init_finish_kernel(double* st, double tol, int mode) {
+  const double rz0 = st[S_RZ0], rz = st[S_RZ + 0], bb = st[S_BB], rr = st[S_RR + 0];
+  st[S_MODE] = mode;
+  st[S_FINAL_RZ] = rz;
+  st[S_FINAL_RR] = rr;
+  if (!isfinite(rz0) || rz0 < 0.0 || !isfinite(rz) || rz < 0.0) {
+    st[S_STATUS] = -2.0;
+    return;
+  }
+  const double thr = (mode == 0) ? tol * rz0 : tol * tol * bb;
+  st[S_THR] = thr;
+  if (!isfinite(thr) || thr < 0.0) {
+    st[S_STATUS] = -2.0;
+    return;
+  }
+  if ((mode == 0 ? rz : rr) <= thr) st[S_STATUS] = 1.0;
+}
+
+// Iteration k, kernel 1: convergence test of iteration k-1, beta, p_k = z + beta p_{k-1}
+// (fused into the gather), q = A p_k, p_k^T A p_k.
+template <typename VT>
+__global__ void __launch_bounds__(kB)
+    pcg3_spmv_kernel(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
+                     const VT* __restrict__ v, uint32_t stride, uint32_t nbr,
+                     const double* __restrict__ z, const double* __restrict__ p_prev,
+                     double* __restrict__ p_new, double* __restrict__ q, double* __restrict__ st,
+                     int k) {
+  if (st[S_STATUS] != 0.0) return;
+  const int cur = k % 3, prev = (k + 2) % 3, prev2 = (k + 1) % 3;
+  const double rz1 = st[S_RZ + prev];
+  const double rr1 = st[S_RR + prev];
+  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > 1) {
+    if (!isfinite(rz1) || rz1 < 0.0) {
+      if (gid == 0) {
+        st[S_STATUS] = -1.0;
+        st[S_ITER] = k - 1;
+      }
+      return;
+    }
+    if ((st[S_MODE] == 0.0 ? rz1 : rr1) <= st[S_THR]) {
+      if (gid == 0) {
+        st[S_STATUS] = 1.0;
+        st[S_ITER] = k - 1;
+        st[S_FINAL_RZ] = rz1;
+        st[S_FINAL_RR] = rr1;
+      }
+      return;
+    }
+  }
+  const double rz2 = st[S_RZ + prev2];
+  double beta = 0.0;
+  if (k > 1) {
+    if (!(rz2 > 0.0) || !isfinite(rz2)) {
+      if (gid == 0) {
+        st[S_STATUS] = -1.0;
+        st[S_ITER] = k - 1;
+      }
+      return;
+    }
+    beta = rz1 / rz2;
+  }
+  if (gid == 0) {
+    st[S_FINAL_RZ] = rz1;
+    st[S_FINAL_RR] = rr1;
+  }
+  const uint32_t blk = xcd_swizzle(blockIdx.x, gridDim.x);
+  const uint32_t r = blk * (kB / kG) + threadIdx.x / kG;
+  const int l = threadIdx.x % kG;
+  double pap = 0.0;
+  if (r < nbr) {
+    double y0, y1, y2;
+    row_product<VT, true>(row_ptr, col, v, stride, p_prev, z, beta, r, l, y0, y1, y2);
+    if (l == 0) {
+      const double p0 = fma(p_prev[3 * r], beta, z[3 * r]);
+      const double p1 = fma(p_prev[3 * r + 1], beta, z[3 * r + 1]);
+      const double p2 = fma(p_prev[3 * r + 2], beta, z[3 * r + 2]);
+      p_new[3 * r] = p0;
+      p_new[3 * r + 1] = p1;
+      p_new[3 * r + 2] = p2;
+      q[3 * r] = y0;
+      q[3 * r + 1] = y1;
+      q[3 * r + 2] = y2;
+      pap = p0 * y0 + p1 * y1 + p2 * y2;
+    }
+  }
+  __shared__ double lds[kB / 64];
+  wg_add(pap, st + S_PAP + cur, lds);
+}
+
+// Iteration k, kernel 2: alpha, x += alpha p, r -= alpha q, z = M^-1 r, r.z and r.r.
+template <typename MT>
+__global__ void __launch_bounds__(kB)
+    pcg3_update_kernel(const MT* __restrict__ minv, double* __restrict__ x, double* __restrict__ r,
+                       const double* __restrict__ p, const double* __restrict__ q,
+                       double* __restrict__ z, uint32_t nv, double* __restrict__ st, int k) {
+  if (st[S_STATUS] != 0.0) return;
+  const int cur = k % 3, prev = (k + 2) % 3, next = (k + 1) % 3;
+  const double pap = st[S_PAP + cur];
+  const double rz1 = st[S_RZ + prev];
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (!isfinite(pap) || pap <= 0.0) {
+    if (v == 0) {
+      st[S_STATUS] = -1.0;
+      st[S_ITER] = k;
+    }
+    return;
+  }
+  const double alpha = rz1 / pap;
+  if (v == 0) {
+    st[S_PAP + next] = 0.0;
+    st[S_RZ + next] = 0.0;
+    st[S_RR + next] = 0.0;
+  }
+  double rz = 0.0, rr = 0.0;
+  if (v < nv) {
+    double rl[3], zl[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const uint32_t d = 3 * v + i;
+      x[d] = fma(p[d], alpha, x[d]);
+      rl[i] = fma(-q[d], alpha, r[d]);
+      r[d] = rl[i];
+    }
+    apply3(minv, v, rl, zl);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      z[3 * v + i] = zl[i];
+      rz += rl[i] * zl[i];
+      rr += rl[i] * rl[i];
+    }
+  }
+  __shared__ double lds[kB / 64];
+  wg_add(rz, st + S_RZ + cur, lds);
+  wg_add(rr, st + S_RR + cur, lds);
+}
+
+double* status_box() {
+  static double* box = nullptr;
+  if (!box && hipHostMalloc(reinterpret_cast<void**>(&box), 8 * sizeof(double),
+                            hipHostMallocDefault) != hipSuccess)
+    box = nullptr;
+  return box;
+}
+
+template <typename VT>
+void launch_full_spmv(const uint32_t* row_ptr, const uint32_t* col, const VT* v, uint32_t nnz,
+                      uint32_t nbr, const double* x, double* y, hipStream_t s) {
+  const unsigned rows_per_block = kB / kG;
+  hipLaunchKernelGGL(bsr3_full_spmv_kernel<VT>, dim3(grid_for(nbr, rows_per_block)), dim3(kB), 0,
+                     s, row_ptr, col, v, pad64(nnz), nbr, x, y);
+}
+
+}  // namespace
+}  // namespace ysp
+
+using namespace ysp;
+
+extern "C" int ysp_bsr3_build(const uint32_t* pos_static, uint32_t nb_static,
+                              const uint32_t* pos_dynamic, uint32_t nb_dynamic, uint32_t n,
+                              uint32_t* row_ptr, uint32_t* col, uint32_t* src, uint32_t capacity,
+                              uint32_t* nnz_out, void* stream) {
+  if (!nnz_out || n % 3) {
+    ysp_set_error(__FILE__, __LINE__, "ysp_bsr3_build: bad arguments");
+    return YSP_ERR_ARG;
+  }
+  *nnz_out = 0;
+  hipStream_t s = as_stream(stream);
+  const uint32_t nbr = n / 3;
+  const size_t cap = 2 * (static_cast<size_t>(nb_static) + nb_dynamic);
+  if (cap == 0) {
+    YSP_CHECK(hipMemsetAsync(row_ptr, 0, sizeof(uint32_t) * (nbr + 1), s));
+    return YSP_OK;
+  }
+  uint64_t* keys = static_cast<uint64_t*>(scratch(40, sizeof(uint64_t) * cap));
+  uint32_t* vals = static_cast<uint32_t*>(scratch(41, sizeof(uint32_t) * cap));
+  uint64_t* keys2 = static_cast<uint64_t*>(scratch(42, sizeof(uint64_t) * cap));
+  uint32_t* cursor = static_cast<uint32_t*>(scratch(43, sizeof(uint32_t) * (nbr + 2)));
+  if (!keys || !vals || !keys2 || !cursor) return YSP_ERR_HIP;
+  YSP_CHECK(hipMemsetAsync(cursor, 0, sizeof(uint32_t), s));
+  if (nb_static)
+    hipLaunchKernelGGL(emit_entries_kernel, dim3(grid_for(nb_static, kB)), dim3(kB), 0, s,
+                       pos_static, nb_static, 0u, nbr, keys, vals, cursor);
+  if (nb_dynamic)
+    hipLaunchKernelGGL(emit_entries_kernel, dim3(grid_for(nb_dynamic, kB)), dim3(kB), 0, s,
+                       pos_dynamic, nb_dynamic, kDyn, nbr, keys, vals, cursor);
+  YSP_CHECK_LAUNCH();
+  uint32_t nnz = 0;
+  YSP_CHECK(hipMemcpyAsync(&nnz, cursor, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  YSP_CHECK(hipStreamSynchronize(s));
+  if (nnz > capacity) {
+    ysp_set_error(__FILE__, __LINE__, "ysp_bsr3_build: capacity too small");
+    return YSP_ERR_ARG;
+  }
+  // sort (row, col) keys; the value keeps the source block and its orientation
+  int end_bit = 1;
+  const uint64_t maxkey = static_cast<uint64_t>(nbr) * nbr;
+  while (end_bit < 64 && (1ull << end_bit) <= maxkey) ++end_bit;
+  hipcub::DoubleBuffer<uint64_t> kb(keys, keys2);
+  hipcub::DoubleBuffer<uint32_t> vb(vals, src);
+  size_t temp = 0;
+  YSP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, temp, kb, vb, nnz, 0, end_bit, s));
+  void* tmp = scratch(44, temp);
+  if (!tmp) return YSP_ERR_HIP;
+  YSP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, temp, kb, vb, nnz, 0, end_bit, s));
+  if (vb.Current() != src)
+    YSP_CHECK(hipMemcpyAsync(src, vb.Current(), sizeof(uint32_t) * nnz, hipMemcpyDeviceToDevice, s));
+  YSP_CHECK(hipMemsetAsync(cursor, 0, sizeof(uint32_t) * (nbr + 1), s));
+  hipLaunchKernelGGL(split_keys_kernel, dim3(grid_for(nnz, kB)), dim3(kB), 0, s, kb.Current(), nnz,
+                     nbr, col, cursor);
+  YSP_CHECK_LAUNCH();
+  size_t t2 = 0;
+  YSP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, cursor, row_ptr, nbr + 1, s));
+  tmp = scratch(44, t2);
+  if (!tmp) return YSP_ERR_HIP;
+  YSP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, t2, cursor, row_ptr, nbr + 1, s));
+  YSP_CHECK(hipStreamSynchronize(s));
+  *nnz_out = nnz;
+  return YSP_OK;
+}
+
+extern "C" int ysp_bsr3_gather_values(const double* static_values, const double* dynamic_values,
+                                      const uint32_t* src, uint32_t nnz, double* out64,
+                                      float* out32, void* stream) {
+  if (nnz == 0) return YSP_OK;
+  hipStream_t s = as_stream(stream);
+  if (out64)
+    hipLaunchKernelGGL(gather_values_kernel<double>, dim3(grid_for(nnz, kB)), dim3(kB), 0, s,
+                       static_values, dynamic_values, src, nnz, out64);
+  if (out32)
+    hipLaunchKernelGGL(gather_values_kernel<float>, dim3(grid_for(nnz, kB)), dim3(kB), 0, s,
+                       static_values, dynamic_values, src, nnz, out32);
+  YSP_CHECK_LAUNCH();
+  return YSP_OK;
+}
+
+extern "C" int ysp_bsr3_spmv(const uint32_t* row_ptr, const uint32_t* col, const void* values,
+                             int32_t values_f32, uint32_t nnz, uint32_t n, const double* x,
+                             double* y, void* stream) {
+  if (n % 3) return YSP_ERR_ARG;
+  if (n == 0) return YSP_OK;
+  hipStream_t s = as_stream(stream);
+  if (values_f32)
+    launch_full_spmv(row_ptr, col, static_cast<const float*>(values), nnz, n / 3, x, y, s);
+  else
+    launch_full_spmv(row_ptr, col, static_cast<const double*>(values), nnz, n / 3, x, y, s);
+  YSP_CHECK_LAUNCH();
+  return YSP_OK;
+}
+
+namespace {
+__global__ void cast_f32_kernel(const double* __restrict__ in, float* __restrict__ out, size_t n) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = static_cast<float>(in[i]);
+}
+}  // namespace
+
+extern "C" int ysp_cast_f32(const double* in, float* out, uint64_t n, void* stream) {
+  if (n == 0) return YSP_OK;
+  hipLaunchKernelGGL(cast_f32_kernel, dim3(grid_for(n, kB)), dim3(kB), 0, as_stream(stream), in,
+                     out, static_cast<size_t>(n));
+  YSP_CHECK_LAUNCH();
+  return YSP_OK;
+}
+
+namespace {
+template <typename VT, typename MT>
+int pcg3_run(uint32_t max_iterations, double threshold, int32_t mode, const uint32_t* row_ptr,
+             const uint32_t* col, const VT* v, uint32_t nnz, const MT* minv, const double* rhs,
+             uint32_t n, double* x, const double* x0, double* work, double* st,
+             ysp_pcg_stats* stats, hipStream_t s) {
+  const uint32_t nv = n / 3;
+  double* r = work;
+  double* z = work + n;
+  double* pa = work + 2 * static_cast<size_t>(n);
+  double* pb = work + 3 * static_cast<size_t>(n);
+  double* q = work + 4 * static_cast<size_t>(n);
+  const uint32_t stride = pad64(nnz);
+  YSP_CHECK(hipMemsetAsync(st, 0, S_COUNT * sizeof(double), s));
+  const double* ax0 = nullptr;
+  if (x0) {
+    launch_full_spmv(row_ptr, col, v, nnz, nv, x0, q, s);
+    ax0 = q;
+  }
+  const dim3 vg(grid_for(nv, kB)), sg(grid_for(nv, kB / kG)), bl(kB);
+  hipLaunchKernelGGL(pcg3_init_kernel<MT>, vg, bl, 0, s, minv, rhs, x0, ax0, x, r, z, pa, pb, nv, st);
+  hipLaunchKernelGGL(pcg3_init_finish_kernel, dim3(1), dim3(1), 0, s, st, threshold, mode);
+  YSP_CHECK_LAUNCH();
+  double* box = status_box();
+  if (!box) {
+    ysp_set_error(__FILE__, __LINE__, "pinned status allocation failed");
+    return YSP_ERR_HIP;
+  }
+  YSP_CHECK(hipMemcpyAsync(box, st, 8 * sizeof(double), hipMemcpyDeviceToHost, s));
+  YSP_CHECK(hipStreamSynchronize(s));
+  uint32_t k = 1;
+  uint32_t chunk = 8;
+  // one extra pass lets the spmv kernel of iteration max+1 record convergence of max
+  while (box[S_STATUS] == 0.0 && k <= max_iterations + 1) {
+    const uint32_t last = std::min<uint32_t>(max_iterations + 1, k + chunk - 1);
+    for (uint32_t it = k; it <= last; ++it) {
+      double* pn = (it & 1) ? pa : pb;
+      double* pp = (it & 1) ? pb : pa;
+      hipLaunchKernelGGL(pcg3_spmv_kernel<VT>, sg, bl, 0, s, row_ptr, col, v, stride, nv, z, pp, pn,
+                         q, st, static_cast<int>(it));
+      if (it <= max_iterations)
+        hipLaunchKernelGGL(pcg3_update_kernel<MT>, vg, bl, 0, s, minv, x, r, pn, q, z, nv, st,
+                           static_cast<int>(it));
+    }
+    YSP_CHECK_LAUNCH();
+    YSP_CHECK(hipMemcpyAsync(box, st, 8 * sizeof(double), hipMemcpyDeviceToHost, s));
+    YSP_CHECK(hipStreamSynchronize(s));
+    k = last + 1;
+    chunk = std::min<uint32_t>(chunk * 2, 64);
+  }
+  double fin[S_COUNT];
+  YSP_CHECK(hipMemcpyAsync(fin, st, sizeof(fin), hipMemcpyDeviceToHost, s));
+  YSP_CHECK(hipStreamSynchronize(s));
+  int code;
+  const int iters = static_cast<int>(fin[S_ITER]);
+  if (fin[S_STATUS] == 1.0) code = iters;
+  else if (fin[S_STATUS] == -1.0) code = -iters - 4;
+  else if (fin[S_STATUS] == -2.0) code = -5;
+  else code = -4;
+  if (stats) {
+    stats->iterations = fin[S_STATUS] == 0.0 ? static_cast<int>(max_iterations) : iters;
+    stats->status = code;
+    stats->delta0 = fin[S_RZ0];
+    stats->delta = fin[S_FINAL_RZ];
+    stats->rr0 = fin[S_BB];
+    stats->rr = fin[S_FINAL_RR];
+  }
+  return code;
+}
+}  // namespace
+
+extern "C" int ysp_pcg3_solve(uint32_t max_iterations, double threshold, int32_t threshold_mode,
+                              const uint32_t* row_ptr, const uint32_t* col, const void* values,
+                              int32_t values_f32, uint32_t nnz, const void* diagonal_inverse,
+                              int32_t inverse_f32, const double* rhs, uint32_t n,
+                              double* solution, const double* initial_guess,
+                              int32_t zero_initial_guess, double* work, double* cg_scalars,
+                              ysp_pcg_stats* stats, void* stream) {
+  if (stats) *stats = ysp_pcg_stats{0, 0, 0, 0, 0, 0};
+  if (n == 0) return 0;
+  if (n % 3 || !work || !cg_scalars || (threshold_mode != 0 && threshold_mode != 1)) {
+    ysp_set_error(__FILE__, __LINE__, "ysp_pcg3_solve: bad arguments");
+    return YSP_ERR_ARG;
+  }
+  hipStream_t s = as_stream(stream);
+  const double* x0 = (!zero_initial_guess && initial_guess) ? initial_guess : nullptr;
+  if (values_f32) {
+    const float* v = static_cast<const float*>(values);
+    if (inverse_f32)
+      return pcg3_run(max_iterations, threshold, threshold_mode, row_ptr, col, v, nnz,
+                      static_cast<const float*>(diagonal_inverse), rhs, n, solution, x0, work,
+                      cg_scalars, stats, s);
+    return pcg3_run(max_iterations, threshold, threshold_mode, row_ptr, col, v, nnz,
+                    static_cast<const double*>(diagonal_inverse), rhs, n, solution, x0, work,
+                    cg_scalars, stats, s);
+  }
+  const double* v = static_cast<const double*>(values);
+  if (inverse_f32)
+    return pcg3_run(max_iterations, threshold, threshold_mode, row_ptr, col, v, nnz,
+                    static_cast<const float*>(diagonal_inverse), rhs, n, solution, x0, work,
+                    cg_scalars, stats, s);
+  return pcg3_run(max_iterations, threshold, threshold_mode, row_ptr, col, v, nnz,
+                  static_cast<const double*>(diagonal_inverse), rhs, n, solution, x0, work,
+                  cg_scalars, stats, s);
+}

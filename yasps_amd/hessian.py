@@ -167,6 +167,19 @@ def build_part(terms: List[TermIndex], stream) -> Part:
     return part
 
 
+@dataclass
+class FullBsr:
+    """Mirrored full 3x3 block-CSR of the current layout (the solve loop's operator)."""
+    key: tuple
+    nbr: int
+    row_ptr: object
+    col: object
+    src: object
+    nnz: int
+    v64: object = None
+    v32: object = None
+
+
 class hessian:
     """Sum of energy terms w.r.t. a list of targets, in symmetric upper block storage."""
 
@@ -191,6 +204,8 @@ class hessian:
         self.dynamic_index: List[TermIndex] = []
         self.static = Part()
         self.dynamic = Part()
+        self.full: Optional[FullBsr] = None
+        self.want_f32 = False
 
     # --------------------------------------------------------------- layout
     def _ensure_setup(self):
@@ -283,11 +298,56 @@ class hessian:
         for (expr, method), ti in zip(self.dynamic_terms, self.dynamic_index):
             if ti.n:
                 expr.evaluate(self._scatter_args(ti, self.dynamic), method)
+        self._update_full()
         return self
 
-    def spmv(self, vec: vector, transpose: bool = False) -> vector:
+    # --------------------------------------------------------------- full block-CSR
+    @property
+    def all_3x3(self) -> bool:
+        return all(w.size == 3 for w in self.wrt) and all(
+            d == 3 for part in (self.static, self.dynamic) for d in part.dims)
+
+    def _update_full(self):
+        if not self.all_3x3:
+            self.full = None
+            return
+        lib = _lib.load()
+        s = dev.stream()
+        key = (id(self.static.positions), self.static.num_blocks,
+               id(self.dynamic.positions), self.dynamic.num_blocks)
+        if self.full is None or self.full.key != key:
+            cap = max(1, 2 * (self.static.num_blocks + self.dynamic.num_blocks))
+            nbr = self.rows // 3
+            row_ptr = dev.empty(nbr + 1, "uint32")
+            col = dev.empty(cap, "uint32")
+            src = dev.empty(cap, "uint32")
+            nnz = C.c_uint32(0)
+            _lib.check(lib.ysp_bsr3_build(dev.ptr(self.static.positions), self.static.num_blocks,
+                                          dev.ptr(self.dynamic.positions), self.dynamic.num_blocks,
+                                          self.rows, dev.ptr(row_ptr), dev.ptr(col), dev.ptr(src),
+                                          cap, C.byref(nnz), s), "ysp_bsr3_build")
+            self.full = FullBsr(key, nbr, row_ptr, col, src, nnz.value)
+        f = self.full
+        stride = (f.nnz + 63) // 64 * 64
+        if f.v64 is None or f.v64.numel() != 9 * stride:
+            f.v64 = dev.empty(9 * stride, "float64")
+        if self.want_f32 and (f.v32 is None or f.v32.numel() != 9 * stride):
+            f.v32 = dev.empty(9 * stride, "float32")
+        _lib.check(lib.ysp_bsr3_gather_values(dev.ptr(self.static.values), dev.ptr(self.dynamic.values),
+                                              dev.ptr(f.src), f.nnz, dev.ptr(f.v64),
+                                              dev.ptr(f.v32) if self.want_f32 else None, s),
+                   "ysp_bsr3_gather_values")
+
+    def spmv(self, vec: vector, transpose: bool = False, use_f32: bool = False) -> vector:
         self._ensure_setup()
         out = vector(self.rows)
+        if self.full is not None and not transpose:
+            f = self.full
+            vals = f.v32 if use_f32 else f.v64
+            _lib.check(_lib.load().ysp_bsr3_spmv(dev.ptr(f.row_ptr), dev.ptr(f.col), dev.ptr(vals),
+                                                 int(use_f32), f.nnz, self.rows, dev.ptr(vec.value),
+                                                 dev.ptr(out.value), dev.stream()), "ysp_bsr3_spmv")
+            return out
         a, b = self.static.ctype(), self.dynamic.ctype()
         _lib.check(_lib.load().ysp_block_spmv(C.byref(a) if a else None, C.byref(b) if b else None,
                                               dev.ptr(vec.value), dev.ptr(out.value), self.rows,

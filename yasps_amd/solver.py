@@ -80,19 +80,29 @@ class jacobianPCGSolver:
             dev.ptr(h.diagonal_blocks), dev.ptr(h.diagonal_blocks_inverse), _lib.u32_array(starts),
             _lib.u32_array(counts), _lib.u32_array(sizes), len(h.wrt), s), "ysp_invert_diagonal_blocks")
         t2 = time.perf_counter()
-        a, b = h.static.ctype(), h.dynamic.ctype()
         stats = _lib.PcgStats()
         guess = None
         if not zero_initial_guess and initial_guess is not None:
             guess = initial_guess.value if hasattr(initial_guess, "value") else initial_guess
-        rc = lib.ysp_pcg_solve(
-            int(maxIterations), float(tolerance), _MODES[self.threshold_mode],
-            C.byref(a) if a else None, C.byref(b) if b else None, dev.ptr(h.diagonal_blocks_inverse),
-            _lib.u32_array(starts), _lib.u32_array(counts), _lib.u32_array(sizes),
-            _lib.u32_array(h.gradient_segments_start_cpu[:-1]), len(h.wrt),
-            dev.ptr(right_hand_side.value), n, dev.ptr(self.__solution), dev.ptr(guess),
-            int(bool(zero_initial_guess) or guess is None), dev.ptr(self.__work),
-            dev.ptr(self.__scalars), C.byref(stats), s)
+        zero = int(bool(zero_initial_guess) or guess is None)
+        if h.full is not None:
+            f = h.full
+            rc = lib.ysp_pcg3_solve(
+                int(maxIterations), float(tolerance), _MODES[self.threshold_mode],
+                dev.ptr(f.row_ptr), dev.ptr(f.col), dev.ptr(f.v64), 0, f.nnz,
+                dev.ptr(h.diagonal_blocks_inverse), 0, dev.ptr(right_hand_side.value), n,
+                dev.ptr(self.__solution), dev.ptr(guess), zero, dev.ptr(self.__work),
+                dev.ptr(self.__scalars), C.byref(stats), s)
+        else:
+            a, b = h.static.ctype(), h.dynamic.ctype()
+            rc = lib.ysp_pcg_solve(
+                int(maxIterations), float(tolerance), _MODES[self.threshold_mode],
+                C.byref(a) if a else None, C.byref(b) if b else None,
+                dev.ptr(h.diagonal_blocks_inverse), _lib.u32_array(starts), _lib.u32_array(counts),
+                _lib.u32_array(sizes), _lib.u32_array(h.gradient_segments_start_cpu[:-1]),
+                len(h.wrt), dev.ptr(right_hand_side.value), n, dev.ptr(self.__solution),
+                dev.ptr(guess), zero, dev.ptr(self.__work), dev.ptr(self.__scalars),
+                C.byref(stats), s)
         _lib.check(rc, "ysp_pcg_solve")
         t3 = time.perf_counter()
         self.iterations = max(0, stats.iterations)
