@@ -258,7 +258,8 @@ int ysp_bsr3_spmv(const uint32_t* row_ptr, const uint32_t* col, const void* valu
  * ysp_pcg_solve: two kernels per iteration, direction update fused into the SpMV
  * gather). Same recurrence, stopping rules and return codes as ysp_pcg_solve; values
  * and the inverse diagonal blocks may be FP32 (FP64 vectors, dots and scalars).
- * work holds 5 * n doubles. Synchronous. */
+ * work holds 5 * n doubles, cg_scalars YSP_PCG3_SCALARS doubles. Synchronous. */
+#define YSP_PCG3_SCALARS 12288
 int ysp_pcg3_solve(uint32_t max_iterations, double threshold, int32_t threshold_mode,
                    const uint32_t* row_ptr, const uint32_t* col, const void* values,
                    int32_t values_f32, uint32_t nnz, const void* diagonal_inverse,

@@ -152,14 +152,54 @@ __global__ void __launch_bounds__(kB)
 }
 
 // ------------------------------------------------------------------ PCG (3x3 blocks)
+// Scalar state: a header of named doubles, then the partial-sum slots. Each workgroup
+// adds its partial dot into one of kSlots slots (blockIdx mod kSlots) spaced one cache
+// line apart, so the device-scope FP64 atomics of thousands of workgroups spread over
+// 64 addresses instead of serialising on one. Consumers reduce the 64 slots with one
+// wavefront load and a fixed shuffle tree, so every workgroup sees the same sum.
 enum : int {
   S_RZ0 = 0, S_BB = 1, S_STATUS = 2, S_ITER = 3, S_THR = 4, S_MODE = 5,
-  S_PAP = 8, S_RZ = 11, S_RR = 14, S_FINAL_RZ = 17, S_FINAL_RR = 18, S_COUNT = 64
+  S_FINAL_RZ = 17, S_FINAL_RR = 18, S_HEADER = 64
 };
+constexpr int kSlots = 64;
+constexpr int kSpace = 16;                       // doubles between slots (128 B)
+constexpr int kSet = kSlots * kSpace;            // one partial-sum array
+enum : int { SET_PAP = 0, SET_RZ = 3, SET_RR = 6, SET_RZ0 = 9, SET_BB = 10, SET_COUNT = 11 };
+
+__host__ __device__ constexpr int slots_base(int set) { return S_HEADER + set * kSet; }
+
+__device__ __forceinline__ double* slot_ptr(double* st, int set) {
+  return st + slots_base(set) + (blockIdx.x % kSlots) * kSpace;
+}
+
+// Sum of one slot array, identical in every thread of the workgroup.
+__device__ __forceinline__ double read_slots(const double* st, int set, double* lds) {
+  if (threadIdx.x < 64) {
+    double v = st[slots_base(set) + threadIdx.x * kSpace];
+    v = wave_sum(v);
+    if (threadIdx.x == 0) lds[0] = v;
+  }
+  __syncthreads();
+  const double r = lds[0];
+  __syncthreads();
+  return r;
+}
+
+// Status read once per workgroup so that every thread takes the same early exit.
+__device__ __forceinline__ bool stopped(const double* st) {
+  __shared__ double s_status;
+  if (threadIdx.x == 0) s_status = st[S_STATUS];
+  __syncthreads();
+  return s_status != 0.0;
+}
 
 __device__ __forceinline__ void wg_add(double v, double* dst, double* lds) {
   v = block_sum<kB>(v, lds);
   if (threadIdx.x == 0) atomic_add_f64(dst, v);
+}
+
+__device__ __forceinline__ void zero_set(double* st, int set) {
+  if (threadIdx.x < kSlots) st[slots_base(set) + threadIdx.x * kSpace] = 0.0;
 }
 
 template <typename MT>
@@ -172,7 +212,7 @@ __device__ __forceinline__ void apply3(const MT* __restrict__ minv, uint32_t v, 
            static_cast<double>(M[3 * i + 2]) * r[2];
 }
 
-// r = b (or b - A x0 held in q), x = x0 or 0, z = M^-1 r, p buffers = 0.
+// r = b (or b - A x0 held in ax0), x = x0 or 0, z = M^-1 r, p buffers = 0.
 template <typename MT>
 __global__ void __launch_bounds__(kB)
     pcg3_init_kernel(const MT* __restrict__ minv, const double* __restrict__ b,
@@ -214,14 +254,21 @@ __global__ void __launch_bounds__(kB)
     }
   }
   __shared__ double lds[kB / 64];
-  wg_add(rz, st + S_RZ + 0, lds);
-  wg_add(rr, st + S_RR + 0, lds);
-  wg_add(bmb, st + S_RZ0, lds);
-  wg_add(bb, st + S_BB, lds);
+  wg_add(rz, slot_ptr(st, SET_RZ + 0), lds);
+  wg_add(rr, slot_ptr(st, SET_RR + 0), lds);
+  wg_add(bmb, slot_ptr(st, SET_RZ0), lds);
+  wg_add(bb, slot_ptr(st, SET_BB), lds);
 }
 
-__global__ void pcg3_init_finish_kernel(double* st, double tol, int mode) {
-  const double rz0 = st[S_RZ0], rz = st[S_RZ + 0], bb = st[S_BB], rr = st[S_RR + 0];
+__global__ void __launch_bounds__(64) pcg3_init_finish_kernel(double* st, double tol, int mode) {
+  __shared__ double lds[1];
+  const double rz0 = read_slots(st, SET_RZ0, lds);
+  const double rz = read_slots(st, SET_RZ + 0, lds);
+  const double bb = read_slots(st, SET_BB, lds);
+  const double rr = read_slots(st, SET_RR + 0, lds);
+  if (threadIdx.x != 0) return;
+  st[S_RZ0] = rz0;
+  st[S_BB] = bb;
   st[S_MODE] = mode;
   st[S_FINAL_RZ] = rz;
   st[S_FINAL_RR] = rr;
@@ -247,21 +294,24 @@ __global__ void __launch_bounds__(kB)
                      const double* __restrict__ z, const double* __restrict__ p_prev,
                      double* __restrict__ p_new, double* __restrict__ q, double* __restrict__ st,
                      int k) {
-  if (st[S_STATUS] != 0.0) return;
+  if (stopped(st)) return;
   const int cur = k % 3, prev = (k + 2) % 3, prev2 = (k + 1) % 3;
-  const double rz1 = st[S_RZ + prev];
-  const double rr1 = st[S_RR + prev];
-  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double lds[kB / 64];
+  const double rz1 = read_slots(st, SET_RZ + prev, lds);
+  const double rr1 = read_slots(st, SET_RR + prev, lds);
+  const double rz2 = read_slots(st, SET_RZ + prev2, lds);
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  double beta = 0.0;
   if (k > 1) {
-    if (!isfinite(rz1) || rz1 < 0.0) {
-      if (gid == 0) {
+    if (!isfinite(rz1) || rz1 < 0.0 || !isfinite(rz2) || !(rz2 > 0.0)) {
+      if (lead) {
         st[S_STATUS] = -1.0;
         st[S_ITER] = k - 1;
       }
       return;
     }
     if ((st[S_MODE] == 0.0 ? rz1 : rr1) <= st[S_THR]) {
-      if (gid == 0) {
+      if (lead) {
         st[S_STATUS] = 1.0;
         st[S_ITER] = k - 1;
         st[S_FINAL_RZ] = rz1;
@@ -269,20 +319,9 @@ __global__ void __launch_bounds__(kB)
       }
       return;
     }
-  }
-  const double rz2 = st[S_RZ + prev2];
-  double beta = 0.0;
-  if (k > 1) {
-    if (!(rz2 > 0.0) || !isfinite(rz2)) {
-      if (gid == 0) {
-        st[S_STATUS] = -1.0;
-        st[S_ITER] = k - 1;
-      }
-      return;
-    }
     beta = rz1 / rz2;
   }
-  if (gid == 0) {
+  if (lead) {
     st[S_FINAL_RZ] = rz1;
     st[S_FINAL_RR] = rr1;
   }
@@ -306,8 +345,7 @@ __global__ void __launch_bounds__(kB)
       pap = p0 * y0 + p1 * y1 + p2 * y2;
     }
   }
-  __shared__ double lds[kB / 64];
-  wg_add(pap, st + S_PAP + cur, lds);
+  wg_add(pap, slot_ptr(st, SET_PAP + cur), lds);
 }
 
 // Iteration k, kernel 2: alpha, x += alpha p, r -= alpha q, z = M^-1 r, r.z and r.r.
@@ -316,10 +354,11 @@ __global__ void __launch_bounds__(kB)
     pcg3_update_kernel(const MT* __restrict__ minv, double* __restrict__ x, double* __restrict__ r,
                        const double* __restrict__ p, const double* __restrict__ q,
                        double* __restrict__ z, uint32_t nv, double* __restrict__ st, int k) {
-  if (st[S_STATUS] != 0.0) return;
+  if (stopped(st)) return;
   const int cur = k % 3, prev = (k + 2) % 3, next = (k + 1) % 3;
-  const double pap = st[S_PAP + cur];
-  const double rz1 = st[S_RZ + prev];
+  __shared__ double lds[kB / 64];
+  const double pap = read_slots(st, SET_PAP + cur, lds);
+  const double rz1 = read_slots(st, SET_RZ + prev, lds);
   const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   if (!isfinite(pap) || pap <= 0.0) {
     if (v == 0) {
@@ -329,10 +368,10 @@ __global__ void __launch_bounds__(kB)
     return;
   }
   const double alpha = rz1 / pap;
-  if (v == 0) {
-    st[S_PAP + next] = 0.0;
-    st[S_RZ + next] = 0.0;
-    st[S_RR + next] = 0.0;
+  if (blockIdx.x == 0) {
+    zero_set(st, SET_PAP + next);
+    zero_set(st, SET_RZ + next);
+    zero_set(st, SET_RR + next);
   }
   double rz = 0.0, rr = 0.0;
   if (v < nv) {
@@ -352,9 +391,8 @@ __global__ void __launch_bounds__(kB)
       rr += rl[i] * rl[i];
     }
   }
-  __shared__ double lds[kB / 64];
-  wg_add(rz, st + S_RZ + cur, lds);
-  wg_add(rr, st + S_RR + cur, lds);
+  wg_add(rz, slot_ptr(st, SET_RZ + cur), lds);
+  wg_add(rr, slot_ptr(st, SET_RR + cur), lds);
 }
 
 double* status_box() {
@@ -498,7 +536,7 @@ int pcg3_run(uint32_t max_iterations, double threshold, int32_t mode, const uint
   double* pb = work + 3 * static_cast<size_t>(n);
   double* q = work + 4 * static_cast<size_t>(n);
   const uint32_t stride = pad64(nnz);
-  YSP_CHECK(hipMemsetAsync(st, 0, S_COUNT * sizeof(double), s));
+  YSP_CHECK(hipMemsetAsync(st, 0, slots_base(SET_COUNT) * sizeof(double), s));
   const double* ax0 = nullptr;
   if (x0) {
     launch_full_spmv(row_ptr, col, v, nnz, nv, x0, q, s);
@@ -506,7 +544,7 @@ int pcg3_run(uint32_t max_iterations, double threshold, int32_t mode, const uint
   }
   const dim3 vg(grid_for(nv, kB)), sg(grid_for(nv, kB / kG)), bl(kB);
   hipLaunchKernelGGL(pcg3_init_kernel<MT>, vg, bl, 0, s, minv, rhs, x0, ax0, x, r, z, pa, pb, nv, st);
-  hipLaunchKernelGGL(pcg3_init_finish_kernel, dim3(1), dim3(1), 0, s, st, threshold, mode);
+  hipLaunchKernelGGL(pcg3_init_finish_kernel, dim3(1), dim3(64), 0, s, st, threshold, mode);
   YSP_CHECK_LAUNCH();
   double* box = status_box();
   if (!box) {
@@ -535,7 +573,7 @@ int pcg3_run(uint32_t max_iterations, double threshold, int32_t mode, const uint
     k = last + 1;
     chunk = std::min<uint32_t>(chunk * 2, 64);
   }
-  double fin[S_COUNT];
+  double fin[S_HEADER];
   YSP_CHECK(hipMemcpyAsync(fin, st, sizeof(fin), hipMemcpyDeviceToHost, s));
   YSP_CHECK(hipStreamSynchronize(s));
   int code;

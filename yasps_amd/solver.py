@@ -49,7 +49,7 @@ class jacobianPCGSolver:
         if self.__solution is None or self.__solution.numel() != n:
             self.__solution = dev.zeros(n, "float64")
             self.__work = dev.zeros(5 * n, "float64")
-            self.__scalars = dev.zeros(64, "float64")
+            self.__scalars = dev.zeros(12288, "float64")   # YSP_PCG3_SCALARS
 
     def computeSolution(self, active_hessian, right_hand_side, initial_guess=None, tolerance=1e-3,
                         maxIterations=20000, zero_initial_guess=False):
