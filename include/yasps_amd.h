@@ -137,6 +137,14 @@ typedef struct ysp_scatter_args {
   double* gradient;
   double* hessian_blocks;
   double* diagonal_blocks;
+  /* Staged (atomic-free) mode, used when block_staging != NULL: the element writes its
+   * k-th oriented 3x3 block to block_staging[9 * (coordinates_outer[e] + k)] and its local
+   * gradient to gradient_staging[e * gradient_local_size ...] with plain stores;
+   * ysp_gather_blocks3 / ysp_gather_gradient3 then sum them in a fixed order. gradient,
+   * hessian_blocks and diagonal_blocks are not touched in this mode. */
+  double* block_staging;
+  double* gradient_staging;
+  uint32_t gradient_local_size;
 } ysp_scatter_args;
 
 /* Rest-shape data of the Stable Neo-Hookean tet (examples/one_bunny/helpers.py:42-54):
@@ -227,6 +235,39 @@ int ysp_pcg_solve(uint32_t max_iterations, double threshold, int32_t threshold_m
                   const double* rhs, uint32_t n,
                   double* solution, const double* initial_guess, int32_t zero_initial_guess,
                   double* work, double* cg_scalars, ysp_pcg_stats* stats, void* stream);
+
+/* --------------------------------------------- atomic-free (staged) assembly */
+/* No reference counterpart: the reference accumulates element blocks with FP64 atomics
+ * (hessianKernelNoProject.pyx:231-258). These entry points sum staged element output in
+ * a fixed order instead (see ysp_scatter_args.block_staging). */
+
+/* Stable inverse of a key map: items lists values[i] (or i when values is NULL) grouped
+ * by keys[i] in ascending key order, offsets[num_keys + 1] delimits each key's run.
+ * Keys >= num_keys (e.g. ~0u for "no contribution") are dropped. */
+int ysp_inverse_map(const uint32_t* keys, const uint32_t* values, uint32_t n, uint32_t num_keys,
+                    uint32_t* offsets, uint32_t* items, void* stream);
+
+/* Gradient contribution keys/values of one term for 3-vector segments: for slot
+ * (e, i) with index >= 2, key = (index - 2) / 3 and value = staging_base + e * local size +
+ * the slot's local offset; other slots get ~0u. keys/values hold N * K entries. */
+int ysp_gradient_contributions(const uint32_t* segment_indices, const uint16_t* segment_sizes,
+                               uint32_t num_instances, uint32_t K, uint64_t staging_base,
+                               uint32_t gradient_local_size, uint32_t* keys, uint32_t* values,
+                               void* stream);
+
+/* values[u] = sum of the staged 3x3 blocks of unique block u (offsets/items from
+ * ysp_inverse_map over the coordinates' unique ordinals); (v, v) blocks also go to
+ * diagonal_blocks (added when accumulate_diagonal is set). */
+int ysp_gather_blocks3(const double* block_staging, const uint32_t* offsets,
+                       const uint32_t* items, uint32_t num_blocks, const uint32_t* positions,
+                       double* values, double* diagonal_blocks,
+                       const uint32_t* diagonal_blocks_start,
+                       const uint32_t* gradient_segments_start, uint32_t num_attributes,
+                       int32_t accumulate_diagonal, void* stream);
+
+/* gradient[3v..3v+2] = sum of the staged gradient segments of vertex v. */
+int ysp_gather_gradient3(const double* gradient_staging, const uint32_t* offsets,
+                         const uint32_t* items, uint32_t n, double* gradient, void* stream);
 
 /* ---------------------------------------------- full 3x3 block-CSR (solve loop) */
 /* The Krylov loop runs on a mirrored full block-CSR built from the upper storage of

@@ -209,3 +209,29 @@ def test_generic_pcg_path_matches_fast_path():
     x_gen = dev.to_host(slv.solution)
     assert abs(slv.statistics["iterations"] - it_fast) <= 1
     assert np.linalg.norm(x_gen - x_fast) / np.linalg.norm(x_fast) < 1e-9
+
+
+def test_staged_and_atomic_assembly_agree_and_staged_is_deterministic():
+    from yasps_amd import device as dev
+    cs = cube_scene(5, seed=7, noise=0.1)
+    out = {}
+    for mode in (True, False):
+        s0, h = build_yasps_scene(cs)
+        hs = s0.minimizer._active()
+        hs.use_staging = mode
+        hs.compute()
+        assert hs.staged == mode
+        out[mode] = (dev.to_host(hs.gradient.value).copy(), dev.to_host(hs.blocks_flattened).copy(),
+                     dev.to_host(hs.diagonal_blocks).copy())
+        if mode:
+            hs.compute()
+            again = (dev.to_host(hs.gradient.value), dev.to_host(hs.blocks_flattened),
+                     dev.to_host(hs.diagonal_blocks))
+            for a, b in zip(out[mode], again):
+                np.testing.assert_array_equal(a, b)   # bitwise reproducible
+    for a, b in zip(out[True], out[False]):
+        assert _rel_err(a, b) < REL
+    st = newton.fast_structure(cs)
+    asm = newton.evaluate(cs, st)
+    assert _rel_err(out[True][1], asm.values) < REL
+    assert _rel_err(out[True][0], asm.gradient) < REL

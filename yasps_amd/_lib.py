@@ -27,7 +27,9 @@ class ScatterArgs(C.Structure):
                 ("coordinates_outer", C.c_void_p), ("max_num_indices", C.c_uint32),
                 ("num_attributes", C.c_uint32), ("diagonal_blocks_start", C.c_void_p),
                 ("gradient_segments_start", C.c_void_p), ("gradient", C.c_void_p),
-                ("hessian_blocks", C.c_void_p), ("diagonal_blocks", C.c_void_p)]
+                ("hessian_blocks", C.c_void_p), ("diagonal_blocks", C.c_void_p),
+                ("block_staging", C.c_void_p), ("gradient_staging", C.c_void_p),
+                ("gradient_local_size", C.c_uint32)]
 
 
 class BlockMatrix(C.Structure):
@@ -75,6 +77,10 @@ SIGNATURES = {
     "ysp_pcg_solve": (C.c_int, [U32, D, I32, C.POINTER(BlockMatrix), C.POINTER(BlockMatrix), P,
                                 C.POINTER(U32), C.POINTER(U32), C.POINTER(U32), C.POINTER(U32),
                                 I32, P, U32, P, P, I32, P, P, C.POINTER(PcgStats), P]),
+    "ysp_inverse_map": (C.c_int, [P, P, U32, U32, P, P, P]),
+    "ysp_gradient_contributions": (C.c_int, [P, P, U32, U32, C.c_uint64, U32, P, P, P]),
+    "ysp_gather_blocks3": (C.c_int, [P, P, P, U32, P, P, P, P, P, U32, I32, P]),
+    "ysp_gather_gradient3": (C.c_int, [P, P, P, U32, P, P]),
     "ysp_bsr3_build": (C.c_int, [P, U32, P, U32, U32, P, P, P, U32, C.POINTER(U32), P]),
     "ysp_bsr3_gather_values": (C.c_int, [P, P, P, U32, P, P, P]),
     "ysp_cast_f32": (C.c_int, [P, P, C.c_uint64, P]),

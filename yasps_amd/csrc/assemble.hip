@@ -1,0 +1,198 @@
+// Atomic-free assembly: element kernels write their oriented 3x3 blocks (one per
+// coordinate) and local gradients to staging buffers; these kernels sum every unique
+// block and every gradient segment over its contributions in a fixed order.
+//
+// The contribution lists are the inverse of the dedupe map: a stable sort of
+// (unique ordinal, coordinate) pairs groups the coordinates of each unique block in
+// ascending coordinate order, so the sum order - and the assembled bits - do not depend
+// on thread scheduling.
+#include "common.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+namespace ysp {
+namespace {
+
+constexpr unsigned kB = 256;
+
+__global__ void iota_kernel(uint32_t* out, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = i;
+}
+
+__global__ void count_keys_kernel(const uint32_t* __restrict__ keys, uint32_t n, uint32_t num_keys,
+                                  uint32_t* __restrict__ counts) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && keys[i] < num_keys) atomicAdd(counts + keys[i], 1u);
+}
+
+// One slot per (element, slot): key = vertex of the gradient segment (or ~0u when the slot
+// does not reach the gradient), value = its offset in the gradient staging buffer.
+__global__ void gradient_contrib_kernel(const uint32_t* __restrict__ idx,
+                                        const uint16_t* __restrict__ sizes, uint32_t n, uint32_t K,
+                                        uint64_t staging_base, uint32_t grad_local,
+                                        uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                        int* __restrict__ error_flag) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  uint32_t col = 0;
+  for (uint32_t i = 0; i < K; ++i) {
+    const size_t o = static_cast<size_t>(e) * K + i;
+    const uint32_t ii = idx[o];
+    const uint32_t sz = sizes[o];
+    if (ii >= 2 && sz != 3) atomicOr(error_flag, 1);
+    const uint64_t pos = staging_base + static_cast<uint64_t>(e) * grad_local + col;
+    if (ii >= 2 && (ii - 2) % 3 == 0 && pos < 0xffffffffull) {
+      keys[o] = (ii - 2) / 3;
+      vals[o] = static_cast<uint32_t>(pos);
+    } else {
+      keys[o] = 0xffffffffu;
+      vals[o] = 0xffffffffu;
+      if (ii >= 2) atomicOr(error_flag, 1);
+    }
+    col += sz;
+  }
+}
+
+__global__ void __launch_bounds__(kB)
+    gather_blocks3_kernel(const double* __restrict__ staging, const uint32_t* __restrict__ offsets,
+                          const uint32_t* __restrict__ items, uint32_t nu,
+                          const uint32_t* __restrict__ positions, double* __restrict__ values,
+                          double* __restrict__ diag_blocks, const uint32_t* __restrict__ diag_start,
+                          const uint32_t* __restrict__ grad_start, uint32_t num_attr,
+                          int accumulate_diag) {
+  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= nu) return;
+  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (uint32_t k = offsets[u]; k < offsets[u + 1]; ++k) {
+    const double* b = staging + 9 * static_cast<size_t>(items[k]);
+#pragma unroll
+    for (int j = 0; j < 9; ++j) acc[j] += b[j];
+  }
+#pragma unroll
+  for (int j = 0; j < 9; ++j) values[9 * static_cast<size_t>(u) + j] = acc[j];
+  const uint32_t r = positions[2 * u], c = positions[2 * u + 1];
+  if (r == c && diag_blocks) {
+    uint32_t a = 0;
+    while (a < num_attr && r >= grad_start[a + 1]) ++a;
+    if (a < num_attr) {
+      double* d = diag_blocks + diag_start[a] + ((r - grad_start[a]) / 3) * 9;
+#pragma unroll
+      for (int j = 0; j < 9; ++j) d[j] = accumulate_diag ? d[j] + acc[j] : acc[j];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kB)
+    gather_gradient3_kernel(const double* __restrict__ staging, const uint32_t* __restrict__ offsets,
+                            const uint32_t* __restrict__ items, uint32_t nv,
+                            double* __restrict__ gradient) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  double g0 = 0, g1 = 0, g2 = 0;
+  for (uint32_t k = offsets[v]; k < offsets[v + 1]; ++k) {
+    const double* s = staging + items[k];
+    g0 += s[0];
+    g1 += s[1];
+    g2 += s[2];
+  }
+  gradient[3 * static_cast<size_t>(v)] = g0;
+  gradient[3 * static_cast<size_t>(v) + 1] = g1;
+  gradient[3 * static_cast<size_t>(v) + 2] = g2;
+}
+
+}  // namespace
+}  // namespace ysp
+
+using namespace ysp;
+
+extern "C" int ysp_inverse_map(const uint32_t* keys, const uint32_t* values, uint32_t n,
+                               uint32_t num_keys, uint32_t* offsets, uint32_t* items,
+                               void* stream) {
+  hipStream_t s = as_stream(stream);
+  if (num_keys == 0) return YSP_OK;
+  YSP_CHECK(hipMemsetAsync(offsets, 0, sizeof(uint32_t) * (num_keys + 1), s));
+  if (n == 0) return YSP_OK;
+  uint32_t* k_sorted = static_cast<uint32_t*>(scratch(50, sizeof(uint32_t) * n));
+  uint32_t* v_in = static_cast<uint32_t*>(scratch(51, sizeof(uint32_t) * n));
+  uint32_t* v_out = static_cast<uint32_t*>(scratch(52, sizeof(uint32_t) * n));
+  uint32_t* counts = static_cast<uint32_t*>(scratch(53, sizeof(uint32_t) * (num_keys + 1)));
+  if (!k_sorted || !v_in || !v_out || !counts) return YSP_ERR_HIP;
+  if (values) {
+    YSP_CHECK(hipMemcpyAsync(v_in, values, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
+  } else {
+    hipLaunchKernelGGL(iota_kernel, dim3(grid_for(n, kB)), dim3(kB), 0, s, v_in, n);
+  }
+  // keys equal to ~0u (no contribution) must sort after every real key: all 32 bits
+  const int end_bit = 32;
+  size_t temp = 0;
+  YSP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, temp, keys, k_sorted, v_in, v_out, n, 0,
+                                               end_bit, s));
+  void* tmp = scratch(54, temp);
+  if (!tmp) return YSP_ERR_HIP;
+  YSP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, temp, keys, k_sorted, v_in, v_out, n, 0,
+                                               end_bit, s));
+  YSP_CHECK(hipMemcpyAsync(items, v_out, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
+  YSP_CHECK(hipMemsetAsync(counts, 0, sizeof(uint32_t) * (num_keys + 1), s));
+  hipLaunchKernelGGL(count_keys_kernel, dim3(grid_for(n, kB)), dim3(kB), 0, s, keys, n, num_keys,
+                     counts);
+  size_t t2 = 0;
+  YSP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, counts, offsets, num_keys + 1, s));
+  tmp = scratch(54, t2);
+  if (!tmp) return YSP_ERR_HIP;
+  YSP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, t2, counts, offsets, num_keys + 1, s));
+  YSP_CHECK_LAUNCH();
+  return YSP_OK;
+}
+
+extern "C" int ysp_gradient_contributions(const uint32_t* segment_indices,
+                                          const uint16_t* segment_sizes, uint32_t num_instances,
+                                          uint32_t K, uint64_t staging_base,
+                                          uint32_t gradient_local_size, uint32_t* keys,
+                                          uint32_t* values, void* stream) {
+  if (num_instances == 0) return YSP_OK;
+  hipStream_t s = as_stream(stream);
+  int* flag = static_cast<int*>(scratch(55, sizeof(int)));
+  if (!flag) return YSP_ERR_HIP;
+  YSP_CHECK(hipMemsetAsync(flag, 0, sizeof(int), s));
+  hipLaunchKernelGGL(gradient_contrib_kernel, dim3(grid_for(num_instances, kB)), dim3(kB), 0, s,
+                     segment_indices, segment_sizes, num_instances, K, staging_base,
+                     gradient_local_size, keys, values, flag);
+  YSP_CHECK_LAUNCH();
+  int h = 0;
+  YSP_CHECK(hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+  YSP_CHECK(hipStreamSynchronize(s));
+  if (h) {
+    ysp_set_error(__FILE__, __LINE__,
+                  "ysp_gradient_contributions: staged gradients need 3-vector segments");
+    return YSP_ERR_UNSUPPORTED;
+  }
+  return YSP_OK;
+}
+
+extern "C" int ysp_gather_blocks3(const double* block_staging, const uint32_t* offsets,
+                                  const uint32_t* items, uint32_t num_blocks,
+                                  const uint32_t* positions, double* values,
+                                  double* diagonal_blocks, const uint32_t* diagonal_blocks_start,
+                                  const uint32_t* gradient_segments_start,
+                                  uint32_t num_attributes, int32_t accumulate_diagonal,
+                                  void* stream) {
+  if (num_blocks == 0) return YSP_OK;
+  hipLaunchKernelGGL(gather_blocks3_kernel, dim3(grid_for(num_blocks, kB)), dim3(kB), 0,
+                     as_stream(stream), block_staging, offsets, items, num_blocks, positions,
+                     values, diagonal_blocks, diagonal_blocks_start, gradient_segments_start,
+                     num_attributes, accumulate_diagonal);
+  YSP_CHECK_LAUNCH();
+  return YSP_OK;
+}
+
+extern "C" int ysp_gather_gradient3(const double* gradient_staging, const uint32_t* offsets,
+                                    const uint32_t* items, uint32_t n, double* gradient,
+                                    void* stream) {
+  if (n % 3) return YSP_ERR_ARG;
+  if (n == 0) return YSP_OK;
+  hipLaunchKernelGGL(gather_gradient3_kernel, dim3(grid_for(n / 3, kB)), dim3(kB), 0,
+                     as_stream(stream), gradient_staging, offsets, items, n / 3, gradient);
+  YSP_CHECK_LAUNCH();
+  return YSP_OK;
+}

@@ -18,6 +18,8 @@
 #include "common.hpp"
 #include "dense.hpp"
 
+#include <type_traits>
+
 namespace ysp {
 namespace {
 
@@ -62,36 +64,47 @@ struct ScatterArgs {
   double* gradient;
   double* blocks;
   double* diag_blocks;
+  double* block_staging;     // staged mode: 9 doubles per coordinate of this term
+  double* grad_staging;      // staged mode: grad_local doubles per element
+  uint32_t grad_local;
 };
 
 ScatterArgs make_scatter(const ysp_scatter_args* a) {
   return ScatterArgs{a->segment_indices, a->segment_sizes, a->local_permutations, a->lookups,
                      a->coordinates_outer, a->max_num_indices, a->num_attributes,
                      a->diagonal_blocks_start, a->gradient_segments_start, a->gradient,
-                     a->hessian_blocks, a->diagonal_blocks};
+                     a->hessian_blocks, a->diagonal_blocks, a->block_staging,
+                     a->gradient_staging, a->gradient_local_size};
 }
 
 constexpr int kMaxSlots = 16;
 
 // Slot bookkeeping of one element (hessianKernelNoProject.pyx:93-163): groups of slots
 // with the same |perm| in first-occurrence order, and the local 3-vector of every slot.
+// KT is the compile-time slot capacity; S.K the slot count (equal when specialised).
+template <int KT>
 struct SlotInfo {
-  uint32_t idx[kMaxSlots];
-  int8_t vtx[kMaxSlots];      // local vertex of the slot, -1 for zero-size padding
-  int8_t group_first[kMaxSlots];
-  uint32_t group_mask[kMaxSlots];
+  uint32_t idx[KT];
+  int8_t vtx[KT];      // local vertex of the slot, -1 for zero-size padding
+  int8_t group_first[KT];
+  uint32_t group_mask[KT];
   int ng;
   bool ok;
 };
 
-__device__ __forceinline__ void load_slots(const ScatterArgs& S, uint32_t e, SlotInfo& si) {
+template <int KT>
+__device__ __forceinline__ void load_slots(const ScatterArgs& S, uint32_t e, SlotInfo<KT>& si) {
   const uint32_t K = S.K;
   si.ng = 0;
-  si.ok = K <= kMaxSlots;
+  si.ok = K <= static_cast<uint32_t>(KT);
   if (!si.ok) return;
-  int placement[kMaxSlots];
+  int placement[KT];
   uint32_t col = 0;
-  for (uint32_t i = 0; i < K; ++i) {
+#pragma unroll
+  for (int i = 0; i < KT; ++i) {
+    si.idx[i] = 0;
+    si.vtx[i] = -1;
+    if (static_cast<uint32_t>(i) >= K) continue;
     const size_t o = static_cast<size_t>(e) * K + i;
     const uint32_t ii = S.seg_idx[o];
     const uint32_t sz = S.seg_size[o];
@@ -99,23 +112,29 @@ __device__ __forceinline__ void load_slots(const ScatterArgs& S, uint32_t e, Slo
     si.idx[i] = ii;
     if (sz == 3) {
       si.vtx[i] = static_cast<int8_t>(col / 3);
-    } else {
-      si.vtx[i] = -1;
-      if (sz != 0) si.ok = false;
+    } else if (sz != 0) {
+      si.ok = false;
     }
     col += sz;
     if (pm == 0) continue;
     const int p = (pm < 0 ? -pm : pm) - 1;
     int g = -1;
-    for (int k = 0; k < si.ng; ++k)
-      if (placement[k] == p) g = k;
+#pragma unroll
+    for (int k = 0; k < KT; ++k)
+      if (k < si.ng && placement[k] == p) g = k;
     if (g < 0) {
       g = si.ng++;
-      placement[g] = p;
-      si.group_first[g] = static_cast<int8_t>(i);
-      si.group_mask[g] = 0;
+#pragma unroll
+      for (int k = 0; k < KT; ++k)
+        if (k == g) {
+          placement[k] = p;
+          si.group_first[k] = static_cast<int8_t>(i);
+          si.group_mask[k] = 0;
+        }
     }
-    si.group_mask[g] |= 1u << i;
+#pragma unroll
+    for (int k = 0; k < KT; ++k)
+      if (k == g) si.group_mask[k] |= 1u << i;
   }
 }
 
@@ -126,11 +145,26 @@ __device__ __forceinline__ uint32_t diag_offset(const ScatterArgs& S, uint32_t s
   return S.diag_start[attr] + inst * size * size;
 }
 
-// Add gradient segments (hessianKernelNoProject.pyx:63-88).
-template <typename GradFn>
-__device__ __forceinline__ void scatter_gradient(const ScatterArgs& S, const SlotInfo& si,
-                                                 GradFn grad) {
-  for (uint32_t i = 0; i < S.K; ++i) {
+// Gradient segments (hessianKernelNoProject.pyx:63-88): FP64 atomics into the gradient,
+// or, staged, the element's whole local gradient written in place.
+template <int KT, typename GradFn>
+__device__ __forceinline__ void scatter_gradient(const ScatterArgs& S, uint32_t e,
+                                                 const SlotInfo<KT>& si, GradFn grad) {
+  if (S.grad_staging) {
+    double* out = S.grad_staging + static_cast<size_t>(e) * S.grad_local;
+#pragma unroll
+    for (int i = 0; i < KT; ++i) {
+      if (si.vtx[i] < 0) continue;
+      double g3[3];
+      grad(si.vtx[i], g3);
+      out[3 * si.vtx[i]] = g3[0];
+      out[3 * si.vtx[i] + 1] = g3[1];
+      out[3 * si.vtx[i] + 2] = g3[2];
+    }
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < KT; ++i) {
     if (si.idx[i] < 2 || si.vtx[i] < 0) continue;
     const uint32_t p = si.idx[i] - 2;
     double g3[3];
@@ -141,44 +175,61 @@ __device__ __forceinline__ void scatter_gradient(const ScatterArgs& S, const Slo
   }
 }
 
-// Add Hessian blocks (hessianKernelNoProject.pyx:165-264). blockFn(va, vb, out[3][3]) adds
-// the local 3x3 block between local vertices va and vb into out.
-template <typename BlockFn>
+// Hessian blocks (hessianKernelNoProject.pyx:165-264). blockFn(va, vb, out[3][3]) adds the
+// local 3x3 block between local vertices va and vb into out. The k-th valid group pair
+// of the element is coordinate coordStart + k: FP64 atomics at lookups[coordinate], or,
+// staged, a plain store of the oriented block at staging[coordinate].
+template <int KT, typename BlockFn>
 __device__ __forceinline__ void scatter_hessian(const ScatterArgs& S, uint32_t e,
-                                                const SlotInfo& si, BlockFn blockfn) {
+                                                const SlotInfo<KT>& si, BlockFn blockfn) {
   uint32_t count = 0;
   const uint32_t cstart = S.coord_outer[e];
-  for (int gi = 0; gi < si.ng; ++gi) {
+#pragma unroll
+  for (int gi = 0; gi < KT; ++gi) {
+    if (gi >= si.ng) continue;
     const uint32_t ii = si.idx[si.group_first[gi]];
     if (ii < 2) continue;
-    for (int gj = gi; gj < si.ng; ++gj) {
+#pragma unroll
+    for (int gj = gi; gj < KT; ++gj) {
+      if (gj >= si.ng) continue;
       const uint32_t jj = si.idx[si.group_first[gj]];
       if (jj < 2) continue;
       double blk[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-      for (uint32_t a = 0; a < S.K; ++a) {
+#pragma unroll
+      for (int a = 0; a < KT; ++a) {
         if (!((si.group_mask[gi] >> a) & 1u) || si.vtx[a] < 0) continue;
-        for (uint32_t b = 0; b < S.K; ++b) {
+#pragma unroll
+        for (int b = 0; b < KT; ++b) {
           if (!((si.group_mask[gj] >> b) & 1u) || si.vtx[b] < 0) continue;
           blockfn(si.vtx[a], si.vtx[b], blk);
         }
       }
-      const uint32_t place = S.lookups[cstart + count];
-      double* dst = S.blocks + place;
-      if (ii < jj) {
+      const uint32_t coord = cstart + count;
+      if (S.block_staging) {
+        double* dst = S.block_staging + 9 * static_cast<size_t>(coord);
+        const bool tr = !(ii < jj);
 #pragma unroll
         for (int k = 0; k < 3; ++k)
 #pragma unroll
-          for (int l = 0; l < 3; ++l) atomic_add_f64(dst + 3 * k + l, blk[k][l]);
+          for (int l = 0; l < 3; ++l) dst[3 * k + l] = tr ? blk[l][k] : blk[k][l];
       } else {
-        double* dd = nullptr;
-        if (gi == gj && S.diag_blocks) dd = S.diag_blocks + diag_offset(S, ii - 2, 3);
+        double* dst = S.blocks + S.lookups[coord];
+        if (ii < jj) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
+          for (int k = 0; k < 3; ++k)
 #pragma unroll
-          for (int l = 0; l < 3; ++l) {
-            atomic_add_f64(dst + 3 * k + l, blk[l][k]);
-            if (dd) atomic_add_f64(dd + 3 * k + l, blk[l][k]);
-          }
+            for (int l = 0; l < 3; ++l) atomic_add_f64(dst + 3 * k + l, blk[k][l]);
+        } else {
+          double* dd = nullptr;
+          if (gi == gj && S.diag_blocks) dd = S.diag_blocks + diag_offset(S, ii - 2, 3);
+#pragma unroll
+          for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int l = 0; l < 3; ++l) {
+              atomic_add_f64(dst + 3 * k + l, blk[l][k]);
+              if (dd) atomic_add_f64(dd + 3 * k + l, blk[l][k]);
+            }
+        }
       }
       ++count;
     }
@@ -353,6 +404,7 @@ __device__ __forceinline__ void snh_load(const UnionView& P, const uint32_t* tet
 }
 
 // Pass 1: gradient, PSD test, scatter of PSD (or unprojected) Hessians, queue the rest.
+template <int KT>
 __global__ void __launch_bounds__(kBlock)
     snh_eval_kernel(UnionView P, const uint32_t* __restrict__ tet2v, uint32_t n, SnhParams prm,
                     ScatterArgs SA, uint32_t* __restrict__ queue, uint32_t* __restrict__ qcount,
@@ -362,13 +414,13 @@ __global__ void __launch_bounds__(kBlock)
   Snh S;
   double mu;
   snh_load(P, tet2v, prm, e, S, mu);
-  SlotInfo si;
-  load_slots(SA, e, si);
+  SlotInfo<KT> si;
+  load_slots<KT>(SA, e, si);
   if (!si.ok) {
     atomicOr(error_flag, 1);
     return;
   }
-  scatter_gradient(SA, si, [&](int a, double* g3) {
+  scatter_gradient<KT>(SA, e, si, [&](int a, double* g3) {
 #pragma unroll
     for (int i = 0; i < 3; ++i) g3[i] = S.s * (S.a1 * S.u[a][i] + S.a3 * S.w[a][i]);
   });
@@ -392,7 +444,7 @@ __global__ void __launch_bounds__(kBlock)
     psd = psd_test_packed<12>(L);
   }
   if (psd) {
-    scatter_hessian(SA, e, si, [&](int a, int b, double out[3][3]) {
+    scatter_hessian<KT>(SA, e, si, [&](int a, int b, double out[3][3]) {
       double blk[3][3];
       snh_block(S, S.g[a], S.g[b], S.u[a], S.u[b], S.w[a], S.w[b], blk);
 #pragma unroll
@@ -414,6 +466,7 @@ __constant__ double kU[4][3] = {
     {0.0, 0.0, -0.86602540378443864676}};
 
 // Pass 2: project the queued (non-PSD) SNH Hessians through the 9x9 reduction.
+template <int KT>
 __global__ void __launch_bounds__(64)
     snh_project_kernel(UnionView P, const uint32_t* __restrict__ tet2v, SnhParams prm,
                        ScatterArgs SA, const uint32_t* __restrict__ queue,
@@ -472,9 +525,9 @@ __global__ void __launch_bounds__(64)
       Mp[i][j] = acc;
       Mp[j][i] = acc;
     }
-  SlotInfo si;
-  load_slots(SA, e, si);
-  scatter_hessian(SA, e, si, [&](int a, int b, double out[3][3]) {
+  SlotInfo<KT> si;
+  load_slots<KT>(SA, e, si);
+  scatter_hessian<KT>(SA, e, si, [&](int a, int b, double out[3][3]) {
 #pragma unroll
     for (int k = 0; k < 3; ++k)
 #pragma unroll
@@ -517,6 +570,7 @@ __device__ __forceinline__ void inertia_diff(const InertiaArgs& I, uint32_t e, d
   }
 }
 
+template <int KT>
 __global__ void __launch_bounds__(kBlock)
     inertia_eval_kernel(InertiaArgs I, uint32_t n, uint32_t method, ScatterArgs SA,
                         int* __restrict__ error_flag) {
@@ -528,17 +582,17 @@ __global__ void __launch_bounds__(kBlock)
   // Hessian m I: PSD for m >= -tol; otherwise the clamp rule on the eigenvalue m.
   double mh = m;
   if (method != 0 && m < -1.0e-10 * fmax(1.0, fabs(m))) mh = (method == 1) ? -m : 0.0;
-  SlotInfo si;
-  load_slots(SA, e, si);
+  SlotInfo<KT> si;
+  load_slots<KT>(SA, e, si);
   if (!si.ok) {
     atomicOr(error_flag, 1);
     return;
   }
-  scatter_gradient(SA, si, [&](int, double* g3) {
+  scatter_gradient<KT>(SA, e, si, [&](int, double* g3) {
 #pragma unroll
     for (int i = 0; i < 3; ++i) g3[i] = m * d[i];
   });
-  scatter_hessian(SA, e, si, [&](int, int, double out[3][3]) {
+  scatter_hessian<KT>(SA, e, si, [&](int, int, double out[3][3]) {
 #pragma unroll
     for (int i = 0; i < 3; ++i) out[i][i] += mh;
   });
@@ -604,14 +658,17 @@ extern "C" int ysp_eval_snh_tet(const ysp_union_f64* positions, const uint32_t* 
   SnhParams prm{dm_inv, vol, mu, mu_stride, lam, lam_stride, dt, projection_method};
   const UnionView P = make_view(positions);
   const ScatterArgs SA = make_scatter(out);
-  hipLaunchKernelGGL(snh_eval_kernel, dim3(grid_for(num_tets, kBlock)), dim3(kBlock), 0, s, P,
-                     tet2v, num_tets, prm, SA, queue, qcount, flag);
+  auto run = [&](auto kt) {
+    constexpr int KT = decltype(kt)::value;
+    hipLaunchKernelGGL(snh_eval_kernel<KT>, dim3(grid_for(num_tets, kBlock)), dim3(kBlock), 0, s, P,
+                       tet2v, num_tets, prm, SA, queue, qcount, flag);
+    if (projection_method != 0)
+      hipLaunchKernelGGL(snh_project_kernel<KT>, dim3(grid_for(num_tets, 64)), dim3(64), 0, s, P,
+                         tet2v, prm, SA, queue, qcount);
+  };
+  if (out->max_num_indices == 8) run(std::integral_constant<int, 8>());
+  else run(std::integral_constant<int, kMaxSlots>());
   YSP_CHECK_LAUNCH();
-  if (projection_method != 0) {
-    hipLaunchKernelGGL(snh_project_kernel, dim3(grid_for(num_tets, 64)), dim3(64), 0, s, P, tet2v,
-                       prm, SA, queue, qcount);
-    YSP_CHECK_LAUNCH();
-  }
   return check_flag(flag, s, "ysp_eval_snh_tet: slot layout not supported (K>16 or size!=3)");
 }
 
@@ -646,8 +703,12 @@ extern "C" int ysp_eval_inertia(const double* position, const double* last_posit
   if (!flag) return YSP_ERR_HIP;
   YSP_CHECK(hipMemsetAsync(flag, 0, sizeof(int), s));
   InertiaArgs I{position, last_position, velocity, mass, dt};
-  hipLaunchKernelGGL(inertia_eval_kernel, dim3(grid_for(num_vertices, kBlock)), dim3(kBlock), 0, s,
-                     I, num_vertices, projection_method, make_scatter(out), flag);
+  if (out->max_num_indices == 1)
+    hipLaunchKernelGGL(inertia_eval_kernel<1>, dim3(grid_for(num_vertices, kBlock)), dim3(kBlock), 0,
+                       s, I, num_vertices, projection_method, make_scatter(out), flag);
+  else
+    hipLaunchKernelGGL(inertia_eval_kernel<kMaxSlots>, dim3(grid_for(num_vertices, kBlock)),
+                       dim3(kBlock), 0, s, I, num_vertices, projection_method, make_scatter(out), flag);
   YSP_CHECK_LAUNCH();
   return check_flag(flag, s, "ysp_eval_inertia: slot layout not supported");
 }

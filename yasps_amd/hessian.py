@@ -59,6 +59,7 @@ class TermIndex:
     dims: object = None
     num_coords: int = 0
     lookups: object = None
+    g_local: int = 0
 
 
 @dataclass
@@ -71,6 +72,11 @@ class Part:
     dims: List[int] = field(default_factory=list)
     total: int = 0
     num_blocks: int = 0
+    ordinals: object = None
+    staging: object = None          # 9 doubles per coordinate (staged assembly)
+    contrib_offsets: object = None
+    contrib_items: object = None
+    coord_bases: List[int] = field(default_factory=list)
     _ct: tuple = ()
 
     def ctype(self) -> Optional[_lib.BlockMatrix]:
@@ -91,7 +97,7 @@ def build_term_index(program, n: int, wrt_start_dev, stream) -> TermIndex:
     conns, unions = [], []
     code = encode(program, conns, unions)
     K = code[1]
-    ti = TermIndex(K=K, n=n)
+    ti = TermIndex(K=K, n=n, g_local=code[2])
     if n == 0:
         return ti
     code_dev = dev.to_device(np.asarray(code, np.int32), "int32")
@@ -142,6 +148,7 @@ def build_part(terms: List[TermIndex], stream) -> Part:
     _lib.check(lib.ysp_get_unique_coords(dev.ptr(coords), dev.ptr(dims), total, dev.ptr(tmp),
                                          dev.ptr(lookup), C.byref(nu), stream), "ysp_get_unique_coords")
     nu = nu.value
+    part.ordinals = lookup.clone()          # coordinate -> unique block ordinal
     positions = dev.empty(2 * nu, "uint32")
     max_cat = 64
     udims = dev.zeros(2 * max_cat, "uint16")
@@ -206,6 +213,8 @@ class hessian:
         self.dynamic = Part()
         self.full: Optional[FullBsr] = None
         self.want_f32 = False
+        self.use_staging = True
+        self._grad_offsets = None
 
     # --------------------------------------------------------------- layout
     def _ensure_setup(self):
@@ -223,6 +232,9 @@ class hessian:
                                               self._wrt_start_dev, s) for e, _ in self.terms]
         self.static = build_part(self.static_index, s)
         self._setup = True
+        self._prepare_staging(self.static, self.static_index)
+        if not self.dynamic_terms:
+            self._prepare_gradient_staging()
 
     def _reindex_dynamic(self):
         if not self.dynamic_terms:
@@ -231,6 +243,69 @@ class hessian:
         self.dynamic_index = [build_term_index(e.program(self._wrt_pos), e.num_instances,
                                                self._wrt_start_dev, s) for e, _ in self.dynamic_terms]
         self.dynamic = build_part(self.dynamic_index, s)
+        self._prepare_staging(self.dynamic, self.dynamic_index)
+        self._prepare_gradient_staging()
+
+    # --------------------------------------------------------------- staged assembly
+    @property
+    def staged(self) -> bool:
+        """Atomic-free assembly (3x3 blocks and 3-vector gradient segments only)."""
+        return self.use_staging and self.all_3x3 and self._grad_offsets is not None
+
+    def _prepare_staging(self, part: Part, index: List[TermIndex]):
+        part.staging = None
+        if not (self.use_staging and self.all_3x3) or part.num_blocks == 0:
+            return
+        lib = _lib.load()
+        total = sum(ti.num_coords for ti in index)
+        bases, acc = [], 0
+        for ti in index:
+            bases.append(acc)
+            acc += ti.num_coords
+        part.coord_bases = bases
+        part.staging = dev.empty(9 * total, "float64")
+        part.contrib_offsets = dev.empty(part.num_blocks + 1, "uint32")
+        part.contrib_items = dev.empty(max(1, total), "uint32")
+        _lib.check(lib.ysp_inverse_map(dev.ptr(part.ordinals), None, total, part.num_blocks,
+                                       dev.ptr(part.contrib_offsets), dev.ptr(part.contrib_items),
+                                       dev.stream()), "ysp_inverse_map")
+
+    def _prepare_gradient_staging(self):
+        self._grad_offsets = None
+        if not (self.use_staging and self.all_3x3):
+            return
+        lib = _lib.load()
+        t = dev.torch()
+        s = dev.stream()
+        index = self.static_index + self.dynamic_index
+        keys, vals, bases, acc = [], [], [], 0
+        try:
+            for ti in index:
+                bases.append(acc)
+                if ti.n == 0:
+                    continue
+                k = dev.empty(ti.n * ti.K, "uint32")
+                v = dev.empty(ti.n * ti.K, "uint32")
+                _lib.check(lib.ysp_gradient_contributions(dev.ptr(ti.indices), dev.ptr(ti.sizes),
+                                                          ti.n, ti.K, acc, ti.g_local, dev.ptr(k),
+                                                          dev.ptr(v), s), "ysp_gradient_contributions")
+                keys.append(k)
+                vals.append(v)
+                acc += ti.n * ti.g_local
+        except _lib.YspError:
+            return      # segments that are not 3-vectors: keep the atomic path
+        self._grad_bases = bases
+        self._grad_staging = dev.empty(max(1, acc), "float64")
+        nslots = sum(k.numel() for k in keys)
+        self._grad_offsets = dev.empty(self.rows // 3 + 1, "uint32")
+        self._grad_items = dev.empty(max(1, nslots), "uint32")
+        if nslots:
+            kk, vv = t.cat(keys), t.cat(vals)
+            _lib.check(lib.ysp_inverse_map(dev.ptr(kk), dev.ptr(vv), nslots, self.rows // 3,
+                                           dev.ptr(self._grad_offsets), dev.ptr(self._grad_items), s),
+                       "ysp_inverse_map")
+        else:
+            self._grad_offsets.zero_()
 
     # --------------------------------------------------------------- storage views
     @property
@@ -286,6 +361,14 @@ class hessian:
         if local_gradient is not None and local_gradient is not self.gradient:
             self.gradient = local_gradient
         self._reindex_dynamic()
+        if self.staged:
+            self._compute_staged()
+        else:
+            self._compute_atomic()
+        self._update_full()
+        return self
+
+    def _compute_atomic(self):
         self.gradient.value.zero_()
         self.diagonal_blocks.zero_()
         if self.static.values is not None:
@@ -298,8 +381,34 @@ class hessian:
         for (expr, method), ti in zip(self.dynamic_terms, self.dynamic_index):
             if ti.n:
                 expr.evaluate(self._scatter_args(ti, self.dynamic), method)
-        self._update_full()
-        return self
+
+    def _compute_staged(self):
+        lib = _lib.load()
+        s = dev.stream()
+        self.diagonal_blocks.zero_()
+        gptr = dev.ptr(self._grad_staging)
+        parts = ((self.terms, self.static_index, self.static),
+                 (self.dynamic_terms, self.dynamic_index, self.dynamic))
+        k = 0
+        for terms, index, part in parts:
+            for (expr, method), ti, base in zip(terms, index, part.coord_bases or [0] * len(index)):
+                if ti.n:
+                    sa = self._scatter_args(ti, part)
+                    sa.block_staging = dev.ptr(part.staging) + 8 * 9 * base
+                    sa.gradient_staging = gptr + 8 * self._grad_bases[k]
+                    sa.gradient_local_size = ti.g_local
+                    expr.evaluate(sa, method)
+                k += 1
+        for i, (_, index, part) in enumerate(parts):
+            if part.num_blocks:
+                _lib.check(lib.ysp_gather_blocks3(
+                    dev.ptr(part.staging), dev.ptr(part.contrib_offsets), dev.ptr(part.contrib_items),
+                    part.num_blocks, dev.ptr(part.positions), dev.ptr(part.values),
+                    dev.ptr(self.diagonal_blocks), dev.ptr(self._diag_start_dev),
+                    dev.ptr(self._wrt_start_dev), len(self.wrt), i, s), "ysp_gather_blocks3")
+        _lib.check(lib.ysp_gather_gradient3(gptr, dev.ptr(self._grad_offsets), dev.ptr(self._grad_items),
+                                            self.rows, dev.ptr(self.gradient.value), s),
+                   "ysp_gather_gradient3")
 
     # --------------------------------------------------------------- full block-CSR
     @property
