@@ -236,6 +236,57 @@ int ysp_pcg_solve(uint32_t max_iterations, double threshold, int32_t threshold_m
                   double* solution, const double* initial_guess, int32_t zero_initial_guess,
                   double* work, double* cg_scalars, ysp_pcg_stats* stats, void* stream);
 
+/* --------------------------------------------------------------- MAS preconditioner */
+/* Multilevel additive Schwarz (solver/mas/*): the hierarchy (domains, identity
+ * aggregation maps) is built on the host once per static structure
+ * (solver/mas/hierarchy.py:150-271); these entry points do the per-solve numeric work
+ * and the apply on the device. All nodes are 3-vectors; domains are padded to 32
+ * scalars (MAS leaf size of the benchmark configuration). */
+
+/* values (level l, AoS 9 doubles per block) = sum of the level-(l-1) blocks listed in
+ * items[offsets[e] .. offsets[e+1]) (identity-aggregation Galerkin product); the previous
+ * level is either the solve loop's full block-CSR (component-major) or an AoS level. */
+int ysp_mas_coarsen(const double* prev_values, int32_t prev_is_full_bsr, uint32_t prev_nnz,
+                    const uint32_t* offsets, const uint32_t* items, uint32_t nnz,
+                    double* values, void* stream);
+
+/* Copy every block e with dest[e] != ~0u into the dense domain matrices (FP64, 32 x 32
+ * row-major per domain) at offset dest[e] (numeric_assembly.py:47-61). */
+int ysp_mas_domain_scatter(const double* values, int32_t is_full_bsr, uint32_t nnz,
+                           const uint32_t* dest, double* domains, void* stream);
+
+/* Explicit inverse of every domain matrix (identity in the padding), FP64 Gauss-Jordan,
+ * stored FP32 column-major (solver/mas/cuda/local_inverse.cu:6-308). *status (device int)
+ * gets bit 0 set when a pivot is below 1e-12 * max|diag|. */
+int ysp_mas_invert_domains(double* domains, const uint32_t* domain_sizes, uint32_t num_domains,
+                           float* inverses, int32_t* status, void* stream);
+
+#define YSP_MAS_MAX_COARSE_LEVELS 8
+
+typedef struct ysp_mas_args {
+  const uint32_t* restrict_offsets;     /* [num_coarse_nodes + 1] */
+  const uint32_t* restrict_items;       /* children of every coarse node: level-0 nodes
+                                           for level 1, coarse ids of level l-1 for l > 1 */
+  uint32_t num_coarse_nodes;            /* coarse levels concatenated */
+  uint32_t num_coarse_levels;           /* <= YSP_MAS_MAX_COARSE_LEVELS */
+  uint32_t level_offsets[YSP_MAS_MAX_COARSE_LEVELS + 1];  /* coarse ids of level l + 1:
+                                           [level_offsets[l], level_offsets[l + 1]) */
+  const uint32_t* prolong_maps;         /* [num_coarse_levels x n/3] coarse ids */
+  const float* inverses;                /* [num_domains x 32 x 32] */
+  const uint32_t* domain_nodes;         /* level-0 nodes or coarse ids */
+  const uint32_t* domain_node_offsets;  /* [num_domains + 1] */
+  uint32_t num_domains;
+  uint32_t level0_domains;              /* domains [0, level0_domains) are on level 0 */
+  const float* weights;                 /* level weights [1 + num_coarse_levels] */
+  double* coarse_residual;              /* work [3 x num_coarse_nodes] */
+  float* fine_correction;               /* work [n] */
+  float* coarse_correction;             /* work [3 x num_coarse_nodes] */
+} ysp_mas_args;
+
+/* z = sum_l w_l P_l B_l^-1 R_l r (solver/mas/preconditioner.py:128-154). */
+int ysp_mas_apply(const ysp_mas_args* args, const double* r, uint32_t n, double* z,
+                  void* stream);
+
 /* --------------------------------------------- atomic-free (staged) assembly */
 /* No reference counterpart: the reference accumulates element blocks with FP64 atomics
  * (hessianKernelNoProject.pyx:231-258). These entry points sum staged element output in
@@ -301,12 +352,23 @@ int ysp_bsr3_spmv(const uint32_t* row_ptr, const uint32_t* col, const void* valu
  * and the inverse diagonal blocks may be FP32 (FP64 vectors, dots and scalars).
  * work holds 5 * n doubles, cg_scalars YSP_PCG3_SCALARS doubles. Synchronous. */
 #define YSP_PCG3_SCALARS 12288
+
 int ysp_pcg3_solve(uint32_t max_iterations, double threshold, int32_t threshold_mode,
                    const uint32_t* row_ptr, const uint32_t* col, const void* values,
                    int32_t values_f32, uint32_t nnz, const void* diagonal_inverse,
                    int32_t inverse_f32, const double* rhs, uint32_t n, double* solution,
                    const double* initial_guess, int32_t zero_initial_guess, double* work,
                    double* cg_scalars, ysp_pcg_stats* stats, void* stream);
+
+/* The same PCG with the MAS preconditioner (solver/mas/pcg.py:23-102 recurrence on the
+ * device): stop on r^T M^-1 r <= tol * r0^T M^-1 r0, or on the relative residual. work
+ * holds 5 * n doubles; the MAS inverses must be current (ysp_mas_invert_domains). */
+int ysp_pcg3_mas_solve(uint32_t max_iterations, double threshold, int32_t threshold_mode,
+                       const uint32_t* row_ptr, const uint32_t* col, const void* values,
+                       int32_t values_f32, uint32_t nnz, const ysp_mas_args* mas,
+                       const double* rhs, uint32_t n, double* solution,
+                       const double* initial_guess, int32_t zero_initial_guess, double* work,
+                       double* cg_scalars, ysp_pcg_stats* stats, void* stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,203 @@
+"""CPU checks of the MAS hierarchy, the device tables built from it, and the MAS oracle:
+partition invariants, the Galerkin identity R A R^T, and an emulation of the device
+apply (table semantics of the restrict / coarsen / scatter / local-solve / prolong
+kernels, FP64) against the dense oracle apply."""
+import numpy as np
+import pytest
+
+from oracle import mas as omas, newton, pcg
+from yasps_amd import mas_hierarchy as mh
+from yasps_amd.mas import DOMAIN_SCALARS, NO_DEST, build_layout, level_weights, static_pairs
+from yasps_amd.scenes import cube_scene
+
+
+@pytest.fixture(scope="module")
+def cube():
+    sc = cube_scene(n=4, noise=0.05, seed=3)
+    st = newton.fast_structure(sc)
+    asm = newton.evaluate(sc, st, 2)
+    A = newton.bsr_matrix(st, asm.values)
+    return sc, A, asm.gradient
+
+
+def _hierarchy(A, **kw):
+    nb = A.shape[0] // 3
+    src = np.zeros(A.indices.size, dtype=np.uint32)
+    pairs = static_pairs(A.indptr, A.indices, src)
+    kw = {"max_domain_dofs": 32, "max_levels": 4, "target_nodes_per_partition": 8} | kw
+    return mh.build_hierarchy(np.full(nb, 3), pairs, **kw)
+
+
+def test_graph_from_pairs_is_symmetric_without_loops():
+    xadj, adj = mh.graph_from_pairs(4, np.array([[0, 1], [1, 0], [2, 2], [3, 1], [0, 1]]))
+    nbrs = [sorted(adj[xadj[i]:xadj[i + 1]].tolist()) for i in range(4)]
+    assert nbrs == [[1], [0, 3], [], [1]]
+
+
+def test_bfs_groups_are_connected_and_bounded():
+    # 6 x 6 grid graph
+    n = 36
+    pairs = [(i, i + 1) for i in range(n) if (i + 1) % 6] + [(i, i + 6) for i in range(n - 6)]
+    xadj, adj = mh.graph_from_pairs(n, np.array(pairs))
+    groups = mh.bfs_groups(xadj, adj, 5)
+    assert sorted(v for g in groups for v in g) == list(range(n))
+    for g in groups:
+        assert 1 <= len(g) <= 5
+        # connected: BFS inside the group reaches every member
+        seen, stack, members = {g[0]}, [g[0]], set(g)
+        while stack:
+            v = stack.pop()
+            for u in adj[xadj[v]:xadj[v + 1]]:
+                if u in members and u not in seen:
+                    seen.add(int(u))
+                    stack.append(int(u))
+        assert seen == members
+
+
+def test_pack_domain_groups_never_crosses_groups_or_capacity():
+    dims = np.array([3, 3, 3, 12, 3, 40, 3])
+    doms = mh.pack_domain_groups([[0, 1, 2, 3], [4, 5, 6]], dims, 12)
+    assert doms == [[0, 1, 2], [3], [4], [5], [6]]
+
+
+def test_hierarchy_partition_invariants(cube):
+    _, A, _ = cube
+    h = _hierarchy(A)
+    assert h.num_levels >= 2
+    for l, lv in enumerate(h.levels):
+        nodes = np.concatenate(lv.domains)
+        assert np.array_equal(np.sort(nodes), np.arange(lv.num_nodes))      # a partition
+        assert all(lv.dims[d].sum() <= 32 for d in lv.domains)
+        if lv.fine_to_parent is not None:
+            m = lv.fine_to_parent
+            # merged nodes share a domain and a dimension
+            dom = np.empty(lv.num_nodes, dtype=np.int64)
+            for k, d in enumerate(lv.domains):
+                dom[d] = k
+            for J in range(m.max() + 1):
+                kids = np.flatnonzero(m == J)
+                assert np.unique(dom[kids]).size == 1
+                assert np.unique(lv.dims[kids]).size == 1
+            assert np.array_equal(h.composed[l + 1], m[h.composed[l]])
+
+
+def test_galerkin_levels_are_restrictions(cube):
+    _, A, _ = cube
+    h = _hierarchy(A)
+    Ad = A.toarray()
+    mats = omas.level_matrices(h, Ad)
+    n = Ad.shape[0]
+    for l in range(h.num_levels):
+        m = omas.scalar_map(h, l)
+        R = np.zeros((h.levels[l].num_dofs, n))
+        R[m, np.arange(n)] = 1.0
+        # brute force: entry (a, b) of level l sums every fine entry mapping onto it
+        B = np.zeros_like(mats[l])
+        np.add.at(B, (m[:, None], m[None, :]), Ad)
+        assert np.allclose(B, mats[l], rtol=0, atol=1e-9 * np.abs(Ad).max())
+        assert np.allclose(mats[l], mats[l].T)
+
+
+def test_mas_operator_is_spd_and_inverts_domains(cube):
+    _, A, _ = cube
+    h = _hierarchy(A)
+    mats = omas.level_matrices(h, A.toarray())
+    invs = omas.domain_inverses(h, mats)
+    for l, lv in enumerate(h.levels):
+        for d, inv in zip(lv.domains, invs[l]):
+            s = omas.domain_scalars(lv, d)
+            assert np.allclose(inv @ mats[l][np.ix_(s, s)], np.eye(s.size), atol=1e-9)
+    M = omas.dense_operator(h, invs, omas.level_weights(h))
+    assert np.allclose(M, M.T, atol=1e-12 * np.abs(M).max())
+    assert np.linalg.eigvalsh(0.5 * (M + M.T)).min() > 0
+
+
+def emulate_device_apply(layout, block_values, r, weights):
+    """FP64 restatement of the device kernels over the layout tables."""
+    D = DOMAIN_SCALARS
+    vals = [block_values]
+    for b in layout.blocks[1:]:
+        prev = vals[-1]
+        out = np.zeros((b.nnz, 3, 3))
+        for e in range(b.nnz):
+            out[e] = prev[b.coarsen_items[b.coarsen_offsets[e]:b.coarsen_offsets[e + 1]]].sum(0)
+        vals.append(out)
+    dom = np.zeros((layout.num_domains * D * D,))
+    for b, v in zip(layout.blocks, vals):
+        for e in np.flatnonzero(b.dest != NO_DEST):
+            base = int(b.dest[e])
+            for i in range(3):
+                dom[base + i * D: base + i * D + 3] = v[e, i]
+    dom = dom.reshape(-1, D, D)
+    for k, P in enumerate(layout.domain_sizes):
+        dom[k, np.arange(P, D), np.arange(P, D)] = 1.0
+    inv = np.linalg.inv(dom)
+    nc = int(layout.level_offsets[-1])
+    rc = np.zeros((nc, 3))
+    r3 = r.reshape(-1, 3)
+    for l in range(layout.num_coarse_levels):
+        for J in range(layout.level_offsets[l], layout.level_offsets[l + 1]):
+            kids = layout.restrict_items[layout.restrict_offsets[J]:layout.restrict_offsets[J + 1]]
+            rc[J] = (r3 if l == 0 else rc)[kids].sum(0)
+    c0 = np.zeros_like(r3)
+    cc = np.zeros((nc, 3))
+    for d in range(layout.num_domains):
+        nodes = layout.domain_nodes[layout.domain_node_offsets[d]:layout.domain_node_offsets[d + 1]]
+        src, dst = (r3, c0) if d < layout.level0_domains else (rc, cc)
+        loc = np.zeros(D)
+        loc[:3 * nodes.size] = src[nodes].reshape(-1)
+        dst[nodes] = (inv[d] @ loc)[:3 * nodes.size].reshape(-1, 3)
+    z = weights[0] * c0
+    for l in range(layout.num_coarse_levels):
+        z = z + weights[l + 1] * cc[layout.prolong_maps[l]]
+    return z.reshape(-1)
+
+
+@pytest.mark.parametrize("levels,coarsest", [(4, 2.5), (2, 1.0), (1, 2.5)])
+def test_device_tables_reproduce_the_oracle_apply(cube, levels, coarsest):
+    _, A, _ = cube
+    h = _hierarchy(A, max_levels=levels)
+    layout = build_layout(h, A.indptr, A.indices)
+    w = level_weights(h.num_levels, coarsest).astype(np.float64)
+    assert np.array_equal(w, np.asarray(omas.level_weights(h, coarsest)))
+    mats = omas.level_matrices(h, A.toarray())
+    invs = omas.domain_inverses(h, mats)
+    rng = np.random.default_rng(7)
+    for _ in range(3):
+        r = rng.standard_normal(A.shape[0])
+        ref = omas.apply(h, invs, r, w)
+        got = emulate_device_apply(layout, A.data, r, w)
+        assert np.abs(got - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+def test_layout_destinations_stay_inside_their_domain(cube):
+    _, A, _ = cube
+    layout = build_layout(_hierarchy(A), A.indptr, A.indices)
+    D = DOMAIN_SCALARS
+    for b in layout.blocks:
+        d = b.dest[b.dest != NO_DEST].astype(np.int64)
+        dom = d // (D * D)
+        row, col = (d % (D * D)) // D, d % D
+        size = layout.domain_sizes[dom]
+        assert np.all(row + 2 < size) and np.all(col + 2 < size)
+        assert np.all(row % 3 == 0) and np.all(col % 3 == 0)
+    assert layout.domain_sizes.max() <= D
+
+
+def test_level_weight_rules():
+    assert level_weights(3).tolist() == [1.0, 1.0, 2.5]
+    assert level_weights(1).tolist() == [2.5]
+    assert level_weights(2, weights=(1.0, 2.0, 3.0)).tolist() == [1.0, 3.0]
+    with pytest.raises(ValueError):
+        level_weights(3, weights=(1.0,))
+
+
+def test_mas_pcg_oracle_converges(cube):
+    _, A, b = cube
+    h = _hierarchy(A)
+    mats = omas.level_matrices(h, A.toarray())
+    invs = omas.domain_inverses(h, mats)
+    w = omas.level_weights(h, 1.0)
+    st_m, x_m, it_m, _ = pcg.pcg(lambda v: A @ v, lambda v: omas.apply(h, invs, v, w), b, 1e-12, 500)
+    assert st_m == it_m > 0                         # reference code: iterations on success
+    assert np.linalg.norm(A @ x_m - b) <= 1e-5 * np.linalg.norm(b)

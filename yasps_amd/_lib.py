@@ -48,6 +48,20 @@ class PcgStats(C.Structure):
                 ("delta", C.c_double), ("rr0", C.c_double), ("rr", C.c_double)]
 
 
+MAS_MAX_COARSE_LEVELS = 8
+
+
+class MasArgs(C.Structure):
+    _fields_ = [("restrict_offsets", C.c_void_p), ("restrict_items", C.c_void_p),
+                ("num_coarse_nodes", C.c_uint32), ("num_coarse_levels", C.c_uint32),
+                ("level_offsets", C.c_uint32 * (MAS_MAX_COARSE_LEVELS + 1)),
+                ("prolong_maps", C.c_void_p), ("inverses", C.c_void_p),
+                ("domain_nodes", C.c_void_p), ("domain_node_offsets", C.c_void_p),
+                ("num_domains", C.c_uint32), ("level0_domains", C.c_uint32),
+                ("weights", C.c_void_p), ("coarse_residual", C.c_void_p),
+                ("fine_correction", C.c_void_p), ("coarse_correction", C.c_void_p)]
+
+
 P = C.c_void_p
 U32 = C.c_uint32
 I32 = C.c_int32
@@ -87,6 +101,12 @@ SIGNATURES = {
     "ysp_bsr3_spmv": (C.c_int, [P, P, P, I32, U32, U32, P, P, P]),
     "ysp_pcg3_solve": (C.c_int, [U32, D, I32, P, P, P, I32, U32, P, I32, P, U32, P, P, I32, P, P,
                                  C.POINTER(PcgStats), P]),
+    "ysp_mas_coarsen": (C.c_int, [P, I32, U32, P, P, U32, P, P]),
+    "ysp_mas_domain_scatter": (C.c_int, [P, I32, U32, P, P, P]),
+    "ysp_mas_invert_domains": (C.c_int, [P, P, U32, P, P, P]),
+    "ysp_mas_apply": (C.c_int, [C.POINTER(MasArgs), P, U32, P, P]),
+    "ysp_pcg3_mas_solve": (C.c_int, [U32, D, I32, P, P, P, I32, U32, C.POINTER(MasArgs), P, U32,
+                                     P, P, I32, P, P, C.POINTER(PcgStats), P]),
 }
 
 _lib = None

@@ -11,6 +11,7 @@
 // to 64) so that 16 lanes reading 16 consecutive blocks issue fully coalesced loads.
 #include "common.hpp"
 #include "dense.hpp"
+#include "mas.hpp"
 
 #include <hipcub/hipcub.hpp>
 #include <algorithm>
@@ -212,7 +213,8 @@ __device__ __forceinline__ void apply3(const MT* __restrict__ minv, uint32_t v, 
            static_cast<double>(M[3 * i + 2]) * r[2];
 }
 
-// r = b (or b - A x0 held in ax0), x = x0 or 0, z = M^-1 r, p buffers = 0.
+// r = b (or b - A x0 held in ax0), x = x0 or 0, p buffers = 0 and, with the block-Jacobi
+// preconditioner (minv != nullptr), z = M^-1 r and the dots r.z, b.M^-1 b.
 template <typename MT>
 __global__ void __launch_bounds__(kB)
     pcg3_init_kernel(const MT* __restrict__ minv, const double* __restrict__ b,
@@ -233,24 +235,23 @@ __global__ void __launch_bounds__(kB)
       pa[d] = 0.0;
       pb[d] = 0.0;
       r[d] = rl[i];
-    }
-    apply3(minv, v, rl, zl);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      z[3 * v + i] = zl[i];
-      rz += rl[i] * zl[i];
       rr += rl[i] * rl[i];
+      bb += bl[i] * bl[i];
     }
-    if (x0) {
-      apply3(minv, v, bl, tmp);
+    if (minv) {
+      apply3(minv, v, rl, zl);
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
-        bmb += bl[i] * tmp[i];
-        bb += bl[i] * bl[i];
+        z[3 * v + i] = zl[i];
+        rz += rl[i] * zl[i];
       }
-    } else {
-      bmb = rz;
-      bb = rr;
+      if (x0) {
+        apply3(minv, v, bl, tmp);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) bmb += bl[i] * tmp[i];
+      } else {
+        bmb = rz;
+      }
     }
   }
   __shared__ double lds[kB / 64];
@@ -260,9 +261,10 @@ __global__ void __launch_bounds__(kB)
   wg_add(bb, slot_ptr(st, SET_BB), lds);
 }
 
-__global__ void __launch_bounds__(64) pcg3_init_finish_kernel(double* st, double tol, int mode) {
+__global__ void __launch_bounds__(64) pcg3_init_finish_kernel(double* st, double tol, int mode,
+                                                             int rz0_is_rz) {
   __shared__ double lds[1];
-  const double rz0 = read_slots(st, SET_RZ0, lds);
+  const double rz0 = read_slots(st, rz0_is_rz ? SET_RZ + 0 : SET_RZ0, lds);
   const double rz = read_slots(st, SET_RZ + 0, lds);
   const double bb = read_slots(st, SET_BB, lds);
   const double rr = read_slots(st, SET_RR + 0, lds);
@@ -383,15 +385,18 @@ __global__ void __launch_bounds__(kB)
       rl[i] = fma(-q[d], alpha, r[d]);
       r[d] = rl[i];
     }
-    apply3(minv, v, rl, zl);
+    if (minv) {
+      apply3(minv, v, rl, zl);
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      z[3 * v + i] = zl[i];
-      rz += rl[i] * zl[i];
-      rr += rl[i] * rl[i];
+      for (int i = 0; i < 3; ++i) {
+        z[3 * v + i] = zl[i];
+        rz += rl[i] * zl[i];
+      }
     }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) rr += rl[i] * rl[i];
   }
-  wg_add(rz, slot_ptr(st, SET_RZ + cur), lds);
+  if (minv) wg_add(rz, slot_ptr(st, SET_RZ + cur), lds);
   wg_add(rr, slot_ptr(st, SET_RR + cur), lds);
 }
 
@@ -526,9 +531,9 @@ extern "C" int ysp_cast_f32(const double* in, float* out, uint64_t n, void* stre
 namespace {
 template <typename VT, typename MT>
 int pcg3_run(uint32_t max_iterations, double threshold, int32_t mode, const uint32_t* row_ptr,
-             const uint32_t* col, const VT* v, uint32_t nnz, const MT* minv, const double* rhs,
-             uint32_t n, double* x, const double* x0, double* work, double* st,
-             ysp_pcg_stats* stats, hipStream_t s) {
+             const uint32_t* col, const VT* v, uint32_t nnz, const MT* minv, const MasApply* mas,
+             const double* rhs, uint32_t n, double* x, const double* x0, double* work,
+             double* st, ysp_pcg_stats* stats, hipStream_t s) {
   const uint32_t nv = n / 3;
   double* r = work;
   double* z = work + n;
@@ -544,7 +549,16 @@ int pcg3_run(uint32_t max_iterations, double threshold, int32_t mode, const uint
   }
   const dim3 vg(grid_for(nv, kB)), sg(grid_for(nv, kB / kG)), bl(kB);
   hipLaunchKernelGGL(pcg3_init_kernel<MT>, vg, bl, 0, s, minv, rhs, x0, ax0, x, r, z, pa, pb, nv, st);
-  hipLaunchKernelGGL(pcg3_init_finish_kernel, dim3(1), dim3(64), 0, s, st, threshold, mode);
+  if (mas) {
+    if (x0) {
+      if (int rc = launch_mas_apply(*mas, rhs, n, z, st + slots_base(SET_RZ0), nullptr, s))
+        return rc;
+    }
+    if (int rc = launch_mas_apply(*mas, r, n, z, st + slots_base(SET_RZ + 0), nullptr, s))
+      return rc;
+  }
+  hipLaunchKernelGGL(pcg3_init_finish_kernel, dim3(1), dim3(64), 0, s, st, threshold, mode,
+                     (mas && !x0) ? 1 : 0);
   YSP_CHECK_LAUNCH();
   double* box = status_box();
   if (!box) {
@@ -563,9 +577,15 @@ int pcg3_run(uint32_t max_iterations, double threshold, int32_t mode, const uint
       double* pp = (it & 1) ? pb : pa;
       hipLaunchKernelGGL(pcg3_spmv_kernel<VT>, sg, bl, 0, s, row_ptr, col, v, stride, nv, z, pp, pn,
                          q, st, static_cast<int>(it));
-      if (it <= max_iterations)
+      if (it <= max_iterations) {
         hipLaunchKernelGGL(pcg3_update_kernel<MT>, vg, bl, 0, s, minv, x, r, pn, q, z, nv, st,
                            static_cast<int>(it));
+        if (mas) {
+          if (int rc = launch_mas_apply(*mas, r, n, z, st + slots_base(SET_RZ + (it % 3)),
+                                        st + S_STATUS, s))
+            return rc;
+        }
+      }
     }
     YSP_CHECK_LAUNCH();
     YSP_CHECK(hipMemcpyAsync(box, st, 8 * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -613,18 +633,48 @@ extern "C" int ysp_pcg3_solve(uint32_t max_iterations, double threshold, int32_t
     const float* v = static_cast<const float*>(values);
     if (inverse_f32)
       return pcg3_run(max_iterations, threshold, threshold_mode, row_ptr, col, v, nnz,
-                      static_cast<const float*>(diagonal_inverse), rhs, n, solution, x0, work,
+                      static_cast<const float*>(diagonal_inverse), nullptr, rhs, n, solution, x0, work,
                       cg_scalars, stats, s);
     return pcg3_run(max_iterations, threshold, threshold_mode, row_ptr, col, v, nnz,
-                    static_cast<const double*>(diagonal_inverse), rhs, n, solution, x0, work,
+                    static_cast<const double*>(diagonal_inverse), nullptr, rhs, n, solution, x0, work,
                     cg_scalars, stats, s);
   }
   const double* v = static_cast<const double*>(values);
   if (inverse_f32)
     return pcg3_run(max_iterations, threshold, threshold_mode, row_ptr, col, v, nnz,
-                    static_cast<const float*>(diagonal_inverse), rhs, n, solution, x0, work,
+                    static_cast<const float*>(diagonal_inverse), nullptr, rhs, n, solution, x0, work,
                     cg_scalars, stats, s);
   return pcg3_run(max_iterations, threshold, threshold_mode, row_ptr, col, v, nnz,
-                  static_cast<const double*>(diagonal_inverse), rhs, n, solution, x0, work,
+                  static_cast<const double*>(diagonal_inverse), nullptr, rhs, n, solution, x0, work,
+                  cg_scalars, stats, s);
+}
+
+extern "C" int ysp_pcg3_mas_solve(uint32_t max_iterations, double threshold,
+                                  int32_t threshold_mode, const uint32_t* row_ptr,
+                                  const uint32_t* col, const void* values, int32_t values_f32,
+                                  uint32_t nnz, const ysp_mas_args* mas, const double* rhs,
+                                  uint32_t n, double* solution, const double* initial_guess,
+                                  int32_t zero_initial_guess, double* work, double* cg_scalars,
+                                  ysp_pcg_stats* stats, void* stream) {
+  if (stats) *stats = ysp_pcg_stats{0, 0, 0, 0, 0, 0};
+  if (n == 0) return 0;
+  if (n % 3 || !work || !cg_scalars || !mas || (threshold_mode != 0 && threshold_mode != 1)) {
+    ysp_set_error(__FILE__, __LINE__, "ysp_pcg3_mas_solve: bad arguments");
+    return YSP_ERR_ARG;
+  }
+  hipStream_t s = as_stream(stream);
+  const double* x0 = (!zero_initial_guess && initial_guess) ? initial_guess : nullptr;
+  const MasApply* m = reinterpret_cast<const MasApply*>(mas);
+  if (check_mas_args(*m, n) != YSP_OK) {
+    ysp_set_error(__FILE__, __LINE__, "ysp_pcg3_mas_solve: inconsistent MAS arguments");
+    return YSP_ERR_ARG;
+  }
+  const float* no_minv = nullptr;
+  if (values_f32)
+    return pcg3_run(max_iterations, threshold, threshold_mode, row_ptr, col,
+                    static_cast<const float*>(values), nnz, no_minv, m, rhs, n, solution, x0, work,
+                    cg_scalars, stats, s);
+  return pcg3_run(max_iterations, threshold, threshold_mode, row_ptr, col,
+                  static_cast<const double*>(values), nnz, no_minv, m, rhs, n, solution, x0, work,
                   cg_scalars, stats, s);
 }

@@ -1,0 +1,325 @@
+// Multilevel additive Schwarz (MAS) preconditioner on the GPU.
+//
+// Per numeric update (each Newton solve):
+//   * coarse operators, level by level: a level-l block is the sum of the level-(l-1)
+//     blocks that map to it (identity aggregation, Galerkin R A R^T); level 0 is the
+//     mirrored full 3x3 block-CSR of the solve loop (component-major values);
+//   * domain matrices: every level block whose two nodes share a domain is copied into
+//     that domain's dense, zero-padded D x D FP64 matrix (one contribution per entry,
+//     no reduction);
+//   * inverses: Gauss-Jordan elimination in LDS, one workgroup per domain, FP64, stored
+//     FP32 column-major (the reference's FP64-factor / FP32-bank split,
+//     solver/mas/cuda/local_inverse.cu:244-308). A pivot below 1e-12 * max|diag| marks the
+//     domain as failed.
+// Per application z = sum_l w_l P_l B_l^-1 R_l r:
+//   restrict level by level (every coarse node sums its children on the level below), one
+//   32-lane group per domain multiplies its inverse with the local residual, prolong adds
+//   the weighted corrections of every level and accumulates r.z.
+#include "mas.hpp"
+
+namespace ysp {
+namespace {
+
+constexpr unsigned kB = 256;
+constexpr int kD = 32;                 // padded domain size (scalars)
+
+__host__ __device__ __forceinline__ uint32_t pad64(uint32_t n) { return (n + 63u) & ~63u; }
+
+// level-l block values (AoS, 9 doubles) from the previous level's blocks
+template <bool kPrevSoA>
+__global__ void __launch_bounds__(kB)
+    coarsen_kernel(const double* __restrict__ prev, uint32_t prev_nnz,
+                   const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ items,
+                   uint32_t nnz, double* __restrict__ out) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nnz) return;
+  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const size_t stride = pad64(prev_nnz);
+  for (uint32_t k = offsets[e]; k < offsets[e + 1]; ++k) {
+    const uint32_t s = items[k];
+#pragma unroll
+    for (int j = 0; j < 9; ++j)
+      acc[j] += kPrevSoA ? prev[j * stride + s] : prev[9 * static_cast<size_t>(s) + j];
+  }
+#pragma unroll
+  for (int j = 0; j < 9; ++j) out[9 * static_cast<size_t>(e) + j] = acc[j];
+}
+
+// copy the in-domain blocks of one level into the dense domain matrices
+template <bool kSoA>
+__global__ void __launch_bounds__(kB)
+    domain_scatter_kernel(const double* __restrict__ vals, uint32_t nnz,
+                          const uint32_t* __restrict__ dest, double* __restrict__ dom) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nnz) return;
+  const uint32_t d = dest[e];
+  if (d == 0xffffffffu) return;
+  const size_t stride = pad64(nnz);
+  double* out = dom + d;  // d = domain * D*D + 3a * D + 3b
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      out[i * kD + j] = kSoA ? vals[(3 * i + j) * stride + e] : vals[9 * static_cast<size_t>(e) + 3 * i + j];
+}
+
+__global__ void pad_identity_kernel(double* __restrict__ dom, const uint32_t* __restrict__ sizes,
+                                    uint32_t ndom) {
+  const uint32_t d = blockIdx.x;
+  if (d >= ndom) return;
+  const uint32_t P = sizes[d];
+  for (uint32_t i = P + threadIdx.x; i < kD; i += blockDim.x)
+    dom[static_cast<size_t>(d) * kD * kD + i * kD + i] = 1.0;
+}
+
+// Gauss-Jordan inversion of one D x D matrix per workgroup (no pivoting: the domain
+// matrices are principal submatrices of an SPD operator).
+__global__ void __launch_bounds__(256)
+    invert_domains_kernel(const double* __restrict__ dom, float* __restrict__ inv,
+                          uint32_t ndom, int* __restrict__ status) {
+  __shared__ double A[kD][kD + 1];
+  __shared__ double Binv[kD][kD + 1];
+  __shared__ double piv_row_a[kD], piv_row_b[kD], piv_col[kD];
+  const uint32_t d = blockIdx.x;
+  if (d >= ndom) return;
+  const double* M = dom + static_cast<size_t>(d) * kD * kD;
+  for (int t = threadIdx.x; t < kD * kD; t += blockDim.x) {
+    const int i = t / kD, j = t % kD;
+    A[i][j] = M[t];
+    Binv[i][j] = (i == j) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  __shared__ double scale;
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < kD; ++i) s = fmax(s, fabs(A[i][i]));
+    scale = s;
+  }
+  __syncthreads();
+  bool bad = false;
+  for (int k = 0; k < kD; ++k) {
+    const double pivot = A[k][k];
+    if (!(fabs(pivot) > 1e-12 * scale)) bad = true;
+    const double ip = 1.0 / pivot;
+    if (threadIdx.x < kD) {
+      piv_row_a[threadIdx.x] = A[k][threadIdx.x] * ip;
+      piv_row_b[threadIdx.x] = Binv[k][threadIdx.x] * ip;
+      piv_col[threadIdx.x] = A[threadIdx.x][k];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < kD * kD; t += blockDim.x) {
+      const int i = t / kD, j = t % kD;
+      if (i == k) {
+        A[i][j] = piv_row_a[j];
+        Binv[i][j] = piv_row_b[j];
+      } else {
+        const double f = piv_col[i];
+        A[i][j] = fma(-f, piv_row_a[j], A[i][j]);
+        Binv[i][j] = fma(-f, piv_row_b[j], Binv[i][j]);
+      }
+    }
+    __syncthreads();
+  }
+  float* out = inv + static_cast<size_t>(d) * kD * kD;
+  for (int t = threadIdx.x; t < kD * kD; t += blockDim.x) {
+    const int col = t / kD, row = t % kD;  // column-major: out[col * D + row]
+    out[t] = static_cast<float>(0.5 * (Binv[row][col] + Binv[col][row]));
+  }
+  if (bad && threadIdx.x == 0) atomicOr(status, 1);
+}
+
+__device__ __forceinline__ bool mas_stopped(const double* stop) {
+  return stop && *stop != 0.0;
+}
+
+// r_l[J] = sum of the residuals of J's children: level-0 nodes of r (kFromFine) or coarse
+// nodes of the level below in rc. One launch per coarse level.
+template <bool kFromFine>
+__global__ void __launch_bounds__(kB)
+    restrict_kernel(const double* __restrict__ r, const uint32_t* __restrict__ offsets,
+                    const uint32_t* __restrict__ items, uint32_t first, uint32_t last,
+                    double* __restrict__ rc, const double* __restrict__ stop) {
+  if (mas_stopped(stop)) return;
+  const uint32_t J = first + blockIdx.x * blockDim.x + threadIdx.x;
+  if (J >= last) return;
+  const double* src = kFromFine ? r : rc;
+  double s0 = 0, s1 = 0, s2 = 0;
+  for (uint32_t k = offsets[J]; k < offsets[J + 1]; ++k) {
+    const uint32_t i = items[k];
+    s0 += src[3 * static_cast<size_t>(i)];
+    s1 += src[3 * static_cast<size_t>(i) + 1];
+    s2 += src[3 * static_cast<size_t>(i) + 2];
+  }
+  rc[3 * static_cast<size_t>(J)] = s0;
+  rc[3 * static_cast<size_t>(J) + 1] = s1;
+  rc[3 * static_cast<size_t>(J) + 2] = s2;
+}
+
+// one 32-lane group per domain: c_loc = B^-1 r_loc
+__global__ void __launch_bounds__(kB)
+    local_solve_kernel(const float* __restrict__ inv, const uint32_t* __restrict__ dom_nodes,
+                       const uint32_t* __restrict__ dom_node_off, uint32_t ndom,
+                       const double* __restrict__ r0, const double* __restrict__ rc,
+                       uint32_t level0_domains, float* __restrict__ c0,
+                       float* __restrict__ cc, const double* __restrict__ stop) {
+  if (mas_stopped(stop)) return;
+  const uint32_t d = blockIdx.x * (kB / kD) + threadIdx.x / kD;
+  const int lane = threadIdx.x % kD;
+  if (d >= ndom) return;
+  const uint32_t n0 = dom_node_off[d], n1 = dom_node_off[d + 1];
+  const uint32_t P = 3 * (n1 - n0);
+  float rl = 0.0f;
+  uint32_t node = 0;
+  if (static_cast<uint32_t>(lane) < P) {
+    node = dom_nodes[n0 + lane / 3];
+    const uint32_t s = 3 * node + lane % 3;
+    rl = static_cast<float>(d < level0_domains ? r0[s] : rc[s]);
+  }
+  const float* B = inv + static_cast<size_t>(d) * kD * kD;
+  float acc = 0.0f;
+#pragma unroll 8
+  for (int j = 0; j < kD; ++j) {
+    const float rj = __shfl(rl, j, kD);
+    acc = fmaf(B[j * kD + lane], rj, acc);
+  }
+  if (static_cast<uint32_t>(lane) < P) {
+    const uint32_t s = 3 * node + lane % 3;
+    if (d < level0_domains) c0[s] = acc;
+    else cc[s] = acc;
+  }
+}
+
+// z = w0 c0 + sum_l w_l c_l[map_l(node)], and r.z
+__global__ void __launch_bounds__(kB)
+    prolong_kernel(const float* __restrict__ c0, const float* __restrict__ cc,
+                   const uint32_t* __restrict__ maps, uint32_t nv, uint32_t ncoarse_levels,
+                   const float* __restrict__ weights, const double* __restrict__ r,
+                   double* __restrict__ z, double* __restrict__ rz_slot,
+                   const double* __restrict__ stop) {
+  if (mas_stopped(stop)) return;
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  double rz = 0.0;
+  if (v < nv) {
+    float s0 = weights[0] * c0[3 * v], s1 = weights[0] * c0[3 * v + 1],
+          s2 = weights[0] * c0[3 * v + 2];
+    for (uint32_t l = 0; l < ncoarse_levels; ++l) {
+      const uint32_t J = maps[static_cast<size_t>(l) * nv + v];
+      const float w = weights[l + 1];
+      s0 = fmaf(w, cc[3 * J], s0);
+      s1 = fmaf(w, cc[3 * J + 1], s1);
+      s2 = fmaf(w, cc[3 * J + 2], s2);
+    }
+    const double z0 = s0, z1 = s1, z2 = s2;
+    z[3 * v] = z0;
+    z[3 * v + 1] = z1;
+    z[3 * v + 2] = z2;
+    rz = r[3 * v] * z0 + r[3 * v + 1] * z1 + r[3 * v + 2] * z2;
+  }
+  if (rz_slot) {
+    __shared__ double lds[kB / 64];
+    rz = block_sum<kB>(rz, lds);
+    if (threadIdx.x == 0) atomic_add_f64(rz_slot + (blockIdx.x % 64) * 16, rz);
+  }
+}
+
+}  // namespace
+}  // namespace ysp
+
+using namespace ysp;
+
+extern "C" int ysp_mas_coarsen(const double* prev_values, int32_t prev_is_full_bsr,
+                               uint32_t prev_nnz, const uint32_t* offsets, const uint32_t* items,
+                               uint32_t nnz, double* values, void* stream) {
+  if (nnz == 0) return YSP_OK;
+  const dim3 g(grid_for(nnz, kB)), b(kB);
+  if (prev_is_full_bsr)
+    hipLaunchKernelGGL(coarsen_kernel<true>, g, b, 0, as_stream(stream), prev_values, prev_nnz,
+                       offsets, items, nnz, values);
+  else
+    hipLaunchKernelGGL(coarsen_kernel<false>, g, b, 0, as_stream(stream), prev_values, prev_nnz,
+                       offsets, items, nnz, values);
+  YSP_CHECK_LAUNCH();
+  return YSP_OK;
+}
+
+extern "C" int ysp_mas_domain_scatter(const double* values, int32_t is_full_bsr, uint32_t nnz,
+                                      const uint32_t* dest, double* domains, void* stream) {
+  if (nnz == 0) return YSP_OK;
+  const dim3 g(grid_for(nnz, kB)), b(kB);
+  if (is_full_bsr)
+    hipLaunchKernelGGL(domain_scatter_kernel<true>, g, b, 0, as_stream(stream), values, nnz, dest,
+                       domains);
+  else
+    hipLaunchKernelGGL(domain_scatter_kernel<false>, g, b, 0, as_stream(stream), values, nnz, dest,
+                       domains);
+  YSP_CHECK_LAUNCH();
+  return YSP_OK;
+}
+
+extern "C" int ysp_mas_invert_domains(double* domains, const uint32_t* domain_sizes,
+                                      uint32_t num_domains, float* inverses, int32_t* status,
+                                      void* stream) {
+  if (num_domains == 0) return YSP_OK;
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(pad_identity_kernel, dim3(num_domains), dim3(64), 0, s, domains, domain_sizes,
+                     num_domains);
+  hipLaunchKernelGGL(invert_domains_kernel, dim3(num_domains), dim3(256), 0, s, domains, inverses,
+                     num_domains, status);
+  YSP_CHECK_LAUNCH();
+  return YSP_OK;
+}
+
+namespace ysp {
+int launch_mas_apply(const MasApply& m, const double* r, uint32_t n, double* z,
+                     double* rz_slots, const double* stop, hipStream_t s) {
+  const uint32_t nv = n / 3;
+  for (uint32_t l = 0; l < m.num_coarse_levels; ++l) {
+    const uint32_t first = m.level_offsets[l], last = m.level_offsets[l + 1];
+    if (last <= first) continue;
+    const dim3 g(grid_for(last - first, kB)), b(kB);
+    if (l == 0)
+      hipLaunchKernelGGL(restrict_kernel<true>, g, b, 0, s, r, m.restrict_offsets,
+                         m.restrict_items, first, last, m.coarse_residual, stop);
+    else
+      hipLaunchKernelGGL(restrict_kernel<false>, g, b, 0, s, r, m.restrict_offsets,
+                         m.restrict_items, first, last, m.coarse_residual, stop);
+  }
+  if (m.num_domains)
+    hipLaunchKernelGGL(local_solve_kernel, dim3(grid_for(m.num_domains, kB / kD)), dim3(kB), 0, s,
+                       m.inverses, m.domain_nodes, m.domain_node_offsets, m.num_domains, r,
+                       m.coarse_residual, m.level0_domains, m.fine_correction,
+                       m.coarse_correction, stop);
+  hipLaunchKernelGGL(prolong_kernel, dim3(grid_for(nv, kB)), dim3(kB), 0, s, m.fine_correction,
+                     m.coarse_correction, m.prolong_maps, nv, m.num_coarse_levels, m.weights, r, z,
+                     rz_slots, stop);
+  YSP_CHECK_LAUNCH();
+  return YSP_OK;
+}
+
+// Host-side check of the argument block against the kernels' assumptions.
+int check_mas_args(const MasApply& m, uint32_t n) {
+  if (n % 3 || m.num_coarse_levels > kMasMaxCoarseLevels || !m.weights || !m.fine_correction ||
+      (m.num_domains && (!m.inverses || !m.domain_nodes || !m.domain_node_offsets)) ||
+      m.level0_domains > m.num_domains ||
+      (m.num_coarse_levels &&
+       (!m.prolong_maps || !m.restrict_offsets || !m.restrict_items || !m.coarse_residual ||
+        !m.coarse_correction || m.level_offsets[0] != 0 ||
+        m.level_offsets[m.num_coarse_levels] != m.num_coarse_nodes)))
+    return YSP_ERR_ARG;
+  for (uint32_t l = 0; l < m.num_coarse_levels; ++l)
+    if (m.level_offsets[l + 1] < m.level_offsets[l]) return YSP_ERR_ARG;
+  return YSP_OK;
+}
+}  // namespace ysp
+
+extern "C" int ysp_mas_apply(const ysp_mas_args* args, const double* r, uint32_t n, double* z,
+                             void* stream) {
+  if (!args) return YSP_ERR_ARG;
+  const MasApply& m = *reinterpret_cast<const MasApply*>(args);
+  if (check_mas_args(m, n) != YSP_OK) {
+    ysp_set_error(__FILE__, __LINE__, "ysp_mas_apply: inconsistent arguments");
+    return YSP_ERR_ARG;
+  }
+  if (n == 0) return YSP_OK;
+  return launch_mas_apply(m, r, n, z, nullptr, nullptr, as_stream(stream));
+}

@@ -1,0 +1,36 @@
+// Internal interface of the MAS apply (shared with the PCG driver).
+#pragma once
+#include "common.hpp"
+
+namespace ysp {
+
+constexpr int kMasMaxCoarseLevels = 8;
+
+struct MasApply {
+  const uint32_t* restrict_offsets;   // [num_coarse_nodes + 1]
+  const uint32_t* restrict_items;     // children: level-0 nodes (level 1), else coarse ids
+  uint32_t num_coarse_nodes;          // all coarse levels, concatenated ids
+  uint32_t num_coarse_levels;
+  uint32_t level_offsets[kMasMaxCoarseLevels + 1];  // coarse id range of every level
+  const uint32_t* prolong_maps;       // [num_coarse_levels x n/3] global coarse ids
+  const float* inverses;              // [num_domains x 32 x 32] column-major
+  const uint32_t* domain_nodes;
+  const uint32_t* domain_node_offsets;  // [num_domains + 1]
+  uint32_t num_domains;
+  uint32_t level0_domains;            // domains [0, level0_domains) are level-0 domains
+  const float* weights;               // [1 + num_coarse_levels]
+  double* coarse_residual;            // [3 x num_coarse_nodes]
+  float* fine_correction;             // [n]
+  float* coarse_correction;           // [3 x num_coarse_nodes]
+};
+
+// z = M r; if rz_slots != nullptr the partial sums of r.z are added to the 64 slots
+// (spaced 16 doubles) starting there. When stop != nullptr and *stop != 0 at run time the
+// kernels do nothing (a converged Krylov loop).
+int launch_mas_apply(const MasApply& m, const double* r, uint32_t n, double* z,
+                     double* rz_slots, const double* stop, hipStream_t s);
+
+// YSP_OK when the argument block is consistent (sizes, level ranges, pointers).
+int check_mas_args(const MasApply& m, uint32_t n);
+
+}  // namespace ysp

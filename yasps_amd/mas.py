@@ -1,0 +1,401 @@
+"""Multilevel additive Schwarz PCG solver (``solver/masSolver.py``, ``solver/mas/``).
+
+``masSolver`` keeps the reference's split between a hierarchy built once per static
+sparsity structure (``solver/mas/solver.py:266-295``) and numeric work on every solve
+(``solver/mas/numeric_assembly.py``, ``local_inverse.py``): on the device, each solve
+coarsens the current Hessian level by level (identity-aggregation Galerkin sums),
+scatters the in-domain blocks into dense 32 x 32 domain matrices, inverts them (FP64
+Gauss-Jordan, FP32 storage) and runs PCG with z = sum_l w_l P_l B_l^-1 R_l r
+(``solver/mas/preconditioner.py:128-154``, ``solver/mas/pcg.py:23-102``).
+
+All variable blocks must be 3 x 3 (free vertices) and domains hold at most 32 scalars.
+The node grouping is a breadth-first replacement of METIS (see ``mas_hierarchy``), so
+domains and iteration counts differ from the reference's; the preconditioner is still
+symmetric positive definite and PCG reaches the same stated residual.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass, field
+from typing import List
+
+import numpy as np
+
+from . import _lib, device as dev
+from .hessian import vector
+from .mas_hierarchy import Hierarchy, build_hierarchy
+
+_MODES = {"reference": 0, "relative_residual": 1}
+DOMAIN_SCALARS = 32          # padded domain size of the device kernels
+NO_DEST = np.uint32(0xFFFFFFFF)
+
+
+@dataclass
+class LevelBlocks:
+    """Blocks of one level: (row, col) node pairs and, for coarse levels, the CSR of the
+    finer level's blocks summed into each block."""
+    rows: np.ndarray
+    cols: np.ndarray
+    coarsen_offsets: np.ndarray = None
+    coarsen_items: np.ndarray = None
+    dest: np.ndarray = None          # offset into the domain matrices or NO_DEST
+
+    @property
+    def nnz(self) -> int:
+        return int(self.rows.size)
+
+
+@dataclass
+class MasLayout:
+    hierarchy: Hierarchy
+    blocks: List[LevelBlocks]
+    level_offsets: np.ndarray            # coarse id ranges [num_coarse_levels + 1]
+    restrict_offsets: np.ndarray
+    restrict_items: np.ndarray
+    prolong_maps: np.ndarray             # [num_coarse_levels, nv]
+    domain_nodes: np.ndarray
+    domain_node_offsets: np.ndarray
+    domain_sizes: np.ndarray             # scalars per domain
+    level0_domains: int
+    weights: np.ndarray = field(default=None)
+
+    @property
+    def num_domains(self) -> int:
+        return int(self.domain_sizes.size)
+
+    @property
+    def num_coarse_levels(self) -> int:
+        return int(self.level_offsets.size - 1)
+
+
+def _csr_by_key(keys: np.ndarray, num_keys: int):
+    """Stable grouping of positions by key: offsets [num_keys + 1], items (positions)."""
+    order = np.argsort(keys, kind="stable")
+    counts = np.bincount(keys, minlength=num_keys)
+    offsets = np.zeros(num_keys + 1, dtype=np.int64)
+    np.cumsum(counts, out=offsets[1:])
+    return offsets, order
+
+
+def level_weights(num_levels: int, coarsest_level_weight: float = 2.5,
+                  weights=None) -> np.ndarray:
+    """(1, ..., 1, coarsest) or configured weights cut to the constructed levels
+    (``solver/mas/solver.py:297-317``, ``preconditioner.py:27-36``)."""
+    if weights is None:
+        w = [1.0] * (num_levels - 1) + [float(coarsest_level_weight)]
+        if num_levels == 1:
+            w = [float(coarsest_level_weight)]
+        return np.asarray(w, dtype=np.float32)
+    weights = tuple(float(x) for x in weights)
+    if len(weights) == num_levels:
+        return np.asarray(weights, dtype=np.float32)
+    if len(weights) > num_levels:
+        w = weights[:1] if num_levels == 1 else weights[:num_levels - 1] + weights[-1:]
+        return np.asarray(w, dtype=np.float32)
+    raise ValueError("level_weights must contain at least one weight per constructed "
+                     f"hierarchy level ({num_levels})")
+
+
+def static_pairs(row_ptr: np.ndarray, col: np.ndarray, src: np.ndarray) -> np.ndarray:
+    """Off-diagonal node pairs of the static blocks of a full block-CSR (src bit 30 marks
+    dynamic blocks)."""
+    nbr = row_ptr.size - 1
+    rows = np.repeat(np.arange(nbr, dtype=np.int64), np.diff(row_ptr.astype(np.int64)))
+    keep = ((src.astype(np.int64) >> 30) & 1) == 0
+    keep &= rows != col
+    return np.stack([rows[keep], col.astype(np.int64)[keep]], 1)
+
+
+def build_layout(h: Hierarchy, row_ptr: np.ndarray, col: np.ndarray) -> MasLayout:
+    """Device tables of the MAS apply and numeric update for the full block-CSR
+    (row_ptr, col) of the current operator and a hierarchy over its block rows."""
+    lv0 = h.levels[0]
+    nv = lv0.num_nodes
+    if np.any(lv0.dims != 3) or row_ptr.size != nv + 1:
+        raise ValueError("MAS layout: every node must be a 3-vector block row")
+    if h.num_levels - 1 > _lib.MAS_MAX_COARSE_LEVELS:
+        raise ValueError(f"MAS layout: at most {_lib.MAS_MAX_COARSE_LEVELS + 1} levels")
+    sizes = [lv.num_nodes for lv in h.levels]
+    bases = np.zeros(h.num_levels, dtype=np.int64)          # global coarse id of node 0
+    bases[1:] = np.concatenate([[0], np.cumsum(sizes[1:])[:-1]]) if h.num_levels > 1 else []
+    level_offsets = np.concatenate([[0], np.cumsum(sizes[1:])]).astype(np.int64)
+
+    # restriction: children of every coarse node on the level below
+    r_off, r_items = [np.zeros(1, dtype=np.int64)], []
+    for l in range(1, h.num_levels):
+        fine = h.levels[l - 1]
+        off, items = _csr_by_key(fine.fine_to_parent, sizes[l])
+        r_off.append(off[1:] + r_off[-1][-1])
+        r_items.append(items + (bases[l - 1] if l >= 2 else 0))
+    restrict_offsets = np.concatenate(r_off)
+    restrict_items = (np.concatenate(r_items) if r_items else np.zeros(0, dtype=np.int64))
+    prolong = (np.stack([bases[l] + h.composed[l] for l in range(1, h.num_levels)])
+               if h.num_levels > 1 else np.zeros((0, nv), dtype=np.int64))
+
+    # domains, numbered level by level
+    dom_nodes, dom_off, dom_sizes = [], [0], []
+    node_dom, node_pos = [], []
+    for l, lv in enumerate(h.levels):
+        nd = np.full(lv.num_nodes, -1, dtype=np.int64)
+        npos = np.zeros(lv.num_nodes, dtype=np.int64)
+        for d in lv.domains:
+            g = len(dom_sizes)
+            if 3 * d.size > DOMAIN_SCALARS:
+                raise ValueError(f"MAS layout: a domain holds {3 * d.size} scalars (max {DOMAIN_SCALARS})")
+            nd[d] = g
+            npos[d] = np.arange(d.size)
+            dom_nodes.append(d + (bases[l] if l else 0))
+            dom_off.append(dom_off[-1] + d.size)
+            dom_sizes.append(3 * d.size)
+        if np.any(nd < 0):
+            raise ValueError(f"MAS layout: level {l} has nodes outside every domain")
+        node_dom.append(nd)
+        node_pos.append(npos)
+
+    # blocks of every level and their destinations in the domain matrices
+    rows0 = np.repeat(np.arange(nv, dtype=np.int64), np.diff(row_ptr.astype(np.int64)))
+    blocks = [LevelBlocks(rows0, col.astype(np.int64))]
+    for l in range(1, h.num_levels):
+        prev = blocks[-1]
+        m = h.levels[l - 1].fine_to_parent
+        n_l = sizes[l]
+        key = m[prev.rows] * n_l + m[prev.cols]
+        uniq, inv = np.unique(key, return_inverse=True)
+        off, items = _csr_by_key(inv.reshape(-1), uniq.size)
+        blocks.append(LevelBlocks(uniq // n_l, uniq % n_l, off, items))
+    D = DOMAIN_SCALARS
+    for l, b in enumerate(blocks):
+        same = node_dom[l][b.rows] == node_dom[l][b.cols]
+        dest = (node_dom[l][b.rows] * (D * D) + 3 * node_pos[l][b.rows] * D
+                + 3 * node_pos[l][b.cols])
+        b.dest = np.where(same, dest, np.int64(NO_DEST)).astype(np.uint32)
+    return MasLayout(h, blocks, level_offsets, restrict_offsets, restrict_items, prolong,
+                     np.concatenate(dom_nodes), np.asarray(dom_off, dtype=np.int64),
+                     np.asarray(dom_sizes, dtype=np.int64), len(h.levels[0].domains))
+
+
+class _DeviceMas:
+    """Device copies of a layout plus the per-solve numeric buffers."""
+
+    def __init__(self, layout: MasLayout, weights: np.ndarray, n: int):
+        self.layout = layout
+        u32 = lambda a: dev.to_device(np.asarray(a, dtype=np.uint32), "uint32")   # noqa: E731
+        self.dest = [u32(b.dest) for b in layout.blocks]
+        self.coarsen = [(u32(b.coarsen_offsets), u32(b.coarsen_items)) if l else (None, None)
+                        for l, b in enumerate(layout.blocks)]
+        self.values = [None] + [dev.empty(9 * b.nnz, "float64") for b in layout.blocks[1:]]
+        nc = int(layout.level_offsets[-1])
+        self.restrict_offsets = u32(layout.restrict_offsets)
+        self.restrict_items = u32(layout.restrict_items)
+        self.prolong = u32(layout.prolong_maps.reshape(-1))
+        self.domain_nodes = u32(layout.domain_nodes)
+        self.domain_node_offsets = u32(layout.domain_node_offsets)
+        self.domain_sizes = u32(layout.domain_sizes)
+        D = DOMAIN_SCALARS
+        self.domains = dev.empty(layout.num_domains * D * D, "float64")
+        self.inverses = dev.empty(layout.num_domains * D * D, "float32")
+        self.status = dev.zeros(1, "int32")
+        self.weights = dev.to_device(weights.astype(np.float32), "float32")
+        self.coarse_residual = dev.empty(max(1, 3 * nc), "float64")
+        self.coarse_correction = dev.empty(max(1, 3 * nc), "float32")
+        self.fine_correction = dev.empty(n, "float32")
+        a = _lib.MasArgs()
+        a.restrict_offsets = dev.ptr(self.restrict_offsets)
+        a.restrict_items = dev.ptr(self.restrict_items)
+        a.num_coarse_nodes = nc
+        a.num_coarse_levels = layout.num_coarse_levels
+        for i, v in enumerate(layout.level_offsets):
+            a.level_offsets[i] = int(v)
+        a.prolong_maps = dev.ptr(self.prolong)
+        a.inverses = dev.ptr(self.inverses)
+        a.domain_nodes = dev.ptr(self.domain_nodes)
+        a.domain_node_offsets = dev.ptr(self.domain_node_offsets)
+        a.num_domains = layout.num_domains
+        a.level0_domains = layout.level0_domains
+        a.weights = dev.ptr(self.weights)
+        a.coarse_residual = dev.ptr(self.coarse_residual)
+        a.fine_correction = dev.ptr(self.fine_correction)
+        a.coarse_correction = dev.ptr(self.coarse_correction)
+        self.args = a
+
+    def update(self, full):
+        """Coarse operators, domain matrices and their inverses from the full block-CSR's
+        current FP64 values (numeric_assembly.py:22-83, local_inverse.py)."""
+        lib = _lib.load()
+        s = dev.stream()
+        self.domains.zero_()
+        self.status.zero_()
+        blocks = self.layout.blocks
+        _lib.check(lib.ysp_mas_domain_scatter(dev.ptr(full.v64), 1, full.nnz, dev.ptr(self.dest[0]),
+                                              dev.ptr(self.domains), s), "ysp_mas_domain_scatter")
+        for l in range(1, len(blocks)):
+            prev_vals = full.v64 if l == 1 else self.values[l - 1]
+            prev_nnz = full.nnz if l == 1 else blocks[l - 1].nnz
+            off, items = self.coarsen[l]
+            _lib.check(lib.ysp_mas_coarsen(dev.ptr(prev_vals), int(l == 1), prev_nnz, dev.ptr(off),
+                                           dev.ptr(items), blocks[l].nnz, dev.ptr(self.values[l]), s),
+                       "ysp_mas_coarsen")
+            _lib.check(lib.ysp_mas_domain_scatter(dev.ptr(self.values[l]), 0, blocks[l].nnz,
+                                                  dev.ptr(self.dest[l]), dev.ptr(self.domains), s),
+                       "ysp_mas_domain_scatter")
+        _lib.check(lib.ysp_mas_invert_domains(dev.ptr(self.domains), dev.ptr(self.domain_sizes),
+                                              self.layout.num_domains, dev.ptr(self.inverses),
+                                              dev.ptr(self.status), s), "ysp_mas_invert_domains")
+
+    def apply(self, r, z):
+        _lib.check(_lib.load().ysp_mas_apply(C.byref(self.args), dev.ptr(r), r.numel(), dev.ptr(z),
+                                             dev.stream()), "ysp_mas_apply")
+
+
+class masSolver:
+    """MAS-preconditioned CG on the device (``solver/masSolver.py:11-133``).
+
+    Options (reference names): ``max_domain_dofs`` (<= 32), ``max_levels``,
+    ``target_nodes_per_partition``, ``coarsest_level_weight``, ``level_weights``,
+    ``global_coarse_domain``; extension ``threshold_mode`` as for ``jacobianPCGSolver``.
+    Returns 0 when converged and -4 otherwise, as the reference.
+    """
+
+    def __init__(self, max_domain_dofs=None, max_levels=None, target_nodes_per_partition=None,
+                 coarsest_level_weight: float = 2.5, level_weights=None,
+                 global_coarse_domain: bool = True, threshold_mode: str = "reference",
+                 inverse_backend: str = "cooperative_gauss_jordan",
+                 allow_cpu_fallback: bool = False):
+        if any(v is not None and v <= 0 for v in (max_domain_dofs, max_levels,
+                                                  target_nodes_per_partition)):
+            raise ValueError("domain capacity/levels must be positive and correction iterations non-negative")
+        if max_domain_dofs is not None and max_domain_dofs > DOMAIN_SCALARS:
+            raise ValueError(f"max_domain_dofs is limited to {DOMAIN_SCALARS} scalars on this device path")
+        if max_levels is not None and max_levels > _lib.MAS_MAX_COARSE_LEVELS + 1:
+            raise ValueError(f"max_levels is limited to {_lib.MAS_MAX_COARSE_LEVELS + 1}")
+        if not np.isfinite(coarsest_level_weight) or coarsest_level_weight <= 0:
+            raise ValueError("coarsest_level_weight must be finite and positive")
+        if level_weights is not None and (not level_weights or any(
+                not np.isfinite(w) or w <= 0 for w in level_weights)):
+            raise ValueError("level_weights must contain finite positive values")
+        if threshold_mode not in _MODES:
+            raise ValueError(f"threshold_mode must be one of {sorted(_MODES)}")
+        if inverse_backend != "cooperative_gauss_jordan":
+            raise ValueError("only the cooperative_gauss_jordan inverse backend exists on this device path")
+        if allow_cpu_fallback:
+            raise ValueError("there is no CPU fallback on this device path")
+        self.max_domain_dofs = int(max_domain_dofs or DOMAIN_SCALARS)
+        self.max_levels = int(max_levels or 6)
+        self.target_nodes_per_partition = int(target_nodes_per_partition or 8)
+        self.coarsest_level_weight = float(coarsest_level_weight)
+        self.level_weights = None if level_weights is None else tuple(float(w) for w in level_weights)
+        self.global_coarse_domain = bool(global_coarse_domain)
+        self.threshold_mode = threshold_mode
+        self.reset()
+
+    def reset(self):
+        self._hierarchy = None
+        self._hierarchy_key = None
+        self._device = None
+        self._device_key = None
+        self._solution = None
+        self._work = None
+        self._scalars = None
+        self._statistics = {}
+        self.hierarchy_build_count = 0
+        self.iterations = 0
+
+    @property
+    def solution(self):
+        return self._solution
+
+    @property
+    def statistics(self) -> dict:
+        return dict(self._statistics)
+
+    @property
+    def hierarchy(self) -> Hierarchy:
+        return self._hierarchy
+
+    @property
+    def device_tables(self) -> _DeviceMas:
+        return self._device
+
+    def _ensure(self, h, n):
+        f = h.full
+        if f is None:
+            raise NotImplementedError("masSolver: the device MAS path needs 3 x 3 variable blocks")
+        skey = f.key[:2]
+        rebuilt = False
+        if self._hierarchy is None or self._hierarchy_key != skey or self._device_key != f.key:
+            row_ptr = dev.to_host(f.row_ptr, "uint32").astype(np.int64)
+            col = dev.to_host(f.col[:f.nnz], "uint32").astype(np.int64)
+            if self._hierarchy is None or self._hierarchy_key != skey:
+                src = dev.to_host(f.src[:f.nnz], "uint32")
+                t0 = time.perf_counter()
+                self._hierarchy = build_hierarchy(
+                    np.full(f.nbr, 3), static_pairs(row_ptr, col, src), self.max_domain_dofs,
+                    self.max_levels, self.target_nodes_per_partition, self.global_coarse_domain)
+                self._hierarchy_key = skey
+                self.hierarchy_build_count += 1
+                self._hierarchy_seconds = time.perf_counter() - t0
+            layout = build_layout(self._hierarchy, row_ptr, col)
+            w = level_weights(self._hierarchy.num_levels, self.coarsest_level_weight, self.level_weights)
+            self._device = _DeviceMas(layout, w, n)
+            self._device_key = f.key
+            rebuilt = True
+        if self._solution is None or self._solution.numel() != n:
+            self._solution = dev.zeros(n, "float64")
+            self._work = dev.zeros(5 * n, "float64")
+            self._scalars = dev.zeros(12288, "float64")     # YSP_PCG3_SCALARS
+        return rebuilt
+
+    def computeSolution(self, active_hessian, right_hand_side, initial_guess=None, tolerance=1e-3,
+                        maxIterations=20000, zero_initial_guess=False):
+        t0 = time.perf_counter()
+        rhs = right_hand_side.value if isinstance(right_hand_side, vector) else right_hand_side
+        if rhs is None or not hasattr(rhs, "is_cuda") or not rhs.is_cuda:
+            raise TypeError("masSolver requires a device-resident right-hand side")
+        n = int(rhs.numel())
+        if active_hessian is None:
+            self._solution = dev.zeros(n, "float64")
+            self._statistics = {"solver": "mas", "converged": True, "iterations": 0,
+                                "solve_seconds": 0.0}
+            return 0
+        if n != int(active_hessian.rows):
+            raise ValueError("masSolver right-hand side size must match the matrix size")
+        guess = None
+        if not zero_initial_guess and initial_guess is not None:
+            guess = initial_guess.value if hasattr(initial_guess, "value") else initial_guess
+            if not hasattr(guess, "is_cuda") or int(guess.numel()) != n:
+                raise ValueError("masSolver initial guess must be a matching device array")
+        h = active_hessian
+        rebuilt = self._ensure(h, n)
+        t1 = time.perf_counter()
+        m = self._device
+        m.update(h.full)
+        t2 = time.perf_counter()
+        lib = _lib.load()
+        stats = _lib.PcgStats()
+        f = h.full
+        rc = lib.ysp_pcg3_mas_solve(
+            int(maxIterations), float(tolerance), _MODES[self.threshold_mode],
+            dev.ptr(f.row_ptr), dev.ptr(f.col), dev.ptr(f.v64), 0, f.nnz, C.byref(m.args),
+            dev.ptr(rhs), n, dev.ptr(self._solution), dev.ptr(guess), int(guess is None),
+            dev.ptr(self._work), dev.ptr(self._scalars), C.byref(stats), dev.stream())
+        _lib.check(rc, "ysp_pcg3_mas_solve")
+        inverse_status = int(m.status.item())
+        t3 = time.perf_counter()
+        self.iterations = max(0, int(stats.iterations))
+        lay = m.layout
+        converged = rc >= 0
+        self._statistics = {
+            "solver": "mas", "converged": converged, "iterations": int(stats.iterations),
+            "result": 0 if converged else -4, "pcg_result": int(rc),
+            "delta0": stats.delta0, "delta": stats.delta, "rr0": stats.rr0, "rr": stats.rr,
+            "hierarchy_build_count": self.hierarchy_build_count,
+            "hierarchy_build_seconds": getattr(self, "_hierarchy_seconds", 0.0),
+            "layout_rebuilt": rebuilt,
+            "nodes_per_level": [lv.num_nodes for lv in lay.hierarchy.levels],
+            "domains_per_level": [len(lv.domains) for lv in lay.hierarchy.levels],
+            "inverse_status": inverse_status,
+            "numeric_update_seconds": t2 - t1, "pcg_seconds": t3 - t2,
+            "solve_seconds": t3 - t0, "matrix_size": n, "tolerance": float(tolerance),
+        }
+        return 0 if converged else -4
