@@ -265,8 +265,7 @@ int ysp_mas_invert_domains(double* domains, const uint32_t* domain_sizes, uint32
 
 typedef struct ysp_mas_args {
   const uint32_t* restrict_offsets;     /* [num_coarse_nodes + 1] */
-  const uint32_t* restrict_items;       /* children of every coarse node: level-0 nodes
-                                           for level 1, coarse ids of level l-1 for l > 1 */
+  const uint32_t* restrict_items;       /* level-0 descendants of every coarse node */
   uint32_t num_coarse_nodes;            /* coarse levels concatenated */
   uint32_t num_coarse_levels;           /* <= YSP_MAS_MAX_COARSE_LEVELS */
   uint32_t level_offsets[YSP_MAS_MAX_COARSE_LEVELS + 1];  /* coarse ids of level l + 1:

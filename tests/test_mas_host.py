@@ -138,7 +138,7 @@ def emulate_device_apply(layout, block_values, r, weights):
     for l in range(layout.num_coarse_levels):
         for J in range(layout.level_offsets[l], layout.level_offsets[l + 1]):
             kids = layout.restrict_items[layout.restrict_offsets[J]:layout.restrict_offsets[J + 1]]
-            rc[J] = (r3 if l == 0 else rc)[kids].sum(0)
+            rc[J] = r3[kids].sum(0)
     c0 = np.zeros_like(r3)
     cc = np.zeros((nc, 3))
     for d in range(layout.num_domains):

@@ -13,6 +13,8 @@
 #include "dense.hpp"
 #include "mas.hpp"
 
+#include <cmath>
+
 #include <hipcub/hipcub.hpp>
 #include <algorithm>
 
@@ -89,30 +91,18 @@ __global__ void gather_values_kernel(const double* __restrict__ sv, const double
 
 constexpr int kG = 16;  // lanes per block row
 
-// One 16-lane group per block row. x_c is p_prev[c] * beta + z[c] when z != nullptr
-// (the CG direction update fused into the gather), x[c] otherwise.
-template <typename VT, bool kFused>
+// One 16-lane group per block row: (y0, y1, y2) = sum over the row's blocks of B x_c.
+template <typename VT>
 __device__ __forceinline__ void row_product(const uint32_t* __restrict__ row_ptr,
                                             const uint32_t* __restrict__ col,
                                             const VT* __restrict__ v, uint32_t stride,
-                                            const double* __restrict__ x,
-                                            const double* __restrict__ z, double beta,
-                                            uint32_t r, int l, double& y0, double& y1,
-                                            double& y2) {
+                                            const double* __restrict__ x, uint32_t r, int l,
+                                            double& y0, double& y1, double& y2) {
   y0 = y1 = y2 = 0.0;
   const uint32_t k0 = row_ptr[r], k1 = row_ptr[r + 1];
   for (uint32_t k = k0 + l; k < k1; k += kG) {
     const uint32_t c = 3 * col[k];
-    double x0, x1, x2;
-    if (kFused) {
-      x0 = fma(x[c], beta, z[c]);
-      x1 = fma(x[c + 1], beta, z[c + 1]);
-      x2 = fma(x[c + 2], beta, z[c + 2]);
-    } else {
-      x0 = x[c];
-      x1 = x[c + 1];
-      x2 = x[c + 2];
-    }
+    const double x0 = x[c], x1 = x[c + 1], x2 = x[c + 2];
     const double b0 = static_cast<double>(v[0 * static_cast<size_t>(stride) + k]);
     const double b1 = static_cast<double>(v[1 * static_cast<size_t>(stride) + k]);
     const double b2 = static_cast<double>(v[2 * static_cast<size_t>(stride) + k]);
@@ -144,7 +134,7 @@ __global__ void __launch_bounds__(kB)
   const int l = threadIdx.x % kG;
   if (r >= nbr) return;
   double y0, y1, y2;
-  row_product<VT, false>(row_ptr, col, v, stride, x, nullptr, 0.0, r, l, y0, y1, y2);
+  row_product<VT>(row_ptr, col, v, stride, x, r, l, y0, y1, y2);
   if (l == 0) {
     y[3 * r] = y0;
     y[3 * r + 1] = y1;
@@ -156,16 +146,19 @@ __global__ void __launch_bounds__(kB)
 // Scalar state: a header of named doubles, then the partial-sum slots. Each workgroup
 // adds its partial dot into one of kSlots slots (blockIdx mod kSlots) spaced one cache
 // line apart, so the device-scope FP64 atomics of thousands of workgroups spread over
-// 64 addresses instead of serialising on one. Consumers reduce the 64 slots with one
-// wavefront load and a fixed shuffle tree, so every workgroup sees the same sum.
+// 64 addresses instead of serialising on one. Every set has its own 64 lines: a line
+// that takes atomics in a kernel is never one that the same kernel's workgroups read
+// (sharing lines across sets measured 25% slower). Consumers reduce the 64 slots with
+// one wavefront load and a fixed shuffle tree, so every workgroup sees the same sum.
 enum : int {
   S_RZ0 = 0, S_BB = 1, S_STATUS = 2, S_ITER = 3, S_THR = 4, S_MODE = 5,
-  S_FINAL_RZ = 17, S_FINAL_RR = 18, S_HEADER = 64
+  S_FINAL_RZ = 17, S_FINAL_RR = 18, S_CUR_ITER = 19, S_HEADER = 64
 };
 constexpr int kSlots = 64;
 constexpr int kSpace = 16;                       // doubles between slots (128 B)
 constexpr int kSet = kSlots * kSpace;            // one partial-sum array
 enum : int { SET_PAP = 0, SET_RZ = 3, SET_RR = 6, SET_RZ0 = 9, SET_BB = 10, SET_COUNT = 11 };
+constexpr int kStateDoubles = S_HEADER + SET_COUNT * kSet;
 
 __host__ __device__ constexpr int slots_base(int set) { return S_HEADER + set * kSet; }
 
@@ -184,6 +177,23 @@ __device__ __forceinline__ double read_slots(const double* st, int set, double* 
   const double r = lds[0];
   __syncthreads();
   return r;
+}
+
+// Three slot sums read by three wavefronts at once (one barrier instead of six).
+__device__ __forceinline__ void read_slots3(const double* st, int set_a, int set_b, int set_c,
+                                            double* lds, double& a, double& b, double& c) {
+  const int w = threadIdx.x / 64;
+  if (w < 3) {
+    const int set = w == 0 ? set_a : (w == 1 ? set_b : set_c);
+    double v = st[slots_base(set) + (threadIdx.x % 64) * kSpace];
+    v = wave_sum(v);
+    if (threadIdx.x % 64 == 0) lds[w] = v;
+  }
+  __syncthreads();
+  a = lds[0];
+  b = lds[1];
+  c = lds[2];
+  __syncthreads();
 }
 
 // Status read once per workgroup so that every thread takes the same early exit.
@@ -213,15 +223,14 @@ __device__ __forceinline__ void apply3(const MT* __restrict__ minv, uint32_t v, 
            static_cast<double>(M[3 * i + 2]) * r[2];
 }
 
-// r = b (or b - A x0 held in ax0), x = x0 or 0, p buffers = 0 and, with the block-Jacobi
+// r = b (or b - A x0 held in ax0), x = x0 or 0 and, with the block-Jacobi
 // preconditioner (minv != nullptr), z = M^-1 r and the dots r.z, b.M^-1 b.
 template <typename MT>
 __global__ void __launch_bounds__(kB)
     pcg3_init_kernel(const MT* __restrict__ minv, const double* __restrict__ b,
                      const double* __restrict__ x0, const double* __restrict__ ax0,
                      double* __restrict__ x, double* __restrict__ r, double* __restrict__ z,
-                     double* __restrict__ pa, double* __restrict__ pb, uint32_t nv,
-                     double* __restrict__ st) {
+                     uint32_t nv, double* __restrict__ st) {
   const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
   double rz = 0, rr = 0, bmb = 0, bb = 0;
   if (v < nv) {
@@ -232,8 +241,6 @@ __global__ void __launch_bounds__(kB)
       bl[i] = b[d];
       rl[i] = ax0 ? bl[i] - ax0[d] : bl[i];
       x[d] = x0 ? x0[d] : 0.0;
-      pa[d] = 0.0;
-      pb[d] = 0.0;
       r[d] = rl[i];
       rr += rl[i] * rl[i];
       bb += bl[i] * bl[i];
@@ -287,21 +294,22 @@ __global__ void __launch_bounds__(64) pcg3_init_finish_kernel(double* st, double
   if ((mode == 0 ? rz : rr) <= thr) st[S_STATUS] = 1.0;
 }
 
-// Iteration k, kernel 1: convergence test of iteration k-1, beta, p_k = z + beta p_{k-1}
-// (fused into the gather), q = A p_k, p_k^T A p_k.
+// Iteration k, kernel 1: convergence test of iteration k-1, beta, then
+//   p_k = z + beta p_{k-1},  q_k = A p_k = A z + beta q_{k-1},  p_k^T q_k.
+// Only z is gathered (one neighbour gather per block); p and q are updated in place per
+// row, so the product costs one pass over the blocks like a plain SpMV.
 template <typename VT>
 __global__ void __launch_bounds__(kB)
     pcg3_spmv_kernel(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
                      const VT* __restrict__ v, uint32_t stride, uint32_t nbr,
-                     const double* __restrict__ z, const double* __restrict__ p_prev,
-                     double* __restrict__ p_new, double* __restrict__ q, double* __restrict__ st,
-                     int k) {
+                     const double* __restrict__ z, double* __restrict__ p,
+                     double* __restrict__ q, double* __restrict__ st, int k,
+                     uint32_t rows_per_wg) {
   if (stopped(st)) return;
   const int cur = k % 3, prev = (k + 2) % 3, prev2 = (k + 1) % 3;
   __shared__ double lds[kB / 64];
-  const double rz1 = read_slots(st, SET_RZ + prev, lds);
-  const double rr1 = read_slots(st, SET_RR + prev, lds);
-  const double rz2 = read_slots(st, SET_RZ + prev2, lds);
+  double rz1, rr1, rz2;
+  read_slots3(st, SET_RZ + prev, SET_RR + prev, SET_RZ + prev2, lds, rz1, rr1, rz2);
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
   double beta = 0.0;
   if (k > 1) {
@@ -326,44 +334,50 @@ __global__ void __launch_bounds__(kB)
   if (lead) {
     st[S_FINAL_RZ] = rz1;
     st[S_FINAL_RR] = rr1;
+    st[S_CUR_ITER] = k - 1;
   }
+  // Each workgroup walks a contiguous chunk of rows_per_wg block rows (16 at a time), so
+  // the grid stays near a few thousand workgroups: fewer partial-sum atomics and fewer
+  // slot reads by the consumers than one workgroup per 16 rows.
   const uint32_t blk = xcd_swizzle(blockIdx.x, gridDim.x);
-  const uint32_t r = blk * (kB / kG) + threadIdx.x / kG;
+  const uint32_t r_end = min(nbr, (blk + 1) * rows_per_wg);
   const int l = threadIdx.x % kG;
   double pap = 0.0;
-  if (r < nbr) {
+  for (uint32_t r = blk * rows_per_wg + threadIdx.x / kG; r < r_end; r += kB / kG) {
     double y0, y1, y2;
-    row_product<VT, true>(row_ptr, col, v, stride, p_prev, z, beta, r, l, y0, y1, y2);
-    if (l == 0) {
-      const double p0 = fma(p_prev[3 * r], beta, z[3 * r]);
-      const double p1 = fma(p_prev[3 * r + 1], beta, z[3 * r + 1]);
-      const double p2 = fma(p_prev[3 * r + 2], beta, z[3 * r + 2]);
-      p_new[3 * r] = p0;
-      p_new[3 * r + 1] = p1;
-      p_new[3 * r + 2] = p2;
-      q[3 * r] = y0;
-      q[3 * r + 1] = y1;
-      q[3 * r + 2] = y2;
-      pap = p0 * y0 + p1 * y1 + p2 * y2;
+    row_product<VT>(row_ptr, col, v, stride, z, r, l, y0, y1, y2);
+    if (l < 3) {                               // lane l owns component l of the row
+      const size_t d = 3 * static_cast<size_t>(r) + l;
+      double pi = z[d], qi = l == 0 ? y0 : (l == 1 ? y1 : y2);
+      if (k > 1) {                             // first direction: p = z, q = A z
+        pi = fma(p[d], beta, pi);
+        qi = fma(q[d], beta, qi);
+      }
+      p[d] = pi;
+      q[d] = qi;
+      pap += pi * qi;
     }
   }
   wg_add(pap, slot_ptr(st, SET_PAP + cur), lds);
 }
 
 // Iteration k, kernel 2: alpha, x += alpha p, r -= alpha q, z = M^-1 r, r.z and r.r.
+// One thread per scalar (coalesced vector traffic); a workgroup of 192 threads holds 64
+// whole vertices, whose residual 3-vectors meet in LDS for the block-Jacobi product.
+constexpr unsigned kUB = 192;
 template <typename MT>
-__global__ void __launch_bounds__(kB)
+__global__ void __launch_bounds__(kUB)
     pcg3_update_kernel(const MT* __restrict__ minv, double* __restrict__ x, double* __restrict__ r,
                        const double* __restrict__ p, const double* __restrict__ q,
-                       double* __restrict__ z, uint32_t nv, double* __restrict__ st, int k) {
+                       double* __restrict__ z, uint32_t n, double* __restrict__ st, int k) {
   if (stopped(st)) return;
   const int cur = k % 3, prev = (k + 2) % 3, next = (k + 1) % 3;
-  __shared__ double lds[kB / 64];
-  const double pap = read_slots(st, SET_PAP + cur, lds);
-  const double rz1 = read_slots(st, SET_RZ + prev, lds);
-  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double lds[4];
+  double pap, rz1, unused;
+  read_slots3(st, SET_PAP + cur, SET_RZ + prev, SET_RZ + prev, lds, pap, rz1, unused);
+  const uint32_t t = blockIdx.x * kUB + threadIdx.x;
   if (!isfinite(pap) || pap <= 0.0) {
-    if (v == 0) {
+    if (t == 0) {
       st[S_STATUS] = -1.0;
       st[S_ITER] = k;
     }
@@ -375,37 +389,62 @@ __global__ void __launch_bounds__(kB)
     zero_set(st, SET_RZ + next);
     zero_set(st, SET_RR + next);
   }
-  double rz = 0.0, rr = 0.0;
-  if (v < nv) {
-    double rl[3], zl[3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      const uint32_t d = 3 * v + i;
-      x[d] = fma(p[d], alpha, x[d]);
-      rl[i] = fma(-q[d], alpha, r[d]);
-      r[d] = rl[i];
-    }
-    if (minv) {
-      apply3(minv, v, rl, zl);
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        z[3 * v + i] = zl[i];
-        rz += rl[i] * zl[i];
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < 3; ++i) rr += rl[i] * rl[i];
+  __shared__ double rs[kUB];
+  double rl = 0.0, zl = 0.0;
+  if (t < n) {
+    x[t] = fma(p[t], alpha, x[t]);
+    rl = fma(-q[t], alpha, r[t]);
+    r[t] = rl;
   }
-  if (minv) wg_add(rz, slot_ptr(st, SET_RZ + cur), lds);
-  wg_add(rr, slot_ptr(st, SET_RR + cur), lds);
+  if (minv) {
+    rs[threadIdx.x] = rl;
+    __syncthreads();
+    if (t < n) {
+      const uint32_t c = threadIdx.x % 3, base = threadIdx.x - c;
+      const MT* M = minv + 3 * static_cast<size_t>(t);    // row c of vertex t / 3
+      zl = static_cast<double>(M[0]) * rs[base] + static_cast<double>(M[1]) * rs[base + 1] +
+           static_cast<double>(M[2]) * rs[base + 2];
+      z[t] = zl;
+    }
+  }
+  // both partial sums in one block reduction
+  double rz = wave_sum(rl * zl), rr = wave_sum(rl * rl);
+  __shared__ double part[2][kUB / 64];
+  if (threadIdx.x % 64 == 0) {
+    part[0][threadIdx.x / 64] = rz;
+    part[1][threadIdx.x / 64] = rr;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    rz = part[0][0] + part[0][1] + part[0][2];
+    rr = part[1][0] + part[1][1] + part[1][2];
+    if (minv) atomic_add_f64(slot_ptr(st, SET_RZ + cur), rz);
+    atomic_add_f64(slot_ptr(st, SET_RR + cur), rr);
+  }
 }
 
+// Pinned ring of header copies, one per chunk of iterations in flight.
+constexpr int kRing = 4;
+constexpr int kBoxDoubles = 32;                  // header entries copied back per chunk
 double* status_box() {
   static double* box = nullptr;
-  if (!box && hipHostMalloc(reinterpret_cast<void**>(&box), 8 * sizeof(double),
+  if (!box && hipHostMalloc(reinterpret_cast<void**>(&box), kRing * kBoxDoubles * sizeof(double),
                             hipHostMallocDefault) != hipSuccess)
     box = nullptr;
   return box;
+}
+
+// Iterations still to enqueue so that the convergence test of the predicted last
+// iteration runs, from two observations of the tested quantity (geometric rate);
+// -1 when no prediction is possible.
+long predict_more(double it_a, double val_a, double it_b, double val_b, double thr,
+                  uint32_t enqueued) {
+  if (!(it_b > it_a) || !(val_b > 0.0) || !(val_a > val_b) || !(thr > 0.0)) return -1;
+  const double rate = (std::log(val_b) - std::log(val_a)) / (it_b - it_a);
+  if (!(rate < 0.0)) return -1;
+  const double left = std::max(0.0, (std::log(thr) - std::log(val_b)) / rate);
+  const double detect = it_b + std::ceil(left) + 1.0;   // spmv pass that sees convergence
+  return static_cast<long>(detect - static_cast<double>(enqueued));
 }
 
 template <typename VT>
@@ -538,17 +577,21 @@ int pcg3_run(uint32_t max_iterations, double threshold, int32_t mode, const uint
   double* r = work;
   double* z = work + n;
   double* pa = work + 2 * static_cast<size_t>(n);
-  double* pb = work + 3 * static_cast<size_t>(n);
   double* q = work + 4 * static_cast<size_t>(n);
   const uint32_t stride = pad64(nnz);
-  YSP_CHECK(hipMemsetAsync(st, 0, slots_base(SET_COUNT) * sizeof(double), s));
+  YSP_CHECK(hipMemsetAsync(st, 0, kStateDoubles * sizeof(double), s));
   const double* ax0 = nullptr;
   if (x0) {
     launch_full_spmv(row_ptr, col, v, nnz, nv, x0, q, s);
     ax0 = q;
   }
-  const dim3 vg(grid_for(nv, kB)), sg(grid_for(nv, kB / kG)), bl(kB);
-  hipLaunchKernelGGL(pcg3_init_kernel<MT>, vg, bl, 0, s, minv, rhs, x0, ax0, x, r, z, pa, pb, nv, st);
+  // product grid: contiguous chunks of block rows, a multiple of 16 rows per workgroup
+  constexpr uint32_t kSpmvWgs = 2048;
+  const uint32_t rows_per_wg =
+      std::max<uint32_t>(kB / kG, (grid_for(nv, kSpmvWgs) + kB / kG - 1) / (kB / kG) * (kB / kG));
+  static_assert(kStateDoubles <= YSP_PCG3_SCALARS, "cg_scalars too small");
+  const dim3 vg(grid_for(nv, kB)), sg(grid_for(nv, rows_per_wg)), ug(grid_for(n, kUB)), bl(kB);
+  hipLaunchKernelGGL(pcg3_init_kernel<MT>, vg, bl, 0, s, minv, rhs, x0, ax0, x, r, z, nv, st);
   if (mas) {
     if (x0) {
       if (int rc = launch_mas_apply(*mas, rhs, n, z, st + slots_base(SET_RZ0), nullptr, s))
@@ -565,21 +608,30 @@ int pcg3_run(uint32_t max_iterations, double threshold, int32_t mode, const uint
     ysp_set_error(__FILE__, __LINE__, "pinned status allocation failed");
     return YSP_ERR_HIP;
   }
-  YSP_CHECK(hipMemcpyAsync(box, st, 8 * sizeof(double), hipMemcpyDeviceToHost, s));
-  YSP_CHECK(hipStreamSynchronize(s));
-  uint32_t k = 1;
-  uint32_t chunk = 8;
-  // one extra pass lets the spmv kernel of iteration max+1 record convergence of max
-  while (box[S_STATUS] == 0.0 && k <= max_iterations + 1) {
-    const uint32_t last = std::min<uint32_t>(max_iterations + 1, k + chunk - 1);
-    for (uint32_t it = k; it <= last; ++it) {
-      double* pn = (it & 1) ? pa : pb;
-      double* pp = (it & 1) ? pb : pa;
-      hipLaunchKernelGGL(pcg3_spmv_kernel<VT>, sg, bl, 0, s, row_ptr, col, v, stride, nv, z, pp, pn,
-                         q, st, static_cast<int>(it));
+  hipEvent_t ev[kRing];
+  for (int i = 0; i < kRing; ++i)
+    YSP_CHECK(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
+  struct EventGuard {
+    hipEvent_t* e;
+    ~EventGuard() {
+      for (int i = 0; i < kRing; ++i) (void)hipEventDestroy(e[i]);
+    }
+  } guard{ev};
+  // Chunks of iterations are enqueued up to two ahead of the one the host waits for, so
+  // the device never waits for the host; chunk sizes shrink as the observed convergence
+  // rate predicts the last iteration, which bounds the no-op passes after convergence.
+  // The extra spmv pass max+1 records convergence of iteration max.
+  const uint32_t end = max_iterations + 1;
+  uint32_t next = 1;
+  int head = 0, pending = 0;
+  auto enqueue = [&](uint32_t size) -> int {
+    const uint32_t last = std::min<uint32_t>(end, next + size - 1);
+    for (uint32_t it = next; it <= last; ++it) {
+      hipLaunchKernelGGL(pcg3_spmv_kernel<VT>, sg, bl, 0, s, row_ptr, col, v, stride, nv, z, pa, q,
+                         st, static_cast<int>(it), rows_per_wg);
       if (it <= max_iterations) {
-        hipLaunchKernelGGL(pcg3_update_kernel<MT>, vg, bl, 0, s, minv, x, r, pn, q, z, nv, st,
-                           static_cast<int>(it));
+        hipLaunchKernelGGL(pcg3_update_kernel<MT>, ug, dim3(kUB), 0, s, minv, x, r, pa, q, z, n,
+                           st, static_cast<int>(it));
         if (mas) {
           if (int rc = launch_mas_apply(*mas, r, n, z, st + slots_base(SET_RZ + (it % 3)),
                                         st + S_STATUS, s))
@@ -588,10 +640,43 @@ int pcg3_run(uint32_t max_iterations, double threshold, int32_t mode, const uint
       }
     }
     YSP_CHECK_LAUNCH();
-    YSP_CHECK(hipMemcpyAsync(box, st, 8 * sizeof(double), hipMemcpyDeviceToHost, s));
-    YSP_CHECK(hipStreamSynchronize(s));
-    k = last + 1;
-    chunk = std::min<uint32_t>(chunk * 2, 64);
+    const int slot = (head + pending) % kRing;
+    YSP_CHECK(hipMemcpyAsync(box + slot * kBoxDoubles, st, kBoxDoubles * sizeof(double),
+                             hipMemcpyDeviceToHost, s));
+    YSP_CHECK(hipEventRecord(ev[slot], s));
+    ++pending;
+    next = last + 1;
+    return YSP_OK;
+  };
+  for (int i = 0; i < 2 && next <= end; ++i)
+    if (int rc = enqueue(8)) return rc;
+  double it_a = -1.0, val_a = 0.0;
+  while (pending > 0) {
+    const int slot = head;
+    YSP_CHECK(hipEventSynchronize(ev[slot]));
+    head = (head + 1) % kRing;
+    --pending;
+    const double* hb = box + slot * kBoxDoubles;
+    if (hb[S_STATUS] != 0.0) break;
+    if (next > end) continue;
+    const double it_b = hb[S_CUR_ITER];
+    const double val_b = hb[S_MODE] == 0.0 ? hb[S_FINAL_RZ] : hb[S_FINAL_RR];
+    const long more = predict_more(it_a, val_a, it_b, val_b, hb[S_THR], next - 1);
+    if (it_b > it_a) {
+      it_a = it_b;
+      val_a = val_b;
+    }
+    if (more < 0) {                        // no prediction yet: keep two chunks of 8 queued
+      if (int rc = enqueue(8)) return rc;
+      if (pending < 2 && next <= end)
+        if (int rc = enqueue(8)) return rc;
+      continue;
+    }
+    if (more == 0 && pending > 0) continue;  // the queued passes should reach convergence
+    const uint32_t first = static_cast<uint32_t>(std::min<long>(16, std::max<long>(1, more)));
+    if (int rc = enqueue(first)) return rc;
+    if (more > 16 && pending < 2 && next <= end)
+      if (int rc = enqueue(static_cast<uint32_t>(std::min<long>(16, more - 16)))) return rc;
   }
   double fin[S_HEADER];
   YSP_CHECK(hipMemcpyAsync(fin, st, sizeof(fin), hipMemcpyDeviceToHost, s));

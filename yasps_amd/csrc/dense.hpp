@@ -1,5 +1,5 @@
-// Small dense FP64 linear algebra held in registers: the packed-Schur PSD test and a
-// cyclic Jacobi symmetric eigensolver (Golub & Van Loan, "Matrix Computations",
+// Small dense FP64 linear algebra held in registers: the packed-Schur PSD test and
+// cyclic Jacobi symmetric eigensolvers (Golub & Van Loan, "Matrix Computations",
 // section 8.5: symmetric Schur 2x2 rotations swept row by row).
 #pragma once
 #include "common.hpp"
@@ -86,6 +86,73 @@ __device__ __forceinline__ void jacobi_eigen(double (&A)[N][N], double (&V)[N][N
           const double vkp = V[k][p], vkq = V[k][q];
           V[k][p] = c * vkp - s * vkq;
           V[k][q] = s * vkp + c * vkq;
+        }
+      }
+  }
+}
+
+// Index of (p, q), p <= q, in the packed upper triangle of an N x N symmetric matrix.
+template <int N>
+__host__ __device__ constexpr int upper_index(int p, int q) {
+  return p * N - p * (p - 1) / 2 + (q - p);
+}
+
+// Cyclic Jacobi on the packed upper triangle a (N(N+1)/2 entries) of a symmetric matrix,
+// with the symmetric two-sided update of each rotation (Golub & Van Loan, section 8.5,
+// in Rutishauser's tau form): only the 2(N-2) entries of rows/columns p and q and the
+// three entries of the 2x2 pivot block change. On return the diagonal holds the
+// eigenvalues and V's columns the eigenvectors.
+template <int N>
+__device__ __forceinline__ void jacobi_eigen_packed(double (&a)[N * (N + 1) / 2],
+                                                    double (&V)[N][N]) {
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = 0; j < N; ++j) V[i][j] = (i == j) ? 1.0 : 0.0;
+  double diag = 0.0, off = 0.0;
+#pragma unroll
+  for (int p = 0; p < N; ++p) {
+    diag += a[upper_index<N>(p, p)] * a[upper_index<N>(p, p)];
+#pragma unroll
+    for (int q = p + 1; q < N; ++q) off += a[upper_index<N>(p, q)] * a[upper_index<N>(p, q)];
+  }
+  const double stop = (diag + 2.0 * off) * 1.0e-32;
+  for (int sweep = 0; sweep < 24; ++sweep) {
+    off = 0.0;
+#pragma unroll
+    for (int p = 0; p < N; ++p)
+#pragma unroll
+      for (int q = p + 1; q < N; ++q) off += a[upper_index<N>(p, q)] * a[upper_index<N>(p, q)];
+    if (off <= stop) break;
+#pragma unroll
+    for (int p = 0; p < N; ++p)
+#pragma unroll
+      for (int q = p + 1; q < N; ++q) {
+        const double apq = a[upper_index<N>(p, q)];
+        if (apq == 0.0) continue;
+        const double app = a[upper_index<N>(p, p)], aqq = a[upper_index<N>(q, q)];
+        const double theta = (aqq - app) / (2.0 * apq);
+        const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(1.0 + theta * theta));
+        const double c = 1.0 / sqrt(1.0 + t * t);
+        const double s = t * c;
+        const double tau = s / (1.0 + c);
+        a[upper_index<N>(p, p)] = app - t * apq;
+        a[upper_index<N>(q, q)] = aqq + t * apq;
+        a[upper_index<N>(p, q)] = 0.0;
+#pragma unroll
+        for (int r = 0; r < N; ++r) {
+          if (r == p || r == q) continue;
+          const int ip = r < p ? upper_index<N>(r, p) : upper_index<N>(p, r);
+          const int iq = r < q ? upper_index<N>(r, q) : upper_index<N>(q, r);
+          const double arp = a[ip], arq = a[iq];
+          a[ip] = arp - s * (arq + tau * arp);
+          a[iq] = arq + s * (arp - tau * arq);
+        }
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          const double vkp = V[k][p], vkq = V[k][q];
+          V[k][p] = vkp - s * (vkq + tau * vkp);
+          V[k][q] = vkq + s * (vkp - tau * vkq);
         }
       }
   }

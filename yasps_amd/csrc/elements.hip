@@ -490,7 +490,8 @@ __global__ void __launch_bounds__(64)
       cw[k][j] = kU[0][k] * S.w[0][j] + kU[1][k] * S.w[1][j] + kU[2][k] * S.w[2][j] +
                  kU[3][k] * S.w[3][j];
     }
-  double M[9][9];
+  constexpr int kP = upper_index<9>(8, 8) + 1;   // 45 packed entries
+  double M[kP];
 #pragma unroll
   for (int k = 0; k < 3; ++k)
 #pragma unroll
@@ -500,21 +501,19 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
       for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) {
-          M[3 * k + i][3 * l + j] = blk[i][j];
-          M[3 * l + j][3 * k + i] = blk[i][j];
-        }
+        for (int j = 0; j < 3; ++j)
+          if (k < l || i <= j) M[upper_index<9>(3 * k + i, 3 * l + j)] = blk[i][j];
     }
   double V[9][9];
-  jacobi_eigen<9>(M, V);
+  jacobi_eigen_packed<9>(M, V);
   double lam_c[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) {
-    const double l = M[i][i];
+    const double l = M[upper_index<9>(i, i)];
     lam_c[i] = (l < 0.0) ? (prm.method == 1 ? -l : 0.0) : l;
   }
-  // projected M' = V diag(lam_c) V^T
-  double Mp[9][9];
+  // projected M' = V diag(lam_c) V^T (packed upper)
+  double Mp[kP];
 #pragma unroll
   for (int i = 0; i < 9; ++i)
 #pragma unroll
@@ -522,8 +521,7 @@ __global__ void __launch_bounds__(64)
       double acc = 0.0;
 #pragma unroll
       for (int k = 0; k < 9; ++k) acc += V[i][k] * lam_c[k] * V[j][k];
-      Mp[i][j] = acc;
-      Mp[j][i] = acc;
+      Mp[upper_index<9>(i, j)] = acc;
     }
   SlotInfo<KT> si;
   load_slots<KT>(SA, e, si);
@@ -536,7 +534,10 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
-          for (int j = 0; j < 3; ++j) out[i][j] += w * Mp[3 * k + i][3 * l + j];
+          for (int j = 0; j < 3; ++j) {
+            const int r = 3 * k + i, c = 3 * l + j;
+            out[i][j] += w * Mp[r <= c ? upper_index<9>(r, c) : upper_index<9>(c, r)];
+          }
       }
   });
 }

@@ -12,9 +12,10 @@
 //     solver/mas/cuda/local_inverse.cu:244-308). A pivot below 1e-12 * max|diag| marks the
 //     domain as failed.
 // Per application z = sum_l w_l P_l B_l^-1 R_l r:
-//   restrict level by level (every coarse node sums its children on the level below), one
-//   32-lane group per domain multiplies its inverse with the local residual, prolong adds
-//   the weighted corrections of every level and accumulates r.z.
+//   restrict (every coarse node of every level sums the residual of its level-0
+//   descendants, one launch), one 32-lane group per domain multiplies its inverse with the
+//   local residual, prolong adds the weighted corrections of every level and accumulates
+//   r.z.
 #include "mas.hpp"
 
 namespace ysp {
@@ -132,27 +133,72 @@ __device__ __forceinline__ bool mas_stopped(const double* stop) {
   return stop && *stop != 0.0;
 }
 
-// r_l[J] = sum of the residuals of J's children: level-0 nodes of r (kFromFine) or coarse
-// nodes of the level below in rc. One launch per coarse level.
-template <bool kFromFine>
+// Sum of r over the level-0 descendants items[offsets[J] .. offsets[J+1]) of coarse node J,
+// by a group of G lanes (lane strides, then a fixed shuffle tree: deterministic).
+template <int G>
+__device__ __forceinline__ void descendant_sum(const double* __restrict__ r,
+                                               const uint32_t* __restrict__ offsets,
+                                               const uint32_t* __restrict__ items, uint32_t J,
+                                               int lane, double s[3]) {
+  s[0] = s[1] = s[2] = 0.0;
+  for (uint32_t k = offsets[J] + lane; k < offsets[J + 1]; k += G) {
+    const size_t i = 3 * static_cast<size_t>(items[k]);
+    s[0] += r[i];
+    s[1] += r[i + 1];
+    s[2] += r[i + 2];
+  }
+  const int width = G < 64 ? G : 64;
+#pragma unroll
+  for (int o = width / 2; o > 0; o >>= 1) {
+    s[0] += __shfl_xor(s[0], o, width);
+    s[1] += __shfl_xor(s[1], o, width);
+    s[2] += __shfl_xor(s[2], o, width);
+  }
+}
+
+// All coarse residuals in one launch, straight from the level-0 residual: workgroups
+// [0, b1) give 8 lanes to each level-1 node, [b1, b1 + b2) a wavefront to each level-2
+// node, the rest a whole workgroup to each node of the deeper levels (few nodes with
+// many descendants).
 __global__ void __launch_bounds__(kB)
     restrict_kernel(const double* __restrict__ r, const uint32_t* __restrict__ offsets,
-                    const uint32_t* __restrict__ items, uint32_t first, uint32_t last,
-                    double* __restrict__ rc, const double* __restrict__ stop) {
+                    const uint32_t* __restrict__ items, uint32_t n1, uint32_t n2, uint32_t nc,
+                    uint32_t b1, uint32_t b2, double* __restrict__ rc,
+                    const double* __restrict__ stop) {
   if (mas_stopped(stop)) return;
-  const uint32_t J = first + blockIdx.x * blockDim.x + threadIdx.x;
-  if (J >= last) return;
-  const double* src = kFromFine ? r : rc;
-  double s0 = 0, s1 = 0, s2 = 0;
-  for (uint32_t k = offsets[J]; k < offsets[J + 1]; ++k) {
-    const uint32_t i = items[k];
-    s0 += src[3 * static_cast<size_t>(i)];
-    s1 += src[3 * static_cast<size_t>(i) + 1];
-    s2 += src[3 * static_cast<size_t>(i) + 2];
+  double sum[3];
+  uint32_t J;
+  bool write;
+  if (blockIdx.x < b1) {
+    J = blockIdx.x * (kB / 8) + threadIdx.x / 8;
+    if (J >= n1) return;                       // whole 8-lane groups leave together
+    descendant_sum<8>(r, offsets, items, J, threadIdx.x % 8, sum);
+    write = threadIdx.x % 8 == 0;
+  } else if (blockIdx.x < b1 + b2) {
+    J = n1 + (blockIdx.x - b1) * (kB / 64) + threadIdx.x / 64;
+    if (J >= n2) return;                       // whole wavefronts leave together
+    descendant_sum<64>(r, offsets, items, J, threadIdx.x % 64, sum);
+    write = threadIdx.x % 64 == 0;
+  } else {
+    J = n2 + (blockIdx.x - b1 - b2);           // grid sized so that J < nc
+    descendant_sum<kB>(r, offsets, items, J, threadIdx.x, sum);
+    __shared__ double part[3][kB / 64];
+    if (threadIdx.x % 64 == 0)
+      for (int c = 0; c < 3; ++c) part[c][threadIdx.x / 64] = sum[c];
+    __syncthreads();
+    for (int c = 0; c < 3; ++c) {
+      double t = 0.0;
+      for (int w = 0; w < static_cast<int>(kB / 64); ++w) t += part[c][w];
+      sum[c] = t;
+    }
+    write = threadIdx.x == 0;
   }
-  rc[3 * static_cast<size_t>(J)] = s0;
-  rc[3 * static_cast<size_t>(J) + 1] = s1;
-  rc[3 * static_cast<size_t>(J) + 2] = s2;
+  if (write) {
+    rc[3 * static_cast<size_t>(J)] = sum[0];
+    rc[3 * static_cast<size_t>(J) + 1] = sum[1];
+    rc[3 * static_cast<size_t>(J) + 2] = sum[2];
+  }
+  (void)nc;
 }
 
 // one 32-lane group per domain: c_loc = B^-1 r_loc
@@ -273,16 +319,15 @@ namespace ysp {
 int launch_mas_apply(const MasApply& m, const double* r, uint32_t n, double* z,
                      double* rz_slots, const double* stop, hipStream_t s) {
   const uint32_t nv = n / 3;
-  for (uint32_t l = 0; l < m.num_coarse_levels; ++l) {
-    const uint32_t first = m.level_offsets[l], last = m.level_offsets[l + 1];
-    if (last <= first) continue;
-    const dim3 g(grid_for(last - first, kB)), b(kB);
-    if (l == 0)
-      hipLaunchKernelGGL(restrict_kernel<true>, g, b, 0, s, r, m.restrict_offsets,
-                         m.restrict_items, first, last, m.coarse_residual, stop);
-    else
-      hipLaunchKernelGGL(restrict_kernel<false>, g, b, 0, s, r, m.restrict_offsets,
-                         m.restrict_items, first, last, m.coarse_residual, stop);
+  if (m.num_coarse_levels) {
+    const uint32_t nc = m.num_coarse_nodes;
+    const uint32_t n1 = m.level_offsets[1];
+    const uint32_t n2 = m.level_offsets[m.num_coarse_levels >= 2 ? 2 : 1];
+    const uint32_t b1 = grid_for(n1, kB / 8), b2 = grid_for(n2 - n1, kB / 64);
+    const uint32_t blocks = b1 + b2 + (nc - n2);
+    if (blocks)
+      hipLaunchKernelGGL(restrict_kernel, dim3(blocks), dim3(kB), 0, s, r, m.restrict_offsets,
+                         m.restrict_items, n1, n2, nc, b1, b2, m.coarse_residual, stop);
   }
   if (m.num_domains)
     hipLaunchKernelGGL(local_solve_kernel, dim3(grid_for(m.num_domains, kB / kD)), dim3(kB), 0, s,

@@ -8,7 +8,7 @@ constexpr int kMasMaxCoarseLevels = 8;
 
 struct MasApply {
   const uint32_t* restrict_offsets;   // [num_coarse_nodes + 1]
-  const uint32_t* restrict_items;     // children: level-0 nodes (level 1), else coarse ids
+  const uint32_t* restrict_items;     // level-0 descendants of every coarse node
   uint32_t num_coarse_nodes;          // all coarse levels, concatenated ids
   uint32_t num_coarse_levels;
   uint32_t level_offsets[kMasMaxCoarseLevels + 1];  // coarse id range of every level

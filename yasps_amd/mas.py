@@ -121,13 +121,12 @@ def build_layout(h: Hierarchy, row_ptr: np.ndarray, col: np.ndarray) -> MasLayou
     bases[1:] = np.concatenate([[0], np.cumsum(sizes[1:])[:-1]]) if h.num_levels > 1 else []
     level_offsets = np.concatenate([[0], np.cumsum(sizes[1:])]).astype(np.int64)
 
-    # restriction: children of every coarse node on the level below
+    # restriction: level-0 descendants of every coarse node
     r_off, r_items = [np.zeros(1, dtype=np.int64)], []
     for l in range(1, h.num_levels):
-        fine = h.levels[l - 1]
-        off, items = _csr_by_key(fine.fine_to_parent, sizes[l])
+        off, items = _csr_by_key(h.composed[l], sizes[l])
         r_off.append(off[1:] + r_off[-1][-1])
-        r_items.append(items + (bases[l - 1] if l >= 2 else 0))
+        r_items.append(items)
     restrict_offsets = np.concatenate(r_off)
     restrict_items = (np.concatenate(r_items) if r_items else np.zeros(0, dtype=np.int64))
     prolong = (np.stack([bases[l] + h.composed[l] for l in range(1, h.num_levels)])
