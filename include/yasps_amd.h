@@ -340,6 +340,15 @@ int ysp_bsr3_gather_values(const double* static_values, const double* dynamic_va
 /* out[i] = (float) in[i] (FP32 copies of FP64 device arrays, e.g. inverse blocks). */
 int ysp_cast_f32(const double* in, float* out, uint64_t n, void* stream);
 
+/* One step of iterative refinement, no reference counterpart (the mixed-precision form of
+ * the solve): solution += correction (skipped when correction is NULL), then
+ * residual = rhs - H solution over the full block-CSR with FP64 arithmetic, and the
+ * squared norms of residual and rhs returned on the host. Synchronous. */
+int ysp_bsr3_refine(const uint32_t* row_ptr, const uint32_t* col, const void* values,
+                    int32_t values_f32, uint32_t nnz, uint32_t n, const double* rhs,
+                    double* solution, const double* correction, double* residual,
+                    double* residual_sq, double* rhs_sq, void* stream);
+
 /* y = H x over the full block-CSR (values FP64 or FP32, arithmetic FP64). */
 int ysp_bsr3_spmv(const uint32_t* row_ptr, const uint32_t* col, const void* values,
                   int32_t values_f32, uint32_t nnz, uint32_t n, const double* x, double* y,

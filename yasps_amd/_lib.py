@@ -99,6 +99,8 @@ SIGNATURES = {
     "ysp_bsr3_gather_values": (C.c_int, [P, P, P, U32, P, P, P]),
     "ysp_cast_f32": (C.c_int, [P, P, C.c_uint64, P]),
     "ysp_bsr3_spmv": (C.c_int, [P, P, P, I32, U32, U32, P, P, P]),
+    "ysp_bsr3_refine": (C.c_int, [P, P, P, I32, U32, U32, P, P, P, P, C.POINTER(D), C.POINTER(D),
+                                  P]),
     "ysp_pcg3_solve": (C.c_int, [U32, D, I32, P, P, P, I32, U32, P, I32, P, U32, P, P, I32, P, P,
                                  C.POINTER(PcgStats), P]),
     "ysp_mas_coarsen": (C.c_int, [P, I32, U32, P, P, U32, P, P]),

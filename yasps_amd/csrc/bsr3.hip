@@ -568,6 +568,91 @@ extern "C" int ysp_cast_f32(const double* in, float* out, uint64_t n, void* stre
 }
 
 namespace {
+__global__ void __launch_bounds__(kB)
+    axpy_kernel(const double* __restrict__ d, double* __restrict__ x, size_t n) {
+  const size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  if (i < n) x[i] += d[i];
+}
+
+// r = b - A x over the full block-CSR, with ||r||^2 and ||b||^2 partial sums (sums[0..1]).
+template <typename VT>
+__global__ void __launch_bounds__(kB)
+    residual_kernel(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
+                    const VT* __restrict__ v, uint32_t stride, uint32_t nbr,
+                    const double* __restrict__ x, const double* __restrict__ b,
+                    double* __restrict__ r, double* __restrict__ sums) {
+  const uint32_t blk = xcd_swizzle(blockIdx.x, gridDim.x);
+  const uint32_t row = blk * (kB / kG) + threadIdx.x / kG;
+  const int l = threadIdx.x % kG;
+  double rr = 0.0, bb = 0.0;
+  if (row < nbr) {
+    double y0, y1, y2;
+    row_product<VT>(row_ptr, col, v, stride, x, row, l, y0, y1, y2);
+    if (l < 3) {
+      const size_t d = 3 * static_cast<size_t>(row) + l;
+      const double bl = b[d];
+      const double rl = bl - (l == 0 ? y0 : (l == 1 ? y1 : y2));
+      r[d] = rl;
+      rr = rl * rl;
+      bb = bl * bl;
+    }
+  }
+  __shared__ double lds[kB / 64];
+  rr = block_sum<kB>(rr, lds);
+  bb = block_sum<kB>(bb, lds);
+  if (threadIdx.x == 0) {
+    atomic_add_f64(sums + 2 * (blockIdx.x % kSlots) * kSpace, rr);
+    atomic_add_f64(sums + (2 * (blockIdx.x % kSlots) + 1) * kSpace, bb);
+  }
+}
+}  // namespace
+
+extern "C" int ysp_bsr3_refine(const uint32_t* row_ptr, const uint32_t* col, const void* values,
+                               int32_t values_f32, uint32_t nnz, uint32_t n, const double* rhs,
+                               double* solution, const double* correction, double* residual,
+                               double* residual_sq, double* rhs_sq, void* stream) {
+  if (n == 0) {
+    if (residual_sq) *residual_sq = 0.0;
+    if (rhs_sq) *rhs_sq = 0.0;
+    return YSP_OK;
+  }
+  if (n % 3 || !row_ptr || !col || !values || !rhs || !solution || !residual) {
+    ysp_set_error(__FILE__, __LINE__, "ysp_bsr3_refine: bad arguments");
+    return YSP_ERR_ARG;
+  }
+  hipStream_t s = as_stream(stream);
+  const size_t nsum = 2 * kSlots * kSpace;
+  double* sums = static_cast<double*>(scratch(33, nsum * sizeof(double)));
+  if (!sums) return YSP_ERR_HIP;
+  YSP_CHECK(hipMemsetAsync(sums, 0, nsum * sizeof(double), s));
+  if (correction)
+    hipLaunchKernelGGL(axpy_kernel, dim3(grid_for(n, kB)), dim3(kB), 0, s, correction, solution,
+                       static_cast<size_t>(n));
+  const uint32_t nbr = n / 3;
+  const dim3 g(grid_for(nbr, kB / kG)), b(kB);
+  if (values_f32)
+    hipLaunchKernelGGL(residual_kernel<float>, g, b, 0, s, row_ptr, col,
+                       static_cast<const float*>(values), pad64(nnz), nbr, solution, rhs, residual,
+                       sums);
+  else
+    hipLaunchKernelGGL(residual_kernel<double>, g, b, 0, s, row_ptr, col,
+                       static_cast<const double*>(values), pad64(nnz), nbr, solution, rhs,
+                       residual, sums);
+  YSP_CHECK_LAUNCH();
+  double h[nsum];
+  YSP_CHECK(hipMemcpyAsync(h, sums, sizeof(h), hipMemcpyDeviceToHost, s));
+  YSP_CHECK(hipStreamSynchronize(s));
+  double rr = 0.0, bb = 0.0;
+  for (int i = 0; i < kSlots; ++i) {
+    rr += h[2 * i * kSpace];
+    bb += h[(2 * i + 1) * kSpace];
+  }
+  if (residual_sq) *residual_sq = rr;
+  if (rhs_sq) *rhs_sq = bb;
+  return YSP_OK;
+}
+
+namespace {
 template <typename VT, typename MT>
 int pcg3_run(uint32_t max_iterations, double threshold, int32_t mode, const uint32_t* row_ptr,
              const uint32_t* col, const VT* v, uint32_t nnz, const MT* minv, const MasApply* mas,

@@ -185,6 +185,7 @@ class FullBsr:
     nnz: int
     v64: object = None
     v32: object = None
+    v32_valid: bool = False      # v32 holds the current values
 
 
 class hessian:
@@ -446,6 +447,23 @@ class hessian:
                                               dev.ptr(f.src), f.nnz, dev.ptr(f.v64),
                                               dev.ptr(f.v32) if self.want_f32 else None, s),
                    "ysp_bsr3_gather_values")
+        f.v32_valid = bool(self.want_f32)
+
+    def full_values_f32(self):
+        """FP32 copy of the full block-CSR values (gathered with the FP64 values when
+        ``want_f32`` is set, else cast on demand)."""
+        f = self.full
+        if f is None:
+            raise NotImplementedError("FP32 block values need 3 x 3 variable blocks")
+        stride = (f.nnz + 63) // 64 * 64
+        if f.v32 is None or f.v32.numel() != 9 * stride:
+            f.v32 = dev.empty(9 * stride, "float32")
+            f.v32_valid = False
+        if not f.v32_valid:
+            _lib.check(_lib.load().ysp_cast_f32(dev.ptr(f.v64), dev.ptr(f.v32), 9 * stride, dev.stream()),
+                       "ysp_cast_f32")
+            f.v32_valid = True
+        return f.v32
 
     def spmv(self, vec: vector, transpose: bool = False, use_f32: bool = False) -> vector:
         self._ensure_setup()

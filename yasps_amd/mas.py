@@ -25,6 +25,7 @@ import numpy as np
 from . import _lib, device as dev
 from .hessian import vector
 from .mas_hierarchy import Hierarchy, build_hierarchy
+from .refine import RefinementBuffers, refine_solve
 
 _MODES = {"reference": 0, "relative_residual": 1}
 DOMAIN_SCALARS = 32          # padded domain size of the device kernels
@@ -252,15 +253,17 @@ class masSolver:
 
     Options (reference names): ``max_domain_dofs`` (<= 32), ``max_levels``,
     ``target_nodes_per_partition``, ``coarsest_level_weight``, ``level_weights``,
-    ``global_coarse_domain``; extension ``threshold_mode`` as for ``jacobianPCGSolver``.
-    Returns 0 when converged and -4 otherwise, as the reference.
+    ``global_coarse_domain``; extensions ``threshold_mode`` as for ``jacobianPCGSolver`` and
+    ``precision="mixed"`` (FP32 block values in the Krylov loop, FP64 iterative refinement
+    on the true relative residual). Returns 0 when converged and -4 otherwise, as the
+    reference.
     """
 
     def __init__(self, max_domain_dofs=None, max_levels=None, target_nodes_per_partition=None,
                  coarsest_level_weight: float = 2.5, level_weights=None,
                  global_coarse_domain: bool = True, threshold_mode: str = "reference",
                  inverse_backend: str = "cooperative_gauss_jordan",
-                 allow_cpu_fallback: bool = False):
+                 allow_cpu_fallback: bool = False, precision: str = "fp64"):
         if any(v is not None and v <= 0 for v in (max_domain_dofs, max_levels,
                                                   target_nodes_per_partition)):
             raise ValueError("domain capacity/levels must be positive and correction iterations non-negative")
@@ -275,6 +278,8 @@ class masSolver:
             raise ValueError("level_weights must contain finite positive values")
         if threshold_mode not in _MODES:
             raise ValueError(f"threshold_mode must be one of {sorted(_MODES)}")
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
         if inverse_backend != "cooperative_gauss_jordan":
             raise ValueError("only the cooperative_gauss_jordan inverse backend exists on this device path")
         if allow_cpu_fallback:
@@ -286,6 +291,8 @@ class masSolver:
         self.level_weights = None if level_weights is None else tuple(float(w) for w in level_weights)
         self.global_coarse_domain = bool(global_coarse_domain)
         self.threshold_mode = threshold_mode
+        self.precision = precision
+        self._refine = RefinementBuffers()
         self.reset()
 
     def reset(self):
@@ -373,6 +380,8 @@ class masSolver:
         lib = _lib.load()
         stats = _lib.PcgStats()
         f = h.full
+        if self.precision == "mixed":
+            return self._solve_mixed(h, rhs, n, tolerance, maxIterations, rebuilt, t0, t1, t2)
         rc = lib.ysp_pcg3_mas_solve(
             int(maxIterations), float(tolerance), _MODES[self.threshold_mode],
             dev.ptr(f.row_ptr), dev.ptr(f.col), dev.ptr(f.v64), 0, f.nnz, C.byref(m.args),
@@ -398,3 +407,39 @@ class masSolver:
             "solve_seconds": t3 - t0, "matrix_size": n, "tolerance": float(tolerance),
         }
         return 0 if converged else -4
+
+    def _solve_mixed(self, h, rhs, n, tolerance, max_iterations, rebuilt, t0, t1, t2):
+        lib = _lib.load()
+        f = h.full
+        v32 = h.full_values_f32()
+        m = self._device
+        stats = _lib.PcgStats()
+        s = dev.stream()
+
+        def inner(r, d, tol_in, max_it):
+            rc = lib.ysp_pcg3_mas_solve(int(max_it), float(tol_in), 1, dev.ptr(f.row_ptr), dev.ptr(f.col),
+                                        dev.ptr(v32), 1, f.nnz, C.byref(m.args), dev.ptr(r), n,
+                                        dev.ptr(d), None, 1, dev.ptr(self._work), dev.ptr(self._scalars),
+                                        C.byref(stats), s)
+            _lib.check(rc, "ysp_pcg3_mas_solve")
+            return rc, int(stats.iterations)
+
+        out = refine_solve(h, rhs, self._solution, float(tolerance), int(max_iterations), inner,
+                           self._refine)
+        inverse_status = int(m.status.item())
+        t3 = time.perf_counter()
+        self.iterations = out["iterations"]
+        lay = m.layout
+        self._statistics = {
+            "solver": "mas", "precision": "mixed", "converged": out["converged"],
+            "iterations": out["iterations"], "outer_iterations": out["outer_iterations"],
+            "relative_residual": out["relative_residual"], "outer_history": out["outer_history"],
+            "result": 0 if out["converged"] else -4,
+            "hierarchy_build_count": self.hierarchy_build_count, "layout_rebuilt": rebuilt,
+            "nodes_per_level": [lv.num_nodes for lv in lay.hierarchy.levels],
+            "domains_per_level": [len(lv.domains) for lv in lay.hierarchy.levels],
+            "inverse_status": inverse_status, "numeric_update_seconds": t2 - t1,
+            "pcg_seconds": t3 - t2, "solve_seconds": t3 - t0, "matrix_size": n,
+            "tolerance": float(tolerance),
+        }
+        return 0 if out["converged"] else -4

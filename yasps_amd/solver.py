@@ -7,7 +7,10 @@ stopping rule r^T M^-1 r <= tol * r0^T M^-1 r0 (tolerance is a squared, precondi
 ratio) and the reference return codes mapped as in ``solverKernel.pyx:800-824``
 (iterations -> 0 on success).
 
-Extension: ``threshold_mode="relative_residual"`` stops on ||r|| <= tol ||b|| instead.
+Extensions: ``threshold_mode="relative_residual"`` stops on ||r|| <= tol ||b|| instead;
+``precision="mixed"`` runs the Krylov loop on FP32 block values and FP32 inverse blocks
+inside an FP64 iterative refinement (``refine.py``) and stops on the true FP64 relative
+residual.
 """
 from __future__ import annotations
 
@@ -16,15 +19,22 @@ import time
 
 from . import _lib, device as dev
 from .hessian import vector
+from .refine import RefinementBuffers, refine_solve
 
 _MODES = {"reference": 0, "relative_residual": 1}
+PRECISIONS = ("fp64", "mixed")
 
 
 class jacobianPCGSolver:
-    def __init__(self, threshold_mode: str = "reference"):
+    def __init__(self, threshold_mode: str = "reference", precision: str = "fp64"):
         if threshold_mode not in _MODES:
             raise ValueError(f"threshold_mode must be one of {sorted(_MODES)}")
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {PRECISIONS}")
         self.threshold_mode = threshold_mode
+        self.precision = precision
+        self._refine = RefinementBuffers()
+        self._minv32 = None
         self.__solution = None
         self.__work = None
         self.__scalars = None
@@ -85,6 +95,8 @@ class jacobianPCGSolver:
         if not zero_initial_guess and initial_guess is not None:
             guess = initial_guess.value if hasattr(initial_guess, "value") else initial_guess
         zero = int(bool(zero_initial_guess) or guess is None)
+        if self.precision == "mixed":
+            return self._solve_mixed(h, right_hand_side, n, tolerance, maxIterations, t0, t1, t2)
         if h.full is not None:
             f = h.full
             rc = lib.ysp_pcg3_solve(
@@ -121,6 +133,42 @@ class jacobianPCGSolver:
             return rc
         return 0
 
+    def _solve_mixed(self, h, rhs, n, tolerance, max_iterations, t0, t1, t2):
+        if h.full is None:
+            raise NotImplementedError("precision='mixed' needs 3 x 3 variable blocks")
+        lib = _lib.load()
+        s = dev.stream()
+        f = h.full
+        v32 = h.full_values_f32()
+        nd = h.diagonal_blocks_inverse.numel()
+        if self._minv32 is None or self._minv32.numel() != nd:
+            self._minv32 = dev.empty(nd, "float32")
+        _lib.check(lib.ysp_cast_f32(dev.ptr(h.diagonal_blocks_inverse), dev.ptr(self._minv32), nd, s),
+                   "ysp_cast_f32")
+        stats = _lib.PcgStats()
+
+        def inner(r, d, tol_in, max_it):
+            rc = lib.ysp_pcg3_solve(int(max_it), float(tol_in), 1, dev.ptr(f.row_ptr), dev.ptr(f.col),
+                                    dev.ptr(v32), 1, f.nnz, dev.ptr(self._minv32), 1, dev.ptr(r), n,
+                                    dev.ptr(d), None, 1, dev.ptr(self.__work), dev.ptr(self.__scalars),
+                                    C.byref(stats), s)
+            _lib.check(rc, "ysp_pcg3_solve")
+            return rc, int(stats.iterations)
+
+        out = refine_solve(h, rhs.value, self.__solution, float(tolerance), int(max_iterations), inner,
+                           self._refine)
+        t3 = time.perf_counter()
+        self.iterations = out["iterations"]
+        self.__statistics = {
+            "solver": "jacobian", "precision": "mixed", "converged": out["converged"],
+            "iterations": out["iterations"], "outer_iterations": out["outer_iterations"],
+            "relative_residual": out["relative_residual"], "outer_history": out["outer_history"],
+            "result": 0 if out["converged"] else -4, "diagonal_inverse_seconds": t2 - t1,
+            "pcg_seconds": t3 - t2, "solve_seconds": t3 - t0, "matrix_size": n,
+            "tolerance": float(tolerance),
+        }
+        return 0 if out["converged"] else -4
+
 
 class solver:
     """Solver dispatch (``solver/solver.py:9-68``)."""
@@ -151,7 +199,7 @@ class solver:
             raise TypeError("solver must be either 'jacobian' or 'mas'")
         name = solver.strip().lower()
         if name == "jacobian":
-            unexpected = set(options) - {"threshold_mode"}
+            unexpected = set(options) - {"threshold_mode", "precision"}
             if unexpected:
                 raise TypeError(f"jacobianPCGSolver does not accept options: {', '.join(sorted(unexpected))}")
             impl = jacobianPCGSolver(**options)
