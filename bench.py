@@ -35,7 +35,34 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the mixed-precision and MAS legs")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="one solve, the SpMV roofline launches and a calibration copy of a "
+                         "known size, no JSON line (for hardware-counter runs)")
     return ap.parse_args()
+
+
+# HBM bytes per launch of bsr3_full_spmv_kernel<double> measured with hardware counters
+# on the same workload (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, over
+# `bench.py --kernels-only`, each scaled by the counter's reading for a 256 MiB device
+# copy: FETCH x1.9998, WRITE x1.0); summary in profiles/spmv_cube48_fp64_pmc.md.
+MEASURED_TRAFFIC = {48: 133998758}
+
+
+def measured_traffic(n):
+    return MEASURED_TRAFFIC.get(n)
+
+
+def calibration_copy(mib=256):
+    """A plain device copy of a known size (the reference point for the counters)."""
+    from yasps_amd import device as dev
+    torch = dev.torch()
+    a = torch.ones(mib * 131072, dtype=torch.float64, device=dev.device())
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
 
 
 def spmv_roofline(hs, reps):
@@ -71,10 +98,65 @@ def spmv_roofline(hs, reps):
     bytes_full = f.nnz * (9 * 8 + 4) + (n // 3 + 1) * 4 + n * 16
     gbs = bytes_alg / (ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(hs_n(hs)),
             "kernel": "bsr3_full_spmv_kernel<double>", "avg_us": round(ms * 1e3, 2),
             "bytes_per_launch": int(bytes_alg), "full_bsr_bytes_per_launch": int(bytes_full),
             "full_bsr_GBs": round(bytes_full / (ms * 1e-3) / 1e9, 1)}
+
+
+def hs_n(hs):
+    """Cube resolution of the benchmark Hessian (rows = 3 (n+1)^2 n)."""
+    nv = hs.rows // 3
+    for n in range(1, 1024):
+        if (n + 1) ** 2 * n == nv:
+            return n
+    return None
+
+
+def timed_solves(s0, steps, warmup, tol):
+    from yasps_amd import device as dev
+    torch = dev.torch()
+    for _ in range(warmup):
+        s0.minimizeEnergy(tolerance=tol)
+    torch.cuda.synchronize()
+    lat, iters = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ts = time.perf_counter()
+        s0.minimizeEnergy(tolerance=tol)
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - ts)
+        iters.append(s0.minimizer.linearSolver.statistics["iterations"])
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, lat, iters
+
+
+def form_leg(cs, args, x64, ref_value, solver, **options):
+    """Mixed-precision or MAS leg on the same scene: solves/s, its ratio to the plain FP64
+    path, the true FP64 residual and the error against the FP64 solution."""
+    from yasps_amd.scenes import build_yasps_scene
+    from yasps_amd import device as dev
+    torch = dev.torch()
+    s1, _ = build_yasps_scene(cs, solver_name=solver, **options)
+    s1.minimizeEnergy(tolerance=args.tol)
+    torch.cuda.synchronize()
+    elapsed, lat, iters = timed_solves(s1, args.steps, args.warmup, args.tol)
+    st = s1.minimizer.linearSolver.statistics
+    hs = s1.minimizer.activeHessian
+    x = s1.minimizer.linearSolver.solution.clone()
+    g = hs.gradient.value
+    from yasps_amd.hessian import vector
+    r = hs.spmv(vector(hs.rows, x)).value - g
+    value = args.steps / elapsed
+    return {"solver": solver, "options": {k: v for k, v in options.items()},
+            "value": round(value, 3), "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+            "vs_fp64_jacobi": round(value / ref_value, 3),
+            "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
+            "pcg_iterations": int(np.median(iters)),
+            "outer_iterations": st.get("outer_iterations"),
+            "true_fp64_rel_residual": float(torch.linalg.norm(r) / torch.linalg.norm(g)),
+            "rel_err_vs_fp64": float(torch.linalg.norm(x - x64) / torch.linalg.norm(x64))}
 
 
 def cpu_baseline(cs, values_host, st, tol):
@@ -135,21 +217,11 @@ def main():
     # setup: static sparsity layout, rest data, first solve
     s0.minimizeEnergy(tolerance=args.tol)
     torch.cuda.synchronize()
-    for _ in range(args.warmup):
-        s0.minimizeEnergy(tolerance=args.tol)
-    torch.cuda.synchronize()
-
-    lat = []
-    iters = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ts = time.perf_counter()
-        s0.minimizeEnergy(tolerance=args.tol)
-        torch.cuda.synchronize()
-        lat.append(time.perf_counter() - ts)
-        iters.append(s0.minimizer.linearSolver.statistics["iterations"])
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    if args.kernels_only:
+        spmv_roofline(s0.minimizer.activeHessian, args.spmv_reps)
+        calibration_copy()
+        return
+    elapsed, lat, iters = timed_solves(s0, args.steps, args.warmup, args.tol)
 
     hs = s0.minimizer.activeHessian
     sol = s0.minimizer.linearSolver.solution
@@ -158,9 +230,20 @@ def main():
     true_rel = float(torch.linalg.norm(r) / torch.linalg.norm(g))
 
     roof = spmv_roofline(hs, args.spmv_reps)
+    value = args.steps / elapsed
+    forms = None
+    if not args.no_forms:
+        x64 = sol.clone()
+        forms = {
+            "mixed_jacobi": form_leg(cs, args, x64, value, "jacobian", precision="mixed"),
+            "mas_fp64": form_leg(cs, args, x64, value, "mas", threshold_mode="relative_residual",
+                                 max_levels=4, coarsest_level_weight=1.0),
+            "mixed_mas": form_leg(cs, args, x64, value, "mas", precision="mixed", max_levels=4,
+                                  coarsest_level_weight=1.0),
+        }
     out = {
         "metric": "Newton solves/s at 1e-6 rel. residual (plain FP64)",
-        "value": round(args.steps / elapsed, 3),
+        "value": round(value, 3),
         "unit": "Newton solves/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -181,6 +264,7 @@ def main():
         "pcg_iterations": int(np.median(iters)),
         "true_rel_residual": true_rel,
         "roofline": roof,
+        "forms": forms,
     }
     if not args.no_cpu:
         from oracle import newton
