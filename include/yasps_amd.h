@@ -181,6 +181,56 @@ int ysp_energy_inertia(const double* position, const double* last_position,
                        const double* velocity, const double* mass, double dt,
                        uint32_t num_vertices, double* energy, void* stream);
 
+/* ------------------------------------------------ generated element energies */
+/* Energies of the reference examples other than SNH/inertia, each evaluated per element
+ * from its vertex data (arity vertices of `width` doubles gathered through the union
+ * view and elem2v), differentiated exactly offline (yasps_amd/codegen/gen_energies.py):
+ *   POINT_POINT/POINT_EDGE/POINT_TRIANGLE/EDGE_EDGE: IPC barriers
+ *     kappa (d - dhat)^2 log^2(d / dhat) on squared distances
+ *     (examples/one_bunny/helpers.py:74-120); scene params [dhat, kappa].
+ *   BENDING: k (theta - theta0)^2 |e0| (examples/many_bunnies_one_cloth/helpers.py:193-212);
+ *     element params [theta0, |e0|], scene params [k].
+ *   BARAFF_WITKIN: stretch/shear (helpers.py:239-259); element params [Minv (2x2), area],
+ *     scene params [stretch, shear, dt, -, -] (the last two are set per element).
+ *   STVK_MEMBRANE: area dt^2 (mu |E|^2 + lam/2 tr(E)^2), no reference counterpart;
+ *     element params [Minv, area], scene params [mu, lam, dt].
+ *   AFFINE_ORTHOGONALITY: k/2 |A^T A - I|_F^2 on a 9-vector A
+ *     (examples/one_bunny_partial_abd/helpers.py:132-139); scene params [k].
+ * They replace the generated compute_hessian_and_gradient kernels of these energies
+ * (hessianKernelHost.pyx:27-50) with the same scatter arguments. */
+#define YSP_ENERGY_POINT_POINT 0
+#define YSP_ENERGY_POINT_EDGE 1
+#define YSP_ENERGY_POINT_TRIANGLE 2
+#define YSP_ENERGY_EDGE_EDGE 3
+#define YSP_ENERGY_BENDING 4
+#define YSP_ENERGY_BARAFF_WITKIN 5
+#define YSP_ENERGY_STVK_MEMBRANE 6
+#define YSP_ENERGY_AFFINE_ORTHOGONALITY 7
+
+/* Shape of an energy kind: vertices per element, doubles per vertex, element and scene
+ * parameter counts. */
+int ysp_energy_arity(int32_t kind, uint32_t* arity, uint32_t* width, uint32_t* element_params,
+                     uint32_t* scene_params);
+
+/* Element parameters from rest positions: BENDING [theta0, |e0|] per hinge,
+ * BARAFF_WITKIN / STVK_MEMBRANE [Minv (4), thickness * rest area] per triangle (rest
+ * frame of helpers.py:214-237). */
+int ysp_element_rest_data(int32_t kind, const ysp_union_f64* rest_positions,
+                          const uint32_t* elem2v, uint32_t n, double thickness,
+                          double* element_params, void* stream);
+
+/* Gradient and (projected) Hessian of every element scattered as ysp_eval_snh_tet does;
+ * segments of any size (3 for vertices, 9 for an affine matrix). Synchronous. */
+int ysp_eval_element_energy(int32_t kind, const ysp_union_f64* positions,
+                            const uint32_t* elem2v, uint32_t n, const double* element_params,
+                            const double* scene_params, uint32_t projection_method,
+                            const ysp_scatter_args* out, void* stream);
+
+/* Per-element energy values. */
+int ysp_element_energy_values(int32_t kind, const ysp_union_f64* positions,
+                              const uint32_t* elem2v, uint32_t n, const double* element_params,
+                              const double* scene_params, double* energy, void* stream);
+
 /* --------------------------------------------------------- block storage */
 /* Symmetric upper block storage of one Hessian part (hessian.pyx:783-808,
  * matrix.pyx:138-282). Host arrays describe the (h, w) categories. */

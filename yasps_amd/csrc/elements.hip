@@ -17,6 +17,7 @@
 // the 12x12 H; only elements that fail it are projected (second kernel, work queue).
 #include "common.hpp"
 #include "dense.hpp"
+#include "element_common.hpp"
 
 #include <type_traits>
 
@@ -24,58 +25,6 @@ namespace ysp {
 namespace {
 
 constexpr unsigned kBlock = 128;
-
-struct UnionView {
-  uint32_t nc;
-  uint32_t width;
-  const uint32_t* counts;
-  const double* data[YSP_MAX_UNION_CHILDREN];
-};
-
-UnionView make_view(const ysp_union_f64* u) {
-  UnionView v{};
-  v.nc = u->num_children;
-  v.width = u->width;
-  v.counts = u->counts;
-  for (uint32_t i = 0; i < YSP_MAX_UNION_CHILDREN; ++i) v.data[i] = u->data[i];
-  return v;
-}
-
-__device__ __forceinline__ const double* union_row(const UnionView& U, uint32_t v) {
-  uint32_t total = 0;
-  for (uint32_t c = 0; c < U.nc; ++c) {
-    const uint32_t cnt = U.counts[c];
-    if (v < total + cnt) return U.data[c] + static_cast<size_t>(v - total) * U.width;
-    total += cnt;
-  }
-  return nullptr;
-}
-
-struct ScatterArgs {
-  const uint32_t* seg_idx;
-  const uint16_t* seg_size;
-  const int16_t* perm;
-  const uint32_t* lookups;
-  const uint32_t* coord_outer;
-  uint32_t K;
-  uint32_t num_attr;
-  const uint32_t* diag_start;
-  const uint32_t* grad_start;
-  double* gradient;
-  double* blocks;
-  double* diag_blocks;
-  double* block_staging;     // staged mode: 9 doubles per coordinate of this term
-  double* grad_staging;      // staged mode: grad_local doubles per element
-  uint32_t grad_local;
-};
-
-ScatterArgs make_scatter(const ysp_scatter_args* a) {
-  return ScatterArgs{a->segment_indices, a->segment_sizes, a->local_permutations, a->lookups,
-                     a->coordinates_outer, a->max_num_indices, a->num_attributes,
-                     a->diagonal_blocks_start, a->gradient_segments_start, a->gradient,
-                     a->hessian_blocks, a->diagonal_blocks, a->block_staging,
-                     a->gradient_staging, a->gradient_local_size};
-}
 
 constexpr int kMaxSlots = 16;
 
@@ -136,13 +85,6 @@ __device__ __forceinline__ void load_slots(const ScatterArgs& S, uint32_t e, Slo
     for (int k = 0; k < KT; ++k)
       if (k == g) si.group_mask[k] |= 1u << i;
   }
-}
-
-__device__ __forceinline__ uint32_t diag_offset(const ScatterArgs& S, uint32_t seg, uint32_t size) {
-  uint32_t attr = 0;
-  while (attr < S.num_attr && seg >= S.grad_start[attr + 1]) ++attr;
-  const uint32_t inst = (seg - S.grad_start[attr]) / size;
-  return S.diag_start[attr] + inst * size * size;
 }
 
 // Gradient segments (hessianKernelNoProject.pyx:63-88): FP64 atomics into the gradient,

@@ -1,0 +1,501 @@
+// Element kernels of the generated energies (IPC barriers, bending, Baraff-Witkin,
+// StVK membrane, affine orthogonality): gather the element's vertex data through the
+// union view, evaluate energy / gradient / packed Hessian (generated_energies.hpp), PSD
+// projection with the reference's rules, and the scatter into block storage for
+// segments of any size (hessianKernelNoProject.pyx:37-264 semantics).
+#include "common.hpp"
+#include "dense.hpp"
+#include "element_common.hpp"
+#include "generated_energies.hpp"
+
+namespace ysp {
+namespace {
+
+constexpr unsigned kEB = 64;
+constexpr int kSlotsMax = 16;
+
+template <int ID>
+struct Kind;
+
+#define YSP_ENERGY_KIND(ID, NAME, ARITY, WIDTH, NPAR, NSCENE)                               \
+  template <>                                                                             \
+  struct Kind<ID> {                                                                       \
+    static constexpr int A = ARITY, W = WIDTH, N = ARITY * WIDTH, P = NPAR, S = NSCENE;   \
+    static_assert(N == gen::kGen_##NAME##_N, "generated size mismatch");                 \
+    __device__ static void eval(const double* x, const double* p, const double* s, double& E, \
+                                double* g, double* H) {                                  \
+      gen::gen_##NAME(x, p, s, E, g, H);                                                 \
+    }                                                                                     \
+  };
+
+YSP_ENERGY_KIND(YSP_ENERGY_POINT_POINT, point_point, 2, 3, 0, 2)
+YSP_ENERGY_KIND(YSP_ENERGY_POINT_EDGE, point_edge, 3, 3, 0, 2)
+YSP_ENERGY_KIND(YSP_ENERGY_POINT_TRIANGLE, point_triangle, 4, 3, 0, 2)
+YSP_ENERGY_KIND(YSP_ENERGY_EDGE_EDGE, edge_edge, 4, 3, 0, 2)
+YSP_ENERGY_KIND(YSP_ENERGY_BENDING, bending, 4, 3, 2, 1)
+YSP_ENERGY_KIND(YSP_ENERGY_BARAFF_WITKIN, baraff_witkin, 3, 3, 5, 5)
+YSP_ENERGY_KIND(YSP_ENERGY_STVK_MEMBRANE, stvk_membrane, 3, 3, 5, 3)
+YSP_ENERGY_KIND(YSP_ENERGY_AFFINE_ORTHOGONALITY, affine_orthogonality, 1, 9, 0, 1)
+#undef YSP_ENERGY_KIND
+
+// ----------------------------------------------------------------- projection
+// hessianKernelHeader.pyx:79-252 for N >= 4: the packed-Schur PSD test on the element
+// Hessian; a failing matrix is replaced by V max(L, 0) V^T (method 2) or V |L| V^T
+// (method 1) from a cyclic Jacobi eigen-decomposition.
+template <int N>
+__device__ __forceinline__ void project_packed(double (&H)[N * (N + 1) / 2], uint32_t method) {
+  double L[N * (N + 1) / 2];
+#pragma unroll
+  for (int r = 0; r < N; ++r)
+#pragma unroll
+    for (int c = 0; c <= r; ++c) L[r * (r + 1) / 2 + c] = H[upper_index<N>(c, r)];
+  if (psd_test_packed<N>(L)) return;
+  double V[N][N];
+  jacobi_eigen_packed<N>(H, V);
+  double lam[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const double l = H[upper_index<N>(i, i)];
+    lam[i] = (l < 0.0) ? (method == 1 ? -l : 0.0) : l;
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = i; j < N; ++j) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < N; ++k) acc += V[i][k] * lam[k] * V[j][k];
+      H[upper_index<N>(i, j)] = acc;
+    }
+}
+
+// ----------------------------------------------------------------- slots (any size)
+// Slot groups by |perm| in first-occurrence order (hessianKernelNoProject.pyx:93-163),
+// with every slot's size and local column offset (sum of the previous slot sizes).
+template <int KT>
+struct GSlots {
+  uint32_t idx[KT];
+  uint8_t size[KT];
+  uint8_t col[KT];
+  int8_t group_first[KT];
+  uint32_t group_mask[KT];
+  int ng;
+  bool ok;
+};
+
+template <int KT>
+__device__ __forceinline__ void load_gslots(const ScatterArgs& S, uint32_t e, int N,
+                                            GSlots<KT>& si) {
+  const uint32_t K = S.K;
+  si.ng = 0;
+  si.ok = K <= static_cast<uint32_t>(KT);
+  if (!si.ok) return;
+  int placement[KT];
+  uint32_t col = 0;
+#pragma unroll
+  for (int i = 0; i < KT; ++i) {
+    si.idx[i] = 0;
+    si.size[i] = 0;
+    si.col[i] = 0;
+    if (static_cast<uint32_t>(i) >= K) continue;
+    const size_t o = static_cast<size_t>(e) * K + i;
+    const uint32_t ii = S.seg_idx[o];
+    const uint32_t sz = S.seg_size[o];
+    const int pm = S.perm[o];
+    si.idx[i] = ii;
+    si.size[i] = static_cast<uint8_t>(sz);
+    si.col[i] = static_cast<uint8_t>(col);
+    col += sz;
+    if (pm == 0) continue;
+    const int p = (pm < 0 ? -pm : pm) - 1;
+    int g = -1;
+#pragma unroll
+    for (int k = 0; k < KT; ++k)
+      if (k < si.ng && placement[k] == p) g = k;
+    if (g < 0) {
+      g = si.ng++;
+#pragma unroll
+      for (int k = 0; k < KT; ++k)
+        if (k == g) {
+          placement[k] = p;
+          si.group_first[k] = static_cast<int8_t>(i);
+          si.group_mask[k] = 0;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < KT; ++k)
+      if (k == g) si.group_mask[k] |= 1u << i;
+  }
+  if (col != static_cast<uint32_t>(N)) si.ok = false;   // slots must tile the local DoFs
+}
+
+template <int N>
+__device__ __forceinline__ double hget(const double (&H)[N * (N + 1) / 2], int i, int j) {
+  return i <= j ? H[upper_index<N>(i, j)] : H[upper_index<N>(j, i)];
+}
+
+template <int KT, int N>
+__device__ __forceinline__ void scatter_generic(const ScatterArgs& S, uint32_t e,
+                                                const GSlots<KT>& si, const double (&g)[N],
+                                                const double (&H)[N * (N + 1) / 2]) {
+  // gradient
+  if (S.grad_staging) {
+    double* out = S.grad_staging + static_cast<size_t>(e) * S.grad_local;
+#pragma unroll
+    for (int i = 0; i < KT; ++i)
+      for (int k = 0; k < si.size[i]; ++k) out[si.col[i] + k] = g[si.col[i] + k];
+  } else {
+#pragma unroll
+    for (int i = 0; i < KT; ++i) {
+      if (si.idx[i] < 2) continue;
+      const uint32_t p = si.idx[i] - 2;
+      for (int k = 0; k < si.size[i]; ++k) atomic_add_f64(S.gradient + p + k, g[si.col[i] + k]);
+    }
+  }
+  // Hessian blocks of every valid group pair, in coordinate order
+  uint32_t count = 0;
+  const uint32_t cstart = S.coord_outer[e];
+#pragma unroll
+  for (int gi = 0; gi < KT; ++gi) {
+    if (gi >= si.ng) continue;
+    const int fi = si.group_first[gi];
+    const uint32_t ii = si.idx[fi];
+    if (ii < 2) continue;
+    const int h = si.size[fi];
+#pragma unroll
+    for (int gj = gi; gj < KT; ++gj) {
+      if (gj >= si.ng) continue;
+      const int fj = si.group_first[gj];
+      const uint32_t jj = si.idx[fj];
+      if (jj < 2) continue;
+      const int w = si.size[fj];
+      const uint32_t coord = cstart + count;
+      double* dst = S.block_staging ? S.block_staging + 9 * static_cast<size_t>(coord)
+                                    : S.blocks + S.lookups[coord];
+      double* dd = nullptr;
+      if (!S.block_staging && gi == gj && S.diag_blocks) dd = S.diag_blocks + diag_offset(S, ii - 2, h);
+      const bool tr = !(ii < jj);
+      for (int k = 0; k < h; ++k)
+        for (int l = 0; l < w; ++l) {
+          double v = 0.0;
+#pragma unroll
+          for (int a = 0; a < KT; ++a) {
+            if (!((si.group_mask[gi] >> a) & 1u) || si.size[a] == 0) continue;
+#pragma unroll
+            for (int b = 0; b < KT; ++b) {
+              if (!((si.group_mask[gj] >> b) & 1u) || si.size[b] == 0) continue;
+              v += hget<N>(H, si.col[a] + k, si.col[b] + l);
+            }
+          }
+          // stored block: rows of the lower global index; (l, k) of a transposed block
+          const int o = tr ? l * h + k : k * w + l;
+          if (S.block_staging) {
+            dst[o] = v;
+          } else {
+            atomic_add_f64(dst + o, v);
+            if (dd) atomic_add_f64(dd + o, v);
+          }
+        }
+      ++count;
+    }
+  }
+}
+
+template <int ID>
+__device__ __forceinline__ void gather(const UnionView& P, const uint32_t* e2v, uint32_t e,
+                                       double (&x)[Kind<ID>::N]) {
+  using T = Kind<ID>;
+#pragma unroll
+  for (int a = 0; a < T::A; ++a) {
+    const uint32_t v = e2v[static_cast<size_t>(e) * T::A + a];
+    const double* row = union_row(P, v);
+#pragma unroll
+    for (int i = 0; i < T::W; ++i) x[T::W * a + i] = row[i];
+  }
+}
+
+// Scene parameters of one element; Baraff-Witkin picks its stretch coefficients from
+// the current F (helpers.py:252-253: 0.01 below unit stretch, else 1).
+template <int ID>
+__device__ __forceinline__ void scene_params(const double* sp, const double* x, const double* p,
+                                             double (&s)[Kind<ID>::S]) {
+#pragma unroll
+  for (int i = 0; i < Kind<ID>::S; ++i) s[i] = sp[i];
+  if constexpr (ID == YSP_ENERGY_BARAFF_WITKIN) {
+    double fu2 = 0.0, fv2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const double d1 = x[3 + i] - x[i], d2 = x[6 + i] - x[i];
+      const double fu = d1 * p[0] + d2 * p[2], fv = d1 * p[1] + d2 * p[3];
+      fu2 += fu * fu;
+      fv2 += fv * fv;
+    }
+    s[3] = sqrt(fu2) < 1.0 ? 0.01 : 1.0;
+    s[4] = sqrt(fv2) < 1.0 ? 0.01 : 1.0;
+  }
+}
+
+template <int ID, int KT>
+__global__ void __launch_bounds__(kEB)
+    energy_eval_kernel(UnionView P, const uint32_t* __restrict__ e2v, uint32_t n,
+                       const double* __restrict__ ep, const double* __restrict__ sp,
+                       uint32_t method, ScatterArgs SA, int* __restrict__ error_flag) {
+  using T = Kind<ID>;
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  GSlots<KT> si;
+  load_gslots<KT>(SA, e, T::N, si);
+  if (!si.ok || (SA.block_staging && T::W != 3)) {
+    atomicOr(error_flag, 1);
+    return;
+  }
+  double x[T::N];
+  gather<ID>(P, e2v, e, x);
+  const double* p = ep + static_cast<size_t>(e) * T::P;
+  double s[T::S];
+  scene_params<ID>(sp, x, p, s);
+  double E, g[T::N], H[T::N * (T::N + 1) / 2];
+  T::eval(x, p, s, E, g, H);
+  if (method != 0) project_packed<T::N>(H, method);
+  scatter_generic<KT, T::N>(SA, e, si, g, H);
+}
+
+template <int ID>
+__global__ void __launch_bounds__(kEB)
+    energy_value_kernel(UnionView P, const uint32_t* __restrict__ e2v, uint32_t n,
+                        const double* __restrict__ ep, const double* __restrict__ sp,
+                        double* __restrict__ energy) {
+  using T = Kind<ID>;
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  double x[T::N];
+  gather<ID>(P, e2v, e, x);
+  const double* p = ep + static_cast<size_t>(e) * T::P;
+  double s[T::S];
+  scene_params<ID>(sp, x, p, s);
+  double E, g[T::N], H[T::N * (T::N + 1) / 2];
+  T::eval(x, p, s, E, g, H);
+  energy[e] = E;
+}
+
+// ----------------------------------------------------------------- rest data
+__device__ __forceinline__ void cross(const double* a, const double* b, double* c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+__device__ __forceinline__ double dot(const double* a, const double* b) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+// Dihedral angle of examples/many_bunnies_one_cloth/helpers.py:193-203.
+__device__ __forceinline__ double edge_theta(const double* q0, const double* q1,
+                                             const double* q2, const double* q3) {
+  double a[3], b[3], c[3], d[3], n0[3], n1[3], axis[3], m[3];
+  for (int i = 0; i < 3; ++i) {
+    a[i] = q0[i] - q2[i];
+    b[i] = q1[i] - q2[i];
+    c[i] = q1[i] - q3[i];
+    d[i] = q0[i] - q3[i];
+    axis[i] = q1[i] - q0[i];
+  }
+  cross(a, b, n0);
+  cross(c, d, n1);
+  cross(n0, n1, m);
+  const double num = dot(m, axis) / sqrt(dot(axis, axis));
+  const double den = dot(n0, n1) + sqrt(dot(n0, n0)) * sqrt(dot(n1, n1));
+  return 2.0 * atan2(num, den);
+}
+
+// 2x2 rest shape of helpers.py:214-237 (triangle rotated so that its normal is +y, then
+// (x, z) coordinates), inverted; area = thickness |v01 x v02| / 2.
+__device__ __forceinline__ void triangle_rest(const double* v0, const double* v1,
+                                              const double* v2, double thickness, double* out) {
+  double e1[3], e2[3], nrm[3];
+  for (int i = 0; i < 3; ++i) {
+    e1[i] = v1[i] - v0[i];
+    e2[i] = v2[i] - v0[i];
+  }
+  cross(e1, e2, nrm);
+  const double len = sqrt(dot(nrm, nrm));
+  for (int i = 0; i < 3; ++i) nrm[i] /= len;
+  const double tgt[3] = {0.0, 1.0, 0.0};
+  double vv[3];
+  cross(nrm, tgt, vv);
+  const double cs = dot(nrm, tgt);
+  const double K[3][3] = {{0.0, -vv[2], vv[1]}, {vv[2], 0.0, -vv[0]}, {-vv[1], vv[0], 0.0}};
+  double R[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double k2 = 0.0;
+      for (int k = 0; k < 3; ++k) k2 += K[i][k] * K[k][j];
+      R[i][j] = (i == j ? 1.0 : 0.0) + K[i][j] + k2 / (1.0 + cs);
+    }
+  double uv[3][2];
+  const double* vs[3] = {v0, v1, v2};
+  for (int t = 0; t < 3; ++t) {
+    uv[t][0] = R[0][0] * vs[t][0] + R[0][1] * vs[t][1] + R[0][2] * vs[t][2];
+    uv[t][1] = R[2][0] * vs[t][0] + R[2][1] * vs[t][1] + R[2][2] * vs[t][2];
+  }
+  const double m00 = uv[1][0] - uv[0][0], m01 = uv[2][0] - uv[0][0];
+  const double m10 = uv[1][1] - uv[0][1], m11 = uv[2][1] - uv[0][1];
+  const double det = m00 * m11 - m01 * m10;
+  out[0] = m11 / det;
+  out[1] = -m01 / det;
+  out[2] = -m10 / det;
+  out[3] = m00 / det;
+  out[4] = thickness * len / 2.0;
+}
+
+__global__ void __launch_bounds__(kEB)
+    rest_kernel(int kind, UnionView R, const uint32_t* __restrict__ e2v, uint32_t n,
+                double thickness, double* __restrict__ ep) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  if (kind == YSP_ENERGY_BENDING) {
+    const double* q[4];
+    for (int a = 0; a < 4; ++a) q[a] = union_row(R, e2v[4 * static_cast<size_t>(e) + a]);
+    double d[3];
+    for (int i = 0; i < 3; ++i) d[i] = q[1][i] - q[0][i];
+    ep[2 * static_cast<size_t>(e)] = edge_theta(q[0], q[1], q[2], q[3]);
+    ep[2 * static_cast<size_t>(e) + 1] = sqrt(dot(d, d));
+  } else {
+    const double* v[3];
+    for (int a = 0; a < 3; ++a) v[a] = union_row(R, e2v[3 * static_cast<size_t>(e) + a]);
+    triangle_rest(v[0], v[1], v[2], thickness, ep + 5 * static_cast<size_t>(e));
+  }
+}
+
+template <int ID>
+int launch_eval(const UnionView& U, const uint32_t* e2v, uint32_t n, const double* ep,
+                const double* sp, uint32_t method, const ScatterArgs& SA, int* flag,
+                hipStream_t s) {
+  const dim3 g(grid_for(n, kEB)), b(kEB);
+  if (SA.K <= 8)
+    hipLaunchKernelGGL((energy_eval_kernel<ID, 8>), g, b, 0, s, U, e2v, n, ep, sp, method, SA, flag);
+  else
+    hipLaunchKernelGGL((energy_eval_kernel<ID, kSlotsMax>), g, b, 0, s, U, e2v, n, ep, sp, method,
+                       SA, flag);
+  return YSP_OK;
+}
+
+template <int ID>
+void launch_values(const UnionView& U, const uint32_t* e2v, uint32_t n, const double* ep,
+                   const double* sp, double* energy, hipStream_t s) {
+  hipLaunchKernelGGL(energy_value_kernel<ID>, dim3(grid_for(n, kEB)), dim3(kEB), 0, s, U, e2v, n,
+                     ep, sp, energy);
+}
+
+template <typename Fn>
+int dispatch_kind(int32_t kind, Fn&& fn) {
+  switch (kind) {
+    case YSP_ENERGY_POINT_POINT: return fn(std::integral_constant<int, YSP_ENERGY_POINT_POINT>());
+    case YSP_ENERGY_POINT_EDGE: return fn(std::integral_constant<int, YSP_ENERGY_POINT_EDGE>());
+    case YSP_ENERGY_POINT_TRIANGLE: return fn(std::integral_constant<int, YSP_ENERGY_POINT_TRIANGLE>());
+    case YSP_ENERGY_EDGE_EDGE: return fn(std::integral_constant<int, YSP_ENERGY_EDGE_EDGE>());
+    case YSP_ENERGY_BENDING: return fn(std::integral_constant<int, YSP_ENERGY_BENDING>());
+    case YSP_ENERGY_BARAFF_WITKIN: return fn(std::integral_constant<int, YSP_ENERGY_BARAFF_WITKIN>());
+    case YSP_ENERGY_STVK_MEMBRANE: return fn(std::integral_constant<int, YSP_ENERGY_STVK_MEMBRANE>());
+    case YSP_ENERGY_AFFINE_ORTHOGONALITY:
+      return fn(std::integral_constant<int, YSP_ENERGY_AFFINE_ORTHOGONALITY>());
+    default: return -1;
+  }
+}
+
+int check_view(const ysp_union_f64* u, uint32_t width, const char* who) {
+  if (!u || u->num_children == 0 || u->num_children > YSP_MAX_UNION_CHILDREN || u->width != width) {
+    ysp_set_error(__FILE__, __LINE__, who);
+    return YSP_ERR_ARG;
+  }
+  return YSP_OK;
+}
+
+}  // namespace
+}  // namespace ysp
+
+using namespace ysp;
+
+extern "C" int ysp_energy_arity(int32_t kind, uint32_t* arity, uint32_t* width,
+                                uint32_t* element_params, uint32_t* scene_params) {
+  return dispatch_kind(kind, [&](auto id) {
+    using T = Kind<decltype(id)::value>;
+    if (arity) *arity = T::A;
+    if (width) *width = T::W;
+    if (element_params) *element_params = T::P;
+    if (scene_params) *scene_params = T::S;
+    return YSP_OK;
+  }) == YSP_OK ? YSP_OK : YSP_ERR_ARG;
+}
+
+extern "C" int ysp_element_rest_data(int32_t kind, const ysp_union_f64* rest_positions,
+                                     const uint32_t* elem2v, uint32_t n, double thickness,
+                                     double* element_params, void* stream) {
+  if (kind != YSP_ENERGY_BENDING && kind != YSP_ENERGY_BARAFF_WITKIN &&
+      kind != YSP_ENERGY_STVK_MEMBRANE) {
+    ysp_set_error(__FILE__, __LINE__, "ysp_element_rest_data: energy has no rest data");
+    return YSP_ERR_ARG;
+  }
+  if (n == 0) return YSP_OK;
+  if (int rc = check_view(rest_positions, 3, "ysp_element_rest_data: bad union view")) return rc;
+  hipLaunchKernelGGL(rest_kernel, dim3(grid_for(n, kEB)), dim3(kEB), 0, as_stream(stream), kind,
+                     make_view(rest_positions), elem2v, n, thickness, element_params);
+  YSP_CHECK_LAUNCH();
+  return YSP_OK;
+}
+
+extern "C" int ysp_eval_element_energy(int32_t kind, const ysp_union_f64* positions,
+                                       const uint32_t* elem2v, uint32_t n,
+                                       const double* element_params, const double* scene_params,
+                                       uint32_t projection_method,
+                                       const ysp_scatter_args* out, void* stream) {
+  uint32_t width = 0;
+  if (ysp_energy_arity(kind, nullptr, &width, nullptr, nullptr) != YSP_OK || !out ||
+      projection_method > 2 || out->max_num_indices > static_cast<uint32_t>(kSlotsMax)) {
+    ysp_set_error(__FILE__, __LINE__, "ysp_eval_element_energy: bad arguments");
+    return YSP_ERR_ARG;
+  }
+  if (n == 0) return YSP_OK;
+  if (int rc = check_view(positions, width, "ysp_eval_element_energy: bad union view")) return rc;
+  hipStream_t s = as_stream(stream);
+  int* flag = static_cast<int*>(scratch(34, sizeof(int)));
+  if (!flag) return YSP_ERR_HIP;
+  YSP_CHECK(hipMemsetAsync(flag, 0, sizeof(int), s));
+  const UnionView U = make_view(positions);
+  const ScatterArgs SA = make_scatter(out);
+  dispatch_kind(kind, [&](auto id) {
+    return launch_eval<decltype(id)::value>(U, elem2v, n, element_params, scene_params,
+                                            projection_method, SA, flag, s);
+  });
+  YSP_CHECK_LAUNCH();
+  int h = 0;
+  YSP_CHECK(hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+  YSP_CHECK(hipStreamSynchronize(s));
+  if (h) {
+    ysp_set_error(__FILE__, __LINE__,
+                  "ysp_eval_element_energy: slot layout does not tile the element DoFs");
+    return YSP_ERR_UNSUPPORTED;
+  }
+  return YSP_OK;
+}
+
+extern "C" int ysp_element_energy_values(int32_t kind, const ysp_union_f64* positions,
+                                         const uint32_t* elem2v, uint32_t n,
+                                         const double* element_params,
+                                         const double* scene_params, double* energy,
+                                         void* stream) {
+  uint32_t width = 0;
+  if (ysp_energy_arity(kind, nullptr, &width, nullptr, nullptr) != YSP_OK) {
+    ysp_set_error(__FILE__, __LINE__, "ysp_element_energy_values: unknown energy");
+    return YSP_ERR_ARG;
+  }
+  if (n == 0) return YSP_OK;
+  if (int rc = check_view(positions, width, "ysp_element_energy_values: bad union view")) return rc;
+  const UnionView U = make_view(positions);
+  hipStream_t s = as_stream(stream);
+  dispatch_kind(kind, [&](auto id) {
+    launch_values<decltype(id)::value>(U, elem2v, n, element_params, scene_params, energy, s);
+    return YSP_OK;
+  });
+  YSP_CHECK_LAUNCH();
+  return YSP_OK;
+}

@@ -242,3 +242,190 @@ def stable_neo_hookean(tet_position_rest, tet_position, mu, lam, dt) -> StableNe
 
 def inertia(x_before, vel, dt, x, mass) -> Inertia:
     return Inertia(x_before, vel, dt, x, mass)
+
+
+# ---------------------------------------------------------------- generated energies
+KIND_IDS = {"point_point": 0, "point_edge": 1, "point_triangle": 2, "edge_edge": 3,
+            "bending": 4, "baraff_witkin": 5, "stvk_membrane": 6, "affine_orthogonality": 7}
+
+
+class ElementEnergy(EnergyExpr):
+    """An energy of ``csrc/generated_energies.hpp`` on the elements of one primitive whose
+    vertex data is a JOIN (``position``) through a fixed-arity connectivity; bending and
+    membrane energies also take the rest JOIN (``rest``) for their element parameters."""
+
+    def __init__(self, kind: str, position, scene_params, rest=None, thickness: float = 1.0):
+        self.kind = kind
+        self.kind_id = KIND_IDS[kind]
+        arity, width, npar, nsc = (C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32())
+        _lib.check(_lib.load().ysp_energy_arity(self.kind_id, C.byref(arity), C.byref(width),
+                                                C.byref(npar), C.byref(nsc)), "ysp_energy_arity")
+        self.arity, self.width, self.npar, self.nsc = arity.value, width.value, npar.value, nsc.value
+        for a, nm in ((position, "position"), (rest, "rest")):
+            if a is None:
+                continue
+            if not isinstance(a, attribute) or a.operator != JOIN or \
+                    a.through.dimension != self.arity or a.source.size != self.width:
+                raise TypeError(f"{kind}: {nm} must be a JOIN of a {self.width}-vector through a "
+                                f"{self.arity}-vertex connectivity")
+        if (rest is None) != (self.npar == 0):
+            raise ValueError(f"{kind}: rest positions are {'required' if self.npar else 'not used'}")
+        if rest is not None and rest.through is not position.through:
+            raise ValueError(f"{kind}: rest and current positions must share the connectivity")
+        if len(scene_params) != self.nsc:
+            raise ValueError(f"{kind}: expected {self.nsc} scene parameters")
+        self.pos, self.rest = position, rest
+        self.scene_params = list(scene_params)
+        self.thickness = float(thickness)
+        self._rest_key = None
+        self._ep = None
+
+    def check_owner(self, owner):
+        if owner is not self.pos.correspondance:
+            raise ValueError(f"{self.kind}: bind the energy to the primitive of its connectivity")
+
+    @property
+    def conn(self):
+        return self.pos.through
+
+    def program(self, wrt_pos: dict):
+        return PConcat([PJoin(self.conn, self.arity, [source_node(self.pos.source, wrt_pos)])])
+
+    def element_params(self):
+        n = self.num_instances
+        if self.npar == 0:
+            return None
+        src = self.rest.source
+        key = (self.conn.version, tuple(c.version for c in (src.children or [src])), n)
+        if key != self._rest_key:
+            self._ep = dev.empty(max(1, self.npar * n), "float64")
+            u = union_view(src)
+            _lib.check(_lib.load().ysp_element_rest_data(
+                self.kind_id, C.byref(u), dev.ptr(self.conn.value), n, self.thickness,
+                dev.ptr(self._ep), dev.stream()), "ysp_element_rest_data")
+            self._rest_key = key
+        return self._ep
+
+    def _scene(self):
+        vals = [_host_scalar(v) for v in self.scene_params]
+        return dev.to_device(np.array(vals, dtype=np.float64), "float64")
+
+    def elem2v(self):
+        return self.conn.value
+
+    def positions(self):
+        return union_view(self.pos.source)
+
+    def evaluate(self, scatter: _lib.ScatterArgs, projection_method: int):
+        n = self.num_instances
+        if n == 0:
+            return
+        ep = self.element_params()
+        sp = self._scene()
+        u = self.positions()
+        _lib.check(_lib.load().ysp_eval_element_energy(
+            self.kind_id, C.byref(u), dev.ptr(self.elem2v()), n, dev.ptr(ep), dev.ptr(sp),
+            max(0, projection_method), C.byref(scatter), dev.stream()), "ysp_eval_element_energy")
+
+    def energy_values(self, att):
+        n = self.num_instances
+        out = dev.empty(n, "float64")
+        if n == 0:
+            return out
+        ep = self.element_params()
+        sp = self._scene()
+        u = self.positions()
+        _lib.check(_lib.load().ysp_element_energy_values(
+            self.kind_id, C.byref(u), dev.ptr(self.elem2v()), n, dev.ptr(ep), dev.ptr(sp),
+            dev.ptr(out), dev.stream()), "ysp_element_energy_values")
+        return out
+
+
+class AffineOrthogonality(ElementEnergy):
+    """``examples/one_bunny_partial_abd/helpers.py:132-139`` on the 9-vector affine matrix
+    of every body (row-major A), times ``stiffness``."""
+
+    def __init__(self, rotation, stiffness=1.0):
+        if not isinstance(rotation, attribute) or rotation.size != 9 or rotation.operator != DATA:
+            raise TypeError("affine_energy: rotation must be a 9-entry DATA attribute (3x3)")
+        self.kind = "affine_orthogonality"
+        self.kind_id = KIND_IDS[self.kind]
+        self.arity, self.width, self.npar, self.nsc = 1, 9, 0, 1
+        self.A = rotation
+        self.scene_params = [stiffness]
+        self.pos = self.rest = None
+        self._iota = None
+
+    def check_owner(self, owner):
+        if owner is not self.A.correspondance and owner is not getattr(self.A.correspondance, "parent", None):
+            raise ValueError("affine_energy: bind the energy to the body primitive or its mesh")
+
+    @property
+    def num_instances(self) -> int:
+        return self.A.numInstances
+
+    def program(self, wrt_pos: dict):
+        return PConcat([PLeaf(9, wrt_pos.get(self.A), True)])
+
+    def element_params(self):
+        return None
+
+    def elem2v(self):
+        n = self.num_instances
+        if self._iota is None or self._iota.numel() != n:
+            self._iota = dev.to_device(np.arange(n, dtype=np.uint32), "uint32")
+        return self._iota
+
+    def positions(self):
+        u = _lib.UnionF64()
+        u.num_children = 1
+        counts = dev.to_device(np.array([self.A.numInstances], np.uint32), "uint32")
+        u.counts = dev.ptr(counts)
+        u.data[0] = dev.ptr(self.A.value)
+        u.width = 9
+        u._keep = [counts, self.A.value]
+        return u
+
+
+def point_point(position, dHat, kappa) -> ElementEnergy:
+    """``examples/one_bunny/helpers.py:74-82``."""
+    return ElementEnergy("point_point", position, [dHat, kappa])
+
+
+def point_edge(position, dHat, kappa) -> ElementEnergy:
+    """``helpers.py:84-94`` (squared distance of the point to the edge's line)."""
+    return ElementEnergy("point_edge", position, [dHat, kappa])
+
+
+def point_triangle(position, dHat, kappa) -> ElementEnergy:
+    """``helpers.py:96-107`` (squared distance of the point to the triangle's plane)."""
+    return ElementEnergy("point_triangle", position, [dHat, kappa])
+
+
+def edge_edge(position, dHat, kappa) -> ElementEnergy:
+    """``helpers.py:109-120`` (squared distance between the two edges' lines)."""
+    return ElementEnergy("edge_edge", position, [dHat, kappa])
+
+
+def bending(x, x_init, bendStiff) -> ElementEnergy:
+    """``examples/many_bunnies_one_cloth/helpers.py:193-212`` on hinges (4 vertices: the
+    shared edge x0-x1 and the two opposite vertices)."""
+    return ElementEnergy("bending", x, [bendStiff], rest=x_init)
+
+
+def baraff_witkin(x_init, x, stretchS, shearS, thickness, dt) -> ElementEnergy:
+    """``examples/many_bunnies_one_cloth/helpers.py:239-259``."""
+    return ElementEnergy("baraff_witkin", x, [stretchS, shearS, dt, 0.0, 0.0], rest=x_init,
+                         thickness=_host_scalar(thickness))
+
+
+def stvk_membrane(x_init, x, mu, lam, thickness, dt) -> ElementEnergy:
+    """StVK membrane area dt^2 (mu |E|^2 + lam/2 tr(E)^2), E = (F^T F - I)/2 with F in the
+    rest frame of ``baraff_witkin`` (no reference counterpart)."""
+    return ElementEnergy("stvk_membrane", x, [mu, lam, dt], rest=x_init,
+                         thickness=_host_scalar(thickness))
+
+
+def affine_energy(rotation, stiffness=1.0) -> AffineOrthogonality:
+    """``examples/one_bunny_partial_abd/helpers.py:132-139``."""
+    return AffineOrthogonality(rotation, stiffness)
