@@ -206,6 +206,24 @@ int ysp_energy_inertia(const double* position, const double* last_position,
 #define YSP_ENERGY_BARAFF_WITKIN 5
 #define YSP_ENERGY_STVK_MEMBRANE 6
 #define YSP_ENERGY_AFFINE_ORTHOGONALITY 7
+#define YSP_ENERGY_SNH 8       /* p [Dm^-1 (9), vol], s [mu, lam, dt] (helpers.py:42-67) */
+#define YSP_ENERGY_INERTIA 9   /* p [x_prev (3), v (3), m], s [dt] (helpers.py:69-71) */
+
+/* Union children whose vertices follow an affine body, x = t + A r
+ * (examples/one_bunny_partial_abd/one_bunny_partial_abd.py:127-144): bit c of mask marks
+ * child c; vertex_body[c] maps its vertices to bodies, rest_position[c] holds r (3 per
+ * vertex). The union view's data for such a child holds the current x. */
+typedef struct ysp_affine_children {
+  uint32_t mask;
+  const uint32_t* vertex_body[YSP_MAX_UNION_CHILDREN];
+  const double* rest_position[YSP_MAX_UNION_CHILDREN];
+} ysp_affine_children;
+
+/* position[v] = translation[b] + affine_matrix[b] rest_position[v], b = vertex_body[v]
+ * (A row-major); the current x of affine vertices for the union view. */
+int ysp_affine_positions(const double* affine_matrix, const double* translation,
+                         const uint32_t* vertex_body, const double* rest_position, uint32_t n,
+                         double* position, void* stream);
 
 /* Shape of an energy kind: vertices per element, doubles per vertex, element and scene
  * parameter counts. */
@@ -220,8 +238,11 @@ int ysp_element_rest_data(int32_t kind, const ysp_union_f64* rest_positions,
                           double* element_params, void* stream);
 
 /* Gradient and (projected) Hessian of every element scattered as ysp_eval_snh_tet does;
- * segments of any size (3 for vertices, 9 for an affine matrix). Synchronous. */
+ * segments of any size (3 for vertices, 9 for an affine matrix). With affine children
+ * (affine may be NULL) the element Hessian in vertex positions is projected and mapped to
+ * the bodies' A and t segments by the chain rule. Synchronous. */
 int ysp_eval_element_energy(int32_t kind, const ysp_union_f64* positions,
+                            const ysp_affine_children* affine,
                             const uint32_t* elem2v, uint32_t n, const double* element_params,
                             const double* scene_params, uint32_t projection_method,
                             const ysp_scatter_args* out, void* stream);

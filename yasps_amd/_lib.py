@@ -32,6 +32,11 @@ class ScatterArgs(C.Structure):
                 ("gradient_local_size", C.c_uint32)]
 
 
+class AffineChildren(C.Structure):
+    _fields_ = [("mask", C.c_uint32), ("vertex_body", C.c_void_p * MAX_UNION_CHILDREN),
+                ("rest_position", C.c_void_p * MAX_UNION_CHILDREN)]
+
+
 class BlockMatrix(C.Structure):
     _fields_ = [("values", C.c_void_p), ("positions", C.c_void_p),
                 ("num_categories", C.c_uint32), ("starts", C.POINTER(C.c_uint32)),
@@ -87,8 +92,9 @@ SIGNATURES = {
     "ysp_energy_arity": (C.c_int, [I32, C.POINTER(U32), C.POINTER(U32), C.POINTER(U32),
                                    C.POINTER(U32)]),
     "ysp_element_rest_data": (C.c_int, [I32, C.POINTER(UnionF64), P, U32, D, P, P]),
-    "ysp_eval_element_energy": (C.c_int, [I32, C.POINTER(UnionF64), P, U32, P, P, U32,
-                                          C.POINTER(ScatterArgs), P]),
+    "ysp_eval_element_energy": (C.c_int, [I32, C.POINTER(UnionF64), C.POINTER(AffineChildren), P,
+                                          U32, P, P, U32, C.POINTER(ScatterArgs), P]),
+    "ysp_affine_positions": (C.c_int, [P, P, P, P, U32, P, P]),
     "ysp_element_energy_values": (C.c_int, [I32, C.POINTER(UnionF64), P, U32, P, P, P, P]),
     "ysp_block_spmv": (C.c_int, [C.POINTER(BlockMatrix), C.POINTER(BlockMatrix), P, P, U32,
                                  C.c_int, C.c_int, P]),

@@ -27,6 +27,11 @@ Energies (reference definitions, restated):
   E = (F^T F - I) / 2; p = [Minv (4), area], s = [mu, lam, dt].
 * affine-body orthogonality 1/2 |A^T A - I|_F^2 (one_bunny_partial_abd/helpers.py:132-139)
   on the 9 entries of A (row-major), times a stiffness s = [k].
+* Stable Neo-Hookean tet (examples/one_bunny/helpers.py:42-67) for the generic path
+  (vertices mapped through affine bodies): p = [Dm^-1 (9, row-major), vol],
+  s = [mu, lam, dt]; F = Ds Dm^-1 with Ds = [x1-x0, x2-x0, x3-x0].
+* inertia 1/2 m |x - (x_prev + v dt - (0, 9.8 dt^2, 0))|^2 (helpers.py:69-71) for the
+  generic path: p = [x_prev (3), v (3), m], s = [dt].
 """
 from __future__ import annotations
 
@@ -122,6 +127,28 @@ def e_affine_orthogonality(x, p, s):
     return s[0] * sp.Rational(1, 2) * sum(D[i, j] ** 2 for i in range(3) for j in range(3))
 
 
+def e_snh(x, p, s):
+    x0, x1, x2, x3 = (vec(x, i) for i in range(4))
+    Ds = sp.Matrix.hstack(x1 - x0, x2 - x0, x3 - x0)
+    Dmi = sp.Matrix(3, 3, list(p[:9]))
+    F = Ds * Dmi
+    mu, lam, dt = s[0], s[1], s[2]
+    ic = sum(F[i, j] ** 2 for i in range(3) for j in range(3))
+    J = F.det()
+    alpha = 1 + sp.Rational(3, 4) * mu / lam
+    return dt * dt * p[9] * (mu / 2 * (ic - 3) - mu / 2 * sp.log(ic + 1) + lam / 2 * (J - alpha) ** 2)
+
+
+def e_inertia(x, p, s):
+    dt = s[0]
+    xv = vec(x, 0)
+    xp = sp.Matrix(p[0:3])
+    v = sp.Matrix(p[3:6])
+    tgt = xp + v * dt - sp.Matrix([0, sp.Rational(98, 10) * dt * dt, 0])
+    d = xv - tgt
+    return p[6] * d.dot(d) / 2
+
+
 # name: (function, number of DoF, per-element params, scene params)
 ENERGIES = {
     "point_point": (e_point_point, 6, 0, 2),
@@ -132,6 +159,8 @@ ENERGIES = {
     "baraff_witkin": (e_baraff_witkin, 9, 5, 5),
     "stvk_membrane": (e_stvk_membrane, 9, 5, 3),
     "affine_orthogonality": (e_affine_orthogonality, 9, 0, 1),
+    "snh": (e_snh, 12, 10, 3),
+    "inertia": (e_inertia, 3, 7, 1),
 }
 
 

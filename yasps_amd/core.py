@@ -82,6 +82,26 @@ class attribute:
     def isPrimitiveLevel(self) -> bool:
         return isinstance(self.correspondance, (primitive, primitiveUnion))
 
+    # ------------------------------------------------------------ algebra
+    def resize(self, rows: int, cols: int) -> "attribute":
+        """Reinterpret the per-instance values as rows x cols (same size), as
+        ``attribute.resize`` does for a JOINed 9-vector used as a 3 x 3 matrix."""
+        if rows * cols != self.size:
+            raise ValueError(f"attribute {self.fullName}: cannot resize {self.size} values to {rows}x{cols}")
+        self.rows, self.cols = int(rows), int(cols)
+        return self
+
+    def __mul__(self, other):
+        from .affine import MatrixVector
+        return MatrixVector(self, other)
+
+    def __add__(self, other):
+        from .affine import AffineMap, MatrixVector
+        if isinstance(other, MatrixVector):
+            return AffineMap(self, other.matrix, other.vector)
+        raise NotImplementedError("attribute algebra: only translation + matrix * rest_position "
+                                  "(an affine-body vertex map) is supported")
+
     # ------------------------------------------------------------ values
     def updateValue(self, values, deepCopy: bool = False):
         """Set the per-instance values (``attribute.pyx:417-436``). Accepts numpy
@@ -144,7 +164,8 @@ class attribute:
             idx = self.through.value.long().reshape(-1)
             self._dev = src[idx].reshape(-1)
         elif self.operator == COMPUTED:
-            self._dev = self.expression.energy_values(self)
+            values = getattr(self.expression, "values", None)
+            self._dev = values(self) if values is not None else self.expression.energy_values(self)
         return self
 
     def __repr__(self):
@@ -233,6 +254,13 @@ class _AttributeOwner:
 
     def addConstant(self, name: str, rows: int = 1, cols: int = 1) -> attribute:
         return self._register(attribute(name, self, rows, cols, CONSTANT))
+
+    def addAttribute(self, name: str, rows: int = 0, cols: int = 0, computed_attribute=None):
+        """Scene- or mesh-level attribute: a DATA value or a computed energy (e.g. the
+        affine orthogonality of every body, one_bunny_partial_abd.py:229-230)."""
+        if computed_attribute is not None:
+            return self._register(computed_attribute.bind(name, self))
+        return self._register(attribute(name, self, rows, cols, DATA))
 
 
 class primitive(_AttributeOwner):

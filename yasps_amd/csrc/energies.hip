@@ -1,8 +1,17 @@
 // Element kernels of the generated energies (IPC barriers, bending, Baraff-Witkin,
-// StVK membrane, affine orthogonality): gather the element's vertex data through the
-// union view, evaluate energy / gradient / packed Hessian (generated_energies.hpp), PSD
-// projection with the reference's rules, and the scatter into block storage for
-// segments of any size (hessianKernelNoProject.pyx:37-264 semantics).
+// StVK membrane, affine orthogonality, and SNH / inertia for vertices driven by affine
+// bodies): gather the element's vertex data through the union view, evaluate energy /
+// gradient / packed Hessian (generated_energies.hpp), PSD projection with the
+// reference's rules, and the scatter into block storage for segments of any size
+// (hessianKernelNoProject.pyx:37-264 semantics).
+//
+// Vertices of an affine union child follow x = t + A r (JOIN of the body's A (9, row-major)
+// and t (3) through a vertex -> body connectivity, one_bunny_partial_abd.py:127-144).
+// The map is linear, so the projection acts on the Hessian in vertex positions and the
+// chain rule maps it to the body's segments (differentiator.pyx:795-801): with slot
+// entries s -> (position entry i(s), coefficient c(s)), g_s = c(s) g_i(s) and
+// H_st = c(s) c(t) H_i(s)i(t): t entries map to x_j with c = 1, A entries (j, l) to x_j
+// with c = r_l.
 #include "common.hpp"
 #include "dense.hpp"
 #include "element_common.hpp"
@@ -36,7 +45,30 @@ YSP_ENERGY_KIND(YSP_ENERGY_BENDING, bending, 4, 3, 2, 1)
 YSP_ENERGY_KIND(YSP_ENERGY_BARAFF_WITKIN, baraff_witkin, 3, 3, 5, 5)
 YSP_ENERGY_KIND(YSP_ENERGY_STVK_MEMBRANE, stvk_membrane, 3, 3, 5, 3)
 YSP_ENERGY_KIND(YSP_ENERGY_AFFINE_ORTHOGONALITY, affine_orthogonality, 1, 9, 0, 1)
+YSP_ENERGY_KIND(YSP_ENERGY_SNH, snh, 4, 3, 10, 3)
+YSP_ENERGY_KIND(YSP_ENERGY_INERTIA, inertia, 1, 3, 7, 1)
 #undef YSP_ENERGY_KIND
+
+struct AffineView {
+  uint32_t mask;                                  // bit c: union child c is affine
+  const uint32_t* body[YSP_MAX_UNION_CHILDREN];   // vertex -> body
+  const double* rest[YSP_MAX_UNION_CHILDREN];     // rest position r per vertex
+};
+
+// Union child of instance v and its index inside the child.
+__device__ __forceinline__ uint32_t union_child(const UnionView& U, uint32_t v, uint32_t& local) {
+  uint32_t total = 0;
+  for (uint32_t c = 0; c < U.nc; ++c) {
+    const uint32_t cnt = U.counts[c];
+    if (v < total + cnt) {
+      local = v - total;
+      return c;
+    }
+    total += cnt;
+  }
+  local = 0;
+  return 0;
+}
 
 // ----------------------------------------------------------------- projection
 // hessianKernelHeader.pyx:79-252 for N >= 4: the packed-Schur PSD test on the element
@@ -77,6 +109,8 @@ struct GSlots {
   uint32_t idx[KT];
   uint8_t size[KT];
   uint8_t col[KT];
+  uint8_t xbase[KT];   // first position entry of the slot's vertex
+  uint8_t amode[KT];   // 1: affine-matrix slot of an affine vertex (x_j, coefficient r_l)
   int8_t group_first[KT];
   uint32_t group_mask[KT];
   int ng;
@@ -97,6 +131,8 @@ __device__ __forceinline__ void load_gslots(const ScatterArgs& S, uint32_t e, in
     si.idx[i] = 0;
     si.size[i] = 0;
     si.col[i] = 0;
+    si.xbase[i] = static_cast<uint8_t>(col);
+    si.amode[i] = 0;
     if (static_cast<uint32_t>(i) >= K) continue;
     const size_t o = static_cast<size_t>(e) * K + i;
     const uint32_t ii = S.seg_idx[o];
@@ -126,7 +162,7 @@ __device__ __forceinline__ void load_gslots(const ScatterArgs& S, uint32_t e, in
     for (int k = 0; k < KT; ++k)
       if (k == g) si.group_mask[k] |= 1u << i;
   }
-  if (col != static_cast<uint32_t>(N)) si.ok = false;   // slots must tile the local DoFs
+  if (N >= 0 && col != static_cast<uint32_t>(N)) si.ok = false;   // slots tile the DoFs
 }
 
 template <int N>
@@ -134,22 +170,44 @@ __device__ __forceinline__ double hget(const double (&H)[N * (N + 1) / 2], int i
   return i <= j ? H[upper_index<N>(i, j)] : H[upper_index<N>(j, i)];
 }
 
+// Position entry and coefficient of entry k of slot i (identity unless the slot is the
+// affine matrix of an affine vertex).
+template <int KT>
+__device__ __forceinline__ int xmap(const GSlots<KT>& si, const double (*r)[3], int i, int k,
+                                    double& c) {
+  if (si.amode[i]) {
+    c = r[si.xbase[i] / 3][k % 3];
+    return si.xbase[i] + k / 3;
+  }
+  c = 1.0;
+  return si.xbase[i] + k;
+}
+
 template <int KT, int N>
 __device__ __forceinline__ void scatter_generic(const ScatterArgs& S, uint32_t e,
-                                                const GSlots<KT>& si, const double (&g)[N],
+                                                const GSlots<KT>& si, const double (*r)[3],
+                                                const double (&g)[N],
                                                 const double (&H)[N * (N + 1) / 2]) {
   // gradient
   if (S.grad_staging) {
     double* out = S.grad_staging + static_cast<size_t>(e) * S.grad_local;
 #pragma unroll
     for (int i = 0; i < KT; ++i)
-      for (int k = 0; k < si.size[i]; ++k) out[si.col[i] + k] = g[si.col[i] + k];
+      for (int k = 0; k < si.size[i]; ++k) {
+        double c;
+        const int x = xmap(si, r, i, k, c);
+        out[si.col[i] + k] = c * g[x];
+      }
   } else {
 #pragma unroll
     for (int i = 0; i < KT; ++i) {
       if (si.idx[i] < 2) continue;
       const uint32_t p = si.idx[i] - 2;
-      for (int k = 0; k < si.size[i]; ++k) atomic_add_f64(S.gradient + p + k, g[si.col[i] + k]);
+      for (int k = 0; k < si.size[i]; ++k) {
+        double c;
+        const int x = xmap(si, r, i, k, c);
+        atomic_add_f64(S.gradient + p + k, c * g[x]);
+      }
     }
   }
   // Hessian blocks of every valid group pair, in coordinate order
@@ -181,10 +239,14 @@ __device__ __forceinline__ void scatter_generic(const ScatterArgs& S, uint32_t e
 #pragma unroll
           for (int a = 0; a < KT; ++a) {
             if (!((si.group_mask[gi] >> a) & 1u) || si.size[a] == 0) continue;
+            double ca;
+            const int xa = xmap(si, r, a, k, ca);
 #pragma unroll
             for (int b = 0; b < KT; ++b) {
               if (!((si.group_mask[gj] >> b) & 1u) || si.size[b] == 0) continue;
-              v += hget<N>(H, si.col[a] + k, si.col[b] + l);
+              double cb;
+              const int xb = xmap(si, r, b, l, cb);
+              v += ca * cb * hget<N>(H, xa, xb);
             }
           }
           // stored block: rows of the lower global index; (l, k) of a transposed block
@@ -237,15 +299,45 @@ __device__ __forceinline__ void scene_params(const double* sp, const double* x, 
 
 template <int ID, int KT>
 __global__ void __launch_bounds__(kEB)
-    energy_eval_kernel(UnionView P, const uint32_t* __restrict__ e2v, uint32_t n,
+    energy_eval_kernel(UnionView P, AffineView AF, const uint32_t* __restrict__ e2v, uint32_t n,
                        const double* __restrict__ ep, const double* __restrict__ sp,
                        uint32_t method, ScatterArgs SA, int* __restrict__ error_flag) {
   using T = Kind<ID>;
   const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   GSlots<KT> si;
-  load_gslots<KT>(SA, e, T::N, si);
-  if (!si.ok || (SA.block_staging && T::W != 3)) {
+  load_gslots<KT>(SA, e, AF.mask ? -1 : T::N, si);
+  double r[T::A][3];
+  if (AF.mask) {
+    // slots of vertex a: [a Kv, (a + 1) Kv); an affine vertex owns A then t
+    const int Kv = static_cast<int>(SA.K) / T::A;
+    if (Kv * T::A != static_cast<int>(SA.K) || T::W != 3) si.ok = false;
+#pragma unroll
+    for (int a = 0; a < T::A; ++a) {
+      uint32_t local;
+      const uint32_t c = union_child(P, e2v[static_cast<size_t>(e) * T::A + a], local);
+      const bool aff = (AF.mask >> c) & 1u;
+      r[a][0] = r[a][1] = r[a][2] = 0.0;
+      if (aff) {
+        const double* rr = AF.rest[c] + 3 * static_cast<size_t>(local);
+        r[a][0] = rr[0];
+        r[a][1] = rr[1];
+        r[a][2] = rr[2];
+      }
+#pragma unroll
+      for (int q = 0; q < KT; ++q) {
+        if (q >= Kv) continue;
+        const int i = a * Kv + q;
+        if (i >= KT) continue;
+        si.xbase[i] = static_cast<uint8_t>(3 * a);
+        si.amode[i] = aff && q == 0;
+        const int want = aff ? (q == 0 ? 9 : (q == 1 ? 3 : -1)) : (q == 0 ? 3 : -1);
+        if (want > 0 && si.size[i] != want) si.ok = false;
+        if (want < 0 && si.idx[i] >= 2) si.ok = false;
+      }
+    }
+  }
+  if (!si.ok || (SA.block_staging && (T::W != 3 || AF.mask))) {
     atomicOr(error_flag, 1);
     return;
   }
@@ -257,7 +349,7 @@ __global__ void __launch_bounds__(kEB)
   double E, g[T::N], H[T::N * (T::N + 1) / 2];
   T::eval(x, p, s, E, g, H);
   if (method != 0) project_packed<T::N>(H, method);
-  scatter_generic<KT, T::N>(SA, e, si, g, H);
+  scatter_generic<KT, T::N>(SA, e, si, r, g, H);
 }
 
 template <int ID>
@@ -367,16 +459,32 @@ __global__ void __launch_bounds__(kEB)
 }
 
 template <int ID>
-int launch_eval(const UnionView& U, const uint32_t* e2v, uint32_t n, const double* ep,
-                const double* sp, uint32_t method, const ScatterArgs& SA, int* flag,
-                hipStream_t s) {
+int launch_eval(const UnionView& U, const AffineView& AF, const uint32_t* e2v, uint32_t n,
+                const double* ep, const double* sp, uint32_t method, const ScatterArgs& SA,
+                int* flag, hipStream_t s) {
   const dim3 g(grid_for(n, kEB)), b(kEB);
   if (SA.K <= 8)
-    hipLaunchKernelGGL((energy_eval_kernel<ID, 8>), g, b, 0, s, U, e2v, n, ep, sp, method, SA, flag);
+    hipLaunchKernelGGL((energy_eval_kernel<ID, 8>), g, b, 0, s, U, AF, e2v, n, ep, sp, method, SA,
+                       flag);
   else
-    hipLaunchKernelGGL((energy_eval_kernel<ID, kSlotsMax>), g, b, 0, s, U, e2v, n, ep, sp, method,
-                       SA, flag);
+    hipLaunchKernelGGL((energy_eval_kernel<ID, kSlotsMax>), g, b, 0, s, U, AF, e2v, n, ep, sp,
+                       method, SA, flag);
   return YSP_OK;
+}
+
+__global__ void __launch_bounds__(256)
+    affine_positions_kernel(const double* __restrict__ A, const double* __restrict__ t,
+                            const uint32_t* __restrict__ body, const double* __restrict__ rest,
+                            uint32_t n, double* __restrict__ x) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const uint32_t b = body[v];
+  const double* a = A + 9 * static_cast<size_t>(b);
+  const double* r = rest + 3 * static_cast<size_t>(v);
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+    x[3 * static_cast<size_t>(v) + j] =
+        t[3 * static_cast<size_t>(b) + j] + a[3 * j] * r[0] + a[3 * j + 1] * r[1] + a[3 * j + 2] * r[2];
 }
 
 template <int ID>
@@ -398,6 +506,8 @@ int dispatch_kind(int32_t kind, Fn&& fn) {
     case YSP_ENERGY_STVK_MEMBRANE: return fn(std::integral_constant<int, YSP_ENERGY_STVK_MEMBRANE>());
     case YSP_ENERGY_AFFINE_ORTHOGONALITY:
       return fn(std::integral_constant<int, YSP_ENERGY_AFFINE_ORTHOGONALITY>());
+    case YSP_ENERGY_SNH: return fn(std::integral_constant<int, YSP_ENERGY_SNH>());
+    case YSP_ENERGY_INERTIA: return fn(std::integral_constant<int, YSP_ENERGY_INERTIA>());
     default: return -1;
   }
 }
@@ -443,7 +553,23 @@ extern "C" int ysp_element_rest_data(int32_t kind, const ysp_union_f64* rest_pos
   return YSP_OK;
 }
 
+extern "C" int ysp_affine_positions(const double* affine_matrix, const double* translation,
+                                    const uint32_t* vertex_body, const double* rest_position,
+                                    uint32_t n, double* position, void* stream) {
+  if (n == 0) return YSP_OK;
+  if (!affine_matrix || !translation || !vertex_body || !rest_position || !position) {
+    ysp_set_error(__FILE__, __LINE__, "ysp_affine_positions: bad arguments");
+    return YSP_ERR_ARG;
+  }
+  hipLaunchKernelGGL(affine_positions_kernel, dim3(grid_for(n, 256)), dim3(256), 0,
+                     as_stream(stream), affine_matrix, translation, vertex_body, rest_position, n,
+                     position);
+  YSP_CHECK_LAUNCH();
+  return YSP_OK;
+}
+
 extern "C" int ysp_eval_element_energy(int32_t kind, const ysp_union_f64* positions,
+                                       const ysp_affine_children* affine,
                                        const uint32_t* elem2v, uint32_t n,
                                        const double* element_params, const double* scene_params,
                                        uint32_t projection_method,
@@ -462,8 +588,20 @@ extern "C" int ysp_eval_element_energy(int32_t kind, const ysp_union_f64* positi
   YSP_CHECK(hipMemsetAsync(flag, 0, sizeof(int), s));
   const UnionView U = make_view(positions);
   const ScatterArgs SA = make_scatter(out);
+  AffineView AF{};
+  if (affine) {
+    AF.mask = affine->mask & ((1u << positions->num_children) - 1u);
+    for (int c = 0; c < YSP_MAX_UNION_CHILDREN; ++c) {
+      AF.body[c] = affine->vertex_body[c];
+      AF.rest[c] = affine->rest_position[c];
+      if (((AF.mask >> c) & 1u) && !AF.rest[c]) {
+        ysp_set_error(__FILE__, __LINE__, "ysp_eval_element_energy: affine child without rest data");
+        return YSP_ERR_ARG;
+      }
+    }
+  }
   dispatch_kind(kind, [&](auto id) {
-    return launch_eval<decltype(id)::value>(U, elem2v, n, element_params, scene_params,
+    return launch_eval<decltype(id)::value>(U, AF, elem2v, n, element_params, scene_params,
                                             projection_method, SA, flag, s);
   });
   YSP_CHECK_LAUNCH();
@@ -472,7 +610,7 @@ extern "C" int ysp_eval_element_energy(int32_t kind, const ysp_union_f64* positi
   YSP_CHECK(hipStreamSynchronize(s));
   if (h) {
     ysp_set_error(__FILE__, __LINE__,
-                  "ysp_eval_element_energy: slot layout does not tile the element DoFs");
+                  "ysp_eval_element_energy: slot layout does not match the element's vertices");
     return YSP_ERR_UNSUPPORTED;
   }
   return YSP_OK;

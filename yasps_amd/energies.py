@@ -15,6 +15,7 @@ import numpy as np
 from . import _lib, device as dev
 from .core import attribute, COMPUTED, DATA, CONSTANT, JOIN, UNION, primitiveUnion
 from .program import PLeaf, PJoin, PUnion, PConcat
+from .affine import is_affine
 
 
 def _leaf(att: attribute, wrt_pos: dict) -> Optional[PLeaf]:
@@ -26,11 +27,24 @@ def _leaf(att: attribute, wrt_pos: dict) -> Optional[PLeaf]:
     return PLeaf(att.size, wrt_pos.get(att), att.isPrimitiveLevel)
 
 
+def _affine_node(att: attribute, wrt_pos: dict) -> PJoin:
+    """Slots of an affine-body vertex: the body's A (9) then t (3) through the vertex ->
+    body connectivity."""
+    m = att.expression
+    return PJoin(m.conn, 1, [PLeaf(9, wrt_pos.get(m.body_matrix), True),
+                             PLeaf(3, wrt_pos.get(m.body_translation), True)])
+
+
 def source_node(src: attribute, wrt_pos: dict):
     """Program node of the attribute a JOIN gathers from."""
+    if is_affine(src):
+        return _affine_node(src, wrt_pos)
     if src.operator == UNION:
         members = []
         for child in src.children:
+            if is_affine(child):
+                members.append([_affine_node(child, wrt_pos)])
+                continue
             leaf = _leaf(child, wrt_pos)
             if leaf is None:
                 raise NotImplementedError(
@@ -53,9 +67,17 @@ def union_view(src: attribute) -> _lib.UnionF64:
             raise NotImplementedError("union with more than 8 children")
         u.num_children = len(kids)
         u.counts = dev.ptr(src.correspondance.counts_device)
-        for i, k in enumerate(kids):
-            u.data[i] = dev.ptr(k.value)
-        u._keep = [src.correspondance.counts_device] + [k.value for k in kids]
+        vals = [k.expression.values() if is_affine(k) else k.value for k in kids]
+        for i, v in enumerate(vals):
+            u.data[i] = dev.ptr(v)
+        u._keep = [src.correspondance.counts_device] + vals
+    elif is_affine(src):
+        u.num_children = 1
+        counts = dev.to_device(np.array([src.numInstances], np.uint32), "uint32")
+        v = src.expression.values()
+        u.counts = dev.ptr(counts)
+        u.data[0] = dev.ptr(v)
+        u._keep = [counts, v]
     else:
         u.num_children = 1
         counts = dev.to_device(np.array([src.numInstances], np.uint32), "uint32")
@@ -64,6 +86,24 @@ def union_view(src: attribute) -> _lib.UnionF64:
         u._keep = [counts, src.value]
     u.width = src.size
     return u
+
+
+def affine_children(src: attribute):
+    """ysp_affine_children of a position source, or None when no child is affine."""
+    kids = src.children if src.operator == UNION else [src]
+    if not any(is_affine(k) for k in kids):
+        return None
+    a = _lib.AffineChildren()
+    keep = []
+    for i, k in enumerate(kids):
+        if is_affine(k):
+            m = k.expression
+            a.mask |= 1 << i
+            a.vertex_body[i] = dev.ptr(m.conn.value)
+            a.rest_position[i] = dev.ptr(m.r.value)
+            keep += [m.conn.value, m.r.value]
+    a._keep = keep
+    return a
 
 
 def _scalar_or_array(att, n_instances: int):
@@ -158,14 +198,33 @@ class StableNeoHookean(EnergyExpr):
         lam, lams = _scalar_or_array(self.lam, n)
         return mu, mus, lam, lams, _host_scalar(self.dt)
 
+    def _generic(self):
+        """Element/scene parameters of the generated SNH (vertices driven by affine
+        bodies take the generic chain-rule kernel)."""
+        n = self.num_instances
+        dm_inv, vol = self.rest_data()
+        t = dev.torch()
+        ep = t.cat([dm_inv.reshape(n, 9), vol.reshape(n, 1)], dim=1).contiguous()
+        sp = dev.to_device(np.array([_host_scalar(self.mu), _host_scalar(self.lam),
+                                     _host_scalar(self.dt)]), "float64")
+        return ep, sp
+
     def evaluate(self, scatter: _lib.ScatterArgs, projection_method: int):
         n = self.num_instances
         if n == 0:
             return
+        method = max(0, projection_method)
+        aff = affine_children(self.pos.source)
+        if aff is not None:
+            ep, sp = self._generic()
+            u = union_view(self.pos.source)
+            _lib.check(_lib.load().ysp_eval_element_energy(
+                KIND_IDS["snh"], C.byref(u), C.byref(aff), dev.ptr(self.conn.value), n, dev.ptr(ep),
+                dev.ptr(sp), method, C.byref(scatter), dev.stream()), "ysp_eval_element_energy")
+            return
         dm_inv, vol = self.rest_data()
         mu, mus, lam, lams, dt = self._params()
         u = union_view(self.pos.source)
-        method = max(0, projection_method)
         _lib.check(_lib.load().ysp_eval_snh_tet(
             C.byref(u), dev.ptr(self.conn.value), n, dev.ptr(dm_inv), dev.ptr(vol),
             dev.ptr(mu), mus, dev.ptr(lam), lams, dt, method, C.byref(scatter), dev.stream()),
@@ -195,16 +254,29 @@ class Inertia(EnergyExpr):
                 raise TypeError(f"inertia: {nm} must be a 3-vector attribute")
         if not (x_before.correspondance is vel.correspondance is x.correspondance):
             raise ValueError("inertia: x_before, vel and x must live on one primitive")
-        if x.operator != DATA:
-            raise TypeError("inertia: x must be a DATA attribute")
+        if x.operator != DATA and not is_affine(x):
+            raise TypeError("inertia: x must be a DATA attribute or an affine-body vertex map")
         self.xp, self.v, self.x, self.mass, self.dt = x_before, vel, x, mass, dt
+        self._iota = None
 
     def check_owner(self, owner):
         if owner is not self.x.correspondance:
             raise ValueError("inertia: bind the energy to the primitive of x")
 
     def program(self, wrt_pos: dict):
+        if is_affine(self.x):
+            return PConcat([_affine_node(self.x, wrt_pos)])
         return PConcat([PLeaf(3, wrt_pos.get(self.x), True)])
+
+    def _generic_args(self):
+        n = self.num_instances
+        t = dev.torch()
+        ep = t.cat([self.xp.value.reshape(n, 3), self.v.value.reshape(n, 3),
+                    self._mass().reshape(n, 1)], dim=1).contiguous()
+        sp = dev.to_device(np.array([_host_scalar(self.dt)]), "float64")
+        if self._iota is None or self._iota.numel() != n:
+            self._iota = dev.to_device(np.arange(n, dtype=np.uint32), "uint32")
+        return ep, sp, union_view(self.x), affine_children(self.x)
 
     def _mass(self):
         n = self.num_instances
@@ -218,6 +290,13 @@ class Inertia(EnergyExpr):
         n = self.num_instances
         if n == 0:
             return
+        if is_affine(self.x):
+            ep, sp, u, aff = self._generic_args()
+            _lib.check(_lib.load().ysp_eval_element_energy(
+                KIND_IDS["inertia"], C.byref(u), C.byref(aff), dev.ptr(self._iota), n, dev.ptr(ep),
+                dev.ptr(sp), max(0, projection_method), C.byref(scatter), dev.stream()),
+                "ysp_eval_element_energy")
+            return
         m = self._mass()
         _lib.check(_lib.load().ysp_eval_inertia(
             dev.ptr(self.x.value), dev.ptr(self.xp.value), dev.ptr(self.v.value), dev.ptr(m),
@@ -228,6 +307,12 @@ class Inertia(EnergyExpr):
         n = self.num_instances
         out = dev.empty(n, "float64")
         if n == 0:
+            return out
+        if is_affine(self.x):
+            ep, sp, u, _ = self._generic_args()
+            _lib.check(_lib.load().ysp_element_energy_values(
+                KIND_IDS["inertia"], C.byref(u), dev.ptr(self._iota), n, dev.ptr(ep), dev.ptr(sp),
+                dev.ptr(out), dev.stream()), "ysp_element_energy_values")
             return out
         m = self._mass()
         _lib.check(_lib.load().ysp_energy_inertia(
@@ -246,7 +331,8 @@ def inertia(x_before, vel, dt, x, mass) -> Inertia:
 
 # ---------------------------------------------------------------- generated energies
 KIND_IDS = {"point_point": 0, "point_edge": 1, "point_triangle": 2, "edge_edge": 3,
-            "bending": 4, "baraff_witkin": 5, "stvk_membrane": 6, "affine_orthogonality": 7}
+            "bending": 4, "baraff_witkin": 5, "stvk_membrane": 6, "affine_orthogonality": 7,
+            "snh": 8, "inertia": 9}
 
 
 class ElementEnergy(EnergyExpr):
@@ -316,6 +402,9 @@ class ElementEnergy(EnergyExpr):
     def positions(self):
         return union_view(self.pos.source)
 
+    def affine(self):
+        return affine_children(self.pos.source)
+
     def evaluate(self, scatter: _lib.ScatterArgs, projection_method: int):
         n = self.num_instances
         if n == 0:
@@ -323,9 +412,11 @@ class ElementEnergy(EnergyExpr):
         ep = self.element_params()
         sp = self._scene()
         u = self.positions()
+        aff = self.affine()
         _lib.check(_lib.load().ysp_eval_element_energy(
-            self.kind_id, C.byref(u), dev.ptr(self.elem2v()), n, dev.ptr(ep), dev.ptr(sp),
-            max(0, projection_method), C.byref(scatter), dev.stream()), "ysp_eval_element_energy")
+            self.kind_id, C.byref(u), C.byref(aff) if aff is not None else None,
+            dev.ptr(self.elem2v()), n, dev.ptr(ep), dev.ptr(sp), max(0, projection_method),
+            C.byref(scatter), dev.stream()), "ysp_eval_element_energy")
 
     def energy_values(self, att):
         n = self.num_instances
@@ -375,6 +466,9 @@ class AffineOrthogonality(ElementEnergy):
         if self._iota is None or self._iota.numel() != n:
             self._iota = dev.to_device(np.arange(n, dtype=np.uint32), "uint32")
         return self._iota
+
+    def affine(self):
+        return None
 
     def positions(self):
         u = _lib.UnionF64()

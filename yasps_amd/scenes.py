@@ -171,3 +171,128 @@ def build_yasps_scene(cs: CubeScene, projection_method: int = 2, solver_name: st
     s0.minimizer.setSolver(solver_name, **solver_options)
     return s0, dict(mesh=m, tets=tets, moving=mv, fixed=fv, snh=snh, inertia=ine, tet2v=tet2v,
                     mu=mu, lam=lam, dt=dt)
+
+
+@dataclass
+class AbdScene:
+    """Tet cube with a fixed bottom face, affine bodies over the top layers and free
+    vertices in between (``examples/one_bunny_partial_abd/one_bunny_partial_abd.py``).
+    Vertex order: fixed, affine-body vertices, free vertices."""
+    cube: CubeScene
+    n_fixed: int
+    n_abd: int
+    n_free: int
+    body_of: np.ndarray          # [n_abd] body index of every affine vertex
+    A: np.ndarray                # [bodies, 3, 3]
+    t: np.ndarray                # [bodies, 3]
+    affine_stiffness: float
+
+    @property
+    def n_bodies(self) -> int:
+        return self.A.shape[0]
+
+    def positions(self) -> np.ndarray:
+        """Current positions of all vertices (affine ones from t + A r)."""
+        cs = self.cube
+        x = cs.position.copy()
+        s = slice(self.n_fixed, self.n_fixed + self.n_abd)
+        r = cs.rest[s]
+        x[s] = self.t[self.body_of] + np.einsum("bij,bj->bi", self.A[self.body_of], r)
+        return x
+
+
+def abd_cube_scene(n: int = 4, abd_layers: int = 2, bodies: int = 2, noise: float = 0.02,
+                   affine_noise: float = 0.03, affine_stiffness: float = 1e3, seed: int = 0,
+                   **kw) -> AbdScene:
+    """Cube scene whose top ``abd_layers`` vertex layers are split (by x) into ``bodies``
+    affine bodies; A = I + N(0, affine_noise^2), t = N(0, (affine_noise h)^2)."""
+    cs = cube_scene(n, noise=noise, seed=seed, **kw)
+    nf = cs.n_fixed
+    rest = cs.rest
+    top = rest[:, 2] >= 1.0 - (abd_layers - 1) / n - 1e-12
+    top[:nf] = False
+    body = np.minimum((rest[:, 0] * bodies).astype(np.int64), bodies - 1)
+    free = ~top
+    free[:nf] = False
+    order = np.concatenate([np.arange(nf), np.flatnonzero(top), np.flatnonzero(free)])
+    inv = np.empty_like(order)
+    inv[order] = np.arange(order.size)
+    cs.rest = rest[order]
+    cs.position = cs.position[order]
+    cs.last_position = cs.last_position[order]
+    cs.velocity = cs.velocity[order]
+    cs.mass = cs.mass[order]
+    cs.tets = inv[cs.tets]
+    n_abd = int(top.sum())
+    rng = np.random.default_rng(seed + 1)
+    A = np.eye(3) + affine_noise * rng.standard_normal((bodies, 3, 3))
+    t = affine_noise / n * rng.standard_normal((bodies, 3))
+    return AbdScene(cube=cs, n_fixed=nf, n_abd=n_abd, n_free=cs.rest.shape[0] - nf - n_abd,
+                    body_of=body[order][nf:nf + n_abd], A=A, t=t,
+                    affine_stiffness=affine_stiffness)
+
+
+def build_abd_scene(sc: AbdScene, projection_method: int = 2, solver_name: str = "jacobian",
+                    **solver_options):
+    """The affine-body scene through the drop-in API, as
+    ``examples/one_bunny_partial_abd/one_bunny_partial_abd.py:102-258``: union
+    [fixed, abd, moving] vertices, abd positions = translation + affine_matrix * rest through
+    a vertex -> body connectivity, SNH on tets, inertia on abd and moving vertices, affine
+    orthogonality on the bodies; targets [moving position, affine_matrix, translation]."""
+    from . import scene, stable_neo_hookean, inertia
+    from .energies import affine_energy
+    cs = sc.cube
+    s0 = scene("scene0")
+    dt = s0.addConstant("dt", rows=1, cols=1)
+    dt.updateValue([cs.dt])
+    m = s0.addMesh("bunny")
+    fv = m.addPrimitive("fixed_vertices", numInstances=sc.n_fixed)
+    av = m.addPrimitive("abd_vertices", numInstances=sc.n_abd)
+    ab = m.addPrimitive("affine_body", numInstances=sc.n_bodies)
+    mv = m.addPrimitive("moving_vertices", numInstances=sc.n_free)
+    m.addPrimitiveUnion("vertices", [fv, av, mv])
+    tets = m.addPrimitive("tets", numInstances=cs.tets.shape[0])
+    ab.addAttribute("affine_matrix", rows=3, cols=3).updateValue(sc.A.reshape(-1))
+    ab.addAttribute("translation", rows=3, cols=1).updateValue(sc.t.reshape(-1))
+    bav2ab = av.addConnectivity("bav2ab", ab, sc.body_of, 1)
+    bavam = av.addAttribute("affine_matrix", through=bav2ab, source=ab["affine_matrix"]).resize(3, 3)
+    bavt = av.addAttribute("translation", through=bav2ab, source=ab["translation"]).resize(3, 1)
+    nf, na = sc.n_fixed, sc.n_abd
+    sa = slice(nf, nf + na)
+    sm = slice(nf + na, None)
+    av.addConstant("rest_position", rows=3, cols=1).updateValue(cs.rest[sa])
+    av.addAttribute("position", computed_attribute=bavt + bavam * av["rest_position"])
+    av.addConstant("last_position", rows=3, cols=1).updateValue(cs.last_position[sa])
+    av.addConstant("velocity", rows=3, cols=1).updateValue(cs.velocity[sa])
+    av.addConstant("mass", rows=1, cols=1).updateValue(cs.mass[sa])
+    fv.addAttribute("position", rows=3, cols=1).updateValue(cs.position[:nf])
+    fv.addConstant("rest_position", rows=3, cols=1).updateValue(cs.rest[:nf])
+    mv.addAttribute("position", rows=3, cols=1).updateValue(cs.position[sm])
+    mv.addConstant("rest_position", rows=3, cols=1).updateValue(cs.rest[sm])
+    mv.addConstant("last_position", rows=3, cols=1).updateValue(cs.last_position[sm])
+    mv.addConstant("velocity", rows=3, cols=1).updateValue(cs.velocity[sm])
+    mv.addConstant("mass", rows=1, cols=1).updateValue(cs.mass[sm])
+    m.vertices.addAttribute("position")
+    m.vertices.addAttribute("rest_position")
+    mu = m.addConstant("mu", rows=1, cols=1)
+    mu.updateValue([cs.mu])
+    lam = m.addConstant("lam", rows=1, cols=1)
+    lam.updateValue([cs.lam])
+    tet2v = tets.addConnectivity("tet2v", m.vertices, cs.tets, 4)
+    tp = tets.addAttribute("positions", through=tet2v, source=m.vertices["position"])
+    trp = tets.addAttribute("rest_positions", through=tet2v, source=m.vertices["rest_position"])
+    snh = tets.addAttribute("stable_neo_hookean",
+                            computed_attribute=stable_neo_hookean(trp, tp, mu, lam, dt))
+    aff = m.addAttribute("affine_energy",
+                         computed_attribute=affine_energy(ab["affine_matrix"], sc.affine_stiffness))
+    ia = av.addAttribute("inertia", computed_attribute=inertia(
+        av["last_position"], av["velocity"], dt, av["position"], av["mass"]))
+    im = mv.addAttribute("inertia", computed_attribute=inertia(
+        mv["last_position"], mv["velocity"], dt, mv["position"], mv["mass"]))
+    s0.addEnergy(snh, projection_method=projection_method)
+    s0.addEnergy(aff, projection_method=projection_method)
+    s0.addEnergy(ia, projection_method=-1)
+    s0.addEnergy(im, projection_method=-1)
+    s0.addMinimizeTarget([mv["position"], ab["affine_matrix"], ab["translation"]])
+    s0.minimizer.setSolver(solver_name, **solver_options)
+    return s0, dict(mesh=m, snh=snh, affine=aff, inertia_abd=ia, inertia_moving=im)
