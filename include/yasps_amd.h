@@ -449,6 +449,41 @@ int ysp_pcg3_mas_solve(uint32_t max_iterations, double threshold, int32_t thresh
                        const double* initial_guess, int32_t zero_initial_guess, double* work,
                        double* cg_scalars, ysp_pcg_stats* stats, void* stream);
 
+/* ------------------------------------------------------------ batched PCG (serving) */
+/* No reference counterpart: several independent 3x3 systems (e.g. the Newton solves of
+ * different requests) advance through the same launches; every workgroup owns a chunk
+ * of one system and reductions are segmented per system (each system's own scalar
+ * state). Block-Jacobi preconditioner, zero initial guess, same recurrence, stopping
+ * rules and return codes as ysp_pcg3_solve. */
+#define YSP_PCG3_RUNNING 0x7fffffff   /* status of a system still iterating */
+
+typedef struct ysp_pcg3_system {
+  const uint32_t* row_ptr;            /* full block-CSR (ysp_bsr3_build) */
+  const uint32_t* col;
+  const void* values;                 /* component-major, FP64 or FP32 (per batch) */
+  uint32_t nnz;
+  const void* diagonal_inverse;       /* 9 per vertex, FP64 or FP32 (per batch) */
+  const double* rhs;
+  uint32_t n;
+  double* solution;
+  double* work;                       /* 5 n doubles */
+  double* cg_scalars;                 /* YSP_PCG3_SCALARS doubles */
+  uint32_t max_iterations;
+  uint32_t passes_done;               /* managed by the library */
+  int32_t status;                     /* YSP_PCG3_RUNNING, or the ysp_pcg3_solve code */
+  int32_t iterations;
+  double delta0, delta, rr;           /* r0^T z0, r^T z, r^T r of the last test */
+} ysp_pcg3_system;
+
+/* Initial residual, preconditioned residual and threshold of one system (asynchronous). */
+int ysp_pcg3_batch_start(ysp_pcg3_system* system, double threshold, int32_t threshold_mode,
+                         int32_t inverse_f32, void* stream);
+
+/* `passes` PCG iterations of every system (finished systems idle), then their status
+ * (synchronous). */
+int ysp_pcg3_batch_run(ysp_pcg3_system* systems, uint32_t count, uint32_t passes,
+                       int32_t values_f32, int32_t inverse_f32, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

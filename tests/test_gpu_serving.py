@@ -1,0 +1,74 @@
+"""Serving: batched multi-system PCG equals the single-system solve of every request,
+continuous batching admits requests while others iterate, every result is the same alone
+and in the mix (1e-8 relative), and the HBM budget bounds the admitted set."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _alone(reqs, precision="fp64"):
+    from yasps_amd.serving import SolveServer, SolveRequest
+    out = {}
+    for r in reqs:
+        srv = SolveServer(max_active=1, keep_solutions=True, precision=precision)
+        q = SolveRequest(id=r.id, n=r.n, young=r.young, seed=r.seed, arrival=0.0)
+        (d,) = srv.serve([q])
+        out[r.id] = d
+    return out
+
+
+def test_batched_pcg_matches_single_system_solver():
+    from yasps_amd import device as dev
+    from yasps_amd.scenes import cube_scene, build_yasps_scene, lame_parameters
+    from yasps_amd.serving import SolveServer, SolveRequest
+    print("batched vs single", flush=True)
+    reqs = [SolveRequest(id=i, n=n, young=y, seed=10 + i)
+            for i, (n, y) in enumerate([(4, 1e5), (7, 3e4), (5, 8e5), (6, 2e5)])]
+    srv = SolveServer(max_active=8, keep_solutions=True)
+    done = {r.id: r for r in srv.serve(reqs)}
+    assert max(srv.batch_sizes) == 4
+    for r in reqs:
+        cs = cube_scene(r.n, young=r.young, seed=0)
+        rng = np.random.default_rng(r.seed)
+        nf = cs.n_fixed
+        cs.position[nf:] = cs.rest[nf:] + rng.normal(0.0, 0.02 / r.n, size=(cs.rest.shape[0] - nf, 3))
+        s0, _ = build_yasps_scene(cs, threshold_mode="relative_residual")
+        x = dev.to_host(s0.minimizeEnergy(tolerance=1e-6)[0])
+        it = s0.minimizer.linearSolver.statistics["iterations"]
+        d = done[r.id]
+        assert d.status >= 0 and abs(d.iterations - it) <= 1
+        assert np.linalg.norm(d.solution - x) <= 1e-8 * np.linalg.norm(x)
+
+
+def test_continuous_batching_same_alone_and_in_mix():
+    from yasps_amd.serving import SolveServer, serving_stream, poisson_arrivals, summary
+    print("continuous batching", flush=True)
+    reqs = serving_stream(count=20, n_min=4, n_max=12, seed=3)
+    srv = SolveServer(max_active=16, keep_solutions=True, passes=4)
+    srv.warm([r.n for r in reqs])
+    poisson_arrivals(reqs, rate=400.0, seed=2)
+    done = srv.serve(reqs)
+    assert len(done) == 20 and all(r.status >= 0 for r in done)
+    # requests joined while others were iterating
+    assert max(srv.batch_sizes) > 1
+    first_finish = min(r.finished for r in done)
+    assert any(r.admitted > min(q.admitted for q in done) and r.admitted < first_finish + 1e9
+               for r in done)
+    alone = _alone(done)
+    for r in done:
+        a = alone[r.id]
+        assert abs(a.iterations - r.iterations) <= 1
+        assert np.linalg.norm(a.solution - r.solution) <= 1e-8 * np.linalg.norm(a.solution)
+    st = summary(done)
+    assert st["p99_ms"] >= st["p50_ms"] > 0
+
+
+def test_hbm_budget_limits_admission():
+    from yasps_amd.serving import SolveServer, SolveRequest
+    reqs = [SolveRequest(id=i, n=6, young=1e5, seed=i) for i in range(4)]
+    srv = SolveServer(max_active=8)
+    srv.warm([6])
+    srv.budget = srv.pool[6][0].footprint          # room for exactly one request
+    done = srv.serve(reqs)
+    assert len(done) == 4 and max(srv.batch_sizes) == 1

@@ -37,6 +37,18 @@ class AffineChildren(C.Structure):
                 ("rest_position", C.c_void_p * MAX_UNION_CHILDREN)]
 
 
+class Pcg3System(C.Structure):
+    _fields_ = [("row_ptr", C.c_void_p), ("col", C.c_void_p), ("values", C.c_void_p),
+                ("nnz", C.c_uint32), ("diagonal_inverse", C.c_void_p), ("rhs", C.c_void_p),
+                ("n", C.c_uint32), ("solution", C.c_void_p), ("work", C.c_void_p),
+                ("cg_scalars", C.c_void_p), ("max_iterations", C.c_uint32),
+                ("passes_done", C.c_uint32), ("status", C.c_int32), ("iterations", C.c_int32),
+                ("delta0", C.c_double), ("delta", C.c_double), ("rr", C.c_double)]
+
+
+PCG3_RUNNING = 0x7FFFFFFF
+
+
 class BlockMatrix(C.Structure):
     _fields_ = [("values", C.c_void_p), ("positions", C.c_void_p),
                 ("num_categories", C.c_uint32), ("starts", C.POINTER(C.c_uint32)),
@@ -115,6 +127,8 @@ SIGNATURES = {
                                   P]),
     "ysp_pcg3_solve": (C.c_int, [U32, D, I32, P, P, P, I32, U32, P, I32, P, U32, P, P, I32, P, P,
                                  C.POINTER(PcgStats), P]),
+    "ysp_pcg3_batch_start": (C.c_int, [C.POINTER(Pcg3System), D, I32, I32, P]),
+    "ysp_pcg3_batch_run": (C.c_int, [C.POINTER(Pcg3System), U32, U32, I32, I32, P]),
     "ysp_mas_coarsen": (C.c_int, [P, I32, U32, P, P, U32, P, P]),
     "ysp_mas_domain_scatter": (C.c_int, [P, I32, U32, P, P, P]),
     "ysp_mas_invert_domains": (C.c_int, [P, P, U32, P, P, P]),

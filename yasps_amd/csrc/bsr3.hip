@@ -166,6 +166,10 @@ __device__ __forceinline__ double* slot_ptr(double* st, int set) {
   return st + slots_base(set) + (blockIdx.x % kSlots) * kSpace;
 }
 
+__device__ __forceinline__ double* slot_ptr_at(double* st, int set, uint32_t block) {
+  return st + slots_base(set) + (block % kSlots) * kSpace;
+}
+
 // Sum of one slot array, identical in every thread of the workgroup.
 __device__ __forceinline__ double read_slots(const double* st, int set, double* lds) {
   if (threadIdx.x < 64) {
@@ -298,19 +302,23 @@ __global__ void __launch_bounds__(64) pcg3_init_finish_kernel(double* st, double
 //   p_k = z + beta p_{k-1},  q_k = A p_k = A z + beta q_{k-1},  p_k^T q_k.
 // Only z is gathered (one neighbour gather per block); p and q are updated in place per
 // row, so the product costs one pass over the blocks like a plain SpMV.
+// Body of one workgroup of the product kernel: blk is the workgroup's chunk of block rows
+// of this system, sb its partial-sum slot, lead the workgroup that writes the status.
 template <typename VT>
-__global__ void __launch_bounds__(kB)
-    pcg3_spmv_kernel(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
-                     const VT* __restrict__ v, uint32_t stride, uint32_t nbr,
-                     const double* __restrict__ z, double* __restrict__ p,
-                     double* __restrict__ q, double* __restrict__ st, int k,
-                     uint32_t rows_per_wg) {
+__device__ __forceinline__ void pcg3_spmv_body(const uint32_t* __restrict__ row_ptr,
+                                               const uint32_t* __restrict__ col,
+                                               const VT* __restrict__ v, uint32_t stride,
+                                               uint32_t nbr, const double* __restrict__ z,
+                                               double* __restrict__ p, double* __restrict__ q,
+                                               double* __restrict__ st, int k,
+                                               uint32_t rows_per_wg, uint32_t blk, uint32_t sb,
+                                               bool lead_wg) {
   if (stopped(st)) return;
   const int cur = k % 3, prev = (k + 2) % 3, prev2 = (k + 1) % 3;
   __shared__ double lds[kB / 64];
   double rz1, rr1, rz2;
   read_slots3(st, SET_RZ + prev, SET_RR + prev, SET_RZ + prev2, lds, rz1, rr1, rz2);
-  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  const bool lead = lead_wg && threadIdx.x == 0;
   double beta = 0.0;
   if (k > 1) {
     if (!isfinite(rz1) || rz1 < 0.0 || !isfinite(rz2) || !(rz2 > 0.0)) {
@@ -339,7 +347,6 @@ __global__ void __launch_bounds__(kB)
   // Each workgroup walks a contiguous chunk of rows_per_wg block rows (16 at a time), so
   // the grid stays near a few thousand workgroups: fewer partial-sum atomics and fewer
   // slot reads by the consumers than one workgroup per 16 rows.
-  const uint32_t blk = xcd_swizzle(blockIdx.x, gridDim.x);
   const uint32_t r_end = min(nbr, (blk + 1) * rows_per_wg);
   const int l = threadIdx.x % kG;
   double pap = 0.0;
@@ -358,7 +365,18 @@ __global__ void __launch_bounds__(kB)
       pap += pi * qi;
     }
   }
-  wg_add(pap, slot_ptr(st, SET_PAP + cur), lds);
+  wg_add(pap, slot_ptr_at(st, SET_PAP + cur, sb), lds);
+}
+
+template <typename VT>
+__global__ void __launch_bounds__(kB)
+    pcg3_spmv_kernel(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
+                     const VT* __restrict__ v, uint32_t stride, uint32_t nbr,
+                     const double* __restrict__ z, double* __restrict__ p,
+                     double* __restrict__ q, double* __restrict__ st, int k,
+                     uint32_t rows_per_wg) {
+  pcg3_spmv_body<VT>(row_ptr, col, v, stride, nbr, z, p, q, st, k, rows_per_wg,
+                     xcd_swizzle(blockIdx.x, gridDim.x), blockIdx.x, blockIdx.x == 0);
 }
 
 // Iteration k, kernel 2: alpha, x += alpha p, r -= alpha q, z = M^-1 r, r.z and r.r.
@@ -366,16 +384,18 @@ __global__ void __launch_bounds__(kB)
 // whole vertices, whose residual 3-vectors meet in LDS for the block-Jacobi product.
 constexpr unsigned kUB = 192;
 template <typename MT>
-__global__ void __launch_bounds__(kUB)
-    pcg3_update_kernel(const MT* __restrict__ minv, double* __restrict__ x, double* __restrict__ r,
-                       const double* __restrict__ p, const double* __restrict__ q,
-                       double* __restrict__ z, uint32_t n, double* __restrict__ st, int k) {
+__device__ __forceinline__ void pcg3_update_body(const MT* __restrict__ minv,
+                                                 double* __restrict__ x, double* __restrict__ r,
+                                                 const double* __restrict__ p,
+                                                 const double* __restrict__ q,
+                                                 double* __restrict__ z, uint32_t n,
+                                                 double* __restrict__ st, int k, uint32_t blk) {
   if (stopped(st)) return;
   const int cur = k % 3, prev = (k + 2) % 3, next = (k + 1) % 3;
   __shared__ double lds[4];
   double pap, rz1, unused;
   read_slots3(st, SET_PAP + cur, SET_RZ + prev, SET_RZ + prev, lds, pap, rz1, unused);
-  const uint32_t t = blockIdx.x * kUB + threadIdx.x;
+  const uint32_t t = blk * kUB + threadIdx.x;
   if (!isfinite(pap) || pap <= 0.0) {
     if (t == 0) {
       st[S_STATUS] = -1.0;
@@ -384,7 +404,7 @@ __global__ void __launch_bounds__(kUB)
     return;
   }
   const double alpha = rz1 / pap;
-  if (blockIdx.x == 0) {
+  if (blk == 0) {
     zero_set(st, SET_PAP + next);
     zero_set(st, SET_RZ + next);
     zero_set(st, SET_RR + next);
@@ -418,9 +438,17 @@ __global__ void __launch_bounds__(kUB)
   if (threadIdx.x == 0) {
     rz = part[0][0] + part[0][1] + part[0][2];
     rr = part[1][0] + part[1][1] + part[1][2];
-    if (minv) atomic_add_f64(slot_ptr(st, SET_RZ + cur), rz);
-    atomic_add_f64(slot_ptr(st, SET_RR + cur), rr);
+    if (minv) atomic_add_f64(slot_ptr_at(st, SET_RZ + cur, blk), rz);
+    atomic_add_f64(slot_ptr_at(st, SET_RR + cur, blk), rr);
   }
+}
+
+template <typename MT>
+__global__ void __launch_bounds__(kUB)
+    pcg3_update_kernel(const MT* __restrict__ minv, double* __restrict__ x, double* __restrict__ r,
+                       const double* __restrict__ p, const double* __restrict__ q,
+                       double* __restrict__ z, uint32_t n, double* __restrict__ st, int k) {
+  pcg3_update_body<MT>(minv, x, r, p, q, z, n, st, k, blockIdx.x);
 }
 
 // Pinned ring of header copies, one per chunk of iterations in flight.
@@ -847,4 +875,194 @@ extern "C" int ysp_pcg3_mas_solve(uint32_t max_iterations, double threshold,
   return pcg3_run(max_iterations, threshold, threshold_mode, row_ptr, col,
                   static_cast<const double*>(values), nnz, no_minv, m, rhs, n, solution, x0, work,
                   cg_scalars, stats, s);
+}
+
+// ------------------------------------------------------------------ batched PCG
+// Several independent 3x3 systems advance together: every product / update launch
+// covers all of them, a workgroup reads which system and which chunk of it it owns from
+// a work table, and each system keeps its own scalar state and partial-sum slots
+// (segmented reductions). The arithmetic of a system does not depend on the others in
+// the batch (same row chunks and slots as the single-system loop).
+namespace {
+struct BatchSystem {
+  const uint32_t* row_ptr;
+  const uint32_t* col;
+  const void* values;
+  const void* minv;
+  double* x;
+  double* r;
+  double* z;
+  double* p;
+  double* q;
+  double* st;
+  uint32_t stride;
+  uint32_t nbr;
+  uint32_t n;
+  uint32_t rows_per_wg;
+  int32_t k0;     // iteration number of the first pass of this run
+  int32_t kmax;   // max_iterations
+};
+
+struct BatchWork {
+  uint32_t sys;
+  uint32_t blk;
+};
+
+uint32_t spmv_rows_per_wg(uint32_t nv) {
+  constexpr uint32_t kSpmvWgs = 2048;
+  return std::max<uint32_t>(kB / kG,
+                            (grid_for(nv, kSpmvWgs) + kB / kG - 1) / (kB / kG) * (kB / kG));
+}
+
+template <typename VT>
+__global__ void __launch_bounds__(kB)
+    batch_spmv_kernel(const BatchSystem* __restrict__ S, const BatchWork* __restrict__ W,
+                      int pass) {
+  const BatchWork w = W[blockIdx.x];
+  const BatchSystem& sy = S[w.sys];
+  const int k = sy.k0 + pass;
+  if (k > sy.kmax + 1) return;
+  pcg3_spmv_body<VT>(sy.row_ptr, sy.col, static_cast<const VT*>(sy.values), sy.stride, sy.nbr,
+                     sy.z, sy.p, sy.q, sy.st, k, sy.rows_per_wg, w.blk, w.blk, w.blk == 0);
+}
+
+template <typename MT>
+__global__ void __launch_bounds__(kUB)
+    batch_update_kernel(const BatchSystem* __restrict__ S, const BatchWork* __restrict__ W,
+                        int pass) {
+  const BatchWork w = W[blockIdx.x];
+  const BatchSystem& sy = S[w.sys];
+  const int k = sy.k0 + pass;
+  if (k > sy.kmax) return;
+  pcg3_update_body<MT>(static_cast<const MT*>(sy.minv), sy.x, sy.r, sy.p, sy.q, sy.z, sy.n, sy.st,
+                       k, w.blk);
+}
+
+constexpr int kBatchOut = 5;
+__global__ void batch_status_kernel(const BatchSystem* __restrict__ S, uint32_t count,
+                                    double* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const double* st = S[i].st;
+  out[kBatchOut * i + 0] = st[S_STATUS];
+  out[kBatchOut * i + 1] = st[S_ITER];
+  out[kBatchOut * i + 2] = st[S_FINAL_RZ];
+  out[kBatchOut * i + 3] = st[S_FINAL_RR];
+  out[kBatchOut * i + 4] = st[S_RZ0];
+}
+
+template <typename VT, typename MT>
+int batch_run(ysp_pcg3_system* systems, uint32_t count, uint32_t passes, hipStream_t s) {
+  std::vector<BatchSystem> host(count);
+  std::vector<BatchWork> wsp, wup;
+  for (uint32_t i = 0; i < count; ++i) {
+    ysp_pcg3_system& y = systems[i];
+    const uint32_t n = y.n, nv = n / 3;
+    BatchSystem& b = host[i];
+    b.row_ptr = y.row_ptr;
+    b.col = y.col;
+    b.values = y.values;
+    b.minv = y.diagonal_inverse;
+    b.x = y.solution;
+    b.r = y.work;
+    b.z = y.work + n;
+    b.p = y.work + 2 * static_cast<size_t>(n);
+    b.q = y.work + 4 * static_cast<size_t>(n);
+    b.st = y.cg_scalars;
+    b.stride = pad64(y.nnz);
+    b.nbr = nv;
+    b.n = n;
+    b.rows_per_wg = spmv_rows_per_wg(nv);
+    b.k0 = static_cast<int32_t>(y.passes_done) + 1;
+    b.kmax = static_cast<int32_t>(y.max_iterations);
+    for (uint32_t k = 0, nb = grid_for(nv, b.rows_per_wg); k < nb; ++k) wsp.push_back({i, k});
+    for (uint32_t k = 0, nb = grid_for(n, kUB); k < nb; ++k) wup.push_back({i, k});
+  }
+  BatchSystem* dsys = static_cast<BatchSystem*>(scratch(35, sizeof(BatchSystem) * count));
+  BatchWork* dsp = static_cast<BatchWork*>(scratch(36, sizeof(BatchWork) * wsp.size()));
+  BatchWork* dup = static_cast<BatchWork*>(scratch(37, sizeof(BatchWork) * wup.size()));
+  double* dout = static_cast<double*>(scratch(38, sizeof(double) * kBatchOut * count));
+  if (!dsys || !dsp || !dup || !dout) return YSP_ERR_HIP;
+  YSP_CHECK(hipMemcpyAsync(dsys, host.data(), sizeof(BatchSystem) * count, hipMemcpyHostToDevice, s));
+  YSP_CHECK(hipMemcpyAsync(dsp, wsp.data(), sizeof(BatchWork) * wsp.size(), hipMemcpyHostToDevice, s));
+  YSP_CHECK(hipMemcpyAsync(dup, wup.data(), sizeof(BatchWork) * wup.size(), hipMemcpyHostToDevice, s));
+  const uint32_t nsp = static_cast<uint32_t>(wsp.size()), nup = static_cast<uint32_t>(wup.size());
+  for (uint32_t pass = 0; pass < passes; ++pass) {
+    hipLaunchKernelGGL(batch_spmv_kernel<VT>, dim3(nsp), dim3(kB), 0, s, dsys, dsp,
+                       static_cast<int>(pass));
+    hipLaunchKernelGGL(batch_update_kernel<MT>, dim3(nup), dim3(kUB), 0, s, dsys, dup,
+                       static_cast<int>(pass));
+  }
+  hipLaunchKernelGGL(batch_status_kernel, dim3(grid_for(count, 64)), dim3(64), 0, s, dsys, count,
+                     dout);
+  YSP_CHECK_LAUNCH();
+  std::vector<double> out(kBatchOut * count);
+  YSP_CHECK(hipMemcpyAsync(out.data(), dout, sizeof(double) * out.size(), hipMemcpyDeviceToHost, s));
+  YSP_CHECK(hipStreamSynchronize(s));
+  for (uint32_t i = 0; i < count; ++i) {
+    ysp_pcg3_system& y = systems[i];
+    y.passes_done += passes;
+    const double st = out[kBatchOut * i], it = out[kBatchOut * i + 1];
+    const int iters = static_cast<int>(it);
+    if (st == 1.0) y.status = iters;
+    else if (st == -1.0) y.status = -iters - 4;
+    else if (st == -2.0) y.status = -5;
+    else y.status = (y.passes_done > y.max_iterations) ? -4 : YSP_PCG3_RUNNING;
+    y.iterations = st == 0.0 ? static_cast<int32_t>(std::min(y.passes_done, y.max_iterations)) : iters;
+    y.delta = out[kBatchOut * i + 2];
+    y.rr = out[kBatchOut * i + 3];
+    y.delta0 = out[kBatchOut * i + 4];
+  }
+  return YSP_OK;
+}
+}  // namespace
+
+extern "C" int ysp_pcg3_batch_start(ysp_pcg3_system* y, double threshold, int32_t threshold_mode,
+                                    int32_t inverse_f32, void* stream) {
+  if (!y || y->n % 3 || !y->work || !y->cg_scalars || !y->rhs || !y->solution ||
+      (threshold_mode != 0 && threshold_mode != 1)) {
+    ysp_set_error(__FILE__, __LINE__, "ysp_pcg3_batch_start: bad arguments");
+    return YSP_ERR_ARG;
+  }
+  y->passes_done = 0;
+  y->status = YSP_PCG3_RUNNING;
+  y->iterations = 0;
+  if (y->n == 0) {
+    y->status = 0;
+    return YSP_OK;
+  }
+  hipStream_t s = as_stream(stream);
+  const uint32_t n = y->n, nv = n / 3;
+  YSP_CHECK(hipMemsetAsync(y->cg_scalars, 0, kStateDoubles * sizeof(double), s));
+  const dim3 vg(grid_for(nv, kB)), bl(kB);
+  if (inverse_f32)
+    hipLaunchKernelGGL(pcg3_init_kernel<float>, vg, bl, 0, s,
+                       static_cast<const float*>(y->diagonal_inverse), y->rhs, nullptr, nullptr,
+                       y->solution, y->work, y->work + n, nv, y->cg_scalars);
+  else
+    hipLaunchKernelGGL(pcg3_init_kernel<double>, vg, bl, 0, s,
+                       static_cast<const double*>(y->diagonal_inverse), y->rhs, nullptr, nullptr,
+                       y->solution, y->work, y->work + n, nv, y->cg_scalars);
+  hipLaunchKernelGGL(pcg3_init_finish_kernel, dim3(1), dim3(64), 0, s, y->cg_scalars, threshold,
+                     threshold_mode, 0);
+  YSP_CHECK_LAUNCH();
+  return YSP_OK;
+}
+
+extern "C" int ysp_pcg3_batch_run(ysp_pcg3_system* systems, uint32_t count, uint32_t passes,
+                                  int32_t values_f32, int32_t inverse_f32, void* stream) {
+  if (count == 0 || passes == 0) return YSP_OK;
+  if (!systems) return YSP_ERR_ARG;
+  for (uint32_t i = 0; i < count; ++i)
+    if (systems[i].n == 0 || systems[i].n % 3 || !systems[i].diagonal_inverse) {
+      ysp_set_error(__FILE__, __LINE__, "ysp_pcg3_batch_run: bad system");
+      return YSP_ERR_ARG;
+    }
+  hipStream_t s = as_stream(stream);
+  if (values_f32) {
+    return inverse_f32 ? batch_run<float, float>(systems, count, passes, s)
+                       : batch_run<float, double>(systems, count, passes, s);
+  }
+  return inverse_f32 ? batch_run<double, float>(systems, count, passes, s)
+                     : batch_run<double, double>(systems, count, passes, s);
 }

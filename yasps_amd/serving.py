@@ -1,0 +1,254 @@
+"""Continuous-batching server of Newton linear solves (device-resident, one GPU).
+
+Requests (a tet-cube mesh of n^3 hexes, a Young's modulus and a noise seed) arrive over
+time. The server admits an arrived request when a slot is free and its HBM footprint
+fits the budget, keeps its mesh, Hessian and Krylov state on the device, and advances all
+admitted requests together: every chunk of PCG passes is one sequence of batched
+launches (``ysp_pcg3_batch_run``) in which each workgroup owns a chunk of one request's
+system and every request reduces into its own scalar state. Requests that arrive while
+others iterate join at the next chunk; finished ones leave and free their slot.
+
+Meshes of one size share a pool of slots whose static sparsity structure (index
+pipeline, block layout, full block-CSR) is built once; a request only reloads positions
+and material, evaluates the elements and assembles (``hessian.compute``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from . import _lib, device as dev
+from .scenes import cube_scene, build_yasps_scene, lame_parameters
+
+_MODES = {"reference": 0, "relative_residual": 1}
+
+
+@dataclass
+class SolveRequest:
+    id: int
+    n: int
+    young: float
+    seed: int
+    arrival: float = 0.0          # seconds after the stream starts
+    noise: float = 0.02
+    # filled by the server
+    admitted: Optional[float] = None
+    finished: Optional[float] = None
+    status: Optional[int] = None
+    iterations: int = 0
+    solution: Optional[np.ndarray] = None
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+class _Slot:
+    """One device-resident problem of mesh size n (reused across requests)."""
+
+    def __init__(self, n: int, precision: str):
+        self.n = n
+        self.precision = precision
+        cs = cube_scene(n)
+        self.scene, self.h = build_yasps_scene(cs, threshold_mode="relative_residual")
+        self.cs = cs
+        self.scene.minimizer.computeNumericValue()       # builds the static structure
+        hs = self.scene.minimizer.activeHessian
+        if hs.full is None:
+            raise RuntimeError("serving needs the 3x3 full block-CSR path")
+        self.hs = hs
+        m = hs.rows
+        self.solution = dev.zeros(m, "float64")
+        self.work = dev.zeros(5 * m, "float64")
+        self.scalars = dev.zeros(12288, "float64")        # YSP_PCG3_SCALARS
+        self.minv32 = dev.empty(hs.diagonal_blocks_inverse.numel(), "float32") \
+            if precision == "mixed" else None
+        f = hs.full
+        staging = getattr(hs.static, "staging", None)
+        self.footprint = int(
+            8 * (hs.static.values.numel() + 9 * f.nnz + 5 * m + 2 * hs.diagonal_blocks.numel())
+            + (4 * 9 * f.nnz if precision == "mixed" else 0)
+            + 4 * (2 * f.nnz + m // 3)
+            + (staging.numel() * staging.element_size() if staging is not None else 0))
+        self.system = _lib.Pcg3System()
+        self.request: Optional[SolveRequest] = None
+
+    def load(self, req: SolveRequest, tolerance: float, max_iterations: int):
+        """Positions and material of the request, element evaluation and assembly, block
+        Jacobi inverse, and the first Krylov state."""
+        rng = np.random.default_rng(req.seed)
+        cs = self.cs
+        nf = cs.n_fixed
+        h = 1.0 / cs.n
+        pos = cs.rest[nf:] + rng.normal(0.0, req.noise * h, size=(cs.rest.shape[0] - nf, 3))
+        mu, lam = lame_parameters(req.young, cs.poisson)
+        self.h["moving"]["position"].updateValue(pos)
+        self.h["mu"].updateValue([mu])
+        self.h["lam"].updateValue([lam])
+        hs = self.scene.minimizer.computeNumericValue()
+        lib = _lib.load()
+        s = dev.stream()
+        counts = [w.numInstances for w in hs.wrt]
+        sizes = [w.size for w in hs.wrt]
+        _lib.check(lib.ysp_invert_diagonal_blocks(
+            dev.ptr(hs.diagonal_blocks), dev.ptr(hs.diagonal_blocks_inverse),
+            _lib.u32_array(hs.diagonal_blocks_start_cpu[:-1]), _lib.u32_array(counts),
+            _lib.u32_array(sizes), len(hs.wrt), s), "ysp_invert_diagonal_blocks")
+        f = hs.full
+        y = self.system
+        mixed = self.precision == "mixed"
+        if mixed:
+            values = hs.full_values_f32()
+            nd = hs.diagonal_blocks_inverse.numel()
+            _lib.check(lib.ysp_cast_f32(dev.ptr(hs.diagonal_blocks_inverse), dev.ptr(self.minv32), nd, s),
+                       "ysp_cast_f32")
+            minv = self.minv32
+        else:
+            values = f.v64
+            minv = hs.diagonal_blocks_inverse
+        self._keep = (values, minv)
+        y.row_ptr, y.col, y.values, y.nnz = dev.ptr(f.row_ptr), dev.ptr(f.col), dev.ptr(values), f.nnz
+        y.diagonal_inverse = dev.ptr(minv)
+        y.rhs = dev.ptr(hs.gradient.value)
+        y.n = hs.rows
+        y.solution, y.work, y.cg_scalars = dev.ptr(self.solution), dev.ptr(self.work), dev.ptr(self.scalars)
+        y.max_iterations = int(max_iterations)
+        _lib.check(lib.ysp_pcg3_batch_start(C.byref(y), float(tolerance), 1, int(mixed), s),
+                   "ysp_pcg3_batch_start")
+        self.request = req
+
+
+class SolveServer:
+    """Admission by HBM footprint and slot count, batched PCG chunks, retirement."""
+
+    def __init__(self, hbm_budget_bytes: int = 64 << 30, max_active: int = 32, passes: int = 8,
+                 tolerance: float = 1e-6, max_iterations: int = 20000, precision: str = "fp64",
+                 keep_solutions: bool = False):
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.budget = int(hbm_budget_bytes)
+        self.max_active = int(max_active)
+        self.passes = int(passes)
+        self.tolerance = float(tolerance)
+        self.max_iterations = int(max_iterations)
+        self.precision = precision
+        self.keep_solutions = keep_solutions
+        self.pool: Dict[int, List[_Slot]] = {}
+        self.chunks = 0
+        self.batch_sizes: List[int] = []
+
+    # ------------------------------------------------------------ slots
+    def _in_use(self) -> int:
+        return sum(s.footprint for slots in self.pool.values() for s in slots if s.request is not None)
+
+    def warm(self, sizes):
+        """Build one idle slot per mesh size (static structures) before serving."""
+        for n in sorted(set(int(x) for x in sizes)):
+            if not self.pool.get(n):
+                self.pool.setdefault(n, []).append(_Slot(n, self.precision))
+        dev.torch().cuda.synchronize()
+
+    def _acquire(self, n: int) -> Optional[_Slot]:
+        for s in self.pool.get(n, []):
+            if s.request is None:
+                return s
+        slot = _Slot(n, self.precision)
+        self.pool.setdefault(n, []).append(slot)
+        return slot
+
+    def _fits(self, n: int) -> bool:
+        idle = [s for s in self.pool.get(n, []) if s.request is None]
+        need = idle[0].footprint if idle else self._estimate(n)
+        return self._in_use() + need <= self.budget
+
+    def _estimate(self, n: int) -> int:
+        known = [s.footprint for s in self.pool.get(n, [])]
+        if known:
+            return known[0]
+        # ~1.5 KB per tet (values, staging, vectors): conservative until a slot exists
+        return int(1536 * 6 * n ** 3)
+
+    # ------------------------------------------------------------ loop
+    def serve(self, requests: List[SolveRequest], clock=time.perf_counter) -> List[SolveRequest]:
+        lib = _lib.load()
+        torch = dev.torch()
+        pending = sorted(requests, key=lambda r: r.arrival)
+        active: List[_Slot] = []
+        done: List[SolveRequest] = []
+        t0 = clock()
+        s = dev.stream()
+        mixed = int(self.precision == "mixed")
+        while pending or active:
+            now = clock() - t0
+            while pending and pending[0].arrival <= now and len(active) < self.max_active \
+                    and self._fits(pending[0].n):
+                req = pending.pop(0)
+                slot = self._acquire(req.n)
+                slot.load(req, self.tolerance, self.max_iterations)
+                req.admitted = clock() - t0
+                active.append(slot)
+            if not active:
+                if pending:
+                    wait = pending[0].arrival - (clock() - t0)
+                    if wait > 0:
+                        time.sleep(wait)
+                continue
+            arr = (_lib.Pcg3System * len(active))(*[sl.system for sl in active])
+            _lib.check(lib.ysp_pcg3_batch_run(arr, len(active), self.passes, mixed, mixed, s),
+                       "ysp_pcg3_batch_run")
+            self.chunks += 1
+            self.batch_sizes.append(len(active))
+            t_end = clock() - t0
+            keep = []
+            for sl, y in zip(active, arr):
+                sl.system = y
+                if y.status != _lib.PCG3_RUNNING:
+                    req = sl.request
+                    req.finished = t_end
+                    req.status = int(y.status)
+                    req.iterations = int(y.iterations)
+                    if self.keep_solutions:
+                        torch.cuda.synchronize()
+                        req.solution = dev.to_host(sl.solution).copy()
+                    sl.request = None
+                    done.append(req)
+                else:
+                    keep.append(sl)
+            active = keep
+        return done
+
+
+def serving_stream(count: int = 512, n_min: int = 6, n_max: int = 48, young_min: float = 1e4,
+                   young_max: float = 1e6, seed: int = 0) -> List[SolveRequest]:
+    """Requests with n log-uniform in [n_min, n_max] and E log-uniform in [young_min,
+    young_max] (arrival times set by ``poisson_arrivals``)."""
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(math.log(n_min), math.log(n_max + 1), count)).astype(int)
+    ns = np.clip(ns, n_min, n_max)
+    young = np.exp(rng.uniform(math.log(young_min), math.log(young_max), count))
+    return [SolveRequest(id=i, n=int(ns[i]), young=float(young[i]), seed=int(seed * 100000 + i))
+            for i in range(count)]
+
+
+def poisson_arrivals(requests: List[SolveRequest], rate: float, seed: int = 1):
+    """Exponential inter-arrival times at ``rate`` requests per second."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, len(requests)))
+    for r, a in zip(requests, t):
+        r.arrival = float(a)
+    return requests
+
+
+def summary(done: List[SolveRequest]) -> dict:
+    lat = np.array([r.latency for r in done])
+    span = max(r.finished for r in done) - min(r.arrival for r in done)
+    return {"requests": len(done), "solves_per_s": len(done) / span,
+            "p50_ms": 1e3 * float(np.percentile(lat, 50)),
+            "p99_ms": 1e3 * float(np.percentile(lat, 99)),
+            "converged": int(sum(1 for r in done if r.status is not None and r.status >= 0)),
+            "mean_iterations": float(np.mean([r.iterations for r in done]))}
