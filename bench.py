@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the mixed-precision and MAS legs")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving stream")
+    ap.add_argument("--serving-requests", type=int, default=512)
     ap.add_argument("--kernels-only", action="store_true",
                     help="one solve, the SpMV roofline launches and a calibration copy of a "
                          "known size, no JSON line (for hardware-counter runs)")
@@ -159,6 +161,30 @@ def form_leg(cs, args, x64, ref_value, solver, **options):
             "rel_err_vs_fp64": float(torch.linalg.norm(x - x64) / torch.linalg.norm(x64))}
 
 
+def serving_leg(args):
+    """Serving stream: requests with n log-uniform in [6, 48] and E log-uniform in
+    [1e4, 1e6], first all at once (capacity), then Poisson arrivals at 80% of that
+    capacity; p50/p99 latency from arrival to solution."""
+    from yasps_amd.serving import SolveServer, serving_stream, poisson_arrivals, summary
+    reqs = serving_stream(count=args.serving_requests, seed=args.seed)
+    srv = SolveServer(max_active=32, passes=8, tolerance=args.tol)
+    t0 = time.perf_counter()
+    srv.warm([r.n for r in reqs])
+    warm_s = time.perf_counter() - t0
+    cap = summary(srv.serve([type(r)(id=r.id, n=r.n, young=r.young, seed=r.seed) for r in reqs]))
+    rate = 0.8 * cap["solves_per_s"]
+    poisson_arrivals(reqs, rate, seed=args.seed + 1)
+    srv.batch_sizes.clear()
+    st = summary(srv.serve(reqs))
+    return {"requests": len(reqs), "capacity_solves_per_s": round(cap["solves_per_s"], 2),
+            "arrival_rate_per_s": round(rate, 2), "value": round(st["solves_per_s"], 2),
+            "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
+            "converged": st["converged"], "mean_pcg_iterations": round(st["mean_iterations"], 1),
+            "mean_batch": round(float(np.mean(srv.batch_sizes)), 2),
+            "max_batch": int(max(srv.batch_sizes)), "warm_seconds": round(warm_s, 2),
+            "solver": "batched block-Jacobi PCG, FP64, ||g-Hx||/||g|| <= %g" % args.tol}
+
+
 def cpu_baseline(cs, values_host, st, tol):
     """The oracle on the host: SNH + inertia assembly timed on 1/8 of the tets (scaled x8)
     and block-Jacobi CG (scipy, one thread) timed in full on the same matrix."""
@@ -265,6 +291,7 @@ def main():
         "true_rel_residual": true_rel,
         "roofline": roof,
         "forms": forms,
+        "serving": None if args.no_serving else serving_leg(args),
     }
     if not args.no_cpu:
         from oracle import newton

@@ -181,6 +181,19 @@ int ysp_energy_inertia(const double* position, const double* last_position,
                        const double* velocity, const double* mass, double dt,
                        uint32_t num_vertices, double* energy, void* stream);
 
+/* Material adjoint (no reference counterpart; the form of a material optimisation like
+ * examples/bunny_material_optimization): dtheta[e] = -adjoint_e . dg_e/dtheta_e for
+ * theta_e = log E_e of every tet, where g_e is the SNH element gradient (mu, lam
+ * proportional to E make dg_e/dtheta_e = g_e) and adjoint_e gathers the adjoint vector
+ * (the solution of H lambda = dL/dx) at the tet's target segments; `slots` carries the
+ * term's segment indices / sizes / permutations (the other fields are unused).
+ * Synchronous. */
+int ysp_snh_theta_adjoint(const ysp_union_f64* positions, const uint32_t* tet2v,
+                          uint32_t num_tets, const double* dm_inv, const double* vol,
+                          const double* mu, uint32_t mu_stride, const double* lam,
+                          uint32_t lam_stride, double dt, const ysp_scatter_args* slots,
+                          const double* adjoint, double* dtheta, void* stream);
+
 /* ------------------------------------------------ generated element energies */
 /* Energies of the reference examples other than SNH/inertia, each evaluated per element
  * from its vertex data (arity vertices of `width` doubles gathered through the union

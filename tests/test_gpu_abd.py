@@ -60,3 +60,40 @@ def test_abd_energy_values():
     ia = energies.inertia_energy(x[nf:nf + na], cs.last_position[nf:nf + na], cs.velocity[nf:nf + na],
                                  cs.mass[nf:nf + na], cs.dt)
     assert _rel(dev.to_host(hd["inertia_abd"].compute().value), ia) < REL
+
+
+def test_abd_index_pipeline_bit_exact():
+    """JOIN inside UNION (affine vertices carry their body's A and t slots): index
+    buffers, permutations, coordinates and the global block layout against the oracle."""
+    from yasps_amd import device as dev
+    from oracle import indices as ix
+    sc = abd_cube_scene(n=3, abd_layers=2, bodies=2, seed=5)
+    s0, hd = build_abd_scene(sc)
+    hs = s0.minimizer.computeNumericValue()
+    cs = sc.cube
+    nf, na, nm, nb = sc.n_fixed, sc.n_abd, sc.n_free, sc.n_bodies
+    body = sc.body_of.astype(np.uint32).reshape(-1, 1)
+    affine = ix.Join(body, [ix.Leaf(9, 1), ix.Leaf(3, 2)])
+    union = ix.Union(counts=[nf, na, nm], members=[[ix.Leaf(3, None)], [affine], [ix.Leaf(3, 0)]])
+    wrt_start = [0, 3 * nm, 3 * nm + 9 * nb]
+    progs = [(ix.Concat([ix.Join(cs.tets.astype(np.uint32), [union])]), cs.tets.shape[0]),
+             (ix.Concat([ix.Leaf(9, 1)]), nb),
+             (ix.Concat([ix.Join(body, [ix.Leaf(9, 1), ix.Leaf(3, 2)])]), na),
+             (ix.Concat([ix.Leaf(3, 0)]), nm)]
+    terms = [ix.term_indices(p, cnt, wrt_start) for p, cnt in progs]
+    layout = ix.build_layout(terms)
+    for ti, ot in zip(hs.static_index, terms):
+        K = ti.K
+        assert K == ot.K
+        np.testing.assert_array_equal(dev.to_host(ti.indices, "uint32").reshape(-1, K), ot.indices)
+        np.testing.assert_array_equal(dev.to_host(ti.sizes, "uint16").reshape(-1, K), ot.sizes)
+        np.testing.assert_array_equal(dev.to_host(ti.permutations).reshape(-1, K),
+                                      ot.compression.permutations)
+        np.testing.assert_array_equal(dev.to_host(ti.coord_outer, "uint32"),
+                                      ot.compression.coordinates_outer)
+    np.testing.assert_array_equal(dev.to_host(hs.static.positions, "uint32"), layout.block_positions)
+    assert hs.static.starts == layout.blocks_start_indices
+    assert hs.static.counts == layout.block_counts
+    assert hs.static.dims == layout.block_dimensions
+    for ti, lk in zip(hs.static_index, layout.lookups):
+        np.testing.assert_array_equal(dev.to_host(ti.lookups, "uint32"), lk)

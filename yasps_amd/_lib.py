@@ -101,6 +101,8 @@ SIGNATURES = {
     "ysp_eval_inertia": (C.c_int, [P, P, P, P, D, U32, U32, C.POINTER(ScatterArgs), P]),
     "ysp_energy_snh_tet": (C.c_int, [C.POINTER(UnionF64), P, U32, P, P, P, U32, P, U32, D, P, P]),
     "ysp_energy_inertia": (C.c_int, [P, P, P, P, D, U32, P, P]),
+    "ysp_snh_theta_adjoint": (C.c_int, [C.POINTER(UnionF64), P, U32, P, P, P, U32, P, U32, D,
+                                        C.POINTER(ScatterArgs), P, P, P]),
     "ysp_energy_arity": (C.c_int, [I32, C.POINTER(U32), C.POINTER(U32), C.POINTER(U32),
                                    C.POINTER(U32)]),
     "ysp_element_rest_data": (C.c_int, [I32, C.POINTER(UnionF64), P, U32, D, P, P]),

@@ -495,6 +495,39 @@ __global__ void __launch_bounds__(kBlock)
   energy[e] = snh_energy_value(S, mu);
 }
 
+// Adjoint of the material parameter theta_e = log E_e of every tet: with mu and lam both
+// proportional to E, the SNH energy (and so its gradient) is homogeneous of degree one
+// in (mu, lam), hence dg_e/dtheta_e = mu dg_e/dmu + lam dg_e/dlam = g_e, and
+// dL/dtheta_e = -lambda_e . g_e for the adjoint solution lambda of H lambda = dL/dx
+// (gathered at the element's target segments).
+template <int KT>
+__global__ void __launch_bounds__(kBlock)
+    snh_theta_adjoint_kernel(UnionView P, const uint32_t* __restrict__ tet2v, uint32_t n,
+                             SnhParams prm, ScatterArgs SA, const double* __restrict__ adjoint,
+                             double* __restrict__ out, int* __restrict__ error_flag) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  Snh S;
+  double mu;
+  snh_load(P, tet2v, prm, e, S, mu);
+  SlotInfo<KT> si;
+  load_slots<KT>(SA, e, si);
+  if (!si.ok) {
+    atomicOr(error_flag, 1);
+    return;
+  }
+  double acc = 0.0;
+#pragma unroll
+  for (int i = 0; i < KT; ++i) {
+    if (si.idx[i] < 2 || si.vtx[i] < 0) continue;
+    const double* l = adjoint + (si.idx[i] - 2);
+    const int a = si.vtx[i];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) acc += l[k] * S.s * (S.a1 * S.u[a][k] + S.a3 * S.w[a][k]);
+  }
+  out[e] = -acc;
+}
+
 // ------------------------------------------------------------------ inertia
 struct InertiaArgs {
   const double* x;
@@ -630,6 +663,36 @@ extern "C" int ysp_energy_snh_tet(const ysp_union_f64* positions, const uint32_t
                      as_stream(stream), make_view(positions), tet2v, num_tets, prm, energy);
   YSP_CHECK_LAUNCH();
   return YSP_OK;
+}
+
+extern "C" int ysp_snh_theta_adjoint(const ysp_union_f64* positions, const uint32_t* tet2v,
+                                     uint32_t num_tets, const double* dm_inv, const double* vol,
+                                     const double* mu, uint32_t mu_stride, const double* lam,
+                                     uint32_t lam_stride, double dt,
+                                     const ysp_scatter_args* slots, const double* adjoint,
+                                     double* dtheta, void* stream) {
+  if (num_tets == 0) return YSP_OK;
+  if (!positions || positions->num_children == 0 ||
+      positions->num_children > YSP_MAX_UNION_CHILDREN || positions->width != 3 || !slots ||
+      slots->max_num_indices > kMaxSlots || !adjoint || !dtheta) {
+    ysp_set_error(__FILE__, __LINE__, "ysp_snh_theta_adjoint: bad arguments");
+    return YSP_ERR_ARG;
+  }
+  hipStream_t s = as_stream(stream);
+  int* flag = static_cast<int*>(scratch(39, sizeof(int)));
+  if (!flag) return YSP_ERR_HIP;
+  YSP_CHECK(hipMemsetAsync(flag, 0, sizeof(int), s));
+  SnhParams prm{dm_inv, vol, mu, mu_stride, lam, lam_stride, dt, 0};
+  const ScatterArgs SA = make_scatter(slots);
+  const dim3 g(grid_for(num_tets, kBlock)), b(kBlock);
+  if (slots->max_num_indices == 8)
+    hipLaunchKernelGGL(snh_theta_adjoint_kernel<8>, g, b, 0, s, make_view(positions), tet2v,
+                       num_tets, prm, SA, adjoint, dtheta, flag);
+  else
+    hipLaunchKernelGGL(snh_theta_adjoint_kernel<kMaxSlots>, g, b, 0, s, make_view(positions),
+                       tet2v, num_tets, prm, SA, adjoint, dtheta, flag);
+  YSP_CHECK_LAUNCH();
+  return check_flag(flag, s, "ysp_snh_theta_adjoint: slot layout not supported");
 }
 
 extern "C" int ysp_eval_inertia(const double* position, const double* last_position,
