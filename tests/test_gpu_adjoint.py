@@ -17,7 +17,7 @@ def setup():
     theta_t = np.log(1e5) + 0.3 * np.random.default_rng(1).standard_normal(prob.num_tets)
     xt = prob.equilibrium(theta_t)
     target = xt.reshape(-1, 3)[prob.top].copy()
-    x = prob.equilibrium(theta)
+    x = prob.equilibrium(theta, tol=1e-13)
     return prob, theta, target, x
 
 

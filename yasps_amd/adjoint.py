@@ -102,33 +102,50 @@ class MaterialProblem:
         return slv.solution.clone()
 
     # ------------------------------------------------------------ forward
-    def equilibrium(self, theta, x0=None, tol: float = 1e-11, max_newton: int = 50,
-                    pcg_tol: float = 1e-10):
+    def equilibrium(self, theta, x0=None, tol: float = 1e-12, max_newton: int = 60,
+                    pcg_tol: float = 1e-11):
         """Static equilibrium x* (free vertex positions, flat) for theta."""
         self.set_theta(theta)
         x = (self.cs.rest[self.cs.n_fixed:].reshape(-1) if x0 is None else np.asarray(x0)).copy()
         self._set_x(x)
         fnorm = float(np.linalg.norm(self.f_host))
         self.newton_iterations = 0
-        for it in range(max_newton):
-            hs, r = self._residual()
-            rn = float(dev.torch().linalg.norm(r))
-            if rn <= tol * fnorm:
-                break
-            d = dev.to_host(self._solve(hs, r, pcg_tol))
-            pi0 = self.potential()
-            slope = -float(dev.to_host(r) @ d)
-            alpha = 1.0
-            while True:
-                self._set_x(x - alpha * d)
-                if self.potential() <= pi0 + 1e-4 * alpha * slope or alpha < 1e-8:
+        history = []
+        hs = self.scene.minimizer._active()
+        saved = list(hs.terms)
+        try:
+            for it in range(max_newton):
+                # projected Hessians far from equilibrium, the exact one near it (quadratic
+                # convergence; the stable equilibrium's Hessian is positive definite)
+                rel = history[-1] / fnorm if history else 1.0
+                method = None if rel > 1e-4 else 0
+                hs.terms = saved if method is None else [(e, method) for e, _ in saved]
+                hs, r = self._residual()
+                rn = float(dev.torch().linalg.norm(r))
+                history.append(rn)
+                if rn <= tol * fnorm:
                     break
-                alpha *= 0.5
-            x = x - alpha * d
-            self._set_x(x)
-            self.newton_iterations = it + 1
-        else:
-            raise RuntimeError("material problem: Newton did not converge")
+                # at the rounding floor the residual stops falling: accept it there
+                if len(history) >= 3 and rn <= 1e-11 * fnorm and rn >= 0.5 * history[-3]:
+                    break
+                d = dev.to_host(self._solve(hs, r, pcg_tol))
+                alpha = 1.0
+                if rn > 1e-6 * fnorm:          # backtracking on the potential
+                    pi0 = self.potential()
+                    slope = -float(dev.to_host(r) @ d)
+                    while True:
+                        self._set_x(x - alpha * d)
+                        if self.potential() <= pi0 + 1e-4 * alpha * slope or alpha < 1e-8:
+                            break
+                        alpha *= 0.5
+                x = x - alpha * d
+                self._set_x(x)
+                self.newton_iterations = it + 1
+            else:
+                raise RuntimeError("material problem: Newton did not converge")
+        finally:
+            hs.terms = saved
+        self.residual_history = history
         return x
 
     def loss(self, x, target):
