@@ -46,9 +46,7 @@ __global__ void emit_entries_kernel(const uint32_t* __restrict__ pos, uint32_t n
   if (b >= nb) return;
   const uint32_t r = pos[2 * b] / 3, c = pos[2 * b + 1] / 3;
   const bool diag = r == c;
-  // wave-aggregated slot reservation
-  const uint32_t need = diag ? 1u : 2u;
-  const uint32_t slot = atomicAdd(cursor, need);
+  const uint32_t slot = wave_reserve(cursor, diag ? 1u : 2u);
   keys[slot] = static_cast<uint64_t>(r) * nbr + c;
   vals[slot] = b | tag;
   if (!diag) {

@@ -41,6 +41,26 @@ __device__ __forceinline__ void atomic_add_f32(float* p, float v) {
   unsafeAtomicAdd(p, v);
 }
 
+// Number of lanes below the calling lane whose bit is set in mask.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+
+// Reserve `need` (0..3) consecutive entries of a counter for each active lane of the wave
+// with one atomic per wave, in lane order; returns the lane's first entry.
+__device__ __forceinline__ uint32_t wave_reserve(uint32_t* counter, uint32_t need) {
+  const uint64_t b0 = __ballot(need & 1u), b1 = __ballot((need >> 1) & 1u);
+  const uint64_t active = __ballot(1);
+  const uint32_t prefix = lanes_below(b0) + 2u * lanes_below(b1);
+  const uint32_t total = __popcll(b0) + 2u * __popcll(b1);
+  const int leader = __ffsll(static_cast<unsigned long long>(active)) - 1;
+  uint32_t base = 0;
+  if (lanes_below(active) == 0 && total) base = atomicAdd(counter, total);
+  base = __shfl(base, leader);
+  return base + prefix;
+}
+
 // Sum over a 64-lane wavefront (all lanes active).
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {

@@ -28,154 +28,158 @@ constexpr unsigned kBlock = 128;
 
 constexpr int kMaxSlots = 16;
 
-// Slot bookkeeping of one element (hessianKernelNoProject.pyx:93-163): groups of slots
-// with the same |perm| in first-occurrence order, and the local 3-vector of every slot.
-// KT is the compile-time slot capacity; S.K the slot count (equal when specialised).
-template <int KT>
-struct SlotInfo {
-  uint32_t idx[KT];
-  int8_t vtx[KT];      // local vertex of the slot, -1 for zero-size padding
-  int8_t group_first[KT];
-  uint32_t group_mask[KT];
-  int ng;
+// Slot bookkeeping of one element (hessianKernelNoProject.pyx:93-163), per local vertex.
+// Slots come in first-occurrence groups of equal |perm|; every 3-wide slot belongs to the
+// next local vertex (its column / 3). Here each local vertex owns at most one slot, so the
+// element is described by, per vertex: its target index, whether the slot is a target
+// (index >= 2, in a group) and the rank of its group among the element's target groups.
+// Everything is indexed by compile-time vertex numbers (a run-time index into a private
+// array would place it in scratch memory). KT is the compile-time slot capacity, S.K the
+// slot count; NV the element's local vertices. An element that repeats a target vertex
+// (two local vertices in one group) is rejected (ok = false).
+template <int NV>
+struct VertexSlots {
+  uint32_t idx[NV];   // seg_idx of the vertex's slot, 0 without one
+  bool has[NV];       // the vertex owns a 3-wide slot
+  bool target[NV];    // ... with a target index (>= 2) in a group
+  int rank[NV];       // rank of the vertex's group among the target groups
+  int ntarget;
   bool ok;
 };
 
-template <int KT>
-__device__ __forceinline__ void load_slots(const ScatterArgs& S, uint32_t e, SlotInfo<KT>& si) {
+template <int KT, int NV>
+__device__ __forceinline__ void load_vertex_slots(const ScatterArgs& S, uint32_t e,
+                                                  VertexSlots<NV>& vs) {
   const uint32_t K = S.K;
-  si.ng = 0;
-  si.ok = K <= static_cast<uint32_t>(KT);
-  if (!si.ok) return;
+  vs.ok = K <= static_cast<uint32_t>(KT);
+  int grp[NV];
+#pragma unroll
+  for (int a = 0; a < NV; ++a) {
+    vs.idx[a] = 0;
+    vs.has[a] = false;
+    grp[a] = -1;
+  }
   int placement[KT];
+  int ng = 0;
   uint32_t col = 0;
 #pragma unroll
   for (int i = 0; i < KT; ++i) {
-    si.idx[i] = 0;
-    si.vtx[i] = -1;
-    if (static_cast<uint32_t>(i) >= K) continue;
+    if (!vs.ok || static_cast<uint32_t>(i) >= K) continue;
     const size_t o = static_cast<size_t>(e) * K + i;
     const uint32_t ii = S.seg_idx[o];
     const uint32_t sz = S.seg_size[o];
     const int pm = S.perm[o];
-    si.idx[i] = ii;
-    if (sz == 3) {
-      si.vtx[i] = static_cast<int8_t>(col / 3);
-    } else if (sz != 0) {
-      si.ok = false;
-    }
-    col += sz;
-    if (pm == 0) continue;
-    const int p = (pm < 0 ? -pm : pm) - 1;
     int g = -1;
-#pragma unroll
-    for (int k = 0; k < KT; ++k)
-      if (k < si.ng && placement[k] == p) g = k;
-    if (g < 0) {
-      g = si.ng++;
+    if (pm != 0) {
+      const int p = (pm < 0 ? -pm : pm) - 1;
 #pragma unroll
       for (int k = 0; k < KT; ++k)
-        if (k == g) {
-          placement[k] = p;
-          si.group_first[k] = static_cast<int8_t>(i);
-          si.group_mask[k] = 0;
-        }
-    }
+        if (k < ng && placement[k] == p) g = k;
+      if (g < 0) {
+        g = ng++;
 #pragma unroll
-    for (int k = 0; k < KT; ++k)
-      if (k == g) si.group_mask[k] |= 1u << i;
+        for (int k = 0; k < KT; ++k)
+          if (k == g) placement[k] = p;
+      }
+    }
+    if (sz == 3) {
+      const uint32_t v = col / 3;
+      if (v >= static_cast<uint32_t>(NV)) vs.ok = false;
+#pragma unroll
+      for (int a = 0; a < NV; ++a)
+        if (v == static_cast<uint32_t>(a)) {
+          vs.idx[a] = ii;
+          vs.has[a] = true;
+          grp[a] = g;
+        }
+    } else if (sz != 0) {
+      vs.ok = false;
+    }
+    col += sz;
+  }
+  vs.ntarget = 0;
+#pragma unroll
+  for (int a = 0; a < NV; ++a) {
+    vs.target[a] = vs.has[a] && vs.idx[a] >= 2 && grp[a] >= 0;
+    vs.ntarget += vs.target[a];
+  }
+#pragma unroll
+  for (int a = 0; a < NV; ++a) {
+    int r = 0;
+#pragma unroll
+    for (int b = 0; b < NV; ++b) {
+      r += vs.target[b] && grp[b] < grp[a];
+      if (b > a && vs.target[a] && vs.target[b] && grp[a] == grp[b]) vs.ok = false;
+    }
+    vs.rank[a] = r;
   }
 }
 
 // Gradient segments (hessianKernelNoProject.pyx:63-88): FP64 atomics into the gradient,
-// or, staged, the element's whole local gradient written in place.
-template <int KT, typename GradFn>
+// or, staged, the element's whole local gradient written in place. grad(a, g3) gives the
+// 3-vector of local vertex a.
+template <int NV, typename GradFn>
 __device__ __forceinline__ void scatter_gradient(const ScatterArgs& S, uint32_t e,
-                                                 const SlotInfo<KT>& si, GradFn grad) {
-  if (S.grad_staging) {
-    double* out = S.grad_staging + static_cast<size_t>(e) * S.grad_local;
+                                                 const VertexSlots<NV>& vs, GradFn grad) {
 #pragma unroll
-    for (int i = 0; i < KT; ++i) {
-      if (si.vtx[i] < 0) continue;
-      double g3[3];
-      grad(si.vtx[i], g3);
-      out[3 * si.vtx[i]] = g3[0];
-      out[3 * si.vtx[i] + 1] = g3[1];
-      out[3 * si.vtx[i] + 2] = g3[2];
-    }
-    return;
-  }
-#pragma unroll
-  for (int i = 0; i < KT; ++i) {
-    if (si.idx[i] < 2 || si.vtx[i] < 0) continue;
-    const uint32_t p = si.idx[i] - 2;
+  for (int a = 0; a < NV; ++a) {
+    if (S.grad_staging ? !vs.has[a] : (!vs.has[a] || vs.idx[a] < 2)) continue;
     double g3[3];
-    grad(si.vtx[i], g3);
-    atomic_add_f64(S.gradient + p, g3[0]);
-    atomic_add_f64(S.gradient + p + 1, g3[1]);
-    atomic_add_f64(S.gradient + p + 2, g3[2]);
+    grad(a, g3);
+    if (S.grad_staging) {
+      double* out = S.grad_staging + static_cast<size_t>(e) * S.grad_local + 3 * a;
+      out[0] = g3[0];
+      out[1] = g3[1];
+      out[2] = g3[2];
+    } else {
+      const uint32_t p = vs.idx[a] - 2;
+      atomic_add_f64(S.gradient + p, g3[0]);
+      atomic_add_f64(S.gradient + p + 1, g3[1]);
+      atomic_add_f64(S.gradient + p + 2, g3[2]);
+    }
   }
 }
 
-// Hessian blocks (hessianKernelNoProject.pyx:165-264). blockFn(va, vb, out[3][3]) adds the
-// local 3x3 block between local vertices va and vb into out. The k-th valid group pair
-// of the element is coordinate coordStart + k: FP64 atomics at lookups[coordinate], or,
-// staged, a plain store of the oriented block at staging[coordinate].
-template <int KT, typename BlockFn>
+// Hessian blocks (hessianKernelNoProject.pyx:165-264). blockfn(a, b, out[3][3]) adds the
+// local 3x3 block between local vertices a and b into out. The target-group pair of ranks
+// lo <= hi is coordinate coordStart + lo*n - lo(lo-1)/2 + (hi-lo) (pairs in first-
+// occurrence order); its block is oriented with the rows of the smaller target index:
+// FP64 atomics at lookups[coordinate], or, staged, a plain store at staging[coordinate].
+template <int NV, typename BlockFn>
 __device__ __forceinline__ void scatter_hessian(const ScatterArgs& S, uint32_t e,
-                                                const SlotInfo<KT>& si, BlockFn blockfn) {
-  uint32_t count = 0;
+                                                const VertexSlots<NV>& vs, BlockFn blockfn) {
   const uint32_t cstart = S.coord_outer[e];
+  const int n = vs.ntarget;
 #pragma unroll
-  for (int gi = 0; gi < KT; ++gi) {
-    if (gi >= si.ng) continue;
-    const uint32_t ii = si.idx[si.group_first[gi]];
-    if (ii < 2) continue;
+  for (int a = 0; a < NV; ++a)
 #pragma unroll
-    for (int gj = gi; gj < KT; ++gj) {
-      if (gj >= si.ng) continue;
-      const uint32_t jj = si.idx[si.group_first[gj]];
-      if (jj < 2) continue;
+    for (int b = a; b < NV; ++b) {
+      if (!vs.target[a] || !vs.target[b]) continue;
       double blk[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-#pragma unroll
-      for (int a = 0; a < KT; ++a) {
-        if (!((si.group_mask[gi] >> a) & 1u) || si.vtx[a] < 0) continue;
-#pragma unroll
-        for (int b = 0; b < KT; ++b) {
-          if (!((si.group_mask[gj] >> b) & 1u) || si.vtx[b] < 0) continue;
-          blockfn(si.vtx[a], si.vtx[b], blk);
-        }
-      }
-      const uint32_t coord = cstart + count;
+      blockfn(a, b, blk);
+      const int lo = min(vs.rank[a], vs.rank[b]), hi = max(vs.rank[a], vs.rank[b]);
+      const uint32_t coord = cstart + static_cast<uint32_t>(lo * n - lo * (lo - 1) / 2 + (hi - lo));
+      const bool tr = !(vs.idx[a] < vs.idx[b]);   // includes the diagonal block a == b
       if (S.block_staging) {
         double* dst = S.block_staging + 9 * static_cast<size_t>(coord);
-        const bool tr = !(ii < jj);
 #pragma unroll
         for (int k = 0; k < 3; ++k)
 #pragma unroll
           for (int l = 0; l < 3; ++l) dst[3 * k + l] = tr ? blk[l][k] : blk[k][l];
       } else {
         double* dst = S.blocks + S.lookups[coord];
-        if (ii < jj) {
+        double* dd = nullptr;
+        if (a == b && S.diag_blocks) dd = S.diag_blocks + diag_offset(S, vs.idx[a] - 2, 3);
 #pragma unroll
-          for (int k = 0; k < 3; ++k)
+        for (int k = 0; k < 3; ++k)
 #pragma unroll
-            for (int l = 0; l < 3; ++l) atomic_add_f64(dst + 3 * k + l, blk[k][l]);
-        } else {
-          double* dd = nullptr;
-          if (gi == gj && S.diag_blocks) dd = S.diag_blocks + diag_offset(S, ii - 2, 3);
-#pragma unroll
-          for (int k = 0; k < 3; ++k)
-#pragma unroll
-            for (int l = 0; l < 3; ++l) {
-              atomic_add_f64(dst + 3 * k + l, blk[l][k]);
-              if (dd) atomic_add_f64(dd + 3 * k + l, blk[l][k]);
-            }
-        }
+          for (int l = 0; l < 3; ++l) {
+            const double v = tr ? blk[l][k] : blk[k][l];
+            atomic_add_f64(dst + 3 * k + l, v);
+            if (dd) atomic_add_f64(dd + 3 * k + l, v);
+          }
       }
-      ++count;
     }
-  }
 }
 
 // ------------------------------------------------------------------ SNH element
@@ -274,6 +278,12 @@ __device__ __forceinline__ void snh_block(const Snh& S, const double* ga, const 
     }
 }
 
+// Gradient of local vertex a.
+__device__ __forceinline__ void snh_gradient_at(const Snh& S, int a, double g3[3]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) g3[i] = S.s * (S.a1 * S.u[a][i] + S.a3 * S.w[a][i]);
+}
+
 __device__ __forceinline__ double snh_energy_value(const Snh& S, double mu) {
   const double alpha_gap = S.a3 / S.lam;  // J - alpha
   return S.s * (0.5 * mu * (S.ic - 3.0) - 0.5 * mu * log(S.ic + 1.0) +
@@ -353,22 +363,32 @@ __global__ void __launch_bounds__(kBlock)
                     int* __restrict__ error_flag) {
   const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
-  Snh S;
-  double mu;
-  snh_load(P, tet2v, prm, e, S, mu);
-  SlotInfo<KT> si;
-  load_slots<KT>(SA, e, si);
-  if (!si.ok) {
+  VertexSlots<4> vs;
+  load_vertex_slots<KT>(SA, e, vs);
+  if (!vs.ok) {
     atomicOr(error_flag, 1);
     return;
   }
-  scatter_gradient<KT>(SA, e, si, [&](int a, double* g3) {
+  auto scatter_blocks = [&](const Snh& S) {
+    scatter_hessian(SA, e, vs, [&](int a, int b, double out[3][3]) {
+      double blk[3][3];
+      snh_block(S, S.g[a], S.g[b], S.u[a], S.u[b], S.w[a], S.w[b], blk);
 #pragma unroll
-    for (int i = 0; i < 3; ++i) g3[i] = S.s * (S.a1 * S.u[a][i] + S.a3 * S.w[a][i]);
-  });
-  bool psd = true;
-  if (prm.method != 0) {
-    double L[78];
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) out[i][k] += blk[i][k];
+    });
+  };
+  double L[78];   // lower packed element Hessian for the PSD test
+  {
+    Snh S;
+    double mu;
+    snh_load(P, tet2v, prm, e, S, mu);
+    scatter_gradient(SA, e, vs, [&](int a, double* g3) { snh_gradient_at(S, a, g3); });
+    if (prm.method == 0) {
+      scatter_blocks(S);
+      return;
+    }
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -383,20 +403,16 @@ __global__ void __launch_bounds__(kBlock)
             if (c <= r) L[r * (r + 1) / 2 + c] = blk[i][k];
           }
       }
-    psd = psd_test_packed<12>(L);
   }
-  if (psd) {
-    scatter_hessian<KT>(SA, e, si, [&](int a, int b, double out[3][3]) {
-      double blk[3][3];
-      snh_block(S, S.g[a], S.g[b], S.u[a], S.u[b], S.w[a], S.w[b], blk);
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) out[i][k] += blk[i][k];
-    });
+  // The test destroys L; a PSD element re-derives its blocks from the tet (the element
+  // state is not kept alive across the test, which keeps the kernel within 256 registers).
+  if (psd_test_packed<12>(L)) {
+    Snh S;
+    double mu;
+    snh_load(P, tet2v, prm, e, S, mu);
+    scatter_blocks(S);
   } else {
-    const uint32_t slot = atomicAdd(qcount, 1u);
-    queue[slot] = e;
+    queue[wave_reserve(qcount, 1u)] = e;
   }
 }
 
@@ -465,9 +481,9 @@ __global__ void __launch_bounds__(64)
       for (int k = 0; k < 9; ++k) acc += V[i][k] * lam_c[k] * V[j][k];
       Mp[upper_index<9>(i, j)] = acc;
     }
-  SlotInfo<KT> si;
-  load_slots<KT>(SA, e, si);
-  scatter_hessian<KT>(SA, e, si, [&](int a, int b, double out[3][3]) {
+  VertexSlots<4> vs;
+  load_vertex_slots<KT>(SA, e, vs);
+  scatter_hessian(SA, e, vs, [&](int a, int b, double out[3][3]) {
 #pragma unroll
     for (int k = 0; k < 3; ++k)
 #pragma unroll
@@ -510,20 +526,21 @@ __global__ void __launch_bounds__(kBlock)
   Snh S;
   double mu;
   snh_load(P, tet2v, prm, e, S, mu);
-  SlotInfo<KT> si;
-  load_slots<KT>(SA, e, si);
-  if (!si.ok) {
+  VertexSlots<4> vs;
+  load_vertex_slots<KT>(SA, e, vs);
+  if (!vs.ok) {
     atomicOr(error_flag, 1);
     return;
   }
   double acc = 0.0;
 #pragma unroll
-  for (int i = 0; i < KT; ++i) {
-    if (si.idx[i] < 2 || si.vtx[i] < 0) continue;
-    const double* l = adjoint + (si.idx[i] - 2);
-    const int a = si.vtx[i];
+  for (int a = 0; a < 4; ++a) {
+    if (!vs.has[a] || vs.idx[a] < 2) continue;
+    const double* l = adjoint + (vs.idx[a] - 2);
+    double g3[3];
+    snh_gradient_at(S, a, g3);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) acc += l[k] * S.s * (S.a1 * S.u[a][k] + S.a3 * S.w[a][k]);
+    for (int k = 0; k < 3; ++k) acc += l[k] * g3[k];
   }
   out[e] = -acc;
 }
@@ -558,17 +575,17 @@ __global__ void __launch_bounds__(kBlock)
   // Hessian m I: PSD for m >= -tol; otherwise the clamp rule on the eigenvalue m.
   double mh = m;
   if (method != 0 && m < -1.0e-10 * fmax(1.0, fabs(m))) mh = (method == 1) ? -m : 0.0;
-  SlotInfo<KT> si;
-  load_slots<KT>(SA, e, si);
-  if (!si.ok) {
+  VertexSlots<1> vs;
+  load_vertex_slots<KT>(SA, e, vs);
+  if (!vs.ok) {
     atomicOr(error_flag, 1);
     return;
   }
-  scatter_gradient<KT>(SA, e, si, [&](int, double* g3) {
+  scatter_gradient(SA, e, vs, [&](int, double* g3) {
 #pragma unroll
     for (int i = 0; i < 3; ++i) g3[i] = m * d[i];
   });
-  scatter_hessian<KT>(SA, e, si, [&](int, int, double out[3][3]) {
+  scatter_hessian(SA, e, vs, [&](int, int, double out[3][3]) {
 #pragma unroll
     for (int i = 0; i < 3; ++i) out[i][i] += mh;
   });
@@ -645,7 +662,7 @@ extern "C" int ysp_eval_snh_tet(const ysp_union_f64* positions, const uint32_t* 
   if (out->max_num_indices == 8) run(std::integral_constant<int, 8>());
   else run(std::integral_constant<int, kMaxSlots>());
   YSP_CHECK_LAUNCH();
-  return check_flag(flag, s, "ysp_eval_snh_tet: slot layout not supported (K>16 or size!=3)");
+  return check_flag(flag, s, "ysp_eval_snh_tet: slot layout not supported (K>16, size!=3 or a repeated vertex)");
 }
 
 extern "C" int ysp_energy_snh_tet(const ysp_union_f64* positions, const uint32_t* tet2v,
