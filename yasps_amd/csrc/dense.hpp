@@ -97,11 +97,29 @@ __host__ __device__ constexpr int upper_index(int p, int q) {
   return p * N - p * (p - 1) / 2 + (q - p);
 }
 
+// 1/x to full double precision for finite normal x: the hardware reciprocal estimate and
+// two Newton steps (each at least doubles the correct bits).
+__device__ __forceinline__ double recip_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(fma(-x, r, 1.0), r, r);
+  return fma(fma(-x, r, 1.0), r, r);
+}
+
+// 1/sqrt(x) for x in [1, 2]: the hardware estimate and two Newton steps.
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double hx = 0.5 * x;
+  y = y * fma(-hx * y, y, 1.5);
+  return y * fma(-hx * y, y, 1.5);
+}
+
 // Cyclic Jacobi on the packed upper triangle a (N(N+1)/2 entries) of a symmetric matrix,
-// with the symmetric two-sided update of each rotation (Golub & Van Loan, section 8.5,
-// in Rutishauser's tau form): only the 2(N-2) entries of rows/columns p and q and the
-// three entries of the 2x2 pivot block change. On return the diagonal holds the
-// eigenvalues and V's columns the eigenvectors.
+// with the symmetric two-sided update of each rotation (Golub & Van Loan, section 8.5):
+// only the 2(N-2) entries of rows/columns p and q and the three entries of the 2x2 pivot
+// block change. The rotation t = tan(phi) = sgn(d) y / (|d| + sqrt(d^2 + y^2)) with
+// d = a_qq - a_pp, y = 2 a_pq (the smaller root of t^2 + 2 (d/y) t - 1 = 0), written with
+// one square root and one reciprocal; c = 1/sqrt(1 + t^2), s = t c. On return the
+// diagonal holds the eigenvalues and V's columns the eigenvectors.
 template <int N>
 __device__ __forceinline__ void jacobi_eigen_packed(double (&a)[N * (N + 1) / 2],
                                                     double (&V)[N][N]) {
@@ -131,28 +149,28 @@ __device__ __forceinline__ void jacobi_eigen_packed(double (&a)[N * (N + 1) / 2]
         const double apq = a[upper_index<N>(p, q)];
         if (apq == 0.0) continue;
         const double app = a[upper_index<N>(p, p)], aqq = a[upper_index<N>(q, q)];
-        const double theta = (aqq - app) / (2.0 * apq);
-        const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(1.0 + theta * theta));
-        const double c = 1.0 / sqrt(1.0 + t * t);
+        const double d = aqq - app, y = 2.0 * apq;
+        const double r = sqrt(fma(d, d, y * y));
+        const double t = (d >= 0.0 ? y : -y) * recip_nr(fabs(d) + r);
+        const double c = rsqrt_nr(fma(t, t, 1.0));
         const double s = t * c;
-        const double tau = s / (1.0 + c);
-        a[upper_index<N>(p, p)] = app - t * apq;
-        a[upper_index<N>(q, q)] = aqq + t * apq;
+        a[upper_index<N>(p, p)] = fma(-t, apq, app);
+        a[upper_index<N>(q, q)] = fma(t, apq, aqq);
         a[upper_index<N>(p, q)] = 0.0;
 #pragma unroll
-        for (int r = 0; r < N; ++r) {
-          if (r == p || r == q) continue;
-          const int ip = r < p ? upper_index<N>(r, p) : upper_index<N>(p, r);
-          const int iq = r < q ? upper_index<N>(r, q) : upper_index<N>(q, r);
-          const double arp = a[ip], arq = a[iq];
-          a[ip] = arp - s * (arq + tau * arp);
-          a[iq] = arq + s * (arp - tau * arq);
+        for (int k = 0; k < N; ++k) {
+          if (k == p || k == q) continue;
+          const int ip = k < p ? upper_index<N>(k, p) : upper_index<N>(p, k);
+          const int iq = k < q ? upper_index<N>(k, q) : upper_index<N>(q, k);
+          const double akp = a[ip], akq = a[iq];
+          a[ip] = fma(c, akp, -s * akq);
+          a[iq] = fma(s, akp, c * akq);
         }
 #pragma unroll
         for (int k = 0; k < N; ++k) {
           const double vkp = V[k][p], vkq = V[k][q];
-          V[k][p] = vkp - s * (vkq + tau * vkp);
-          V[k][q] = vkq + s * (vkp - tau * vkq);
+          V[k][p] = fma(c, vkp, -s * vkq);
+          V[k][q] = fma(s, vkp, c * vkq);
         }
       }
   }
