@@ -321,7 +321,7 @@ int ysp_pcg_solve(uint32_t max_iterations, double threshold, int32_t threshold_m
                   double* work, double* cg_scalars, ysp_pcg_stats* stats, void* stream);
 
 /* --------------------------------------------------------------- MAS preconditioner */
-/* Multilevel additive Schwarz (solver/mas/*): the hierarchy (domains, identity
+/* Multilevel additive Schwarz (solver/mas): the hierarchy (domains, identity
  * aggregation maps) is built on the host once per static structure
  * (solver/mas/hierarchy.py:150-271); these entry points do the per-solve numeric work
  * and the apply on the device. All nodes are 3-vectors; domains are padded to 32

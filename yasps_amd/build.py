@@ -57,12 +57,21 @@ def hip_runtime_dir() -> str:
     return "/opt/rocm/lib"
 
 
-def build(verbose: bool = True) -> Path:
+def build(verbose: bool = True, only=None) -> Path:
+    """Compile every source (or only the named stems, reusing the other objects from an
+    earlier build) and link the library."""
     BUILD.mkdir(exist_ok=True)
     sources = sorted(CSRC.glob("*.hip"))
-    workers = max(1, min(len(sources), int(os.environ.get("MAX_JOBS", "8"))))
+    todo = sources if not only else [s for s in sources if s.stem in set(only)]
+    if only and len(todo) != len(set(only)):
+        raise ValueError(f"unknown sources in {only}")
+    workers = max(1, min(len(todo), int(os.environ.get("MAX_JOBS", "8"))))
     with ThreadPoolExecutor(workers) as ex:
-        objs = list(ex.map(_compile, sources))
+        list(ex.map(_compile, todo))
+    objs = [BUILD / (s.stem + ".o") for s in sources]
+    missing = [o.name for o in objs if not o.exists()]
+    if missing:
+        raise RuntimeError(f"objects missing (build everything first): {missing}")
     tmp = LIB.with_suffix(".so.tmp")
     rt = hip_runtime_dir()
     # plain host link: device code is already bundled in every object
@@ -78,5 +87,5 @@ def build(verbose: bool = True) -> Path:
 
 
 if __name__ == "__main__":
-    build()
+    build(only=sys.argv[1:] or None)
     sys.exit(0)

@@ -89,6 +89,16 @@ __global__ void gather_values_kernel(const double* __restrict__ sv, const double
 
 constexpr int kG = 16;  // lanes per block row
 
+// Block rows per workgroup of the fused PCG product: about 2048 workgroups, in multiples
+// of the 16 rows one pass of a workgroup covers, at most kMaxRowsPerWg.
+constexpr uint32_t kMaxRowsPerWg = 64;
+static_assert(3 * kMaxRowsPerWg <= kB, "one thread per scalar of a workgroup's rows");
+inline uint32_t spmv_rows_per_wg(uint32_t nv) {
+  constexpr uint32_t kSpmvWgs = 2048, kPass = kB / kG;
+  const uint32_t rows = (grid_for(nv, kSpmvWgs) + kPass - 1) / kPass * kPass;
+  return std::min<uint32_t>(kMaxRowsPerWg, std::max<uint32_t>(kPass, rows));
+}
+
 // One 16-lane group per block row: (y0, y1, y2) = sum over the row's blocks of B x_c.
 template <typename VT>
 __device__ __forceinline__ void row_product(const uint32_t* __restrict__ row_ptr,
@@ -342,26 +352,32 @@ __device__ __forceinline__ void pcg3_spmv_body(const uint32_t* __restrict__ row_
     st[S_FINAL_RR] = rr1;
     st[S_CUR_ITER] = k - 1;
   }
-  // Each workgroup walks a contiguous chunk of rows_per_wg block rows (16 at a time), so
-  // the grid stays near a few thousand workgroups: fewer partial-sum atomics and fewer
-  // slot reads by the consumers than one workgroup per 16 rows.
-  const uint32_t r_end = min(nbr, (blk + 1) * rows_per_wg);
+  // Each workgroup walks a contiguous chunk of rows_per_wg (<= kMaxRowsPerWg) block rows,
+  // 16 at a time, so the grid stays near a few thousand workgroups: fewer partial-sum
+  // atomics and fewer slot reads by the consumers than one workgroup per 16 rows. The
+  // row products meet in LDS; the chunk's 3 * rows scalars of p, q and z are then
+  // updated one per thread, in coalesced order.
+  __shared__ double ys[3 * kMaxRowsPerWg];
+  const uint32_t rbase = blk * rows_per_wg;
+  const uint32_t r_end = min(nbr, rbase + rows_per_wg);
   const int l = threadIdx.x % kG;
-  double pap = 0.0;
-  for (uint32_t r = blk * rows_per_wg + threadIdx.x / kG; r < r_end; r += kB / kG) {
+  for (uint32_t r = rbase + threadIdx.x / kG; r < r_end; r += kB / kG) {
     double y0, y1, y2;
     row_product<VT>(row_ptr, col, v, stride, z, r, l, y0, y1, y2);
-    if (l < 3) {                               // lane l owns component l of the row
-      const size_t d = 3 * static_cast<size_t>(r) + l;
-      double pi = z[d], qi = l == 0 ? y0 : (l == 1 ? y1 : y2);
-      if (k > 1) {                             // first direction: p = z, q = A z
-        pi = fma(p[d], beta, pi);
-        qi = fma(q[d], beta, qi);
-      }
-      p[d] = pi;
-      q[d] = qi;
-      pap += pi * qi;
+    if (l < 3) ys[3 * (r - rbase) + l] = l == 0 ? y0 : (l == 1 ? y1 : y2);
+  }
+  __syncthreads();
+  double pap = 0.0;
+  if (threadIdx.x < 3 * (r_end - rbase)) {
+    const size_t d = 3 * static_cast<size_t>(rbase) + threadIdx.x;
+    double pi = z[d], qi = ys[threadIdx.x];
+    if (k > 1) {                               // first direction: p = z, q = A z
+      pi = fma(p[d], beta, pi);
+      qi = fma(q[d], beta, qi);
     }
+    p[d] = pi;
+    q[d] = qi;
+    pap = pi * qi;
   }
   wg_add(pap, slot_ptr_at(st, SET_PAP + cur, sb), lds);
 }
@@ -697,9 +713,7 @@ int pcg3_run(uint32_t max_iterations, double threshold, int32_t mode, const uint
     ax0 = q;
   }
   // product grid: contiguous chunks of block rows, a multiple of 16 rows per workgroup
-  constexpr uint32_t kSpmvWgs = 2048;
-  const uint32_t rows_per_wg =
-      std::max<uint32_t>(kB / kG, (grid_for(nv, kSpmvWgs) + kB / kG - 1) / (kB / kG) * (kB / kG));
+  const uint32_t rows_per_wg = spmv_rows_per_wg(nv);
   static_assert(kStateDoubles <= YSP_PCG3_SCALARS, "cg_scalars too small");
   const dim3 vg(grid_for(nv, kB)), sg(grid_for(nv, rows_per_wg)), ug(grid_for(n, kUB)), bl(kB);
   hipLaunchKernelGGL(pcg3_init_kernel<MT>, vg, bl, 0, s, minv, rhs, x0, ax0, x, r, z, nv, st);
@@ -906,11 +920,6 @@ struct BatchWork {
   uint32_t blk;
 };
 
-uint32_t spmv_rows_per_wg(uint32_t nv) {
-  constexpr uint32_t kSpmvWgs = 2048;
-  return std::max<uint32_t>(kB / kG,
-                            (grid_for(nv, kSpmvWgs) + kB / kG - 1) / (kB / kG) * (kB / kG));
-}
 
 template <typename VT>
 __global__ void __launch_bounds__(kB)
