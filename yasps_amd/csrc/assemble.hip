@@ -54,6 +54,30 @@ __global__ void gradient_contrib_kernel(const uint32_t* __restrict__ idx,
   }
 }
 
+// Lanes (unit, entry): each wavefront holds kUnitsPerWave whole units (global blocks of 9
+// entries, or gradient vertices of 3) with one lane per entry, so that the entries of a
+// contribution are read as one contiguous run and the result is written contiguously.
+template <int W>
+struct LaneUnit {
+  static constexpr uint32_t kUnitsPerWave = 64 / W;
+  uint32_t unit;
+  int entry;
+  bool valid;
+  __device__ LaneUnit(uint32_t count) {
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = tid % 64, slot = lane / W;
+    unit = (tid / 64) * kUnitsPerWave + slot;
+    entry = static_cast<int>(lane % W);
+    valid = slot < kUnitsPerWave && unit < count;
+  }
+  static unsigned grid(uint32_t count) {
+    const size_t waves = (static_cast<size_t>(count) + kUnitsPerWave - 1) / kUnitsPerWave;
+    return grid_for(waves * 64, kB);
+  }
+};
+
+// Sum of the staged contributions of every global block (contributions in their sorted
+// order, so the sums are reproducible), and its copy into the diagonal blocks.
 __global__ void __launch_bounds__(kB)
     gather_blocks3_kernel(const double* __restrict__ staging, const uint32_t* __restrict__ offsets,
                           const uint32_t* __restrict__ items, uint32_t nu,
@@ -61,24 +85,22 @@ __global__ void __launch_bounds__(kB)
                           double* __restrict__ diag_blocks, const uint32_t* __restrict__ diag_start,
                           const uint32_t* __restrict__ grad_start, uint32_t num_attr,
                           int accumulate_diag) {
-  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
-  if (u >= nu) return;
-  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (uint32_t k = offsets[u]; k < offsets[u + 1]; ++k) {
-    const double* b = staging + 9 * static_cast<size_t>(items[k]);
-#pragma unroll
-    for (int j = 0; j < 9; ++j) acc[j] += b[j];
-  }
-#pragma unroll
-  for (int j = 0; j < 9; ++j) values[9 * static_cast<size_t>(u) + j] = acc[j];
+  const LaneUnit<9> lu(nu);
+  if (!lu.valid) return;
+  const uint32_t u = lu.unit;
+  const int j = lu.entry;
+  double acc = 0.0;
+  const uint32_t k1 = offsets[u + 1];
+#pragma unroll 4
+  for (uint32_t k = offsets[u]; k < k1; ++k) acc += staging[9 * static_cast<size_t>(items[k]) + j];
+  values[9 * static_cast<size_t>(u) + j] = acc;
   const uint32_t r = positions[2 * u], c = positions[2 * u + 1];
   if (r == c && diag_blocks) {
     uint32_t a = 0;
     while (a < num_attr && r >= grad_start[a + 1]) ++a;
     if (a < num_attr) {
-      double* d = diag_blocks + diag_start[a] + ((r - grad_start[a]) / 3) * 9;
-#pragma unroll
-      for (int j = 0; j < 9; ++j) d[j] = accumulate_diag ? d[j] + acc[j] : acc[j];
+      double* d = diag_blocks + diag_start[a] + ((r - grad_start[a]) / 3) * 9 + j;
+      *d = accumulate_diag ? *d + acc : acc;
     }
   }
 }
@@ -87,18 +109,14 @@ __global__ void __launch_bounds__(kB)
     gather_gradient3_kernel(const double* __restrict__ staging, const uint32_t* __restrict__ offsets,
                             const uint32_t* __restrict__ items, uint32_t nv,
                             double* __restrict__ gradient) {
-  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nv) return;
-  double g0 = 0, g1 = 0, g2 = 0;
-  for (uint32_t k = offsets[v]; k < offsets[v + 1]; ++k) {
-    const double* s = staging + items[k];
-    g0 += s[0];
-    g1 += s[1];
-    g2 += s[2];
-  }
-  gradient[3 * static_cast<size_t>(v)] = g0;
-  gradient[3 * static_cast<size_t>(v) + 1] = g1;
-  gradient[3 * static_cast<size_t>(v) + 2] = g2;
+  const LaneUnit<3> lu(nv);
+  if (!lu.valid) return;
+  const uint32_t v = lu.unit;
+  double g = 0.0;
+  const uint32_t k1 = offsets[v + 1];
+#pragma unroll 4
+  for (uint32_t k = offsets[v]; k < k1; ++k) g += staging[items[k] + lu.entry];
+  gradient[3 * static_cast<size_t>(v) + lu.entry] = g;
 }
 
 }  // namespace
@@ -178,7 +196,7 @@ extern "C" int ysp_gather_blocks3(const double* block_staging, const uint32_t* o
                                   uint32_t num_attributes, int32_t accumulate_diagonal,
                                   void* stream) {
   if (num_blocks == 0) return YSP_OK;
-  hipLaunchKernelGGL(gather_blocks3_kernel, dim3(grid_for(num_blocks, kB)), dim3(kB), 0,
+  hipLaunchKernelGGL(gather_blocks3_kernel, dim3(LaneUnit<9>::grid(num_blocks)), dim3(kB), 0,
                      as_stream(stream), block_staging, offsets, items, num_blocks, positions,
                      values, diagonal_blocks, diagonal_blocks_start, gradient_segments_start,
                      num_attributes, accumulate_diagonal);
@@ -191,7 +209,7 @@ extern "C" int ysp_gather_gradient3(const double* gradient_staging, const uint32
                                     void* stream) {
   if (n % 3) return YSP_ERR_ARG;
   if (n == 0) return YSP_OK;
-  hipLaunchKernelGGL(gather_gradient3_kernel, dim3(grid_for(n / 3, kB)), dim3(kB), 0,
+  hipLaunchKernelGGL(gather_gradient3_kernel, dim3(LaneUnit<3>::grid(n / 3)), dim3(kB), 0,
                      as_stream(stream), gradient_staging, offsets, items, n / 3, gradient);
   YSP_CHECK_LAUNCH();
   return YSP_OK;
