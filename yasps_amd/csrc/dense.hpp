@@ -113,16 +113,38 @@ __device__ __forceinline__ double rsqrt_nr(double x) {
   return y * fma(-hx * y, y, 1.5);
 }
 
-// Cyclic Jacobi on the packed upper triangle a (N(N+1)/2 entries) of a symmetric matrix,
-// with the symmetric two-sided update of each rotation (Golub & Van Loan, section 8.5):
-// only the 2(N-2) entries of rows/columns p and q and the three entries of the 2x2 pivot
-// block change. The rotation t = tan(phi) = sgn(d) y / (|d| + sqrt(d^2 + y^2)) with
+// Round-robin (circle method) ordering of the N(N-1)/2 index pairs: with M = N rounded up
+// to even, index M-1 stays fixed and round r pairs r with M-1 and (r+i) mod (M-1) with
+// (r-i) mod (M-1), i = 1..M/2-1. The pairs of a round are disjoint (a pair with index N,
+// the stand-in for odd N, is skipped).
+template <int N>
+struct RoundRobin {
+  static constexpr int M = N + (N & 1);
+  static constexpr int kRounds = M - 1, kPerRound = M / 2;
+  __host__ __device__ static constexpr int a(int r, int i) { return i == 0 ? r : (r + i) % (M - 1); }
+  __host__ __device__ static constexpr int b(int r, int i) {
+    return i == 0 ? M - 1 : (r - i + M - 1) % (M - 1);
+  }
+  __host__ __device__ static constexpr int p(int r, int i) { return a(r, i) < b(r, i) ? a(r, i) : b(r, i); }
+  __host__ __device__ static constexpr int q(int r, int i) { return a(r, i) < b(r, i) ? b(r, i) : a(r, i); }
+};
+
+// Jacobi on the packed upper triangle a (N(N+1)/2 entries) of a symmetric matrix, with the
+// symmetric two-sided update of each rotation (Golub & Van Loan, section 8.5): only the
+// 2(N-2) entries of rows/columns p and q and the three entries of the 2x2 pivot block
+// change. The rotation t = tan(phi) = sgn(d) y / (|d| + sqrt(d^2 + y^2)) with
 // d = a_qq - a_pp, y = 2 a_pq (the smaller root of t^2 + 2 (d/y) t - 1 = 0), written with
-// one square root and one reciprocal; c = 1/sqrt(1 + t^2), s = t c. On return the
-// diagonal holds the eigenvalues and V's columns the eigenvectors.
+// one square root and one reciprocal; c = 1/sqrt(1 + t^2), s = t c; a_pq = 0 gives the
+// identity. Sweeps follow the round-robin ordering: the rotations of a round touch
+// disjoint pivot blocks, so their parameters are computed together from the state at the
+// start of the round (the same values the one-at-a-time order computes) and the
+// rotations are then applied in turn. Branch-free rounds let the scheduler overlap the
+// independent parameter chains. On return the diagonal holds the eigenvalues and V's
+// columns the eigenvectors.
 template <int N>
 __device__ __forceinline__ void jacobi_eigen_packed(double (&a)[N * (N + 1) / 2],
                                                     double (&V)[N][N]) {
+  using RR = RoundRobin<N>;
 #pragma unroll
   for (int i = 0; i < N; ++i)
 #pragma unroll
@@ -143,19 +165,30 @@ __device__ __forceinline__ void jacobi_eigen_packed(double (&a)[N * (N + 1) / 2]
       for (int q = p + 1; q < N; ++q) off += a[upper_index<N>(p, q)] * a[upper_index<N>(p, q)];
     if (off <= stop) break;
 #pragma unroll
-    for (int p = 0; p < N; ++p)
+    for (int rd = 0; rd < RR::kRounds; ++rd) {
+      double cr[RR::kPerRound], sr[RR::kPerRound], tr[RR::kPerRound];
 #pragma unroll
-      for (int q = p + 1; q < N; ++q) {
+      for (int i = 0; i < RR::kPerRound; ++i) {
+        const int p = RR::p(rd, i), q = RR::q(rd, i);
+        if (q >= N) continue;
         const double apq = a[upper_index<N>(p, q)];
-        if (apq == 0.0) continue;
-        const double app = a[upper_index<N>(p, p)], aqq = a[upper_index<N>(q, q)];
-        const double d = aqq - app, y = 2.0 * apq;
+        const double d = a[upper_index<N>(q, q)] - a[upper_index<N>(p, p)], y = 2.0 * apq;
         const double r = sqrt(fma(d, d, y * y));
-        const double t = (d >= 0.0 ? y : -y) * recip_nr(fabs(d) + r);
+        double t = (d >= 0.0 ? y : -y) * recip_nr(fabs(d) + r);
+        t = apq == 0.0 ? 0.0 : t;
         const double c = rsqrt_nr(fma(t, t, 1.0));
-        const double s = t * c;
-        a[upper_index<N>(p, p)] = fma(-t, apq, app);
-        a[upper_index<N>(q, q)] = fma(t, apq, aqq);
+        cr[i] = c;
+        sr[i] = t * c;
+        tr[i] = t;
+      }
+#pragma unroll
+      for (int i = 0; i < RR::kPerRound; ++i) {
+        const int p = RR::p(rd, i), q = RR::q(rd, i);
+        if (q >= N) continue;
+        const double c = cr[i], s = sr[i], t = tr[i];
+        const double apq = a[upper_index<N>(p, q)];
+        a[upper_index<N>(p, p)] = fma(-t, apq, a[upper_index<N>(p, p)]);
+        a[upper_index<N>(q, q)] = fma(t, apq, a[upper_index<N>(q, q)]);
         a[upper_index<N>(p, q)] = 0.0;
 #pragma unroll
         for (int k = 0; k < N; ++k) {
@@ -173,6 +206,7 @@ __device__ __forceinline__ void jacobi_eigen_packed(double (&a)[N * (N + 1) / 2]
           V[k][q] = fma(s, vkp, c * vkq);
         }
       }
+    }
   }
 }
 
