@@ -95,6 +95,22 @@ def spmv_roofline(hs, reps):
     end.record(stream)
     end.synchronize()
     ms = start.elapsed_time(end) / reps
+    # cold: reading 1 GiB between launches evicts L2 and the 256 MB Infinity Cache (with
+    # clean lines: a write here would leave dirty lines whose write-back competes with the
+    # next launch), so each timed launch streams its operands from HBM
+    flush = torch.ones(1 << 27, dtype=torch.float64, device=dev.device())
+    cold = []
+    for _ in range(min(reps, 20)):
+        flush.sum()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        launch()
+        e1.record(stream)
+        cold.append((e0, e1))
+    torch.cuda.synchronize()
+    ms_cold = float(np.mean([a.elapsed_time(b) for a, b in cold]))
+    del flush
     nb = hs.static.num_blocks
     n = hs.rows
     # algorithmic: 72 B values + 8 B (row, col) per stored upper block, x read, y written
@@ -106,7 +122,12 @@ def spmv_roofline(hs, reps):
             "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(hs_n(hs)),
             "kernel": "bsr3_full_spmv_kernel<double>", "avg_us": round(ms * 1e3, 2),
             "bytes_per_launch": int(bytes_alg), "full_bsr_bytes_per_launch": int(bytes_full),
-            "full_bsr_GBs": round(bytes_full / (ms * 1e-3) / 1e9, 1)}
+            "full_bsr_GBs": round(bytes_full / (ms * 1e-3) / 1e9, 1),
+            "cold": {"avg_us": round(ms_cold * 1e3, 2),
+                     "achieved": round(bytes_alg / (ms_cold * 1e-3) / 1e9, 1),
+                     "frac": round(bytes_alg / (ms_cold * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "full_bsr_GBs": round(bytes_full / (ms_cold * 1e-3) / 1e9, 1),
+                     "note": "1 GiB read between launches (L2 and MALL evicted); single-launch event pairs"}}
 
 
 def hs_n(hs):

@@ -7,7 +7,7 @@
 //   * domain matrices: every level block whose two nodes share a domain is copied into
 //     that domain's dense, zero-padded D x D FP64 matrix (one contribution per entry,
 //     no reduction);
-//   * inverses: Gauss-Jordan elimination in LDS, one workgroup per domain, FP64, stored
+//   * inverses: Gauss-Jordan elimination in registers, one wavefront per domain, FP64, stored
 //     FP32 column-major (the reference's FP64-factor / FP32-bank split,
 //     solver/mas/cuda/local_inverse.cu:244-308). A pivot below 1e-12 * max|diag| marks the
 //     domain as failed.
@@ -73,60 +73,73 @@ __global__ void pad_identity_kernel(double* __restrict__ dom, const uint32_t* __
     dom[static_cast<size_t>(d) * kD * kD + i * kD + i] = 1.0;
 }
 
-// Gauss-Jordan inversion of one D x D matrix per workgroup (no pivoting: the domain
-// matrices are principal submatrices of an SPD operator).
-__global__ void __launch_bounds__(256)
+// Gauss-Jordan inversion, one wavefront per domain (no pivoting: the domain matrices are
+// principal submatrices of an SPD operator). Lane j holds column j of the augmented
+// [A | I] (D x 2D) in registers. At step k lane k's column goes through LDS to the whole
+// wavefront (broadcast reads) and every lane updates its own column: x_k <- x_k / a_kk,
+// x_i <- x_i - a_ik x_k (i != k). The wavefront works alone: no workgroup barriers.
+constexpr int kInvWaves = 4;                   // domains per workgroup
+
+// LDS written by some lanes of a wavefront and then read by others of the same wavefront.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+__global__ void __launch_bounds__(64 * kInvWaves)
     invert_domains_kernel(const double* __restrict__ dom, float* __restrict__ inv,
                           uint32_t ndom, int* __restrict__ status) {
-  __shared__ double A[kD][kD + 1];
-  __shared__ double Binv[kD][kD + 1];
-  __shared__ double piv_row_a[kD], piv_row_b[kD], piv_col[kD];
-  const uint32_t d = blockIdx.x;
-  if (d >= ndom) return;
+  static_assert(2 * kD == 64, "one lane per column of [A | I]");
+  __shared__ double colk[kInvWaves][kD];
+  __shared__ double tr[kInvWaves][kD][kD + 1];
+  const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const uint32_t d = blockIdx.x * kInvWaves + w;
+  if (d >= ndom) return;                       // whole wavefronts leave together
   const double* M = dom + static_cast<size_t>(d) * kD * kD;
-  for (int t = threadIdx.x; t < kD * kD; t += blockDim.x) {
-    const int i = t / kD, j = t % kD;
-    A[i][j] = M[t];
-    Binv[i][j] = (i == j) ? 1.0 : 0.0;
+  double x[kD];
+  double scale = 0.0;
+  if (lane < kD) {                             // column `lane` of A (symmetric, row-major)
+#pragma unroll
+    for (int i = 0; i < kD; ++i) x[i] = M[i * kD + lane];
+    scale = fabs(M[lane * kD + lane]);
+  } else {                                     // column lane - D of I
+#pragma unroll
+    for (int i = 0; i < kD; ++i) x[i] = i == lane - kD ? 1.0 : 0.0;
   }
-  __syncthreads();
-  __shared__ double scale;
-  if (threadIdx.x == 0) {
-    double s = 0.0;
-    for (int i = 0; i < kD; ++i) s = fmax(s, fabs(A[i][i]));
-    scale = s;
-  }
-  __syncthreads();
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) scale = fmax(scale, __shfl_xor(scale, o));
   bool bad = false;
+#pragma unroll
   for (int k = 0; k < kD; ++k) {
-    const double pivot = A[k][k];
-    if (!(fabs(pivot) > 1e-12 * scale)) bad = true;
-    const double ip = 1.0 / pivot;
-    if (threadIdx.x < kD) {
-      piv_row_a[threadIdx.x] = A[k][threadIdx.x] * ip;
-      piv_row_b[threadIdx.x] = Binv[k][threadIdx.x] * ip;
-      piv_col[threadIdx.x] = A[threadIdx.x][k];
+    if (lane == k) {
+#pragma unroll
+      for (int i = 0; i < kD; ++i) colk[w][i] = x[i];
     }
-    __syncthreads();
-    for (int t = threadIdx.x; t < kD * kD; t += blockDim.x) {
-      const int i = t / kD, j = t % kD;
-      if (i == k) {
-        A[i][j] = piv_row_a[j];
-        Binv[i][j] = piv_row_b[j];
-      } else {
-        const double f = piv_col[i];
-        A[i][j] = fma(-f, piv_row_a[j], A[i][j]);
-        Binv[i][j] = fma(-f, piv_row_b[j], Binv[i][j]);
-      }
-    }
-    __syncthreads();
+    wave_lds_sync();
+    const double akk = colk[w][k];
+    if (!(fabs(akk) > 1e-12 * scale)) bad = true;
+    const double xk = x[k] / akk;
+#pragma unroll
+    for (int i = 0; i < kD; ++i)
+      if (i != k) x[i] = fma(-colk[w][i], xk, x[i]);
+    x[k] = xk;
+    wave_lds_sync();
   }
+  // inverse B = columns D..2D-1; stored FP32 column-major, symmetrised:
+  // out[c * D + r] = (B[r][c] + B[c][r]) / 2
+  if (lane >= kD) {
+#pragma unroll
+    for (int i = 0; i < kD; ++i) tr[w][i][lane - kD] = x[i];
+  }
+  wave_lds_sync();
   float* out = inv + static_cast<size_t>(d) * kD * kD;
-  for (int t = threadIdx.x; t < kD * kD; t += blockDim.x) {
-    const int col = t / kD, row = t % kD;  // column-major: out[col * D + row]
-    out[t] = static_cast<float>(0.5 * (Binv[row][col] + Binv[col][row]));
+  const int r = lane % kD;
+#pragma unroll
+  for (int it = 0; it < kD / 2; ++it) {
+    const int c = 2 * it + lane / kD;
+    out[c * kD + r] = static_cast<float>(0.5 * (tr[w][r][c] + tr[w][c][r]));
   }
-  if (bad && threadIdx.x == 0) atomicOr(status, 1);
+  if (bad && lane == 0) atomicOr(status, 1);
 }
 
 __device__ __forceinline__ bool mas_stopped(const double* stop) {
@@ -309,7 +322,8 @@ extern "C" int ysp_mas_invert_domains(double* domains, const uint32_t* domain_si
   hipStream_t s = as_stream(stream);
   hipLaunchKernelGGL(pad_identity_kernel, dim3(num_domains), dim3(64), 0, s, domains, domain_sizes,
                      num_domains);
-  hipLaunchKernelGGL(invert_domains_kernel, dim3(num_domains), dim3(256), 0, s, domains, inverses,
+  hipLaunchKernelGGL(invert_domains_kernel, dim3(grid_for(num_domains, kInvWaves)),
+                     dim3(64 * kInvWaves), 0, s, domains, inverses,
                      num_domains, status);
   YSP_CHECK_LAUNCH();
   return YSP_OK;
