@@ -7,7 +7,7 @@
 //   * domain matrices: every level block whose two nodes share a domain is copied into
 //     that domain's dense, zero-padded D x D FP64 matrix (one contribution per entry,
 //     no reduction);
-//   * inverses: Gauss-Jordan elimination in registers, one wavefront per domain, FP64, stored
+//   * inverses: in-place Gauss-Jordan in registers, two domains per wavefront, FP64, stored
 //     FP32 column-major (the reference's FP64-factor / FP32-bank split,
 //     solver/mas/cuda/local_inverse.cu:244-308). A pivot below 1e-12 * max|diag| marks the
 //     domain as failed.
@@ -73,12 +73,15 @@ __global__ void pad_identity_kernel(double* __restrict__ dom, const uint32_t* __
     dom[static_cast<size_t>(d) * kD * kD + i * kD + i] = 1.0;
 }
 
-// Gauss-Jordan inversion, one wavefront per domain (no pivoting: the domain matrices are
-// principal submatrices of an SPD operator). Lane j holds column j of the augmented
-// [A | I] (D x 2D) in registers. At step k lane k's column goes through LDS to the whole
-// wavefront (broadcast reads) and every lane updates its own column: x_k <- x_k / a_kk,
-// x_i <- x_i - a_ik x_k (i != k). The wavefront works alone: no workgroup barriers.
-constexpr int kInvWaves = 4;                   // domains per workgroup
+// In-place Gauss-Jordan inversion (no pivoting: the domain matrices are principal
+// submatrices of an SPD operator), two domains per wavefront: in each 32-lane half, lane j
+// holds column j of its domain in registers. At step k the half's lane k publishes its
+// column c = A[:, k] through LDS (broadcast reads) and, with p = c_k,
+//   lane k:      A_kk <- 1/p,  A_ik <- -c_i / p      (i != k)
+//   lane j != k: A_kj <- A_kj / p,  A_ij <- A_ij - c_i A_kj   (i != k)
+// which leaves A^-1 in place after D steps. The wavefront works alone: no workgroup
+// barriers.
+constexpr int kInvWaves = 4;                   // wavefronts per workgroup
 
 // LDS written by some lanes of a wavefront and then read by others of the same wavefront.
 __device__ __forceinline__ void wave_lds_sync() {
@@ -89,57 +92,58 @@ __device__ __forceinline__ void wave_lds_sync() {
 __global__ void __launch_bounds__(64 * kInvWaves)
     invert_domains_kernel(const double* __restrict__ dom, float* __restrict__ inv,
                           uint32_t ndom, int* __restrict__ status) {
-  static_assert(2 * kD == 64, "one lane per column of [A | I]");
-  __shared__ double colk[kInvWaves][kD];
-  __shared__ double tr[kInvWaves][kD][kD + 1];
-  const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
-  const uint32_t d = blockIdx.x * kInvWaves + w;
-  if (d >= ndom) return;                       // whole wavefronts leave together
-  const double* M = dom + static_cast<size_t>(d) * kD * kD;
+  static_assert(2 * kD == 64, "two domains of D columns per wavefront");
+  __shared__ double colk[kInvWaves][2][kD];
+  __shared__ double tr[kInvWaves][2][kD][kD + 1];
+  const int w = threadIdx.x / 64, h = (threadIdx.x % 64) / kD, j = threadIdx.x % kD;
+  const uint32_t d = (blockIdx.x * kInvWaves + w) * 2 + h;
+  const bool live = d < ndom;                  // a half past the end idles along
+  const double* M = dom + static_cast<size_t>(live ? d : 0) * kD * kD;
   double x[kD];
   double scale = 0.0;
-  if (lane < kD) {                             // column `lane` of A (symmetric, row-major)
+  if (live) {                                  // column j (the matrix is symmetric)
 #pragma unroll
-    for (int i = 0; i < kD; ++i) x[i] = M[i * kD + lane];
-    scale = fabs(M[lane * kD + lane]);
-  } else {                                     // column lane - D of I
+    for (int i = 0; i < kD; ++i) x[i] = M[i * kD + j];
+    scale = fabs(M[j * kD + j]);
+  } else {
 #pragma unroll
-    for (int i = 0; i < kD; ++i) x[i] = i == lane - kD ? 1.0 : 0.0;
+    for (int i = 0; i < kD; ++i) x[i] = i == j ? 1.0 : 0.0;
   }
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) scale = fmax(scale, __shfl_xor(scale, o));
+  for (int o = kD / 2; o > 0; o >>= 1) scale = fmax(scale, __shfl_xor(scale, o, kD));
   bool bad = false;
 #pragma unroll
   for (int k = 0; k < kD; ++k) {
-    if (lane == k) {
+    if (j == k) {
 #pragma unroll
-      for (int i = 0; i < kD; ++i) colk[w][i] = x[i];
+      for (int i = 0; i < kD; ++i) colk[w][h][i] = x[i];
     }
     wave_lds_sync();
-    const double akk = colk[w][k];
-    if (!(fabs(akk) > 1e-12 * scale)) bad = true;
-    const double xk = x[k] / akk;
+    const double p = colk[w][h][k];
+    if (!(fabs(p) > 1e-12 * scale)) bad = true;
+    const double ip = 1.0 / p;
+    if (j == k) {
 #pragma unroll
-    for (int i = 0; i < kD; ++i)
-      if (i != k) x[i] = fma(-colk[w][i], xk, x[i]);
-    x[k] = xk;
+      for (int i = 0; i < kD; ++i) x[i] = i == k ? ip : -x[i] * ip;
+    } else {
+      const double xk = x[k] * ip;
+#pragma unroll
+      for (int i = 0; i < kD; ++i)
+        if (i != k) x[i] = fma(-colk[w][h][i], xk, x[i]);
+      x[k] = xk;
+    }
     wave_lds_sync();
   }
-  // inverse B = columns D..2D-1; stored FP32 column-major, symmetrised:
-  // out[c * D + r] = (B[r][c] + B[c][r]) / 2
-  if (lane >= kD) {
+  // stored FP32 column-major, symmetrised: out[c * D + r] = (B[r][c] + B[c][r]) / 2
 #pragma unroll
-    for (int i = 0; i < kD; ++i) tr[w][i][lane - kD] = x[i];
-  }
+  for (int i = 0; i < kD; ++i) tr[w][h][i][j] = x[i];
   wave_lds_sync();
-  float* out = inv + static_cast<size_t>(d) * kD * kD;
-  const int r = lane % kD;
+  if (live) {
+    float* out = inv + static_cast<size_t>(d) * kD * kD;
 #pragma unroll
-  for (int it = 0; it < kD / 2; ++it) {
-    const int c = 2 * it + lane / kD;
-    out[c * kD + r] = static_cast<float>(0.5 * (tr[w][r][c] + tr[w][c][r]));
+    for (int c = 0; c < kD; ++c) out[c * kD + j] = static_cast<float>(0.5 * (tr[w][h][j][c] + tr[w][h][c][j]));
+    if (bad && j == 0) atomicOr(status, 1);
   }
-  if (bad && lane == 0) atomicOr(status, 1);
 }
 
 __device__ __forceinline__ bool mas_stopped(const double* stop) {
@@ -322,7 +326,7 @@ extern "C" int ysp_mas_invert_domains(double* domains, const uint32_t* domain_si
   hipStream_t s = as_stream(stream);
   hipLaunchKernelGGL(pad_identity_kernel, dim3(num_domains), dim3(64), 0, s, domains, domain_sizes,
                      num_domains);
-  hipLaunchKernelGGL(invert_domains_kernel, dim3(grid_for(num_domains, kInvWaves)),
+  hipLaunchKernelGGL(invert_domains_kernel, dim3(grid_for(num_domains, 2 * kInvWaves)),
                      dim3(64 * kInvWaves), 0, s, domains, inverses,
                      num_domains, status);
   YSP_CHECK_LAUNCH();
