@@ -238,12 +238,23 @@ __global__ void __launch_bounds__(kB)
     const uint32_t s = 3 * node + lane % 3;
     rl = static_cast<float>(d < level0_domains ? r0[s] : rc[s]);
   }
-  const float* B = inv + static_cast<size_t>(d) * kD * kD;
+  // the domain's residual goes through LDS (its 32 lanes sit in one wavefront); lane i
+  // reads row i of the inverse, which is its column i (the stored inverse is exactly
+  // symmetric), as eight 16-byte loads
+  __shared__ __align__(16) float rs[kB / kD][kD];
+  const int g = threadIdx.x / kD;
+  rs[g][lane] = rl;
+  wave_lds_sync();
+  const float4* Bi = reinterpret_cast<const float4*>(inv + (static_cast<size_t>(d) * kD + lane) * kD);
+  const float4* R = reinterpret_cast<const float4*>(rs[g]);
   float acc = 0.0f;
-#pragma unroll 8
-  for (int j = 0; j < kD; ++j) {
-    const float rj = __shfl(rl, j, kD);
-    acc = fmaf(B[j * kD + lane], rj, acc);
+#pragma unroll
+  for (int q = 0; q < kD / 4; ++q) {
+    const float4 b = Bi[q], rq = R[q];
+    acc = fmaf(b.x, rq.x, acc);
+    acc = fmaf(b.y, rq.y, acc);
+    acc = fmaf(b.z, rq.z, acc);
+    acc = fmaf(b.w, rq.w, acc);
   }
   if (static_cast<uint32_t>(lane) < P) {
     const uint32_t s = 3 * node + lane % 3;
