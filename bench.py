@@ -335,6 +335,9 @@ def main():
                                  max_levels=4, coarsest_level_weight=1.0),
             "mixed_mas": form_leg(cs, args, x64, value, "mas", precision="mixed", max_levels=4,
                                   coarsest_level_weight=1.0),
+            "mixed_mas_bf16_coarse": form_leg(cs, args, x64, value, "mas", precision="mixed",
+                                              max_levels=4, coarsest_level_weight=1.0,
+                                              coarse_bank="bf16"),
         }
     out = {
         "metric": "Newton solves/s at 1e-6 rel. residual (plain FP64)",

@@ -340,10 +340,14 @@ int ysp_mas_domain_scatter(const double* values, int32_t is_full_bsr, uint32_t n
                            const uint32_t* dest, double* domains, void* stream);
 
 /* Explicit inverse of every domain matrix (identity in the padding), FP64 Gauss-Jordan,
- * stored FP32 column-major (solver/mas/cuda/local_inverse.cu:6-308). *status (device int)
+ * stored FP32 column-major (solver/mas/cuda/local_inverse.cu:6-308). With inverses_bf16
+ * non-NULL, domains d >= bf16_first_domain are stored BF16 (round to nearest even) at
+ * inverses_bf16[(d - bf16_first_domain) * 32 * 32] instead (the BF16 coarse banks of the
+ * mixed-precision MAS); the others stay in inverses[d * 32 * 32]. *status (device int)
  * gets bit 0 set when a pivot is below 1e-12 * max|diag|. */
 int ysp_mas_invert_domains(double* domains, const uint32_t* domain_sizes, uint32_t num_domains,
-                           float* inverses, int32_t* status, void* stream);
+                           float* inverses, uint16_t* inverses_bf16, uint32_t bf16_first_domain,
+                           int32_t* status, void* stream);
 
 #define YSP_MAS_MAX_COARSE_LEVELS 8
 
@@ -364,6 +368,8 @@ typedef struct ysp_mas_args {
   double* coarse_residual;              /* work [3 x num_coarse_nodes] */
   float* fine_correction;               /* work [n] */
   float* coarse_correction;             /* work [3 x num_coarse_nodes] */
+  const uint16_t* inverses_bf16;        /* NULL, or the BF16 bank of domains >= */
+  uint32_t bf16_first_domain;           /*   bf16_first_domain (ysp_mas_invert_domains) */
 } ysp_mas_args;
 
 /* z = sum_l w_l P_l B_l^-1 R_l r (solver/mas/preconditioner.py:128-154). */

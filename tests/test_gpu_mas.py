@@ -65,6 +65,56 @@ def test_mas_apply_and_inverses_match_oracle(levels, coarsest):
         assert np.linalg.norm(got - ref) <= APPLY_TOL * np.linalg.norm(ref)
 
 
+def test_mas_bf16_coarse_bank():
+    """Coarse-level inverses stored BF16, level 0 FP32: every coarse inverse within BF16
+    rounding (2^-8 of the domain's largest entry) of the oracle's, level 0 as before, the
+    apply within 1e-2 norm-wise, and the mixed MAS-PCG still reaches the requested true
+    FP64 residual."""
+    from yasps_amd import device as dev
+    from yasps_amd.mas import DOMAIN_SCALARS
+    print("mas bf16 coarse bank n=5", flush=True)
+    cs, s0, hs = _mas_scene(5, seed=1, max_levels=4, coarsest_level_weight=2.5, coarse_bank="bf16")
+    slv = s0.minimizer.linearSolver.implementation
+    slv._ensure(hs, hs.rows)
+    m = slv.device_tables
+    m.update(hs.full)
+    dev.torch().cuda.synchronize()
+    assert int(m.status.item()) == 0
+    h = slv.hierarchy
+    A, _ = _oracle_matrix(cs)
+    invs = omas.domain_inverses(h, omas.level_matrices(h, A.toarray()))
+    D = DOMAIN_SCALARS
+    n0 = len(h.levels[0].domains)
+    assert m.bf16_first == n0 and m.inverses_bf16 is not None
+    f32 = dev.to_host(m.inverses).reshape(-1, D, D)
+    u16 = dev.to_host(m.inverses_bf16, "uint16").astype(np.uint32) << 16
+    b16 = u16.view(np.float32).reshape(-1, D, D)
+    k = 0
+    for l, lv in enumerate(h.levels):
+        for ref in invs[l]:
+            P = ref.shape[0]
+            if k < n0:
+                got, tol = f32[k].T[:P, :P].astype(np.float64), 1e-5
+            else:
+                got, tol = b16[k - n0].T[:P, :P].astype(np.float64), 2.0 ** -8
+            assert np.abs(got - ref).max() <= tol * np.abs(ref).max(), (l, k)
+            k += 1
+    assert k == n0 + b16.shape[0]
+    w = np.asarray(omas.level_weights(h, 2.5))
+    r = np.random.default_rng(12).standard_normal(hs.rows)
+    z = dev.empty(hs.rows, "float64")
+    m.apply(dev.to_device(r, "float64"), z)
+    ref = omas.apply(h, invs, r, w)
+    assert np.linalg.norm(dev.to_host(z) - ref) <= 1e-2 * np.linalg.norm(ref)
+    cs8 = cube_scene(8, seed=0)
+    s8, _ = build_yasps_scene(cs8, solver_name="mas", threshold_mode="relative_residual",
+                              max_levels=4, precision="mixed", coarse_bank="bf16")
+    x = dev.to_host(s8.minimizeEnergy(tolerance=1e-8)[0]).copy()
+    assert s8.minimizer.linearSolver.statistics["converged"]
+    A8, b8 = _oracle_matrix(cs8)
+    assert opcg.relative_residual(lambda v: A8 @ v, x, b8) < 1.5e-8
+
+
 def test_mas_pcg_relative_residual():
     from yasps_amd import device as dev
     print("mas pcg n=8 relative residual 1e-8", flush=True)

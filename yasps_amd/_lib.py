@@ -76,7 +76,8 @@ class MasArgs(C.Structure):
                 ("domain_nodes", C.c_void_p), ("domain_node_offsets", C.c_void_p),
                 ("num_domains", C.c_uint32), ("level0_domains", C.c_uint32),
                 ("weights", C.c_void_p), ("coarse_residual", C.c_void_p),
-                ("fine_correction", C.c_void_p), ("coarse_correction", C.c_void_p)]
+                ("fine_correction", C.c_void_p), ("coarse_correction", C.c_void_p),
+                ("inverses_bf16", C.c_void_p), ("bf16_first_domain", C.c_uint32)]
 
 
 P = C.c_void_p
@@ -133,7 +134,7 @@ SIGNATURES = {
     "ysp_pcg3_batch_run": (C.c_int, [C.POINTER(Pcg3System), U32, U32, I32, I32, P]),
     "ysp_mas_coarsen": (C.c_int, [P, I32, U32, P, P, U32, P, P]),
     "ysp_mas_domain_scatter": (C.c_int, [P, I32, U32, P, P, P]),
-    "ysp_mas_invert_domains": (C.c_int, [P, P, U32, P, P, P]),
+    "ysp_mas_invert_domains": (C.c_int, [P, P, U32, P, P, U32, P, P]),
     "ysp_mas_apply": (C.c_int, [C.POINTER(MasArgs), P, U32, P, P]),
     "ysp_pcg3_mas_solve": (C.c_int, [U32, D, I32, P, P, P, I32, U32, C.POINTER(MasArgs), P, U32,
                                      P, P, I32, P, P, C.POINTER(PcgStats), P]),

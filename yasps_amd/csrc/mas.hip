@@ -91,7 +91,8 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 __global__ void __launch_bounds__(64 * kInvWaves)
     invert_domains_kernel(const double* __restrict__ dom, float* __restrict__ inv,
-                          uint32_t ndom, int* __restrict__ status) {
+                          uint16_t* __restrict__ inv_bf16, uint32_t bf16_first, uint32_t ndom,
+                          int* __restrict__ status) {
   static_assert(2 * kD == 64, "two domains of D columns per wavefront");
   __shared__ double colk[kInvWaves][2][kD];
   __shared__ double tr[kInvWaves][2][kD][kD + 1];
@@ -139,9 +140,17 @@ __global__ void __launch_bounds__(64 * kInvWaves)
   for (int i = 0; i < kD; ++i) tr[w][h][i][j] = x[i];
   wave_lds_sync();
   if (live) {
-    float* out = inv + static_cast<size_t>(d) * kD * kD;
+    if (inv_bf16 && d >= bf16_first) {
+      uint16_t* out = inv_bf16 + static_cast<size_t>(d - bf16_first) * kD * kD;
 #pragma unroll
-    for (int c = 0; c < kD; ++c) out[c * kD + j] = static_cast<float>(0.5 * (tr[w][h][j][c] + tr[w][h][c][j]));
+      for (int c = 0; c < kD; ++c)
+        out[c * kD + j] = to_bf16(static_cast<float>(0.5 * (tr[w][h][j][c] + tr[w][h][c][j])));
+    } else {
+      float* out = inv + static_cast<size_t>(d) * kD * kD;
+#pragma unroll
+      for (int c = 0; c < kD; ++c)
+        out[c * kD + j] = static_cast<float>(0.5 * (tr[w][h][j][c] + tr[w][h][c][j]));
+    }
     if (bad && j == 0) atomicOr(status, 1);
   }
 }
@@ -218,17 +227,16 @@ __global__ void __launch_bounds__(kB)
   (void)nc;
 }
 
-// one 32-lane group per domain: c_loc = B^-1 r_loc
-__global__ void __launch_bounds__(kB)
-    local_solve_kernel(const float* __restrict__ inv, const uint32_t* __restrict__ dom_nodes,
-                       const uint32_t* __restrict__ dom_node_off, uint32_t ndom,
-                       const double* __restrict__ r0, const double* __restrict__ rc,
-                       uint32_t level0_domains, float* __restrict__ c0,
-                       float* __restrict__ cc, const double* __restrict__ stop) {
-  if (mas_stopped(stop)) return;
-  const uint32_t d = blockIdx.x * (kB / kD) + threadIdx.x / kD;
-  const int lane = threadIdx.x % kD;
-  if (d >= ndom) return;
+// One 32-lane group per domain: c_loc = B^-1 r_loc. Lane i reads row i of the inverse,
+// which is its column i (the stored inverse is exactly symmetric): eight 16-byte loads
+// from the FP32 bank, or four from the BF16 bank; the domain's residual reaches every
+// lane through LDS (the 32 lanes sit in one wavefront).
+template <bool kBF16>
+__device__ __forceinline__ void local_solve_domain(
+    const void* __restrict__ bank, const uint32_t* __restrict__ dom_nodes,
+    const uint32_t* __restrict__ dom_node_off, uint32_t d, size_t row,
+    const double* __restrict__ r0, const double* __restrict__ rc, uint32_t level0_domains,
+    float* __restrict__ c0, float* __restrict__ cc, float* __restrict__ rs, int lane) {
   const uint32_t n0 = dom_node_off[d], n1 = dom_node_off[d + 1];
   const uint32_t P = 3 * (n1 - n0);
   float rl = 0.0f;
@@ -238,28 +246,68 @@ __global__ void __launch_bounds__(kB)
     const uint32_t s = 3 * node + lane % 3;
     rl = static_cast<float>(d < level0_domains ? r0[s] : rc[s]);
   }
-  // the domain's residual goes through LDS (its 32 lanes sit in one wavefront); lane i
-  // reads row i of the inverse, which is its column i (the stored inverse is exactly
-  // symmetric), as eight 16-byte loads
-  __shared__ __align__(16) float rs[kB / kD][kD];
-  const int g = threadIdx.x / kD;
-  rs[g][lane] = rl;
+  rs[lane] = rl;
   wave_lds_sync();
-  const float4* Bi = reinterpret_cast<const float4*>(inv + (static_cast<size_t>(d) * kD + lane) * kD);
-  const float4* R = reinterpret_cast<const float4*>(rs[g]);
+  const float4* R = reinterpret_cast<const float4*>(rs);
   float acc = 0.0f;
+  if (kBF16) {
+    const uint4* Bi = reinterpret_cast<const uint4*>(static_cast<const uint16_t*>(bank) + row * kD);
 #pragma unroll
-  for (int q = 0; q < kD / 4; ++q) {
-    const float4 b = Bi[q], rq = R[q];
-    acc = fmaf(b.x, rq.x, acc);
-    acc = fmaf(b.y, rq.y, acc);
-    acc = fmaf(b.z, rq.z, acc);
-    acc = fmaf(b.w, rq.w, acc);
+    for (int q = 0; q < kD / 8; ++q) {
+      const uint4 b = Bi[q];
+      const float4 ra = R[2 * q], rb = R[2 * q + 1];
+      acc = fmaf(__uint_as_float(b.x << 16), ra.x, acc);
+      acc = fmaf(__uint_as_float(b.x & 0xffff0000u), ra.y, acc);
+      acc = fmaf(__uint_as_float(b.y << 16), ra.z, acc);
+      acc = fmaf(__uint_as_float(b.y & 0xffff0000u), ra.w, acc);
+      acc = fmaf(__uint_as_float(b.z << 16), rb.x, acc);
+      acc = fmaf(__uint_as_float(b.z & 0xffff0000u), rb.y, acc);
+      acc = fmaf(__uint_as_float(b.w << 16), rb.z, acc);
+      acc = fmaf(__uint_as_float(b.w & 0xffff0000u), rb.w, acc);
+    }
+  } else {
+    const float4* Bi = reinterpret_cast<const float4*>(static_cast<const float*>(bank) + row * kD);
+#pragma unroll
+    for (int q = 0; q < kD / 4; ++q) {
+      const float4 b = Bi[q], rq = R[q];
+      acc = fmaf(b.x, rq.x, acc);
+      acc = fmaf(b.y, rq.y, acc);
+      acc = fmaf(b.z, rq.z, acc);
+      acc = fmaf(b.w, rq.w, acc);
+    }
   }
   if (static_cast<uint32_t>(lane) < P) {
     const uint32_t s = 3 * node + lane % 3;
     if (d < level0_domains) c0[s] = acc;
     else cc[s] = acc;
+  }
+}
+
+// Workgroups [0, nb32) hold the FP32-bank domains [0, bf16_first), the rest the BF16-bank
+// domains [bf16_first, ndom): the bank type is uniform per workgroup.
+__global__ void __launch_bounds__(kB)
+    local_solve_kernel(const float* __restrict__ inv, const uint16_t* __restrict__ inv_bf16,
+                       uint32_t bf16_first, const uint32_t* __restrict__ dom_nodes,
+                       const uint32_t* __restrict__ dom_node_off, uint32_t ndom,
+                       const double* __restrict__ r0, const double* __restrict__ rc,
+                       uint32_t level0_domains, float* __restrict__ c0,
+                       float* __restrict__ cc, const double* __restrict__ stop) {
+  if (mas_stopped(stop)) return;
+  constexpr uint32_t kPer = kB / kD;
+  __shared__ __align__(16) float rs[kPer][kD];
+  const int g = threadIdx.x / kD, lane = threadIdx.x % kD;
+  const uint32_t nb32 = (bf16_first + kPer - 1) / kPer;
+  if (blockIdx.x < nb32) {
+    const uint32_t d = blockIdx.x * kPer + g;
+    if (d >= bf16_first) return;
+    local_solve_domain<false>(inv, dom_nodes, dom_node_off, d, static_cast<size_t>(d) * kD + lane,
+                              r0, rc, level0_domains, c0, cc, rs[g], lane);
+  } else {
+    const uint32_t d = bf16_first + (blockIdx.x - nb32) * kPer + g;
+    if (d >= ndom) return;
+    local_solve_domain<true>(inv_bf16, dom_nodes, dom_node_off, d,
+                             static_cast<size_t>(d - bf16_first) * kD + lane, r0, rc,
+                             level0_domains, c0, cc, rs[g], lane);
   }
 }
 
@@ -331,15 +379,22 @@ extern "C" int ysp_mas_domain_scatter(const double* values, int32_t is_full_bsr,
 }
 
 extern "C" int ysp_mas_invert_domains(double* domains, const uint32_t* domain_sizes,
-                                      uint32_t num_domains, float* inverses, int32_t* status,
-                                      void* stream) {
+                                      uint32_t num_domains, float* inverses,
+                                      uint16_t* inverses_bf16, uint32_t bf16_first_domain,
+                                      int32_t* status, void* stream) {
   if (num_domains == 0) return YSP_OK;
+  const uint32_t fp32_domains = inverses_bf16 ? bf16_first_domain : num_domains;
+  if (!domains || !domain_sizes || !status || (inverses_bf16 && bf16_first_domain > num_domains) ||
+      (fp32_domains && !inverses)) {
+    ysp_set_error(__FILE__, __LINE__, "ysp_mas_invert_domains: bad arguments");
+    return YSP_ERR_ARG;
+  }
   hipStream_t s = as_stream(stream);
   hipLaunchKernelGGL(pad_identity_kernel, dim3(num_domains), dim3(64), 0, s, domains, domain_sizes,
                      num_domains);
   hipLaunchKernelGGL(invert_domains_kernel, dim3(grid_for(num_domains, 2 * kInvWaves)),
-                     dim3(64 * kInvWaves), 0, s, domains, inverses,
-                     num_domains, status);
+                     dim3(64 * kInvWaves), 0, s, domains, inverses, inverses_bf16,
+                     bf16_first_domain, num_domains, status);
   YSP_CHECK_LAUNCH();
   return YSP_OK;
 }
@@ -358,11 +413,14 @@ int launch_mas_apply(const MasApply& m, const double* r, uint32_t n, double* z,
       hipLaunchKernelGGL(restrict_kernel, dim3(blocks), dim3(kB), 0, s, r, m.restrict_offsets,
                          m.restrict_items, n1, n2, nc, b1, b2, m.coarse_residual, stop);
   }
-  if (m.num_domains)
-    hipLaunchKernelGGL(local_solve_kernel, dim3(grid_for(m.num_domains, kB / kD)), dim3(kB), 0, s,
-                       m.inverses, m.domain_nodes, m.domain_node_offsets, m.num_domains, r,
-                       m.coarse_residual, m.level0_domains, m.fine_correction,
-                       m.coarse_correction, stop);
+  if (m.num_domains) {
+    const uint32_t first = m.inverses_bf16 ? m.bf16_first_domain : m.num_domains;
+    hipLaunchKernelGGL(local_solve_kernel,
+                       dim3(grid_for(first, kB / kD) + grid_for(m.num_domains - first, kB / kD)),
+                       dim3(kB), 0, s, m.inverses, m.inverses_bf16, first, m.domain_nodes,
+                       m.domain_node_offsets, m.num_domains, r, m.coarse_residual,
+                       m.level0_domains, m.fine_correction, m.coarse_correction, stop);
+  }
   hipLaunchKernelGGL(prolong_kernel, dim3(grid_for(nv, kB)), dim3(kB), 0, s, m.fine_correction,
                      m.coarse_correction, m.prolong_maps, nv, m.num_coarse_levels, m.weights, r, z,
                      rz_slots, stop);
@@ -373,7 +431,9 @@ int launch_mas_apply(const MasApply& m, const double* r, uint32_t n, double* z,
 // Host-side check of the argument block against the kernels' assumptions.
 int check_mas_args(const MasApply& m, uint32_t n) {
   if (n % 3 || m.num_coarse_levels > kMasMaxCoarseLevels || !m.weights || !m.fine_correction ||
-      (m.num_domains && (!m.inverses || !m.domain_nodes || !m.domain_node_offsets)) ||
+      (m.num_domains && (!m.domain_nodes || !m.domain_node_offsets)) ||
+      (m.inverses_bf16 && m.bf16_first_domain > m.num_domains) ||
+      ((m.inverses_bf16 ? m.bf16_first_domain : m.num_domains) && !m.inverses) ||
       m.level0_domains > m.num_domains ||
       (m.num_coarse_levels &&
        (!m.prolong_maps || !m.restrict_offsets || !m.restrict_items || !m.coarse_residual ||

@@ -2,6 +2,8 @@
 #pragma once
 #include "common.hpp"
 
+#include <cstddef>
+
 namespace ysp {
 
 constexpr int kMasMaxCoarseLevels = 8;
@@ -22,7 +24,19 @@ struct MasApply {
   double* coarse_residual;            // [3 x num_coarse_nodes]
   float* fine_correction;             // [n]
   float* coarse_correction;           // [3 x num_coarse_nodes]
+  const uint16_t* inverses_bf16;      // nullptr, or BF16 inverses of domains >= bf16_first
+  uint32_t bf16_first_domain;
 };
+static_assert(sizeof(MasApply) == sizeof(ysp_mas_args) &&
+                  offsetof(MasApply, bf16_first_domain) == offsetof(ysp_mas_args, bf16_first_domain),
+              "MasApply mirrors ysp_mas_args");
+
+// BF16 (round to nearest even) and back.
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return static_cast<uint16_t>(u >> 16);
+}
 
 // z = M r; if rz_slots != nullptr the partial sums of r.z are added to the 64 slots
 // (spaced 16 doubles) starting there. When stop != nullptr and *stop != 0 at run time the

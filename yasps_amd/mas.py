@@ -178,7 +178,7 @@ def build_layout(h: Hierarchy, row_ptr: np.ndarray, col: np.ndarray) -> MasLayou
 class _DeviceMas:
     """Device copies of a layout plus the per-solve numeric buffers."""
 
-    def __init__(self, layout: MasLayout, weights: np.ndarray, n: int):
+    def __init__(self, layout: MasLayout, weights: np.ndarray, n: int, coarse_bank: str = "fp32"):
         self.layout = layout
         u32 = lambda a: dev.to_device(np.asarray(a, dtype=np.uint32), "uint32")   # noqa: E731
         self.dest = [u32(b.dest) for b in layout.blocks]
@@ -194,7 +194,11 @@ class _DeviceMas:
         self.domain_sizes = u32(layout.domain_sizes)
         D = DOMAIN_SCALARS
         self.domains = dev.empty(layout.num_domains * D * D, "float64")
-        self.inverses = dev.empty(layout.num_domains * D * D, "float32")
+        # domains >= bf16_first keep BF16 inverses when the coarse bank is BF16
+        self.bf16_first = layout.level0_domains if coarse_bank == "bf16" else layout.num_domains
+        self.inverses = dev.empty(max(1, self.bf16_first) * D * D, "float32")
+        nb16 = layout.num_domains - self.bf16_first
+        self.inverses_bf16 = dev.empty(nb16 * D * D, "int16") if nb16 else None
         self.status = dev.zeros(1, "int32")
         self.weights = dev.to_device(weights.astype(np.float32), "float32")
         self.coarse_residual = dev.empty(max(1, 3 * nc), "float64")
@@ -217,6 +221,8 @@ class _DeviceMas:
         a.coarse_residual = dev.ptr(self.coarse_residual)
         a.fine_correction = dev.ptr(self.fine_correction)
         a.coarse_correction = dev.ptr(self.coarse_correction)
+        a.inverses_bf16 = dev.ptr(self.inverses_bf16)
+        a.bf16_first_domain = self.bf16_first
         self.args = a
 
     def update(self, full):
@@ -241,6 +247,7 @@ class _DeviceMas:
                        "ysp_mas_domain_scatter")
         _lib.check(lib.ysp_mas_invert_domains(dev.ptr(self.domains), dev.ptr(self.domain_sizes),
                                               self.layout.num_domains, dev.ptr(self.inverses),
+                                              dev.ptr(self.inverses_bf16), self.bf16_first,
                                               dev.ptr(self.status), s), "ysp_mas_invert_domains")
 
     def apply(self, r, z):
@@ -255,15 +262,17 @@ class masSolver:
     ``target_nodes_per_partition``, ``coarsest_level_weight``, ``level_weights``,
     ``global_coarse_domain``; extensions ``threshold_mode`` as for ``jacobianPCGSolver`` and
     ``precision="mixed"`` (FP32 block values in the Krylov loop, FP64 iterative refinement
-    on the true relative residual). Returns 0 when converged and -4 otherwise, as the
-    reference.
+    on the true relative residual) and ``coarse_bank="bf16"`` (the coarse levels' domain
+    inverses stored BF16, level 0 FP32; halves the coarse banks' bytes in every apply).
+    Returns 0 when converged and -4 otherwise, as the reference.
     """
 
     def __init__(self, max_domain_dofs=None, max_levels=None, target_nodes_per_partition=None,
                  coarsest_level_weight: float = 2.5, level_weights=None,
                  global_coarse_domain: bool = True, threshold_mode: str = "reference",
                  inverse_backend: str = "cooperative_gauss_jordan",
-                 allow_cpu_fallback: bool = False, precision: str = "fp64"):
+                 allow_cpu_fallback: bool = False, precision: str = "fp64",
+                 coarse_bank: str = "fp32"):
         if any(v is not None and v <= 0 for v in (max_domain_dofs, max_levels,
                                                   target_nodes_per_partition)):
             raise ValueError("domain capacity/levels must be positive and correction iterations non-negative")
@@ -280,6 +289,8 @@ class masSolver:
             raise ValueError(f"threshold_mode must be one of {sorted(_MODES)}")
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
+        if coarse_bank not in ("fp32", "bf16"):
+            raise ValueError("coarse_bank must be 'fp32' or 'bf16'")
         if inverse_backend != "cooperative_gauss_jordan":
             raise ValueError("only the cooperative_gauss_jordan inverse backend exists on this device path")
         if allow_cpu_fallback:
@@ -292,6 +303,7 @@ class masSolver:
         self.global_coarse_domain = bool(global_coarse_domain)
         self.threshold_mode = threshold_mode
         self.precision = precision
+        self.coarse_bank = coarse_bank
         self._refine = RefinementBuffers()
         self.reset()
 
@@ -343,7 +355,7 @@ class masSolver:
                 self._hierarchy_seconds = time.perf_counter() - t0
             layout = build_layout(self._hierarchy, row_ptr, col)
             w = level_weights(self._hierarchy.num_levels, self.coarsest_level_weight, self.level_weights)
-            self._device = _DeviceMas(layout, w, n)
+            self._device = _DeviceMas(layout, w, n, self.coarse_bank)
             self._device_key = f.key
             rebuilt = True
         if self._solution is None or self._solution.numel() != n:
