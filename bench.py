@@ -187,18 +187,23 @@ def form_leg(cs, args, x64, ref_value, solver, **options):
 
 def serving_leg(args):
     """Serving stream: requests with n log-uniform in [6, 48] and E log-uniform in
-    [1e4, 1e6], first all at once (capacity), then Poisson arrivals at 80% of that
-    capacity; p50/p99 latency from arrival to solution."""
-    from yasps_amd.serving import SolveServer, serving_stream, poisson_arrivals, summary
-    reqs = serving_stream(count=args.serving_requests, seed=args.seed)
+    [1e4, 1e6], each carrying its positions (pinned host payload), first all at once
+    (capacity), then Poisson arrivals at 80% of that capacity; p50/p99 latency from
+    arrival to solution."""
+    from yasps_amd.serving import (SolveServer, attach_payloads, serving_stream,
+                                   poisson_arrivals, summary)
+    reqs = attach_payloads(serving_stream(count=args.serving_requests, seed=args.seed))
     srv = SolveServer(max_active=32, passes=8, tolerance=args.tol)
     t0 = time.perf_counter()
     srv.warm([r.n for r in reqs])
     # the first closed-loop pass grows the slot pool to the concurrency it needs (static
     # structures of meshes that are in flight more than once); the second one measures
-    srv.serve([type(r)(id=r.id, n=r.n, young=r.young, seed=r.seed) for r in reqs])
+    def fresh():
+        return [type(r)(id=r.id, n=r.n, young=r.young, seed=r.seed, positions=r.positions)
+                for r in reqs]
+    srv.serve(fresh())
     warm_s = time.perf_counter() - t0
-    cap = summary(srv.serve([type(r)(id=r.id, n=r.n, young=r.young, seed=r.seed) for r in reqs]))
+    cap = summary(srv.serve(fresh()))
     rate = 0.8 * cap["solves_per_s"]
     poisson_arrivals(reqs, rate, seed=args.seed + 1)
     srv.batch_sizes.clear()
@@ -206,6 +211,8 @@ def serving_leg(args):
     return {"requests": len(reqs), "capacity_solves_per_s": round(cap["solves_per_s"], 2),
             "arrival_rate_per_s": round(rate, 2), "value": round(st["solves_per_s"], 2),
             "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
+            "admission_wait_p50_ms": round(st["wait_p50_ms"], 3),
+            "admission_wait_p99_ms": round(st["wait_p99_ms"], 3),
             "converged": st["converged"], "mean_pcg_iterations": round(st["mean_iterations"], 1),
             "mean_batch": round(float(np.mean(srv.batch_sizes)), 2),
             "max_batch": int(max(srv.batch_sizes)), "warm_seconds": round(warm_s, 2),

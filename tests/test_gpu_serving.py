@@ -72,3 +72,19 @@ def test_hbm_budget_limits_admission():
     srv.budget = srv.pool[6][0].footprint          # room for exactly one request
     done = srv.serve(reqs)
     assert len(done) == 4 and max(srv.batch_sizes) == 1
+
+
+def test_payload_positions_match_seeded_positions():
+    """Requests carrying their positions (pinned payload, asynchronous upload) give the
+    same solutions, bit for bit, as requests whose positions the server derives."""
+    from yasps_amd.serving import SolveServer, SolveRequest, attach_payloads
+    print("payload positions", flush=True)
+    spec = [(5, 2e5), (7, 6e4), (9, 4e5)]
+    plain = [SolveRequest(id=i, n=n, young=y, seed=40 + i) for i, (n, y) in enumerate(spec)]
+    carried = attach_payloads([SolveRequest(id=i, n=n, young=y, seed=40 + i)
+                               for i, (n, y) in enumerate(spec)])
+    a = {r.id: r for r in SolveServer(max_active=4, keep_solutions=True).serve(plain)}
+    b = {r.id: r for r in SolveServer(max_active=4, keep_solutions=True).serve(carried)}
+    for i in a:
+        assert a[i].status >= 0 and a[i].iterations == b[i].iterations
+        np.testing.assert_array_equal(a[i].solution, b[i].solution)

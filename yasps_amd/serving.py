@@ -36,6 +36,9 @@ class SolveRequest:
     seed: int
     arrival: float = 0.0          # seconds after the stream starts
     noise: float = 0.02
+    # moving-vertex positions (flat, pinned host tensor) carried by the request; when None
+    # the server derives them from the seed
+    positions: Optional[object] = None
     # filled by the server
     admitted: Optional[float] = None
     finished: Optional[float] = None
@@ -76,18 +79,19 @@ class _Slot:
             + 4 * (2 * f.nnz + m // 3)
             + (staging.numel() * staging.element_size() if staging is not None else 0))
         self.system = _lib.Pcg3System()
+        self.pos_dev = dev.empty(3 * (cs.rest.shape[0] - cs.n_fixed), "float64")
         self.request: Optional[SolveRequest] = None
 
     def load(self, req: SolveRequest, tolerance: float, max_iterations: int):
         """Positions and material of the request, element evaluation and assembly, block
         Jacobi inverse, and the first Krylov state."""
-        rng = np.random.default_rng(req.seed)
-        cs = self.cs
-        nf = cs.n_fixed
-        h = 1.0 / cs.n
-        pos = cs.rest[nf:] + rng.normal(0.0, req.noise * h, size=(cs.rest.shape[0] - nf, 3))
-        mu, lam = lame_parameters(req.young, cs.poisson)
-        self.h["moving"]["position"].updateValue(pos)
+        if req.positions is not None:
+            # asynchronous upload from the request's pinned payload into the slot's buffer
+            self.pos_dev.copy_(req.positions, non_blocking=True)
+            self.h["moving"]["position"].updateValue(self.pos_dev)
+        else:
+            self.h["moving"]["position"].updateValue(request_positions(self.cs, req))
+        mu, lam = lame_parameters(req.young, self.cs.poisson)
         self.h["mu"].updateValue([mu])
         self.h["lam"].updateValue([lam])
         hs = self.scene.minimizer.computeNumericValue()
@@ -223,6 +227,27 @@ class SolveServer:
         return done
 
 
+def request_positions(cs, req: SolveRequest) -> np.ndarray:
+    """Moving-vertex positions of a request: rest + N(0, (noise h)^2), seeded."""
+    rng = np.random.default_rng(req.seed)
+    nf = cs.n_fixed
+    h = 1.0 / cs.n
+    return cs.rest[nf:] + rng.normal(0.0, req.noise * h, size=(cs.rest.shape[0] - nf, 3))
+
+
+def attach_payloads(requests: List[SolveRequest]) -> List[SolveRequest]:
+    """Give every request its positions as a pinned host tensor (the data a client would
+    send), so that the server's loop only uploads them."""
+    torch = dev.torch()
+    scenes = {}
+    for r in requests:
+        if r.n not in scenes:
+            scenes[r.n] = cube_scene(r.n)
+        pos = request_positions(scenes[r.n], r).reshape(-1)
+        r.positions = torch.from_numpy(np.ascontiguousarray(pos)).pin_memory()
+    return requests
+
+
 def serving_stream(count: int = 512, n_min: int = 6, n_max: int = 48, young_min: float = 1e4,
                    young_max: float = 1e6, seed: int = 0) -> List[SolveRequest]:
     """Requests with n log-uniform in [n_min, n_max] and E log-uniform in [young_min,
@@ -246,9 +271,12 @@ def poisson_arrivals(requests: List[SolveRequest], rate: float, seed: int = 1):
 
 def summary(done: List[SolveRequest]) -> dict:
     lat = np.array([r.latency for r in done])
+    wait = np.array([r.admitted - r.arrival for r in done])
     span = max(r.finished for r in done) - min(r.arrival for r in done)
     return {"requests": len(done), "solves_per_s": len(done) / span,
             "p50_ms": 1e3 * float(np.percentile(lat, 50)),
             "p99_ms": 1e3 * float(np.percentile(lat, 99)),
+            "wait_p50_ms": 1e3 * float(np.percentile(wait, 50)),
+            "wait_p99_ms": 1e3 * float(np.percentile(wait, 99)),
             "converged": int(sum(1 for r in done if r.status is not None and r.status >= 0)),
             "mean_iterations": float(np.mean([r.iterations for r in done]))}
