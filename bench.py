@@ -189,35 +189,45 @@ def serving_leg(args):
     """Serving stream: requests with n log-uniform in [6, 48] and E log-uniform in
     [1e4, 1e6], each carrying its positions (pinned host payload), first all at once
     (capacity), then Poisson arrivals at 80% of that capacity; p50/p99 latency from
-    arrival to solution."""
+    arrival to solution. The default server overlaps request loading (a loader thread
+    and stream) with the batched PCG chunks; the same stream through the sequential
+    server (loads between chunks: larger batches in bursts, longer waits) is reported
+    beside it."""
     from yasps_amd.serving import (SolveServer, attach_payloads, serving_stream,
                                    poisson_arrivals, summary)
-    reqs = attach_payloads(serving_stream(count=args.serving_requests, seed=args.seed))
-    srv = SolveServer(max_active=32, passes=8, tolerance=args.tol)
-    t0 = time.perf_counter()
-    srv.warm([r.n for r in reqs])
-    # the first closed-loop pass grows the slot pool to the concurrency it needs (static
-    # structures of meshes that are in flight more than once); the second one measures
-    def fresh():
-        return [type(r)(id=r.id, n=r.n, young=r.young, seed=r.seed, positions=r.positions)
-                for r in reqs]
-    srv.serve(fresh())
-    warm_s = time.perf_counter() - t0
-    cap = summary(srv.serve(fresh()))
-    rate = 0.8 * cap["solves_per_s"]
-    poisson_arrivals(reqs, rate, seed=args.seed + 1)
-    srv.batch_sizes.clear()
-    st = summary(srv.serve(reqs))
-    return {"requests": len(reqs), "capacity_solves_per_s": round(cap["solves_per_s"], 2),
-            "arrival_rate_per_s": round(rate, 2), "value": round(st["solves_per_s"], 2),
-            "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
-            "admission_wait_p50_ms": round(st["wait_p50_ms"], 3),
-            "admission_wait_p99_ms": round(st["wait_p99_ms"], 3),
-            "converged": st["converged"], "mean_pcg_iterations": round(st["mean_iterations"], 1),
-            "mean_batch": round(float(np.mean(srv.batch_sizes)), 2),
-            "max_batch": int(max(srv.batch_sizes)), "warm_seconds": round(warm_s, 2),
-            "slots": int(sum(len(v) for v in srv.pool.values())),
-            "solver": "batched block-Jacobi PCG, FP64, ||g-Hx||/||g|| <= %g" % args.tol}
+    base = attach_payloads(serving_stream(count=args.serving_requests, seed=args.seed))
+
+    def fresh(arrivals=False):
+        return [type(r)(id=r.id, n=r.n, young=r.young, seed=r.seed, positions=r.positions,
+                        arrival=r.arrival if arrivals else 0.0) for r in base]
+
+    def run(overlap):
+        srv = SolveServer(max_active=32, passes=8, tolerance=args.tol, overlap_loads=overlap)
+        t0 = time.perf_counter()
+        srv.warm([r.n for r in base])
+        # the first closed-loop pass grows the slot pool to the concurrency it needs (static
+        # structures of meshes that are in flight more than once); the second one measures
+        srv.serve(fresh())
+        warm_s = time.perf_counter() - t0
+        cap = summary(srv.serve(fresh()))
+        rate = 0.8 * cap["solves_per_s"]
+        reqs = poisson_arrivals(fresh(), rate, seed=args.seed + 1)
+        srv.batch_sizes.clear()
+        st = summary(srv.serve(reqs))
+        return {"capacity_solves_per_s": round(cap["solves_per_s"], 2),
+                "arrival_rate_per_s": round(rate, 2), "value": round(st["solves_per_s"], 2),
+                "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
+                "admission_wait_p50_ms": round(st["wait_p50_ms"], 3),
+                "admission_wait_p99_ms": round(st["wait_p99_ms"], 3),
+                "converged": st["converged"], "mean_pcg_iterations": round(st["mean_iterations"], 1),
+                "mean_batch": round(float(np.mean(srv.batch_sizes)), 2),
+                "max_batch": int(max(srv.batch_sizes)), "warm_seconds": round(warm_s, 2),
+                "slots": int(sum(len(v) for v in srv.pool.values()))}
+
+    out = {"requests": len(base), **run(True),
+           "solver": "batched block-Jacobi PCG, FP64, ||g-Hx||/||g|| <= %g" % args.tol}
+    out["sequential_loads"] = run(False)
+    return out
 
 
 def adjoint_leg(args):

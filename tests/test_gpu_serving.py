@@ -25,7 +25,8 @@ def test_batched_pcg_matches_single_system_solver():
     print("batched vs single", flush=True)
     reqs = [SolveRequest(id=i, n=n, young=y, seed=10 + i)
             for i, (n, y) in enumerate([(4, 1e5), (7, 3e4), (5, 8e5), (6, 2e5)])]
-    srv = SolveServer(max_active=8, keep_solutions=True)
+    # sequential admission: the four requests arriving together share every chunk
+    srv = SolveServer(max_active=8, keep_solutions=True, overlap_loads=False)
     done = {r.id: r for r in srv.serve(reqs)}
     assert max(srv.batch_sizes) == 4
     for r in reqs:
@@ -76,7 +77,8 @@ def test_hbm_budget_limits_admission():
 
 def test_payload_positions_match_seeded_positions():
     """Requests carrying their positions (pinned payload, asynchronous upload) give the
-    same solutions, bit for bit, as requests whose positions the server derives."""
+    same solutions, bit for bit, as requests whose positions the server derives, and the
+    loader-thread server gives the same bits as the sequential one."""
     from yasps_amd.serving import SolveServer, SolveRequest, attach_payloads
     print("payload positions", flush=True)
     spec = [(5, 2e5), (7, 6e4), (9, 4e5)]
@@ -85,6 +87,22 @@ def test_payload_positions_match_seeded_positions():
                                for i, (n, y) in enumerate(spec)])
     a = {r.id: r for r in SolveServer(max_active=4, keep_solutions=True).serve(plain)}
     b = {r.id: r for r in SolveServer(max_active=4, keep_solutions=True).serve(carried)}
+    seq = [SolveRequest(id=r.id, n=r.n, young=r.young, seed=r.seed, positions=r.positions)
+           for r in carried]
+    c = {r.id: r for r in SolveServer(max_active=4, keep_solutions=True,
+                                      overlap_loads=False).serve(seq)}
     for i in a:
-        assert a[i].status >= 0 and a[i].iterations == b[i].iterations
+        assert a[i].status >= 0 and a[i].iterations == b[i].iterations == c[i].iterations
         np.testing.assert_array_equal(a[i].solution, b[i].solution)
+        np.testing.assert_array_equal(b[i].solution, c[i].solution)
+
+
+def test_request_larger_than_the_budget_is_rejected():
+    from yasps_amd.serving import SolveServer, SolveRequest
+    print("budget rejection", flush=True)
+    for overlap in (True, False):
+        srv = SolveServer(max_active=2, overlap_loads=overlap)
+        srv.warm([5])
+        srv.budget = srv.pool[5][0].footprint - 1
+        with pytest.raises(ValueError, match="does not fit"):
+            srv.serve([SolveRequest(id=0, n=5, young=1e5, seed=0)])

@@ -10,12 +10,16 @@ others iterate join at the next chunk; finished ones leave and free their slot.
 
 Meshes of one size share a pool of slots whose static sparsity structure (index
 pipeline, block layout, full block-CSR) is built once; a request only reloads positions
-and material, evaluates the elements and assembles (``hessian.compute``).
+and material, evaluates the elements and assembles (``hessian.compute``). By default a
+loader thread does that on a second stream, so the element kernels of arriving requests
+overlap the batched PCG chunks of the admitted ones.
 """
 from __future__ import annotations
 
 import ctypes as C
 import math
+import queue
+import threading
 import time
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional
@@ -132,9 +136,11 @@ class SolveServer:
 
     def __init__(self, hbm_budget_bytes: int = 64 << 30, max_active: int = 32, passes: int = 8,
                  tolerance: float = 1e-6, max_iterations: int = 20000, precision: str = "fp64",
-                 keep_solutions: bool = False):
+                 keep_solutions: bool = False, overlap_loads: bool = True):
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.overlap_loads = bool(overlap_loads)
+        self._lock = threading.Lock()           # the slot pool, shared with the loader
         self.budget = int(hbm_budget_bytes)
         self.max_active = int(max_active)
         self.passes = int(passes)
@@ -179,14 +185,135 @@ class SolveServer:
 
     # ------------------------------------------------------------ loop
     def serve(self, requests: List[SolveRequest], clock=time.perf_counter) -> List[SolveRequest]:
-        lib = _lib.load()
+        """Serve a request stream; returns the finished requests (arrival, admission and
+        finish times, status, PCG iterations)."""
+        if self.overlap_loads:
+            return self._serve_overlapped(requests, clock)
+        return self._serve_sequential(requests, clock)
+
+    def _retire(self, active, arr, t_end, done):
         torch = dev.torch()
+        keep = []
+        for sl, y in zip(active, arr):
+            sl.system = y
+            if y.status != _lib.PCG3_RUNNING:
+                req = sl.request
+                req.finished = t_end
+                req.status = int(y.status)
+                req.iterations = int(y.iterations)
+                if self.keep_solutions:
+                    torch.cuda.synchronize()
+                    req.solution = dev.to_host(sl.solution).copy()
+                with self._lock:
+                    sl.request = None
+                done.append(req)
+            else:
+                keep.append(sl)
+        return keep
+
+    def _run_chunk(self, active, s):
+        lib = _lib.load()
+        mixed = int(self.precision == "mixed")
+        arr = (_lib.Pcg3System * len(active))(*[sl.system for sl in active])
+        _lib.check(lib.ysp_pcg3_batch_run(arr, len(active), self.passes, mixed, mixed, s),
+                   "ysp_pcg3_batch_run")
+        self.chunks += 1
+        self.batch_sizes.append(len(active))
+        return arr
+
+    def _serve_overlapped(self, requests, clock):
+        """Admission hands requests to a loader thread that evaluates and assembles them on
+        its own stream (element kernels of new requests overlap the batched PCG chunks);
+        a loaded request joins the next chunk after the solver stream waits on its load
+        event."""
+        torch = dev.torch()
+        pending = sorted(requests, key=lambda r: r.arrival)
+        active: List[_Slot] = []
+        done: List[SolveRequest] = []
+        todo: "queue.Queue" = queue.Queue()
+        ready = []
+        reserved = {}                           # request id -> footprint held while loading
+        load_stream = torch.cuda.Stream()
+        main = torch.cuda.current_stream()
+
+        def loader():
+            with torch.cuda.stream(load_stream):
+                while True:
+                    req = todo.get()
+                    if req is None:
+                        return
+                    try:
+                        with self._lock:
+                            slot = next((sl for sl in self.pool.get(req.n, [])
+                                         if sl.request is None), None)
+                            if slot is not None:
+                                slot.request = req
+                        if slot is None:              # a new slot, built outside the lock
+                            slot = _Slot(req.n, self.precision)
+                            slot.request = req
+                            with self._lock:
+                                self.pool.setdefault(req.n, []).append(slot)
+                        slot.load(req, self.tolerance, self.max_iterations)
+                        ev = torch.cuda.Event()
+                        ev.record(load_stream)
+                        item = (slot, req, ev, None)
+                    except BaseException as e:         # reported by the serving loop
+                        item = (None, req, None, e)
+                    with self._lock:
+                        ready.append(item)
+
+        th = threading.Thread(target=loader, name="yasps-loader", daemon=True)
+        t0 = clock()
+        th.start()
+        s = main.cuda_stream
+        try:
+            while pending or active or reserved:
+                now = clock() - t0
+                while pending and pending[0].arrival <= now and \
+                        len(active) + len(reserved) < self.max_active:
+                    n = pending[0].n
+                    with self._lock:
+                        idle = [sl for sl in self.pool.get(n, []) if sl.request is None]
+                        need = idle[0].footprint if idle else self._estimate(n)
+                        if self._in_use() + sum(reserved.values()) + need > self.budget:
+                            break
+                    req = pending.pop(0)
+                    req.admitted = clock() - t0
+                    reserved[req.id] = need
+                    todo.put(req)
+                with self._lock:
+                    got, ready[:] = list(ready), []
+                for slot, req, ev, err in got:
+                    if err is not None:
+                        raise err
+                    main.wait_event(ev)
+                    del reserved[req.id]
+                    active.append(slot)
+                if not active:
+                    if reserved:
+                        time.sleep(2e-5)
+                        continue
+                    if pending and pending[0].arrival <= clock() - t0:
+                        raise ValueError(f"request {pending[0].id} (n={pending[0].n}) does not "
+                                         "fit the HBM budget of an empty server")
+                    if pending:
+                        wait = pending[0].arrival - (clock() - t0)
+                        if wait > 0:
+                            time.sleep(wait)
+                    continue
+                arr = self._run_chunk(active, s)
+                active = self._retire(active, arr, clock() - t0, done)
+        finally:
+            todo.put(None)
+            th.join()
+        return done
+
+    def _serve_sequential(self, requests, clock):
         pending = sorted(requests, key=lambda r: r.arrival)
         active: List[_Slot] = []
         done: List[SolveRequest] = []
         t0 = clock()
         s = dev.stream()
-        mixed = int(self.precision == "mixed")
         while pending or active:
             now = clock() - t0
             while pending and pending[0].arrival <= now and len(active) < self.max_active \
@@ -197,33 +324,16 @@ class SolveServer:
                 req.admitted = clock() - t0
                 active.append(slot)
             if not active:
+                if pending and pending[0].arrival <= clock() - t0:
+                    raise ValueError(f"request {pending[0].id} (n={pending[0].n}) does not "
+                                     "fit the HBM budget of an empty server")
                 if pending:
                     wait = pending[0].arrival - (clock() - t0)
                     if wait > 0:
                         time.sleep(wait)
                 continue
-            arr = (_lib.Pcg3System * len(active))(*[sl.system for sl in active])
-            _lib.check(lib.ysp_pcg3_batch_run(arr, len(active), self.passes, mixed, mixed, s),
-                       "ysp_pcg3_batch_run")
-            self.chunks += 1
-            self.batch_sizes.append(len(active))
-            t_end = clock() - t0
-            keep = []
-            for sl, y in zip(active, arr):
-                sl.system = y
-                if y.status != _lib.PCG3_RUNNING:
-                    req = sl.request
-                    req.finished = t_end
-                    req.status = int(y.status)
-                    req.iterations = int(y.iterations)
-                    if self.keep_solutions:
-                        torch.cuda.synchronize()
-                        req.solution = dev.to_host(sl.solution).copy()
-                    sl.request = None
-                    done.append(req)
-                else:
-                    keep.append(sl)
-            active = keep
+            arr = self._run_chunk(active, s)
+            active = self._retire(active, arr, clock() - t0, done)
         return done
 
 
