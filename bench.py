@@ -305,8 +305,17 @@ def cpu_baseline(cs, values_host, st, tol):
         t_cg = time.perf_counter() - t0
         rel = float(np.linalg.norm(b - A @ x) / np.linalg.norm(b))
     total = t_asm + t_cg
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": round(1.0 / total, 4), "unit": "Newton solves/s", "cores": threads,
-            "kind": "port",
+            "kind": "port", "host_cores_available": len(os.sched_getaffinity(0)),
+            "cpu_model": model, "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "threads_note": "numpy/BLAS limited to 1 thread (threadpoolctl); scipy CG and "
+                            "SpMV are single-threaded",
             "sample": (f"n={cs.n} cube: SNH assembly+projection on 1/{frac} of the {T} tets "
                        f"timed ({t_asm / frac:.2f}s) and scaled x{frac}; block-Jacobi CG "
                        f"(scipy, tol {tol:g}) timed in full ({t_cg:.2f}s, info={info}, "
