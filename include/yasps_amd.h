@@ -32,7 +32,15 @@ extern "C" {
 /* ---------------------------------------------------------------- runtime */
 const char* ysp_last_error(void);
 int ysp_version(void);
-int ysp_sync(void* stream);
+/* Entry points whose kernels flag an unsupported input (the element evaluations' slot
+ * layouts) do not synchronise: the flag is copied back in stream order and examined by
+ * the next ysp_check_deferred (or ysp_sync) on the same stream, which synchronises the
+ * stream and returns YSP_ERR_UNSUPPORTED (message in ysp_last_error) if any queued flag
+ * was raised. The reference's kernel calls are synchronous and raise at once
+ * (hessianAndGradientKernel.pyx:348-349); the Python facade restores that by checking
+ * at the end of every assembly, or after the solve in a Newton step. */
+int ysp_check_deferred(void* stream);
+int ysp_sync(void* stream);   /* ysp_check_deferred */
 /* Frees the library's grow-on-demand scratch buffers. */
 void ysp_release_scratch(void);
 

@@ -90,6 +90,7 @@ SIGNATURES = {
     "ysp_last_error": (C.c_char_p, []),
     "ysp_version": (C.c_int, []),
     "ysp_sync": (C.c_int, [P]),
+    "ysp_check_deferred": (C.c_int, [P]),
     "ysp_release_scratch": (None, []),
     "ysp_get_indices": (C.c_int, [P, I32, P, I32, P, I32, P, P, P, U32, U32, P]),
     "ysp_compress_indices": (C.c_int, [P, P, P, P, P, P, P, P, P, U32, U32, U32, P]),

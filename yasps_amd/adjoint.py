@@ -182,6 +182,7 @@ class MaterialProblem:
             C.byref(u), dev.ptr(e.conn.value), self.num_tets, dev.ptr(dm_inv), dev.ptr(vol),
             dev.ptr(self.mu_att.value), 1, dev.ptr(self.lam_att.value), 1, 1.0, C.byref(sa),
             dev.ptr(lam), dev.ptr(out), dev.stream()), "ysp_snh_theta_adjoint")
+        _lib.check(_lib.load().ysp_check_deferred(dev.stream()), "ysp_snh_theta_adjoint")
         return dev.to_host(out)
 
 

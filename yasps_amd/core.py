@@ -145,10 +145,14 @@ class attribute:
         return self._dev
 
     def host_value(self) -> np.ndarray:
-        if self.operator in (DATA, CONSTANT) and self._dev is None:
-            if self._host is None:
+        """Host copy of the values. DATA/CONSTANT values set from the host keep that copy
+        (the device copy is made from it and replaced, not modified, by updateValue), so
+        reading a scalar parameter does not synchronise the stream."""
+        if self.operator in (DATA, CONSTANT):
+            if self._host is not None:
+                return self._host.copy()
+            if self._dev is None:
                 return np.zeros(self.size * self.numInstances)
-            return self._host.copy()
         return dev.to_host(self.value)
 
     def compute(self):

@@ -29,6 +29,10 @@ inline unsigned grid_for(size_t n, unsigned block) {
   return static_cast<unsigned>(g == 0 ? 1 : g);
 }
 
+// Queue a check of a device flag word written by the kernels just launched on s; the
+// flag is examined (error `what` when nonzero) at the next ysp_check_deferred(s).
+int defer_flag_check(const int* flag, hipStream_t s, const char* what);
+
 // Device scratch that grows on demand and is released by ysp_release_scratch().
 // Each slot is an independent buffer so nested users do not alias.
 void* scratch(int slot, size_t bytes);

@@ -601,14 +601,7 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 int check_flag(int* flag, hipStream_t s, const char* what) {
-  int h = 0;
-  YSP_CHECK(hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, s));
-  YSP_CHECK(hipStreamSynchronize(s));
-  if (h) {
-    ysp_set_error(__FILE__, __LINE__, what);
-    return YSP_ERR_UNSUPPORTED;
-  }
-  return YSP_OK;
+  return defer_flag_check(flag, s, what);
 }
 
 }  // namespace

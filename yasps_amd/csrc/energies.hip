@@ -605,14 +605,9 @@ extern "C" int ysp_eval_element_energy(int32_t kind, const ysp_union_f64* positi
                                             projection_method, SA, flag, s);
   });
   YSP_CHECK_LAUNCH();
-  int h = 0;
-  YSP_CHECK(hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, s));
-  YSP_CHECK(hipStreamSynchronize(s));
-  if (h) {
-    ysp_set_error(__FILE__, __LINE__,
-                  "ysp_eval_element_energy: slot layout does not match the element's vertices");
-    return YSP_ERR_UNSUPPORTED;
-  }
+  if (int rc = defer_flag_check(flag, s,
+                                 "ysp_eval_element_energy: slot layout does not match the element's vertices"))
+    return rc;
   return YSP_OK;
 }
 

@@ -19,6 +19,11 @@ from . import _lib, device as dev
 from .program import encode
 
 
+def check_deferred():
+    """Synchronise the current stream and raise if a queued kernel flag was set."""
+    _lib.check(_lib.load().ysp_check_deferred(dev.stream()), "element evaluation")
+
+
 class vector:
     """A device FP64 vector (``matrixAndVector/vector``)."""
 
@@ -357,7 +362,11 @@ class hessian:
                                 dev.ptr(self.gradient.value), dev.ptr(part.values),
                                 dev.ptr(self.diagonal_blocks))
 
-    def compute(self, local_gradient=None):
+    def compute(self, local_gradient=None, check: bool = True):
+        """Evaluate, project and assemble every term. The element kernels' flags (slot
+        layouts they do not implement) are examined at the end (check=True: synchronises
+        the stream) or by the caller's later ``check_deferred`` (a Newton step checks
+        after its solve, so that assembly and solve run without a host round trip)."""
         self._ensure_setup()
         if local_gradient is not None and local_gradient is not self.gradient:
             self.gradient = local_gradient
@@ -367,6 +376,8 @@ class hessian:
         else:
             self._compute_atomic()
         self._update_full()
+        if check:
+            check_deferred()
         return self
 
     def _compute_atomic(self):

@@ -10,7 +10,7 @@ from __future__ import annotations
 from typing import List, Optional
 
 from . import device as dev
-from .hessian import hessian
+from .hessian import check_deferred, hessian
 from .solver import solver
 
 
@@ -83,9 +83,9 @@ class minimizer:
     def activeHessian(self) -> Optional[hessian]:
         return self.__hessian
 
-    def computeNumericValue(self) -> hessian:
+    def computeNumericValue(self, check: bool = True) -> hessian:
         h = self._active()
-        h.compute()
+        h.compute(check=check)
         return h
 
     @property
@@ -110,9 +110,10 @@ class minimizer:
         return self.__segments
 
     def computeHessianAndGradient(self, tolerance=1e-3, maxIterations=20000):
-        h = self.computeNumericValue()
+        h = self.computeNumericValue(check=False)
         rc = self.__solver.computeSolution(h, h.gradient, None, tolerance=tolerance,
                                            maxIterations=maxIterations, zero_initial_guess=True)
+        check_deferred()                  # element flags of the assembly, after the solve
         sol = self.__solver.solution
         st = h.gradient_segments_start_cpu
         self.__segments = [sol[st[i]:st[i + 1]] for i in range(len(st) - 1)]
