@@ -39,6 +39,8 @@ def parse():
                     help="skip the mixed-precision and MAS legs")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving stream")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the material adjoint")
+    ap.add_argument("--no-dynamic", action="store_true",
+                    help="skip the dynamic contact-term leg")
     ap.add_argument("--adjoint-n", type=int, default=32)
     ap.add_argument("--adjoint-fd", type=int, default=64)
     ap.add_argument("--serving-requests", type=int, default=512)
@@ -271,6 +273,53 @@ def adjoint_leg(args):
             "loss": prob.loss(x, target)}
 
 
+def dynamic_leg(args):
+    """The n=48 cube with a dynamic point-point barrier over grid-neighbour pairs of moving
+    vertices (axis and face-diagonal neighbours, squared distances below d^2 = 1e-3):
+    every Newton solve re-indexes the dynamic term (index generation, local compression,
+    coordinates, global dedupe, staging layout), evaluates, assembles the static and
+    dynamic parts and solves. Reports solves/s and the re-index time alone."""
+    from yasps_amd import device as dev
+    from yasps_amd.energies import point_point
+    from yasps_amd.scenes import cube_scene, build_yasps_scene
+    torch = dev.torch()
+    cs = cube_scene(args.n, seed=args.seed)
+    s0, hd = build_yasps_scene(cs, threshold_mode="relative_residual")
+    n1 = args.n + 1
+    nf = cs.n_fixed
+    # moving vertices sit on the (n+1)^2 x n grid above the fixed bottom layer
+    ijk = np.round(cs.rest[nf:] * args.n).astype(np.int64)
+    key = {tuple(v): nf + i for i, v in enumerate(ijk)}
+    pairs = []
+    for d in ((1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 0), (1, 0, 1), (0, 1, 1)):
+        for i, v in enumerate(ijk):
+            w = (v[0] + d[0], v[1] + d[1], v[2] + d[2])
+            j = key.get(w)
+            if j is not None:
+                pairs.append((nf + i, j))
+    pairs = np.array(pairs, dtype=np.uint32)
+    m = hd["mesh"]
+    contacts = m.addPrimitive("contacts", numInstances=len(pairs), isDynamic=True)
+    c2v = contacts.addConnectivity("c2v", m.vertices, pairs, 2)
+    cp = contacts.addAttribute("positions", through=c2v, source=m.vertices["position"])
+    pp = contacts.addAttribute("pp", computed_attribute=point_point(cp, 1e-3, 1e-2))
+    s0.addEnergy(pp, projection_method=2, dynamic_instances=True)
+    total, lat, iters = timed_solves(s0, max(3, args.steps // 2), 2, args.tol)
+    hs = s0.minimizer.activeHessian
+    reps = 5
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        hs._reindex_dynamic()
+    torch.cuda.synchronize()
+    reindex_ms = 1e3 * (time.perf_counter() - t0) / reps
+    return {"pairs": int(len(pairs)), "dynamic_blocks": int(hs.dynamic.num_blocks),
+            "static_blocks": int(hs.static.num_blocks), "grid": n1,
+            "value": round(len(lat) / total, 3), "unit": "Newton solves/s",
+            "ms_per_step": round(1e3 * total / len(lat), 3),
+            "pcg_iterations": int(np.median(iters)), "reindex_ms": round(reindex_ms, 3)}
+
+
 def cpu_baseline(cs, values_host, st, tol):
     """The oracle on the host: SNH + inertia assembly timed on 1/8 of the tets (scaled x8)
     and block-Jacobi CG (scipy, one thread) timed in full on the same matrix."""
@@ -391,6 +440,7 @@ def main():
         "forms": forms,
         "serving": None if args.no_serving else serving_leg(args),
         "adjoint": None if args.no_adjoint else adjoint_leg(args),
+        "dynamic_contacts": None if args.no_dynamic else dynamic_leg(args),
     }
     if not args.no_cpu:
         from oracle import newton
