@@ -39,29 +39,57 @@ __device__ __forceinline__ uint32_t xcd_swizzle(uint32_t bid, uint32_t nblocks) 
   return xcd * per + slot;
 }
 
+// Block b of a part writes the entries 2b and 2b + 1 of the part's range: (r, c) and
+// its mirror (c, r), or, for a diagonal block, the sentinel key nbr^2 that sorts past
+// every real entry. Fixed slots keep the order of equal keys (a block present in both
+// parts) independent of scheduling, so the sums of the SpMV are reproducible.
 __global__ void emit_entries_kernel(const uint32_t* __restrict__ pos, uint32_t nb, uint32_t tag,
                                     uint32_t nbr, uint64_t* __restrict__ keys,
-                                    uint32_t* __restrict__ vals, uint32_t* __restrict__ cursor) {
+                                    uint32_t* __restrict__ vals) {
   const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   const uint32_t r = pos[2 * b] / 3, c = pos[2 * b + 1] / 3;
-  const bool diag = r == c;
-  const uint32_t slot = wave_reserve(cursor, diag ? 1u : 2u);
-  keys[slot] = static_cast<uint64_t>(r) * nbr + c;
-  vals[slot] = b | tag;
-  if (!diag) {
-    keys[slot + 1] = static_cast<uint64_t>(c) * nbr + r;
-    vals[slot + 1] = b | tag | kTrans;
-  }
+  const uint64_t sentinel = static_cast<uint64_t>(nbr) * nbr;
+  keys[2 * static_cast<size_t>(b)] = static_cast<uint64_t>(r) * nbr + c;
+  vals[2 * static_cast<size_t>(b)] = b | tag;
+  keys[2 * static_cast<size_t>(b) + 1] = r == c ? sentinel : static_cast<uint64_t>(c) * nbr + r;
+  vals[2 * static_cast<size_t>(b) + 1] = b | tag | kTrans;
 }
 
-__global__ void split_keys_kernel(const uint64_t* __restrict__ keys, uint32_t nnz, uint32_t nbr,
-                                  uint32_t* __restrict__ col, uint32_t* __restrict__ row_count) {
+// key / nbr for keys below 2^52: the double-precision quotient, corrected by one.
+__device__ __forceinline__ uint32_t key_row(uint64_t key, uint32_t nbr) {
+  uint64_t q = static_cast<uint64_t>(static_cast<double>(key) / static_cast<double>(nbr));
+  if (q * nbr > key) --q;
+  else if ((q + 1) * nbr <= key) ++q;
+  return static_cast<uint32_t>(q);
+}
+
+// Number of real entries: the first sentinel in the sorted keys (one thread, binary
+// search).
+__global__ void count_entries_kernel(const uint64_t* __restrict__ keys, uint32_t n,
+                                     uint64_t sentinel, uint32_t* __restrict__ nnz) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (keys[mid] < sentinel) lo = mid + 1;
+    else hi = mid;
+  }
+  *nnz = lo;
+}
+
+// Columns and row starts from the sorted keys, without atomics: entry k starts every
+// row after the previous entry's row up to its own (empty rows included); k = nnz closes
+// the remaining rows.
+__global__ void split_keys_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ nnz_p,
+                                  uint32_t nbr, uint32_t* __restrict__ col,
+                                  uint32_t* __restrict__ row_ptr) {
+  const uint32_t nnz = *nnz_p;
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nnz) return;
-  const uint64_t key = keys[k];
-  col[k] = static_cast<uint32_t>(key % nbr);
-  atomicAdd(row_count + static_cast<uint32_t>(key / nbr), 1u);
+  if (k > nnz) return;
+  const uint32_t rk = k < nnz ? key_row(keys[k], nbr) : nbr;
+  const uint32_t first = k == 0 ? 0 : key_row(keys[k - 1], nbr) + 1;
+  for (uint32_t rr = first; rr <= rk; ++rr) row_ptr[rr] = k;
+  if (k < nnz) col[k] = static_cast<uint32_t>(keys[k] - static_cast<uint64_t>(rk) * nbr);
 }
 
 template <typename VT>
@@ -518,48 +546,50 @@ extern "C" int ysp_bsr3_build(const uint32_t* pos_static, uint32_t nb_static,
     YSP_CHECK(hipMemsetAsync(row_ptr, 0, sizeof(uint32_t) * (nbr + 1), s));
     return YSP_OK;
   }
+  if (cap > 0xffffffffull) {
+    ysp_set_error(__FILE__, __LINE__, "ysp_bsr3_build: too many blocks");
+    return YSP_ERR_ARG;
+  }
   uint64_t* keys = static_cast<uint64_t*>(scratch(40, sizeof(uint64_t) * cap));
   uint32_t* vals = static_cast<uint32_t*>(scratch(41, sizeof(uint32_t) * cap));
   uint64_t* keys2 = static_cast<uint64_t*>(scratch(42, sizeof(uint64_t) * cap));
-  uint32_t* cursor = static_cast<uint32_t*>(scratch(43, sizeof(uint32_t) * (nbr + 2)));
-  if (!keys || !vals || !keys2 || !cursor) return YSP_ERR_HIP;
-  YSP_CHECK(hipMemsetAsync(cursor, 0, sizeof(uint32_t), s));
+  uint32_t* count = static_cast<uint32_t*>(scratch(43, sizeof(uint32_t)));
+  uint32_t* vals2 = static_cast<uint32_t*>(scratch(45, sizeof(uint32_t) * cap));
+  if (!keys || !vals || !keys2 || !count || !vals2) return YSP_ERR_HIP;
   if (nb_static)
     hipLaunchKernelGGL(emit_entries_kernel, dim3(grid_for(nb_static, kB)), dim3(kB), 0, s,
-                       pos_static, nb_static, 0u, nbr, keys, vals, cursor);
+                       pos_static, nb_static, 0u, nbr, keys, vals);
   if (nb_dynamic)
     hipLaunchKernelGGL(emit_entries_kernel, dim3(grid_for(nb_dynamic, kB)), dim3(kB), 0, s,
-                       pos_dynamic, nb_dynamic, kDyn, nbr, keys, vals, cursor);
+                       pos_dynamic, nb_dynamic, kDyn, nbr, keys + 2 * static_cast<size_t>(nb_static),
+                       vals + 2 * static_cast<size_t>(nb_static));
   YSP_CHECK_LAUNCH();
+  // sort (row, col) keys (stable); the value keeps the source block and its orientation
+  const uint64_t sentinel = static_cast<uint64_t>(nbr) * nbr;
+  int end_bit = 1;
+  while (end_bit < 64 && (1ull << end_bit) <= sentinel) ++end_bit;
+  hipcub::DoubleBuffer<uint64_t> kb(keys, keys2);
+  hipcub::DoubleBuffer<uint32_t> vb(vals, vals2);
+  const uint32_t total = static_cast<uint32_t>(cap);
+  size_t temp = 0;
+  YSP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, temp, kb, vb, total, 0, end_bit, s));
+  void* tmp = scratch(44, temp);
+  if (!tmp) return YSP_ERR_HIP;
+  YSP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, temp, kb, vb, total, 0, end_bit, s));
+  hipLaunchKernelGGL(count_entries_kernel, dim3(1), dim3(1), 0, s, kb.Current(), total, sentinel,
+                     count);
   uint32_t nnz = 0;
-  YSP_CHECK(hipMemcpyAsync(&nnz, cursor, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  YSP_CHECK(hipMemcpyAsync(&nnz, count, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   YSP_CHECK(hipStreamSynchronize(s));
   if (nnz > capacity) {
     ysp_set_error(__FILE__, __LINE__, "ysp_bsr3_build: capacity too small");
     return YSP_ERR_ARG;
   }
-  // sort (row, col) keys; the value keeps the source block and its orientation
-  int end_bit = 1;
-  const uint64_t maxkey = static_cast<uint64_t>(nbr) * nbr;
-  while (end_bit < 64 && (1ull << end_bit) <= maxkey) ++end_bit;
-  hipcub::DoubleBuffer<uint64_t> kb(keys, keys2);
-  hipcub::DoubleBuffer<uint32_t> vb(vals, src);
-  size_t temp = 0;
-  YSP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, temp, kb, vb, nnz, 0, end_bit, s));
-  void* tmp = scratch(44, temp);
-  if (!tmp) return YSP_ERR_HIP;
-  YSP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, temp, kb, vb, nnz, 0, end_bit, s));
-  if (vb.Current() != src)
+  if (nnz)
     YSP_CHECK(hipMemcpyAsync(src, vb.Current(), sizeof(uint32_t) * nnz, hipMemcpyDeviceToDevice, s));
-  YSP_CHECK(hipMemsetAsync(cursor, 0, sizeof(uint32_t) * (nbr + 1), s));
-  hipLaunchKernelGGL(split_keys_kernel, dim3(grid_for(nnz, kB)), dim3(kB), 0, s, kb.Current(), nnz,
-                     nbr, col, cursor);
+  hipLaunchKernelGGL(split_keys_kernel, dim3(grid_for(static_cast<size_t>(nnz) + 1, kB)), dim3(kB),
+                     0, s, kb.Current(), count, nbr, col, row_ptr);
   YSP_CHECK_LAUNCH();
-  size_t t2 = 0;
-  YSP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, cursor, row_ptr, nbr + 1, s));
-  tmp = scratch(44, t2);
-  if (!tmp) return YSP_ERR_HIP;
-  YSP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, t2, cursor, row_ptr, nbr + 1, s));
   YSP_CHECK(hipStreamSynchronize(s));
   *nnz_out = nnz;
   return YSP_OK;
