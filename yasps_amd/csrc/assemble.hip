@@ -20,10 +20,21 @@ __global__ void iota_kernel(uint32_t* out, uint32_t n) {
   if (i < n) out[i] = i;
 }
 
-__global__ void count_keys_kernel(const uint32_t* __restrict__ keys, uint32_t n, uint32_t num_keys,
-                                  uint32_t* __restrict__ counts) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && keys[i] < num_keys) atomicAdd(counts + keys[i], 1u);
+// Run starts of the sorted keys (keys >= num_keys sort last and are dropped): entry k
+// starts every key after the previous entry's key up to its own (absent keys included);
+// the first dropped entry (or k = n) closes the remaining keys up to num_keys.
+__global__ void run_starts_kernel(const uint32_t* __restrict__ sorted, uint32_t n,
+                                  uint32_t num_keys, uint32_t* __restrict__ offsets) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > n) return;
+  const uint32_t key = k < n ? sorted[k] : num_keys;
+  const uint32_t rk = key < num_keys ? key : num_keys;
+  uint32_t first = 0;
+  if (k > 0) {
+    const uint32_t prev = sorted[k - 1];
+    first = prev < num_keys ? prev + 1 : num_keys + 1;
+  }
+  for (uint32_t r = first; r <= rk; ++r) offsets[r] = k;
 }
 
 // One slot per (element, slot): key = vertex of the gradient segment (or ~0u when the slot
@@ -129,13 +140,14 @@ extern "C" int ysp_inverse_map(const uint32_t* keys, const uint32_t* values, uin
                                void* stream) {
   hipStream_t s = as_stream(stream);
   if (num_keys == 0) return YSP_OK;
-  YSP_CHECK(hipMemsetAsync(offsets, 0, sizeof(uint32_t) * (num_keys + 1), s));
-  if (n == 0) return YSP_OK;
+  if (n == 0) {
+    YSP_CHECK(hipMemsetAsync(offsets, 0, sizeof(uint32_t) * (num_keys + 1), s));
+    return YSP_OK;
+  }
   uint32_t* k_sorted = static_cast<uint32_t*>(scratch(50, sizeof(uint32_t) * n));
   uint32_t* v_in = static_cast<uint32_t*>(scratch(51, sizeof(uint32_t) * n));
   uint32_t* v_out = static_cast<uint32_t*>(scratch(52, sizeof(uint32_t) * n));
-  uint32_t* counts = static_cast<uint32_t*>(scratch(53, sizeof(uint32_t) * (num_keys + 1)));
-  if (!k_sorted || !v_in || !v_out || !counts) return YSP_ERR_HIP;
+  if (!k_sorted || !v_in || !v_out) return YSP_ERR_HIP;
   if (values) {
     YSP_CHECK(hipMemcpyAsync(v_in, values, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
   } else {
@@ -151,14 +163,8 @@ extern "C" int ysp_inverse_map(const uint32_t* keys, const uint32_t* values, uin
   YSP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, temp, keys, k_sorted, v_in, v_out, n, 0,
                                                end_bit, s));
   YSP_CHECK(hipMemcpyAsync(items, v_out, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
-  YSP_CHECK(hipMemsetAsync(counts, 0, sizeof(uint32_t) * (num_keys + 1), s));
-  hipLaunchKernelGGL(count_keys_kernel, dim3(grid_for(n, kB)), dim3(kB), 0, s, keys, n, num_keys,
-                     counts);
-  size_t t2 = 0;
-  YSP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, counts, offsets, num_keys + 1, s));
-  tmp = scratch(54, t2);
-  if (!tmp) return YSP_ERR_HIP;
-  YSP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, t2, counts, offsets, num_keys + 1, s));
+  hipLaunchKernelGGL(run_starts_kernel, dim3(grid_for(static_cast<size_t>(n) + 1, kB)), dim3(kB), 0,
+                     s, k_sorted, n, num_keys, offsets);
   YSP_CHECK_LAUNCH();
   return YSP_OK;
 }

@@ -206,21 +206,55 @@ __global__ void coordinates_kernel(const uint32_t* __restrict__ indices,
 
 __device__ __forceinline__ uint32_t bit_length(uint32_t v) { return v ? 32u - __clz(v) : 0u; }
 
-__global__ void max_fields_kernel(const uint32_t* __restrict__ coords,
-                                  const uint16_t* __restrict__ dims, uint32_t n,
-                                  uint32_t* __restrict__ out /* [2]: max coord, max dim */) {
+// Largest coordinate and dimension, in two passes without atomics: every workgroup
+// writes its maxima to partial[2 * block], then one workgroup reduces the partials.
+constexpr unsigned kRedBlock = 256;
+__device__ __forceinline__ void block_max2(uint32_t& a, uint32_t& b) {
+  __shared__ uint32_t sa[kRedBlock / 64], sb[kRedBlock / 64];
+  for (int off = 32; off > 0; off >>= 1) {
+    a = max(a, static_cast<uint32_t>(__shfl_xor(static_cast<int>(a), off, 64)));
+    b = max(b, static_cast<uint32_t>(__shfl_xor(static_cast<int>(b), off, 64)));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    sa[threadIdx.x / 64] = a;
+    sb[threadIdx.x / 64] = b;
+  }
+  __syncthreads();
+  a = sa[0];
+  b = sb[0];
+  for (unsigned w = 1; w < kRedBlock / 64; ++w) {
+    a = max(a, sa[w]);
+    b = max(b, sb[w]);
+  }
+}
+
+__global__ void __launch_bounds__(kRedBlock)
+    max_fields_kernel(const uint32_t* __restrict__ coords, const uint16_t* __restrict__ dims,
+                      uint32_t n, uint32_t* __restrict__ partial) {
   uint32_t mc = 0, md = 0;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     mc = max(mc, max(coords[2 * i], coords[2 * i + 1]));
     md = max(md, static_cast<uint32_t>(max(dims[2 * i], dims[2 * i + 1])));
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    mc = max(mc, static_cast<uint32_t>(__shfl_xor(static_cast<int>(mc), off, 64)));
-    md = max(md, static_cast<uint32_t>(__shfl_xor(static_cast<int>(md), off, 64)));
+  block_max2(mc, md);
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = mc;
+    partial[2 * blockIdx.x + 1] = md;
   }
-  if ((threadIdx.x & 63) == 0) {
-    atomicMax(out, mc);
-    atomicMax(out + 1, md);
+}
+
+__global__ void __launch_bounds__(kRedBlock)
+    max_partials_kernel(const uint32_t* __restrict__ partial, uint32_t nblocks,
+                        uint32_t* __restrict__ out /* [2]: max coord, max dim */) {
+  uint32_t mc = 0, md = 0;
+  for (uint32_t i = threadIdx.x; i < nblocks; i += blockDim.x) {
+    mc = max(mc, partial[2 * i]);
+    md = max(md, partial[2 * i + 1]);
+  }
+  block_max2(mc, md);
+  if (threadIdx.x == 0) {
+    out[0] = mc;
+    out[1] = md;
   }
 }
 
@@ -500,15 +534,17 @@ extern "C" int ysp_get_unique_coords(const uint32_t* coords, const uint16_t* dim
   const uint32_t n = num_coordinates;
   if (n == 0) return YSP_OK;
   hipStream_t s = as_stream(stream);
-  uint32_t* maxes = static_cast<uint32_t*>(scratch(10, 2 * sizeof(uint32_t)));
+  const unsigned mblocks = std::min<unsigned>(grid_for(n, kRedBlock), 1024);
+  uint32_t* maxes = static_cast<uint32_t*>(scratch(10, 2 * sizeof(uint32_t) * (mblocks + 1)));
   uint64_t* keys = static_cast<uint64_t*>(scratch(11, sizeof(uint64_t) * n));
   uint32_t* vals = static_cast<uint32_t*>(scratch(12, sizeof(uint32_t) * n));
   uint32_t* flags = static_cast<uint32_t*>(scratch(13, sizeof(uint32_t) * n));
   uint32_t* ords = static_cast<uint32_t*>(scratch(14, sizeof(uint32_t) * n));
   if (!maxes || !keys || !vals || !flags || !ords) return YSP_ERR_HIP;
-  YSP_CHECK(hipMemsetAsync(maxes, 0, 2 * sizeof(uint32_t), s));
-  hipLaunchKernelGGL(max_fields_kernel, dim3(std::min<unsigned>(grid_for(n, kBlock), 1024)),
-                     dim3(kBlock), 0, s, coords, dims, n, maxes);
+  hipLaunchKernelGGL(max_fields_kernel, dim3(mblocks), dim3(kRedBlock), 0, s, coords, dims, n,
+                     maxes + 2);
+  hipLaunchKernelGGL(max_partials_kernel, dim3(1), dim3(kRedBlock), 0, s, maxes + 2, mblocks,
+                     maxes);
   uint32_t hm[2];
   YSP_CHECK(hipMemcpyAsync(hm, maxes, sizeof(hm), hipMemcpyDeviceToHost, s));
   YSP_CHECK(hipStreamSynchronize(s));
