@@ -99,8 +99,8 @@ def test_large_noise_exercises_projection():
     assert _rel_err(dev.to_host(hs.gradient.value), grad) < REL
 
 
-def test_config1_cube_fast_structure_parity():
-    """Config-1 size (n=8, 3072 tets, 1944 unknowns, 4335 blocks)."""
+def test_cube_n8_fast_structure_parity():
+    """The n=8 cube (3072 tets, 1944 unknowns, 4335 blocks)."""
     from yasps_amd import device as dev
     cs, s0, hs = _gpu(8)
     st = newton.fast_structure(cs)

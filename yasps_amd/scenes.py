@@ -103,7 +103,7 @@ class CubeScene:
 
 def cube_scene(n: int = 8, young: float = 1e5, poisson: float = 0.4, density: float = 1000.0,
                dt: float = 1e-2, noise: float = 0.02, seed: int = 0) -> CubeScene:
-    """Config-1 style scene: [0,1]^3 cube, n^3 hexes x 6 tets, bottom face (z=0) fixed.
+    """Tet cube scene: [0,1]^3, n^3 hexes x 6 tets, bottom face (z=0) fixed.
 
     Moving vertices get positions rest + N(0, (noise*h)^2) from ``default_rng(seed)``;
     fixed vertices stay at rest. last_position = rest, velocity = 0.
