@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the material adjoint")
     ap.add_argument("--no-dynamic", action="store_true",
                     help="skip the dynamic contact-term leg")
+    ap.add_argument("--no-elements", action="store_true",
+                    help="skip the per-energy assembly leg")
     ap.add_argument("--adjoint-n", type=int, default=32)
     ap.add_argument("--adjoint-fd", type=int, default=64)
     ap.add_argument("--serving-requests", type=int, default=512)
@@ -320,6 +322,91 @@ def dynamic_leg(args):
             "pcg_iterations": int(np.median(iters)), "reindex_ms": round(reindex_ms, 3)}
 
 
+def _energy_compute_ms(s0, reps=10):
+    """Average time of one assembly (element evaluation, PSD projection, staged assembly)
+    by HIP events on the stream the kernels run on."""
+    from yasps_amd import device as dev
+    torch = dev.torch()
+    hs = s0.minimizer.computeNumericValue()          # static structure, warm-up
+    stream = torch.cuda.current_stream()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(reps):
+        hs.compute(check=False)
+    end.record(stream)
+    end.synchronize()
+    hs.compute()                                       # flags of the timed assemblies
+    return start.elapsed_time(end) / reps
+
+
+def element_leg(args):
+    """Per-energy assembly on meshes the size of the reference's published per-element
+    figures (RTX 4090: SNH 79,935 tets, bending 104,065 hinges, Baraff-Witkin 20,000
+    triangles): SNH on the n=24 cube (82,944 tets, bottom fixed), dihedral bending on the
+    hinges of a 188 x 188 cloth grid (104,533), Baraff-Witkin on a 101 x 101 grid (20,000
+    triangles); every vertex of the cloths free. Projection method 2. The reference's
+    figures are gradient and Hessian times; these include projection and assembly."""
+    from yasps_amd import scene, energies as en, stable_neo_hookean
+    from yasps_amd.scenes import cube_scene, build_yasps_scene, cloth_scene
+    out = {}
+    cs = cube_scene(24, seed=args.seed)
+    s0 = scene("snh")
+    dt = s0.addConstant("dt", rows=1, cols=1)
+    dt.updateValue([cs.dt])
+    m = s0.addMesh("cube")
+    nf = cs.n_fixed
+    fv = m.addPrimitive("fixed_vertices", numInstances=nf)
+    mv = m.addPrimitive("moving_vertices", numInstances=cs.n_moving)
+    m.addPrimitiveUnion("vertices", [fv, mv])
+    fv.addAttribute("position", rows=3, cols=1).updateValue(cs.position[:nf])
+    fv.addConstant("rest_position", rows=3, cols=1).updateValue(cs.rest[:nf])
+    mv.addAttribute("position", rows=3, cols=1).updateValue(cs.position[nf:])
+    mv.addConstant("rest_position", rows=3, cols=1).updateValue(cs.rest[nf:])
+    m.vertices.addAttribute("position")
+    m.vertices.addAttribute("rest_position")
+    mu = m.addConstant("mu", rows=1, cols=1)
+    mu.updateValue([cs.mu])
+    lam = m.addConstant("lam", rows=1, cols=1)
+    lam.updateValue([cs.lam])
+    tets = m.addPrimitive("tets", numInstances=cs.tets.shape[0])
+    t2v = tets.addConnectivity("tet2v", m.vertices, cs.tets, 4)
+    tp = tets.addAttribute("positions", through=t2v, source=m.vertices["position"])
+    trp = tets.addAttribute("rest_positions", through=t2v, source=m.vertices["rest_position"])
+    snh = tets.addAttribute("snh", computed_attribute=stable_neo_hookean(trp, tp, mu, lam, dt))
+    s0.addEnergy(snh, projection_method=2)
+    s0.addMinimizeTarget([mv["position"]])
+    ms = _energy_compute_ms(s0)
+    out["snh"] = {"elements": int(cs.tets.shape[0]), "ms": round(ms, 4),
+                  "ns_per_element": round(1e6 * ms / cs.tets.shape[0], 2),
+                  "reference_rtx4090": {"elements": 79935, "gradient_ms": 0.19728,
+                                        "hessian_ms": 0.59}}
+    for kind, mgrid, ref in (("bending", 188, {"elements": 104065, "gradient_ms": 0.17,
+                                                "hessian_ms": 2.81}),
+                             ("baraff_witkin", 101, {"elements": 20000, "gradient_ms": 0.04,
+                                                     "hessian_ms": 0.12})):
+        cl = cloth_scene(mgrid, seed=args.seed)
+        s1 = scene(kind)
+        mc = s1.addMesh("cloth")
+        verts = mc.addPrimitive("vertices", numInstances=mgrid * mgrid)
+        verts.addAttribute("position", rows=3, cols=1).updateValue(cl.position)
+        verts.addConstant("rest_position", rows=3, cols=1).updateValue(cl.rest)
+        conn = cl.hinges if kind == "bending" else cl.triangles
+        el = mc.addPrimitive("elements", numInstances=conn.shape[0])
+        e2v = el.addConnectivity("e2v", verts, conn, conn.shape[1])
+        pos = el.addAttribute("positions", through=e2v, source=verts["position"])
+        rp = el.addAttribute("rest_positions", through=e2v, source=verts["rest_position"])
+        e = (en.bending(pos, rp, 1e-3) if kind == "bending"
+             else en.baraff_witkin(rp, pos, 1e3, 1e2, 1e-3, 1e-2))
+        att = el.addAttribute(kind, computed_attribute=e)
+        s1.addEnergy(att, projection_method=2)
+        s1.addMinimizeTarget([verts["position"]])
+        ms = _energy_compute_ms(s1)
+        out[kind] = {"elements": int(conn.shape[0]), "ms": round(ms, 4),
+                     "ns_per_element": round(1e6 * ms / conn.shape[0], 2),
+                     "reference_rtx4090": ref}
+    return out
+
+
 def cpu_baseline(cs, values_host, st, tol):
     """The oracle on the host: SNH + inertia assembly timed on 1/8 of the tets (scaled x8)
     and block-Jacobi CG (scipy, one thread) timed in full on the same matrix."""
@@ -441,6 +528,7 @@ def main():
         "serving": None if args.no_serving else serving_leg(args),
         "adjoint": None if args.no_adjoint else adjoint_leg(args),
         "dynamic_contacts": None if args.no_dynamic else dynamic_leg(args),
+        "element_kernels": None if args.no_elements else element_leg(args),
     }
     if not args.no_cpu:
         from oracle import newton

@@ -17,6 +17,9 @@
 #include "element_common.hpp"
 #include "generated_energies.hpp"
 
+#include <algorithm>
+#include <type_traits>
+
 namespace ysp {
 namespace {
 
@@ -297,14 +300,15 @@ __device__ __forceinline__ void scene_params(const double* sp, const double* x, 
   }
 }
 
+// One element through the general slot bookkeeping (any slot sizes, affine vertices).
 template <int ID, int KT>
-__global__ void __launch_bounds__(kEB)
-    energy_eval_kernel(UnionView P, AffineView AF, const uint32_t* __restrict__ e2v, uint32_t n,
-                       const double* __restrict__ ep, const double* __restrict__ sp,
-                       uint32_t method, ScatterArgs SA, int* __restrict__ error_flag) {
+__device__ __forceinline__ void eval_element_general(const UnionView& P, const AffineView& AF,
+                                                     const uint32_t* __restrict__ e2v, uint32_t e,
+                                                     const double* __restrict__ ep,
+                                                     const double* __restrict__ sp, uint32_t method,
+                                                     const ScatterArgs& SA,
+                                                     int* __restrict__ error_flag) {
   using T = Kind<ID>;
-  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
   GSlots<KT> si;
   load_gslots<KT>(SA, e, AF.mask ? -1 : T::N, si);
   double r[T::A][3];
@@ -350,6 +354,187 @@ __global__ void __launch_bounds__(kEB)
   T::eval(x, p, s, E, g, H);
   if (method != 0) project_packed<T::N>(H, method);
   scatter_generic<KT, T::N>(SA, e, si, r, g, H);
+}
+
+// General kernel: every element (queue == nullptr) or the elements queued by the vertex
+// kernel (count in flags[1]), in a grid-stride loop.
+template <int ID, int KT>
+__global__ void __launch_bounds__(kEB)
+    energy_eval_kernel(UnionView P, AffineView AF, const uint32_t* __restrict__ e2v, uint32_t n,
+                       const double* __restrict__ ep, const double* __restrict__ sp,
+                       uint32_t method, ScatterArgs SA, int* __restrict__ flags,
+                       const uint32_t* __restrict__ queue) {
+  const uint32_t count = queue ? static_cast<uint32_t>(flags[1]) : n;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += gridDim.x * blockDim.x)
+    eval_element_general<ID, KT>(P, AF, e2v, queue ? queue[t] : t, ep, sp, method, SA, flags);
+}
+
+// ----------------------------------------------------------------- vertex fast path
+// Elements without affine vertices whose slots are one W-wide slot per local vertex (or
+// padding): slot bookkeeping and scatter at compile-time vertex indices, so that no
+// per-element array is indexed at run time (such an array would be placed in scratch
+// memory). An element whose layout is anything else (a target vertex repeated inside
+// the element) goes to a queue for the general kernel.
+template <int NV>
+struct VSlots {
+  uint32_t idx[NV];   // seg_idx of the vertex's slot, 0 without one
+  bool has[NV];       // the vertex owns a W-wide slot
+  bool target[NV];    // ... with a target index (>= 2) in a group
+  int rank[NV];       // rank of its group among the element's target groups
+  int ntarget;
+  bool ok;
+};
+
+template <int KT, int NV, int W>
+__device__ __forceinline__ void load_vslots(const ScatterArgs& S, uint32_t e, VSlots<NV>& vs) {
+  const uint32_t K = S.K;
+  vs.ok = K <= static_cast<uint32_t>(KT);
+  int grp[NV];
+#pragma unroll
+  for (int a = 0; a < NV; ++a) {
+    vs.idx[a] = 0;
+    vs.has[a] = false;
+    grp[a] = -1;
+  }
+  int placement[KT];
+  int ng = 0;
+  uint32_t col = 0;
+#pragma unroll
+  for (int i = 0; i < KT; ++i) {
+    if (!vs.ok || static_cast<uint32_t>(i) >= K) continue;
+    const size_t o = static_cast<size_t>(e) * K + i;
+    const uint32_t ii = S.seg_idx[o];
+    const uint32_t sz = S.seg_size[o];
+    const int pm = S.perm[o];
+    int g = -1;
+    if (pm != 0) {
+      const int p = (pm < 0 ? -pm : pm) - 1;
+#pragma unroll
+      for (int k = 0; k < KT; ++k)
+        if (k < ng && placement[k] == p) g = k;
+      if (g < 0) {
+        g = ng++;
+#pragma unroll
+        for (int k = 0; k < KT; ++k)
+          if (k == g) placement[k] = p;
+      }
+    }
+    if (sz == W) {
+      const uint32_t v = col / W;
+      if (v >= static_cast<uint32_t>(NV)) vs.ok = false;
+#pragma unroll
+      for (int a = 0; a < NV; ++a)
+        if (v == static_cast<uint32_t>(a)) {
+          vs.idx[a] = ii;
+          vs.has[a] = true;
+          grp[a] = g;
+        }
+    } else if (sz != 0) {
+      vs.ok = false;
+    }
+    col += sz;
+  }
+  if (col != static_cast<uint32_t>(NV * W)) vs.ok = false;      // slots tile the DoFs
+  vs.ntarget = 0;
+#pragma unroll
+  for (int a = 0; a < NV; ++a) {
+    vs.target[a] = vs.has[a] && vs.idx[a] >= 2 && grp[a] >= 0;
+    vs.ntarget += vs.target[a];
+  }
+#pragma unroll
+  for (int a = 0; a < NV; ++a) {
+    int r = 0;
+#pragma unroll
+    for (int b = 0; b < NV; ++b) {
+      r += vs.target[b] && grp[b] < grp[a];
+      if (b > a && vs.target[a] && vs.target[b] && grp[a] == grp[b]) vs.ok = false;
+    }
+    vs.rank[a] = r;
+  }
+}
+
+// Gradient segments and Hessian blocks of a vertex-layout element (the semantics of
+// scatter_generic: pair of target-group ranks lo <= hi at coordinate coordStart +
+// lo*n - lo(lo-1)/2 + (hi-lo), block oriented with the rows of the smaller target index).
+template <int NV, int W, int N>
+__device__ __forceinline__ void scatter_vertex(const ScatterArgs& S, uint32_t e,
+                                               const VSlots<NV>& vs, const double (&g)[N],
+                                               const double (&H)[N * (N + 1) / 2]) {
+#pragma unroll
+  for (int a = 0; a < NV; ++a) {
+    if (S.grad_staging ? !vs.has[a] : (!vs.has[a] || vs.idx[a] < 2)) continue;
+    if (S.grad_staging) {
+      double* out = S.grad_staging + static_cast<size_t>(e) * S.grad_local + W * a;
+#pragma unroll
+      for (int k = 0; k < W; ++k) out[k] = g[W * a + k];
+    } else {
+      const uint32_t p = vs.idx[a] - 2;
+#pragma unroll
+      for (int k = 0; k < W; ++k) atomic_add_f64(S.gradient + p + k, g[W * a + k]);
+    }
+  }
+  const uint32_t cstart = S.coord_outer[e];
+  const int n = vs.ntarget;
+#pragma unroll
+  for (int a = 0; a < NV; ++a)
+#pragma unroll
+    for (int b = a; b < NV; ++b) {
+      if (!vs.target[a] || !vs.target[b]) continue;
+      const int lo = min(vs.rank[a], vs.rank[b]), hi = max(vs.rank[a], vs.rank[b]);
+      const uint32_t coord = cstart + static_cast<uint32_t>(lo * n - lo * (lo - 1) / 2 + (hi - lo));
+      const bool tr = !(vs.idx[a] < vs.idx[b]);   // includes the diagonal block a == b
+      if (S.block_staging) {
+        double* dst = S.block_staging + 9 * static_cast<size_t>(coord);
+#pragma unroll
+        for (int k = 0; k < W; ++k)
+#pragma unroll
+          for (int l = 0; l < W; ++l)
+            dst[tr ? l * W + k : k * W + l] = hget<N>(H, W * a + k, W * b + l);
+      } else {
+        double* dst = S.blocks + S.lookups[coord];
+        double* dd = nullptr;
+        if (a == b && S.diag_blocks) dd = S.diag_blocks + diag_offset(S, vs.idx[a] - 2, W);
+#pragma unroll
+        for (int k = 0; k < W; ++k)
+#pragma unroll
+          for (int l = 0; l < W; ++l) {
+            const double v = hget<N>(H, W * a + k, W * b + l);
+            const int o = tr ? l * W + k : k * W + l;
+            atomic_add_f64(dst + o, v);
+            if (dd) atomic_add_f64(dd + o, v);
+          }
+      }
+    }
+}
+
+template <int ID, int KT>
+__global__ void __launch_bounds__(kEB)
+    energy_eval_vertex_kernel(UnionView P, const uint32_t* __restrict__ e2v, uint32_t n,
+                              const double* __restrict__ ep, const double* __restrict__ sp,
+                              uint32_t method, ScatterArgs SA, int* __restrict__ flags,
+                              uint32_t* __restrict__ queue) {
+  using T = Kind<ID>;
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  VSlots<T::A> vs;
+  load_vslots<KT, T::A, T::W>(SA, e, vs);
+  if (!vs.ok) {
+    queue[wave_reserve(reinterpret_cast<uint32_t*>(flags) + 1, 1u)] = e;
+    return;
+  }
+  if (SA.block_staging && T::W != 3) {
+    atomicOr(flags, 1);
+    return;
+  }
+  double x[T::N];
+  gather<ID>(P, e2v, e, x);
+  const double* p = ep + static_cast<size_t>(e) * T::P;
+  double s[T::S];
+  scene_params<ID>(sp, x, p, s);
+  double E, g[T::N], H[T::N * (T::N + 1) / 2];
+  T::eval(x, p, s, E, g, H);
+  if (method != 0) project_packed<T::N>(H, method);
+  scatter_vertex<T::A, T::W, T::N>(SA, e, vs, g, H);
 }
 
 template <int ID>
@@ -461,14 +646,23 @@ __global__ void __launch_bounds__(kEB)
 template <int ID>
 int launch_eval(const UnionView& U, const AffineView& AF, const uint32_t* e2v, uint32_t n,
                 const double* ep, const double* sp, uint32_t method, const ScatterArgs& SA,
-                int* flag, hipStream_t s) {
+                int* flags, uint32_t* queue, hipStream_t s) {
   const dim3 g(grid_for(n, kEB)), b(kEB);
-  if (SA.K <= 8)
-    hipLaunchKernelGGL((energy_eval_kernel<ID, 8>), g, b, 0, s, U, AF, e2v, n, ep, sp, method, SA,
-                       flag);
-  else
-    hipLaunchKernelGGL((energy_eval_kernel<ID, kSlotsMax>), g, b, 0, s, U, AF, e2v, n, ep, sp,
-                       method, SA, flag);
+  auto run = [&](auto kt) {
+    constexpr int KT = decltype(kt)::value;
+    if (AF.mask) {
+      hipLaunchKernelGGL((energy_eval_kernel<ID, KT>), g, b, 0, s, U, AF, e2v, n, ep, sp, method,
+                         SA, flags, nullptr);
+      return;
+    }
+    hipLaunchKernelGGL((energy_eval_vertex_kernel<ID, KT>), g, b, 0, s, U, e2v, n, ep, sp, method,
+                       SA, flags, queue);
+    // the queued elements (usually none): a small grid-stride launch
+    hipLaunchKernelGGL((energy_eval_kernel<ID, KT>), dim3(std::min<unsigned>(g.x, 256)), b, 0, s,
+                       U, AF, e2v, n, ep, sp, method, SA, flags, queue);
+  };
+  if (SA.K <= 8) run(std::integral_constant<int, 8>());
+  else run(std::integral_constant<int, kSlotsMax>());
   return YSP_OK;
 }
 
@@ -583,9 +777,11 @@ extern "C" int ysp_eval_element_energy(int32_t kind, const ysp_union_f64* positi
   if (n == 0) return YSP_OK;
   if (int rc = check_view(positions, width, "ysp_eval_element_energy: bad union view")) return rc;
   hipStream_t s = as_stream(stream);
-  int* flag = static_cast<int*>(scratch(34, sizeof(int)));
-  if (!flag) return YSP_ERR_HIP;
-  YSP_CHECK(hipMemsetAsync(flag, 0, sizeof(int), s));
+  // flag[0]: unsupported layout; flag[1]: elements queued for the general kernel
+  int* flag = static_cast<int*>(scratch(34, 2 * sizeof(int)));
+  uint32_t* queue = static_cast<uint32_t*>(scratch(46, sizeof(uint32_t) * n));
+  if (!flag || !queue) return YSP_ERR_HIP;
+  YSP_CHECK(hipMemsetAsync(flag, 0, 2 * sizeof(int), s));
   const UnionView U = make_view(positions);
   const ScatterArgs SA = make_scatter(out);
   AffineView AF{};
@@ -602,7 +798,7 @@ extern "C" int ysp_eval_element_energy(int32_t kind, const ysp_union_f64* positi
   }
   dispatch_kind(kind, [&](auto id) {
     return launch_eval<decltype(id)::value>(U, AF, elem2v, n, element_params, scene_params,
-                                            projection_method, SA, flag, s);
+                                            projection_method, SA, flag, queue, s);
   });
   YSP_CHECK_LAUNCH();
   if (int rc = defer_flag_check(flag, s,

@@ -296,3 +296,40 @@ def build_abd_scene(sc: AbdScene, projection_method: int = 2, solver_name: str =
     s0.addMinimizeTarget([mv["position"], ab["affine_matrix"], ab["translation"]])
     s0.minimizer.setSolver(solver_name, **solver_options)
     return s0, dict(mesh=m, snh=snh, affine=aff, inertia_abd=ia, inertia_moving=im)
+
+
+@dataclass
+class ClothScene:
+    """An m x m vertex grid in the plane z = 0 (spacing 1/(m-1)) cut into triangles, with
+    its hinges (interior edges: the shared edge x0-x1 and the two opposite vertices, the
+    layout of ``examples/many_bunnies_one_cloth/helpers.py:193-212``). Positions are the
+    rest grid plus N(0, (noise h)^2) in every coordinate."""
+    m: int
+    rest: np.ndarray          # [m*m, 3]
+    position: np.ndarray      # [m*m, 3]
+    triangles: np.ndarray     # [2 (m-1)^2, 3], counter-clockwise seen from +z
+    hinges: np.ndarray        # [interior edges, 4]
+
+
+def cloth_scene(m: int, noise: float = 0.02, seed: int = 0) -> ClothScene:
+    h = 1.0 / (m - 1)
+    j, i = np.meshgrid(np.arange(m), np.arange(m), indexing="ij")
+    rest = np.stack([i.ravel() * h, j.ravel() * h, np.zeros(m * m)], axis=1)
+    v = lambda a, b: b * m + a                                      # noqa: E731
+    tri = []
+    for b in range(m - 1):
+        for a in range(m - 1):
+            tri.append((v(a, b), v(a + 1, b), v(a + 1, b + 1)))
+            tri.append((v(a, b), v(a + 1, b + 1), v(a, b + 1)))
+    tri = np.array(tri, dtype=np.int64)
+    # a directed edge u -> v of one triangle meets v -> u of its neighbour
+    opposite = {}
+    for t in tri:
+        for k in range(3):
+            opposite[(int(t[k]), int(t[(k + 1) % 3]))] = int(t[(k + 2) % 3])
+    hinges = [(a, b, c, opposite[(b, a)]) for (a, b), c in opposite.items() if (b, a) in opposite
+              and a < b]
+    rng = np.random.default_rng(seed)
+    position = rest + rng.normal(0.0, noise * h, size=rest.shape)
+    return ClothScene(m=m, rest=rest, position=position, triangles=tri,
+                      hinges=np.array(hinges, dtype=np.int64))
