@@ -315,7 +315,7 @@ typedef struct ysp_pcg_stats {
 
 /* Replaces computeSolution (kernel/Solver/solverKernel.pyx:477-687): block-Jacobi PCG
  * with device-resident recurrence scalars. threshold_mode selects the stopping rule.
- * work must hold 5 * n doubles, cg_scalars 64 doubles. Synchronous. Returns the
+ * work must hold 5 * n doubles, cg_scalars YSP_PCG3_SCALARS doubles. Synchronous. Returns the
  * iteration count (>= 0), -4 when max_iterations is reached, -5 for an invalid
  * initial residual, -(it + 4) for a non-positive curvature at iteration it. */
 int ysp_pcg_solve(uint32_t max_iterations, double threshold, int32_t threshold_mode,

@@ -5,8 +5,11 @@
 //   (2) alpha = rz / pAp, x += alpha p, r -= alpha q, q = 0, z = M^-1 r and the partial
 //       dots r.z, r.r (one pass per diagonal block);
 //   (3) beta = rz_new / rz_old, p = z + beta p and the convergence test.
-// Scalars live in rotating slots (k mod 3) so that every kernel of iteration k reads
-// complete sums and no kernel overwrites a slot another kernel of the same launch reads.
+// Scalars live in rotating sets (k mod 3) so that every kernel of iteration k reads
+// complete sums and no kernel overwrites a set another kernel of the same launch reads.
+// Each dot is a set of 64 partial-sum slots (one per 128-byte line, workgroup w adds into
+// slot w mod 64); consumers reduce a set with one wavefront load and a fixed shuffle
+// tree, so every workgroup sees the same sum.
 // Every kernel returns immediately once a status was recorded, so the host can enqueue
 // several iterations ahead and read the status once per chunk (the reference copies two
 // doubles to the host after every iteration, solverKernel.pyx:667-668).
@@ -26,13 +29,38 @@ enum : int {
   S_ITER = 3,     // iteration at which the status was set
   S_THR = 4,      // absolute threshold on rz (mode 0) or rr (mode 1)
   S_MODE = 5,
-  S_PAP = 8,      // [3]
-  S_RZ = 11,      // [3]
-  S_RR = 14,      // [3]
   S_FINAL_RZ = 17,
   S_FINAL_RR = 18,
-  S_COUNT = 64
+  S_COUNT = 64    // header; the partial-sum sets follow
 };
+enum : int { SET_PAP = 0, SET_RZ = 3, SET_RR = 6, SET_RZ0 = 9, SET_BB = 10, SET_COUNT = 11 };
+constexpr int kSet = kQuadSlots * kQuadSpace;
+constexpr int kStateDoubles = S_COUNT + SET_COUNT * kSet;
+static_assert(kStateDoubles <= YSP_PCG3_SCALARS, "cg_scalars too small");
+
+__host__ __device__ constexpr int set_base(int set) { return S_COUNT + set * kSet; }
+
+// This workgroup's slot of a set.
+__device__ __forceinline__ double* my_slot(double* st, int set) {
+  return st + set_base(set) + (blockIdx.x % kQuadSlots) * kQuadSpace;
+}
+
+// Sums of up to four sets, one wavefront each (blockDim >= 64 * count), identical in
+// every thread of the workgroup.
+template <int C>
+__device__ __forceinline__ void set_sums(const double* st, const int (&sets)[C], double* lds,
+                                         double (&out)[C]) {
+  const int w = threadIdx.x / 64;
+  if (w < C) {
+    double v = st[set_base(sets[w]) + (threadIdx.x % 64) * kQuadSpace];
+    v = wave_sum(v);
+    if (threadIdx.x % 64 == 0) lds[w] = v;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < C; ++i) out[i] = lds[i];
+  __syncthreads();
+}
 
 constexpr int kMaxAttr = 8;
 
@@ -110,18 +138,23 @@ __global__ void __launch_bounds__(kVecBlock)
   bmb = block_sum<kVecBlock>(bmb, red);
   bb = block_sum<kVecBlock>(bb, red);
   if (threadIdx.x == 0) {
-    atomic_add_f64(st + S_RZ + 0, rz);
-    atomic_add_f64(st + S_RR + 0, rr);
-    atomic_add_f64(st + S_RZ0, bmb);
-    atomic_add_f64(st + S_BB, bb);
+    atomic_add_f64(my_slot(st, SET_RZ + 0), rz);
+    atomic_add_f64(my_slot(st, SET_RR + 0), rr);
+    atomic_add_f64(my_slot(st, SET_RZ0), bmb);
+    atomic_add_f64(my_slot(st, SET_BB), bb);
   }
 }
 
-__global__ void init_finish_kernel(double* st, double tol, int mode) {
-  const double rz0 = st[S_RZ0];
-  const double rz = st[S_RZ + 0];
-  const double bb = st[S_BB];
-  const double rr = st[S_RR + 0];
+// One workgroup of 256 threads.
+__global__ void __launch_bounds__(256) init_finish_kernel(double* st, double tol, int mode) {
+  __shared__ double lds[4];
+  double v[4];
+  const int sets[4] = {SET_RZ0, SET_RZ + 0, SET_BB, SET_RR + 0};
+  set_sums<4>(st, sets, lds, v);
+  if (threadIdx.x != 0) return;
+  const double rz0 = v[0], rz = v[1], bb = v[2], rr = v[3];
+  st[S_RZ0] = rz0;
+  st[S_BB] = bb;
   st[S_MODE] = mode;
   if (!isfinite(rz0) || rz0 < 0.0 || !isfinite(rz) || rz < 0.0) {
     st[S_STATUS] = -2.0;
@@ -149,8 +182,11 @@ __global__ void __launch_bounds__(kVecBlock)
                   double* __restrict__ z, double* __restrict__ st, int k) {
   if (st[S_STATUS] != 0.0) return;
   const int cur = k % 3, prev = (k + 2) % 3, next = (k + 1) % 3;
-  const double pap = st[S_PAP + cur];
-  const double rzo = st[S_RZ + prev];
+  __shared__ double lds[2];
+  double v[2];
+  const int sets[2] = {SET_PAP + cur, SET_RZ + prev};
+  set_sums<2>(st, sets, lds, v);
+  const double pap = v[0], rzo = v[1];
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (!isfinite(pap) || pap <= 0.0) {
     if (t == 0) {
@@ -160,10 +196,10 @@ __global__ void __launch_bounds__(kVecBlock)
     return;
   }
   const double alpha = rzo / pap;
-  if (t == 0) {
-    st[S_PAP + next] = 0.0;
-    st[S_RZ + next] = 0.0;
-    st[S_RR + next] = 0.0;
+  if (blockIdx.x == 0 && threadIdx.x < kQuadSlots) {
+    st[set_base(SET_PAP + next) + threadIdx.x * kQuadSpace] = 0.0;
+    st[set_base(SET_RZ + next) + threadIdx.x * kQuadSpace] = 0.0;
+    st[set_base(SET_RR + next) + threadIdx.x * kQuadSpace] = 0.0;
   }
   double rz = 0.0, rr = 0.0;
   uint32_t dof0, inv0, s;
@@ -183,8 +219,8 @@ __global__ void __launch_bounds__(kVecBlock)
   rz = block_sum<kVecBlock>(rz, red);
   rr = block_sum<kVecBlock>(rr, red);
   if (threadIdx.x == 0) {
-    atomic_add_f64(st + S_RZ + cur, rz);
-    atomic_add_f64(st + S_RR + cur, rr);
+    atomic_add_f64(my_slot(st, SET_RZ + cur), rz);
+    atomic_add_f64(my_slot(st, SET_RR + cur), rr);
   }
 }
 
@@ -193,9 +229,11 @@ __global__ void __launch_bounds__(kVecBlock)
                      uint32_t n, int k) {
   if (st[S_STATUS] != 0.0) return;
   const int cur = k % 3, prev = (k + 2) % 3;
-  const double rzn = st[S_RZ + cur];
-  const double rzo = st[S_RZ + prev];
-  const double rr = st[S_RR + cur];
+  __shared__ double lds[3];
+  double v[3];
+  const int sets[3] = {SET_RZ + cur, SET_RZ + prev, SET_RR + cur};
+  set_sums<3>(st, sets, lds, v);
+  const double rzn = v[0], rzo = v[1], rr = v[2];
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (!isfinite(rzn) || rzn < 0.0 || !isfinite(rzo) || rzo <= 0.0) {
     if (t == 0) {
@@ -341,7 +379,7 @@ extern "C" int ysp_pcg_solve(uint32_t max_iterations, double threshold, int32_t 
   const dim3 vgrid(grid_for(nblocks, kVecBlock)), vblock(kVecBlock);
   const dim3 dgrid(std::min<unsigned>(grid_for(n, kVecBlock), 2048u));
 
-  YSP_CHECK(hipMemsetAsync(st, 0, S_COUNT * sizeof(double), s));
+  YSP_CHECK(hipMemsetAsync(st, 0, kStateDoubles * sizeof(double), s));
   const int have_guess = (!zero_initial_guess && initial_guess) ? 1 : 0;
   if (have_guess) {
     YSP_CHECK(hipMemsetAsync(q, 0, sizeof(double) * n, s));
@@ -352,7 +390,7 @@ extern "C" int ysp_pcg_solve(uint32_t max_iterations, double threshold, int32_t 
   }
   hipLaunchKernelGGL(init_kernel, vgrid, vblock, 0, s, L, diagonal_blocks_inverse, rhs, x,
                      initial_guess, r, p, z, q, st, have_guess);
-  hipLaunchKernelGGL(init_finish_kernel, dim3(1), dim3(1), 0, s, st, threshold, threshold_mode);
+  hipLaunchKernelGGL(init_finish_kernel, dim3(1), dim3(256), 0, s, st, threshold, threshold_mode);
   YSP_CHECK_LAUNCH();
 
   double* box = status_mailbox();
@@ -375,9 +413,10 @@ extern "C" int ysp_pcg_solve(uint32_t max_iterations, double threshold, int32_t 
     const uint32_t last = std::min<uint32_t>(max_iterations, k + chunk - 1);
     for (uint32_t it = k; it <= last; ++it) {
       const int cur = static_cast<int>(it % 3);
-      int rc = launch_part_spmv(part, p, q, 0, st + S_PAP + cur, st + S_STATUS, s);
+      double* pap = st + set_base(SET_PAP + cur);
+      int rc = launch_part_spmv(part, p, q, 0, pap, st + S_STATUS, s);
       if (rc) return rc;
-      rc = launch_part_spmv(dynamic_part, p, q, 0, st + S_PAP + cur, st + S_STATUS, s);
+      rc = launch_part_spmv(dynamic_part, p, q, 0, pap, st + S_STATUS, s);
       if (rc) return rc;
       hipLaunchKernelGGL(update_kernel, vgrid, vblock, 0, s, L, diagonal_blocks_inverse, x, r, p,
                          q, z, st, static_cast<int>(it));

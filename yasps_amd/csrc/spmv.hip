@@ -5,7 +5,8 @@
 // share a block row (blocks are sorted by (row, col), so they are contiguous lanes) and
 // flushed with one FP64 atomic per row segment; the transposed part is scattered with
 // FP64 atomics. Optionally the kernel also accumulates the quadratic form x^T H x
-// (sum over blocks of x_r^T B x_c, doubled off the diagonal) used by CG as p^T A p.
+// (sum over blocks of x_r^T B x_c, doubled off the diagonal) used by CG as p^T A p, one
+// partial sum per workgroup into a set of 64 slots.
 #include "spmv.hpp"
 
 namespace ysp {
@@ -64,7 +65,7 @@ __global__ void __launch_bounds__(256)
   if (quad) {
     __shared__ double red[4];
     const double s = block_sum<256>(q, red);
-    if (threadIdx.x == 0) atomic_add_f64(quad, s);
+    if (threadIdx.x == 0) atomic_add_f64(quad + (blockIdx.x % kQuadSlots) * kQuadSpace, s);
   }
 }
 
@@ -103,7 +104,7 @@ __global__ void __launch_bounds__(256)
   if (quad) {
     __shared__ double red[4];
     const double s = block_sum<256>(q, red);
-    if (threadIdx.x == 0) atomic_add_f64(quad, s);
+    if (threadIdx.x == 0) atomic_add_f64(quad + (blockIdx.x % kQuadSlots) * kQuadSpace, s);
   }
 }
 
