@@ -328,6 +328,21 @@ int ysp_pcg_solve(uint32_t max_iterations, double threshold, int32_t threshold_m
                   double* solution, const double* initial_guess, int32_t zero_initial_guess,
                   double* work, double* cg_scalars, ysp_pcg_stats* stats, void* stream);
 
+/* ysp_pcg_solve with the operator given as the split mirrored full block-CSR of
+ * ysp_bsr3_build_split / ysp_bsr3_gather_split (gathers instead of FP64 atomics; same
+ * block-Jacobi preconditioner over the diagonal blocks of every target, recurrence,
+ * stopping rules and return codes). n must be a multiple of 3. */
+int ysp_pcg_solve_full(uint32_t max_iterations, double threshold, int32_t threshold_mode,
+                       const uint32_t* row_ptr, const uint32_t* col, const double* values,
+                       uint32_t nnz, const double* diagonal_blocks_inverse,
+                       const uint32_t* diagonal_blocks_start,
+                       const uint32_t* diagonal_blocks_count,
+                       const uint32_t* diagonal_blocks_size,
+                       const uint32_t* gradient_segments_start, int32_t num_attributes,
+                       const double* rhs, uint32_t n, double* solution,
+                       const double* initial_guess, int32_t zero_initial_guess, double* work,
+                       double* cg_scalars, ysp_pcg_stats* stats, void* stream);
+
 /* --------------------------------------------------------------- MAS preconditioner */
 /* Multilevel additive Schwarz (solver/mas): the hierarchy (domains, identity
  * aggregation maps) is built on the host once per static structure
@@ -434,6 +449,23 @@ int ysp_bsr3_build(const uint32_t* pos_static, uint32_t nb_static,
 int ysp_bsr3_gather_values(const double* static_values, const double* dynamic_values,
                            const uint32_t* src, uint32_t nnz, double* out64, float* out32,
                            void* stream);
+
+/* The same mirrored full 3x3 block-CSR for upper storage whose block dimensions are
+ * multiples of 3 (affine-body segments give 9x9, 9x3, 3x9 next to 3x3 blocks): every
+ * stored h x w block is split into (h/3)(w/3) 3x3 sub-blocks, each entered at (r, c) and
+ * mirrored at (c, r) (the sub-blocks of a diagonal block, which holds both triangles, are
+ * entered once). src [nnz] (64-bit): bits 0-39 value offset of the sub-block's first
+ * element in its part, bits 40-41 w/3 - 1, bit 42 transposed, bit 43 dynamic part.
+ * capacity bounds nnz (the sum of 2 (h/3)(w/3) over all blocks suffices). No reference
+ * counterpart: the operator of spmvWithSystem (kernel/Solver/solverKernel.pyx:217-274)
+ * laid out for gathers. Synchronous. */
+int ysp_bsr3_build_split(const ysp_block_matrix* part, const ysp_block_matrix* dynamic_part,
+                         uint32_t n, uint32_t* row_ptr, uint32_t* col, uint64_t* src,
+                         uint32_t capacity, uint32_t* nnz, void* stream);
+
+/* Component-major FP64 values of the split block-CSR (layout as ysp_bsr3_gather_values). */
+int ysp_bsr3_gather_split(const double* static_values, const double* dynamic_values,
+                          const uint64_t* src, uint32_t nnz, double* out64, void* stream);
 
 /* out[i] = (float) in[i] (FP32 copies of FP64 device arrays, e.g. inverse blocks). */
 int ysp_cast_f32(const double* in, float* out, uint64_t n, void* stream);

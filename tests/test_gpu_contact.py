@@ -67,6 +67,12 @@ def test_contact_scene_matches_oracle_and_solves():
         assert hs.dynamic.num_blocks > 0
         assert _rel(dev.to_host(hs.gradient.value), g) < REL
         assert _rel(hs.dense(), H) < REL
+        # the solve operator: every block split into mirrored 3x3 sub-blocks
+        assert hs.full is None and hs.split is not None
+        from yasps_amd.hessian import vector
+        v = np.random.default_rng(step).standard_normal(H.shape[0])
+        y = dev.to_host(hs.spmv(vector(H.shape[0], dev.to_device(v, "float64"))).value)
+        assert np.abs(y - H @ v).max() <= 1e-12 * np.abs(H @ v).max()
         sol = s0.minimizeEnergy(tolerance=1e-10)
         x = np.concatenate([dev.to_host(v) for v in sol])
         assert [v.numel() for v in sol] == [3 * cs.n_free, 9 * cs.n_bodies, 3 * cs.n_bodies]

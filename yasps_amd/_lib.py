@@ -119,6 +119,12 @@ SIGNATURES = {
     "ysp_pcg_solve": (C.c_int, [U32, D, I32, C.POINTER(BlockMatrix), C.POINTER(BlockMatrix), P,
                                 C.POINTER(U32), C.POINTER(U32), C.POINTER(U32), C.POINTER(U32),
                                 I32, P, U32, P, P, I32, P, P, C.POINTER(PcgStats), P]),
+    "ysp_pcg_solve_full": (C.c_int, [U32, D, I32, P, P, P, U32, P, C.POINTER(U32),
+                                     C.POINTER(U32), C.POINTER(U32), C.POINTER(U32), I32, P, U32,
+                                     P, P, I32, P, P, C.POINTER(PcgStats), P]),
+    "ysp_bsr3_build_split": (C.c_int, [C.POINTER(BlockMatrix), C.POINTER(BlockMatrix), U32, P, P,
+                                       P, U32, C.POINTER(U32), P]),
+    "ysp_bsr3_gather_split": (C.c_int, [P, P, P, U32, P, P]),
     "ysp_inverse_map": (C.c_int, [P, P, U32, U32, P, P, P]),
     "ysp_gradient_contributions": (C.c_int, [P, P, U32, U32, C.c_uint64, U32, P, P, P]),
     "ysp_gather_blocks3": (C.c_int, [P, P, P, U32, P, P, P, P, P, U32, I32, P]),

@@ -610,6 +610,163 @@ extern "C" int ysp_bsr3_gather_values(const double* static_values, const double*
   return YSP_OK;
 }
 
+// ------------------------------------------------------------------ split build
+// Upper block storage whose block dimensions are multiples of 3 (affine-body segments:
+// 9x9, 9x3, 3x9 next to 3x3) as a mirrored full 3x3 block-CSR: every stored h x w block
+// becomes (h/3)(w/3) sub-blocks, each entered at (r, c) and mirrored at (c, r); the
+// sub-blocks of a diagonal block already hold both triangles and are entered once. The
+// source of entry k is a 64-bit word: bits 0-39 the value offset of the sub-block's first
+// element in its part, 40-41 w/3 - 1 (the row stride of the source block), 42 transposed,
+// 43 dynamic part.
+namespace {
+constexpr uint64_t kSplitOffset = (1ull << 40) - 1;
+constexpr uint64_t kSplitTrans = 1ull << 42;
+constexpr uint64_t kSplitDyn = 1ull << 43;
+
+// Entries 2i and 2i + 1 of this category's range for sub-block i = (block, a, b).
+__global__ void emit_split_kernel(const uint32_t* __restrict__ pos, uint32_t nb, uint32_t h,
+                                  uint32_t w, uint64_t value_start, uint64_t tag, uint32_t nbr,
+                                  uint64_t* __restrict__ keys, uint64_t* __restrict__ vals) {
+  const uint32_t sh = h / 3, sw = w / 3;
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= static_cast<size_t>(nb) * sh * sw) return;
+  const uint32_t b = static_cast<uint32_t>(i / (sh * sw));
+  const uint32_t sub = static_cast<uint32_t>(i % (sh * sw));
+  const uint32_t a = sub / sw, bb = sub % sw;
+  const uint32_t pr = pos[2 * b], pc = pos[2 * b + 1];
+  const uint32_t r = pr / 3 + a, c = pc / 3 + bb;
+  const uint64_t off = value_start + static_cast<uint64_t>(b) * h * w + 3ull * a * w + 3ull * bb;
+  const uint64_t word = off | (static_cast<uint64_t>(sw - 1) << 40) | tag;
+  keys[2 * i] = static_cast<uint64_t>(r) * nbr + c;
+  vals[2 * i] = word;
+  keys[2 * i + 1] = pr == pc ? static_cast<uint64_t>(nbr) * nbr : static_cast<uint64_t>(c) * nbr + r;
+  vals[2 * i + 1] = word | kSplitTrans;
+}
+
+__global__ void gather_split_kernel(const double* __restrict__ sv, const double* __restrict__ dv,
+                                    const uint64_t* __restrict__ src, uint32_t nnz,
+                                    double* __restrict__ out) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  const uint64_t word = src[k];
+  const double* B = ((word & kSplitDyn) ? dv : sv) + (word & kSplitOffset);
+  const uint32_t w = 3 * (static_cast<uint32_t>((word >> 40) & 3u) + 1);
+  const bool tr = (word & kSplitTrans) != 0;
+  const size_t stride = pad64(nnz);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      out[(3 * i + j) * stride + k] = tr ? B[j * w + i] : B[i * w + j];
+}
+
+// Sub-block count of a part, 0 with *ok = false when a block dimension is not a
+// multiple of 3 or wider than 12.
+size_t split_entries(const ysp_block_matrix* part, bool* ok) {
+  size_t total = 0;
+  if (!part) return 0;
+  for (uint32_t c = 0; c < part->num_categories; ++c) {
+    const uint32_t h = part->dims[2 * c], w = part->dims[2 * c + 1];
+    if (h % 3 || w % 3 || h == 0 || w == 0 || w > 12) *ok = false;
+    total += 2ull * part->counts[c] * (h / 3) * (w / 3);
+  }
+  return total;
+}
+}  // namespace
+
+extern "C" int ysp_bsr3_build_split(const ysp_block_matrix* part, const ysp_block_matrix* dynamic_part,
+                                    uint32_t n, uint32_t* row_ptr, uint32_t* col, uint64_t* src,
+                                    uint32_t capacity, uint32_t* nnz_out, void* stream) {
+  if (!nnz_out || n % 3) {
+    ysp_set_error(__FILE__, __LINE__, "ysp_bsr3_build_split: bad arguments");
+    return YSP_ERR_ARG;
+  }
+  *nnz_out = 0;
+  bool ok = true;
+  const size_t cap = split_entries(part, &ok) + split_entries(dynamic_part, &ok);
+  if (!ok) {
+    ysp_set_error(__FILE__, __LINE__,
+                  "ysp_bsr3_build_split: block dimensions must be multiples of 3, at most 12");
+    return YSP_ERR_UNSUPPORTED;
+  }
+  hipStream_t s = as_stream(stream);
+  const uint32_t nbr = n / 3;
+  if (cap == 0) {
+    YSP_CHECK(hipMemsetAsync(row_ptr, 0, sizeof(uint32_t) * (nbr + 1), s));
+    return YSP_OK;
+  }
+  if (cap > 0xffffffffull) {
+    ysp_set_error(__FILE__, __LINE__, "ysp_bsr3_build_split: too many blocks");
+    return YSP_ERR_ARG;
+  }
+  uint64_t* keys = static_cast<uint64_t*>(scratch(60, sizeof(uint64_t) * cap));
+  uint64_t* vals = static_cast<uint64_t*>(scratch(61, sizeof(uint64_t) * cap));
+  uint64_t* keys2 = static_cast<uint64_t*>(scratch(62, sizeof(uint64_t) * cap));
+  uint64_t* vals2 = static_cast<uint64_t*>(scratch(63, sizeof(uint64_t) * cap));
+  uint32_t* count = static_cast<uint32_t*>(scratch(64, sizeof(uint32_t)));
+  if (!keys || !vals || !keys2 || !vals2 || !count) return YSP_ERR_HIP;
+  size_t base = 0;
+  for (int which = 0; which < 2; ++which) {
+    const ysp_block_matrix* m = which ? dynamic_part : part;
+    if (!m) continue;
+    uint32_t pos_offset = 0;
+    for (uint32_t c = 0; c < m->num_categories; ++c) {
+      const uint32_t nb = m->counts[c], h = m->dims[2 * c], w = m->dims[2 * c + 1];
+      const size_t subs = static_cast<size_t>(nb) * (h / 3) * (w / 3);
+      if (subs)
+        hipLaunchKernelGGL(emit_split_kernel, dim3(grid_for(subs, kB)), dim3(kB), 0, s,
+                           m->positions + 2 * static_cast<size_t>(pos_offset), nb, h, w,
+                           static_cast<uint64_t>(m->starts[c]), which ? kSplitDyn : 0ull, nbr,
+                           keys + base, vals + base);
+      base += 2 * subs;
+      pos_offset += nb;
+    }
+  }
+  YSP_CHECK_LAUNCH();
+  const uint64_t sentinel = static_cast<uint64_t>(nbr) * nbr;
+  int end_bit = 1;
+  while (end_bit < 64 && (1ull << end_bit) <= sentinel) ++end_bit;
+  hipcub::DoubleBuffer<uint64_t> kb(keys, keys2);
+  hipcub::DoubleBuffer<uint64_t> vb(vals, vals2);
+  const uint32_t total = static_cast<uint32_t>(cap);
+  size_t temp = 0;
+  YSP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, temp, kb, vb, total, 0, end_bit, s));
+  void* tmp = scratch(65, temp);
+  if (!tmp) return YSP_ERR_HIP;
+  YSP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, temp, kb, vb, total, 0, end_bit, s));
+  hipLaunchKernelGGL(count_entries_kernel, dim3(1), dim3(1), 0, s, kb.Current(), total, sentinel,
+                     count);
+  uint32_t nnz = 0;
+  YSP_CHECK(hipMemcpyAsync(&nnz, count, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  YSP_CHECK(hipStreamSynchronize(s));
+  if (nnz > capacity) {
+    ysp_set_error(__FILE__, __LINE__, "ysp_bsr3_build_split: capacity too small");
+    return YSP_ERR_ARG;
+  }
+  if (nnz)
+    YSP_CHECK(hipMemcpyAsync(src, vb.Current(), sizeof(uint64_t) * nnz, hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(split_keys_kernel, dim3(grid_for(static_cast<size_t>(nnz) + 1, kB)), dim3(kB),
+                     0, s, kb.Current(), count, nbr, col, row_ptr);
+  YSP_CHECK_LAUNCH();
+  YSP_CHECK(hipStreamSynchronize(s));
+  *nnz_out = nnz;
+  return YSP_OK;
+}
+
+extern "C" int ysp_bsr3_gather_split(const double* static_values, const double* dynamic_values,
+                                     const uint64_t* src, uint32_t nnz, double* out64,
+                                     void* stream) {
+  if (nnz == 0) return YSP_OK;
+  if (!src || !out64) {
+    ysp_set_error(__FILE__, __LINE__, "ysp_bsr3_gather_split: bad arguments");
+    return YSP_ERR_ARG;
+  }
+  hipLaunchKernelGGL(gather_split_kernel, dim3(grid_for(nnz, kB)), dim3(kB), 0, as_stream(stream),
+                     static_values, dynamic_values, src, nnz, out64);
+  YSP_CHECK_LAUNCH();
+  return YSP_OK;
+}
+
 extern "C" int ysp_bsr3_spmv(const uint32_t* row_ptr, const uint32_t* col, const void* values,
                              int32_t values_f32, uint32_t nnz, uint32_t n, const double* x,
                              double* y, void* stream) {

@@ -302,6 +302,51 @@ int launch_invert(const double* in, double* out, uint32_t count, uint32_t size, 
   return YSP_OK;
 }
 
+// y = A x over a mirrored full 3x3 block-CSR (component-major values, `stride` entries
+// per component): one 16-lane group per block row, every row written; with quad, the
+// workgroup's sum of x_r^T y_r goes into its partial-sum slot.
+constexpr unsigned kFullThreads = 256;
+constexpr unsigned kFullLanes = 16;
+constexpr unsigned kFullRows = kFullThreads / kFullLanes;
+__global__ void __launch_bounds__(kFullThreads)
+    full3_product_kernel(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
+                         const double* __restrict__ v, uint32_t stride, uint32_t nbr,
+                         const double* __restrict__ x, double* __restrict__ y,
+                         double* __restrict__ quad, const double* __restrict__ stop) {
+  if (stop && *stop != 0.0) return;
+  const uint32_t r = blockIdx.x * kFullRows + threadIdx.x / kFullLanes;
+  const int l = threadIdx.x % kFullLanes;
+  double y0 = 0.0, y1 = 0.0, y2 = 0.0, xq = 0.0;
+  if (r < nbr) {
+    const uint32_t k0 = row_ptr[r], k1 = row_ptr[r + 1];
+    for (uint32_t k = k0 + l; k < k1; k += kFullLanes) {
+      const uint32_t c = 3 * col[k];
+      const double x0 = x[c], x1 = x[c + 1], x2 = x[c + 2];
+      const size_t st = stride;
+      y0 += v[k] * x0 + v[st + k] * x1 + v[2 * st + k] * x2;
+      y1 += v[3 * st + k] * x0 + v[4 * st + k] * x1 + v[5 * st + k] * x2;
+      y2 += v[6 * st + k] * x0 + v[7 * st + k] * x1 + v[8 * st + k] * x2;
+    }
+#pragma unroll
+    for (int off = kFullLanes / 2; off > 0; off >>= 1) {
+      y0 += __shfl_xor(y0, off, kFullLanes);
+      y1 += __shfl_xor(y1, off, kFullLanes);
+      y2 += __shfl_xor(y2, off, kFullLanes);
+    }
+    if (l == 0) {
+      y[3 * r] = y0;
+      y[3 * r + 1] = y1;
+      y[3 * r + 2] = y2;
+      xq = x[3 * r] * y0 + x[3 * r + 1] * y1 + x[3 * r + 2] * y2;
+    }
+  }
+  if (quad) {
+    __shared__ double red[kFullThreads / 64];
+    const double sum = block_sum<kFullThreads>(xq, red);
+    if (threadIdx.x == 0) atomic_add_f64(quad + (blockIdx.x % kQuadSlots) * kQuadSpace, sum);
+  }
+}
+
 // Pinned status mailbox shared by the solves issued from this process.
 double* status_mailbox() {
   static double* box = nullptr;
@@ -331,17 +376,19 @@ extern "C" int ysp_invert_diagonal_blocks(const double* diagonal_blocks, double*
   return YSP_OK;
 }
 
-extern "C" int ysp_pcg_solve(uint32_t max_iterations, double threshold, int32_t threshold_mode,
-                             const ysp_block_matrix* part, const ysp_block_matrix* dynamic_part,
-                             const double* diagonal_blocks_inverse,
-                             const uint32_t* diagonal_blocks_start,
-                             const uint32_t* diagonal_blocks_count,
-                             const uint32_t* diagonal_blocks_size,
-                             const uint32_t* gradient_segments_start, int32_t num_attributes,
-                             const double* rhs, uint32_t n, double* solution,
-                             const double* initial_guess, int32_t zero_initial_guess,
-                             double* work, double* cg_scalars, ysp_pcg_stats* stats,
-                             void* stream) {
+namespace ysp {
+namespace {
+// The PCG loop of ysp_pcg_solve over any operator: product(x, y, quad, stop) launches
+// y = A x (accumulate == 0: y is overwritten for every row; otherwise y is zero on entry
+// and receives sums), adding x^T A x into the partial-sum set quad when it is not null.
+template <typename Product>
+int generic_run(Product&& product, uint32_t max_iterations, double threshold,
+                int32_t threshold_mode, const double* diagonal_blocks_inverse,
+                const uint32_t* diagonal_blocks_start, const uint32_t* diagonal_blocks_count,
+                const uint32_t* diagonal_blocks_size, const uint32_t* gradient_segments_start,
+                int32_t num_attributes, const double* rhs, uint32_t n, double* solution,
+                const double* initial_guess, int32_t zero_initial_guess, double* work,
+                double* cg_scalars, ysp_pcg_stats* stats, hipStream_t s) {
   if (stats) *stats = ysp_pcg_stats{0, 0, 0, 0, 0, 0};
   if (n == 0) return 0;
   if (num_attributes <= 0 || num_attributes > kMaxAttr || !work || !cg_scalars ||
@@ -349,7 +396,6 @@ extern "C" int ysp_pcg_solve(uint32_t max_iterations, double threshold, int32_t 
     ysp_set_error(__FILE__, __LINE__, "ysp_pcg_solve: bad arguments");
     return YSP_ERR_ARG;
   }
-  hipStream_t s = as_stream(stream);
   DiagLayout L{};
   L.num_attr = num_attributes;
   L.cum[0] = 0;
@@ -383,10 +429,7 @@ extern "C" int ysp_pcg_solve(uint32_t max_iterations, double threshold, int32_t 
   const int have_guess = (!zero_initial_guess && initial_guess) ? 1 : 0;
   if (have_guess) {
     YSP_CHECK(hipMemsetAsync(q, 0, sizeof(double) * n, s));
-    int rc = launch_part_spmv(part, initial_guess, q, 0, nullptr, nullptr, s);
-    if (rc) return rc;
-    rc = launch_part_spmv(dynamic_part, initial_guess, q, 0, nullptr, nullptr, s);
-    if (rc) return rc;
+    if (int rc = product(initial_guess, q, nullptr, nullptr)) return rc;
   }
   hipLaunchKernelGGL(init_kernel, vgrid, vblock, 0, s, L, diagonal_blocks_inverse, rhs, x,
                      initial_guess, r, p, z, q, st, have_guess);
@@ -413,11 +456,7 @@ extern "C" int ysp_pcg_solve(uint32_t max_iterations, double threshold, int32_t 
     const uint32_t last = std::min<uint32_t>(max_iterations, k + chunk - 1);
     for (uint32_t it = k; it <= last; ++it) {
       const int cur = static_cast<int>(it % 3);
-      double* pap = st + set_base(SET_PAP + cur);
-      int rc = launch_part_spmv(part, p, q, 0, pap, st + S_STATUS, s);
-      if (rc) return rc;
-      rc = launch_part_spmv(dynamic_part, p, q, 0, pap, st + S_STATUS, s);
-      if (rc) return rc;
+      if (int rc = product(p, q, st + set_base(SET_PAP + cur), st + S_STATUS)) return rc;
       hipLaunchKernelGGL(update_kernel, vgrid, vblock, 0, s, L, diagonal_blocks_inverse, x, r, p,
                          q, z, st, static_cast<int>(it));
       hipLaunchKernelGGL(direction_kernel, dgrid, vblock, 0, s, z, p, st, n,
@@ -452,4 +491,61 @@ extern "C" int ysp_pcg_solve(uint32_t max_iterations, double threshold, int32_t 
     stats->rr = fin[S_FINAL_RR];
   }
   return code;
+}
+}  // namespace
+}  // namespace ysp
+
+extern "C" int ysp_pcg_solve(uint32_t max_iterations, double threshold, int32_t threshold_mode,
+                             const ysp_block_matrix* part, const ysp_block_matrix* dynamic_part,
+                             const double* diagonal_blocks_inverse,
+                             const uint32_t* diagonal_blocks_start,
+                             const uint32_t* diagonal_blocks_count,
+                             const uint32_t* diagonal_blocks_size,
+                             const uint32_t* gradient_segments_start, int32_t num_attributes,
+                             const double* rhs, uint32_t n, double* solution,
+                             const double* initial_guess, int32_t zero_initial_guess,
+                             double* work, double* cg_scalars, ysp_pcg_stats* stats,
+                             void* stream) {
+  hipStream_t s = as_stream(stream);
+  auto product = [&](const double* x, double* y, double* quad, const double* stop) -> int {
+    if (int rc = launch_part_spmv(part, x, y, 0, quad, stop, s)) return rc;
+    return launch_part_spmv(dynamic_part, x, y, 0, quad, stop, s);
+  };
+  return generic_run(product, max_iterations, threshold, threshold_mode, diagonal_blocks_inverse,
+                     diagonal_blocks_start, diagonal_blocks_count, diagonal_blocks_size,
+                     gradient_segments_start, num_attributes, rhs, n, solution, initial_guess,
+                     zero_initial_guess, work, cg_scalars, stats, s);
+}
+
+extern "C" int ysp_pcg_solve_full(uint32_t max_iterations, double threshold,
+                                  int32_t threshold_mode, const uint32_t* row_ptr,
+                                  const uint32_t* col, const double* values, uint32_t nnz,
+                                  const double* diagonal_blocks_inverse,
+                                  const uint32_t* diagonal_blocks_start,
+                                  const uint32_t* diagonal_blocks_count,
+                                  const uint32_t* diagonal_blocks_size,
+                                  const uint32_t* gradient_segments_start,
+                                  int32_t num_attributes, const double* rhs, uint32_t n,
+                                  double* solution, const double* initial_guess,
+                                  int32_t zero_initial_guess, double* work, double* cg_scalars,
+                                  ysp_pcg_stats* stats, void* stream) {
+  if (n % 3 || (nnz && (!row_ptr || !col || !values))) {
+    if (stats) *stats = ysp_pcg_stats{0, 0, 0, 0, 0, 0};
+    ysp_set_error(__FILE__, __LINE__, "ysp_pcg_solve_full: bad arguments");
+    return YSP_ERR_ARG;
+  }
+  hipStream_t s = as_stream(stream);
+  const uint32_t nbr = n / 3;
+  const uint32_t stride = (nnz + 63u) & ~63u;
+  auto product = [&](const double* x, double* y, double* quad, const double* stop) -> int {
+    if (nbr == 0) return YSP_OK;
+    hipLaunchKernelGGL(full3_product_kernel, dim3(grid_for(nbr, kFullRows)), dim3(kFullThreads), 0,
+                       s, row_ptr, col, values, stride, nbr, x, y, quad, stop);
+    YSP_CHECK_LAUNCH();
+    return YSP_OK;
+  };
+  return generic_run(product, max_iterations, threshold, threshold_mode, diagonal_blocks_inverse,
+                     diagonal_blocks_start, diagonal_blocks_count, diagonal_blocks_size,
+                     gradient_segments_start, num_attributes, rhs, n, solution, initial_guess,
+                     zero_initial_guess, work, cg_scalars, stats, s);
 }

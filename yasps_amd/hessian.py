@@ -218,6 +218,7 @@ class hessian:
         self.static = Part()
         self.dynamic = Part()
         self.full: Optional[FullBsr] = None
+        self.split: Optional[FullBsr] = None     # 3x3 sub-blocks of 9x9 / 9x3 / 3x9 layouts
         self.want_f32 = False
         self.use_staging = True
         self._grad_offsets = None
@@ -428,10 +429,53 @@ class hessian:
         return all(w.size == 3 for w in self.wrt) and all(
             d == 3 for part in (self.static, self.dynamic) for d in part.dims)
 
+    @property
+    def splittable(self) -> bool:
+        """Every target segment and block dimension is a multiple of 3 (at most 12)."""
+        return all(w.size % 3 == 0 for w in self.wrt) and all(
+            d % 3 == 0 and 0 < d <= 12 for part in (self.static, self.dynamic) for d in part.dims)
+
+    def _update_split(self):
+        """The mirrored full 3x3 block-CSR of a layout with affine-body segments (every
+        block split into 3x3 sub-blocks); the solve loop's operator there."""
+        if not self.splittable:
+            self.split = None
+            return
+        lib = _lib.load()
+        s = dev.stream()
+        key = (id(self.static.positions), self.static.num_blocks,
+               id(self.dynamic.positions), self.dynamic.num_blocks)
+        if self.split is None or self.split.key != key:
+            cap = 0
+            for part in (self.static, self.dynamic):
+                for c, cnt in enumerate(part.counts):
+                    cap += 2 * cnt * (part.dims[2 * c] // 3) * (part.dims[2 * c + 1] // 3)
+            cap = max(1, cap)
+            nbr = self.rows // 3
+            row_ptr = dev.empty(nbr + 1, "uint32")
+            col = dev.empty(cap, "uint32")
+            src = dev.empty(cap, "int64")
+            nnz = C.c_uint32(0)
+            a, b = self.static.ctype(), self.dynamic.ctype()
+            _lib.check(lib.ysp_bsr3_build_split(C.byref(a) if a else None, C.byref(b) if b else None,
+                                                self.rows, dev.ptr(row_ptr), dev.ptr(col),
+                                                dev.ptr(src), cap, C.byref(nnz), s),
+                       "ysp_bsr3_build_split")
+            self.split = FullBsr(key, nbr, row_ptr, col, src, nnz.value)
+        f = self.split
+        stride = (f.nnz + 63) // 64 * 64
+        if f.v64 is None or f.v64.numel() != 9 * stride:
+            f.v64 = dev.empty(9 * stride, "float64")
+        _lib.check(lib.ysp_bsr3_gather_split(dev.ptr(self.static.values), dev.ptr(self.dynamic.values),
+                                             dev.ptr(f.src), f.nnz, dev.ptr(f.v64), s),
+                   "ysp_bsr3_gather_split")
+
     def _update_full(self):
         if not self.all_3x3:
             self.full = None
+            self._update_split()
             return
+        self.split = None
         lib = _lib.load()
         s = dev.stream()
         key = (id(self.static.positions), self.static.num_blocks,
@@ -479,8 +523,8 @@ class hessian:
     def spmv(self, vec: vector, transpose: bool = False, use_f32: bool = False) -> vector:
         self._ensure_setup()
         out = vector(self.rows)
-        if self.full is not None and not transpose:
-            f = self.full
+        if (self.full is not None or (self.split is not None and not use_f32)) and not transpose:
+            f = self.full if self.full is not None else self.split
             vals = f.v32 if use_f32 else f.v64
             _lib.check(_lib.load().ysp_bsr3_spmv(dev.ptr(f.row_ptr), dev.ptr(f.col), dev.ptr(vals),
                                                  int(use_f32), f.nnz, self.rows, dev.ptr(vec.value),
