@@ -331,10 +331,12 @@ int ysp_pcg_solve(uint32_t max_iterations, double threshold, int32_t threshold_m
 /* ysp_pcg_solve with the operator given as the split mirrored full block-CSR of
  * ysp_bsr3_build_split / ysp_bsr3_gather_split (gathers instead of FP64 atomics; same
  * block-Jacobi preconditioner over the diagonal blocks of every target, recurrence,
- * stopping rules and return codes). n must be a multiple of 3. */
+ * stopping rules and return codes). n must be a multiple of 3. Block rows with more than
+ * long_row_threshold entries (0: 128) get a workgroup each. */
 int ysp_pcg_solve_full(uint32_t max_iterations, double threshold, int32_t threshold_mode,
                        const uint32_t* row_ptr, const uint32_t* col, const double* values,
-                       uint32_t nnz, const double* diagonal_blocks_inverse,
+                       uint32_t nnz, uint32_t long_row_threshold,
+                       const double* diagonal_blocks_inverse,
                        const uint32_t* diagonal_blocks_start,
                        const uint32_t* diagonal_blocks_count,
                        const uint32_t* diagonal_blocks_size,

@@ -119,7 +119,7 @@ SIGNATURES = {
     "ysp_pcg_solve": (C.c_int, [U32, D, I32, C.POINTER(BlockMatrix), C.POINTER(BlockMatrix), P,
                                 C.POINTER(U32), C.POINTER(U32), C.POINTER(U32), C.POINTER(U32),
                                 I32, P, U32, P, P, I32, P, P, C.POINTER(PcgStats), P]),
-    "ysp_pcg_solve_full": (C.c_int, [U32, D, I32, P, P, P, U32, P, C.POINTER(U32),
+    "ysp_pcg_solve_full": (C.c_int, [U32, D, I32, P, P, P, U32, U32, P, C.POINTER(U32),
                                      C.POINTER(U32), C.POINTER(U32), C.POINTER(U32), I32, P, U32,
                                      P, P, I32, P, P, C.POINTER(PcgStats), P]),
     "ysp_bsr3_build_split": (C.c_int, [C.POINTER(BlockMatrix), C.POINTER(BlockMatrix), U32, P, P,

@@ -18,6 +18,7 @@
 #include "spmv.hpp"
 
 #include <algorithm>
+#include <vector>
 
 namespace ysp {
 namespace {
@@ -99,6 +100,34 @@ __device__ __forceinline__ void apply_block(const double* __restrict__ minv, con
 }
 
 constexpr int kVecBlock = 256;
+
+// One diagonal block of the update: x += alpha p, r -= alpha q, q = 0, z = Minv r, and
+// its r.z, r.r terms (same arithmetic order as apply_block).
+template <int S>
+__device__ __forceinline__ void update_block(const double* __restrict__ minv, uint32_t dof0,
+                                             double alpha, double* __restrict__ x,
+                                             double* __restrict__ r, const double* __restrict__ p,
+                                             double* __restrict__ q, double* __restrict__ z,
+                                             double& rz, double& rr) {
+  double rl[S];
+#pragma unroll
+  for (int i = 0; i < S; ++i) {
+    const uint32_t d = dof0 + i;
+    x[d] = x[d] + p[d] * alpha;
+    rl[i] = r[d] - q[d] * alpha;
+    r[d] = rl[i];
+    q[d] = 0.0;
+  }
+#pragma unroll
+  for (int i = 0; i < S; ++i) {
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < S; ++j) acc += minv[i * S + j] * rl[j];
+    z[dof0 + i] = acc;
+    rz += rl[i] * acc;
+    rr += rl[i] * rl[i];
+  }
+}
 
 __global__ void __launch_bounds__(kVecBlock)
     init_kernel(DiagLayout L, const double* __restrict__ minv, const double* __restrict__ b,
@@ -204,16 +233,28 @@ __global__ void __launch_bounds__(kVecBlock)
   double rz = 0.0, rr = 0.0;
   uint32_t dof0, inv0, s;
   if (locate(L, t, dof0, inv0, s)) {
-    double rl[kMaxBlock], zl[kMaxBlock];
-    for (uint32_t i = 0; i < s; ++i) {
-      const uint32_t d = dof0 + i;
-      x[d] = x[d] + p[d] * alpha;
-      rl[i] = r[d] - q[d] * alpha;
-      r[d] = rl[i];
-      q[d] = 0.0;
+    // compile-time block sizes keep the per-block arrays in registers
+    switch (s) {
+      case 3: update_block<3>(minv + inv0, dof0, alpha, x, r, p, q, z, rz, rr); break;
+      case 9: update_block<9>(minv + inv0, dof0, alpha, x, r, p, q, z, rz, rr); break;
+      case 12: update_block<12>(minv + inv0, dof0, alpha, x, r, p, q, z, rz, rr); break;
+      case 1: update_block<1>(minv + inv0, dof0, alpha, x, r, p, q, z, rz, rr); break;
+      case 2: update_block<2>(minv + inv0, dof0, alpha, x, r, p, q, z, rz, rr); break;
+      case 4: update_block<4>(minv + inv0, dof0, alpha, x, r, p, q, z, rz, rr); break;
+      case 6: update_block<6>(minv + inv0, dof0, alpha, x, r, p, q, z, rz, rr); break;
+      default: {
+        double rl[kMaxBlock], zl[kMaxBlock];
+        for (uint32_t i = 0; i < s; ++i) {
+          const uint32_t d = dof0 + i;
+          x[d] = x[d] + p[d] * alpha;
+          rl[i] = r[d] - q[d] * alpha;
+          r[d] = rl[i];
+          q[d] = 0.0;
+        }
+        apply_block(minv + inv0, rl, zl, s, rz, rr);
+        for (uint32_t i = 0; i < s; ++i) z[dof0 + i] = zl[i];
+      }
     }
-    apply_block(minv + inv0, rl, zl, s, rz, rr);
-    for (uint32_t i = 0; i < s; ++i) z[dof0 + i] = zl[i];
   }
   __shared__ double red[kVecBlock / 64];
   rz = block_sum<kVecBlock>(rz, red);
@@ -308,16 +349,20 @@ int launch_invert(const double* in, double* out, uint32_t count, uint32_t size, 
 constexpr unsigned kFullThreads = 256;
 constexpr unsigned kFullLanes = 16;
 constexpr unsigned kFullRows = kFullThreads / kFullLanes;
+// Rows with more entries than this (by default) go to full3_long_row_kernel (a
+// workgroup per row) so that a few long rows (the coupling rows of affine bodies) do not
+// set the time.
+constexpr uint32_t kLongRow = 128;
 __global__ void __launch_bounds__(kFullThreads)
     full3_product_kernel(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
                          const double* __restrict__ v, uint32_t stride, uint32_t nbr,
-                         const double* __restrict__ x, double* __restrict__ y,
+                         uint32_t long_row, const double* __restrict__ x, double* __restrict__ y,
                          double* __restrict__ quad, const double* __restrict__ stop) {
   if (stop && *stop != 0.0) return;
   const uint32_t r = blockIdx.x * kFullRows + threadIdx.x / kFullLanes;
   const int l = threadIdx.x % kFullLanes;
   double y0 = 0.0, y1 = 0.0, y2 = 0.0, xq = 0.0;
-  if (r < nbr) {
+  if (r < nbr && row_ptr[r + 1] - row_ptr[r] <= long_row) {
     const uint32_t k0 = row_ptr[r], k1 = row_ptr[r + 1];
     for (uint32_t k = k0 + l; k < k1; k += kFullLanes) {
       const uint32_t c = 3 * col[k];
@@ -344,6 +389,39 @@ __global__ void __launch_bounds__(kFullThreads)
     __shared__ double red[kFullThreads / 64];
     const double sum = block_sum<kFullThreads>(xq, red);
     if (threadIdx.x == 0) atomic_add_f64(quad + (blockIdx.x % kQuadSlots) * kQuadSpace, sum);
+  }
+}
+
+// The rows longer than kLongRow (listed in long_rows): one workgroup per row.
+__global__ void __launch_bounds__(kFullThreads)
+    full3_long_row_kernel(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
+                          const double* __restrict__ v, uint32_t stride,
+                          const uint32_t* __restrict__ long_rows, const double* __restrict__ x,
+                          double* __restrict__ y, double* __restrict__ quad,
+                          const double* __restrict__ stop) {
+  if (stop && *stop != 0.0) return;
+  const uint32_t r = long_rows[blockIdx.x];
+  const uint32_t k0 = row_ptr[r], k1 = row_ptr[r + 1];
+  double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+  const size_t st = stride;
+  for (uint32_t k = k0 + threadIdx.x; k < k1; k += kFullThreads) {
+    const uint32_t c = 3 * col[k];
+    const double x0 = x[c], x1 = x[c + 1], x2 = x[c + 2];
+    y0 += v[k] * x0 + v[st + k] * x1 + v[2 * st + k] * x2;
+    y1 += v[3 * st + k] * x0 + v[4 * st + k] * x1 + v[5 * st + k] * x2;
+    y2 += v[6 * st + k] * x0 + v[7 * st + k] * x1 + v[8 * st + k] * x2;
+  }
+  __shared__ double red[kFullThreads / 64];
+  y0 = block_sum<kFullThreads>(y0, red);
+  y1 = block_sum<kFullThreads>(y1, red);
+  y2 = block_sum<kFullThreads>(y2, red);
+  if (threadIdx.x == 0) {
+    y[3 * r] = y0;
+    y[3 * r + 1] = y1;
+    y[3 * r + 2] = y2;
+    if (quad)
+      atomic_add_f64(quad + (blockIdx.x % kQuadSlots) * kQuadSpace,
+                     x[3 * r] * y0 + x[3 * r + 1] * y1 + x[3 * r + 2] * y2);
   }
 }
 
@@ -520,6 +598,7 @@ extern "C" int ysp_pcg_solve(uint32_t max_iterations, double threshold, int32_t 
 extern "C" int ysp_pcg_solve_full(uint32_t max_iterations, double threshold,
                                   int32_t threshold_mode, const uint32_t* row_ptr,
                                   const uint32_t* col, const double* values, uint32_t nnz,
+                                  uint32_t long_row_threshold,
                                   const double* diagonal_blocks_inverse,
                                   const uint32_t* diagonal_blocks_start,
                                   const uint32_t* diagonal_blocks_count,
@@ -537,10 +616,34 @@ extern "C" int ysp_pcg_solve_full(uint32_t max_iterations, double threshold,
   hipStream_t s = as_stream(stream);
   const uint32_t nbr = n / 3;
   const uint32_t stride = (nnz + 63u) & ~63u;
+  // rows longer than the threshold, listed once per solve from the row starts
+  const uint32_t long_row = long_row_threshold ? long_row_threshold : kLongRow;
+  uint32_t num_long_rows = 0;
+  uint32_t* long_rows = nullptr;
+  if (nbr && nnz) {
+    std::vector<uint32_t> rp(static_cast<size_t>(nbr) + 1);
+    YSP_CHECK(hipMemcpyAsync(rp.data(), row_ptr, sizeof(uint32_t) * rp.size(),
+                             hipMemcpyDeviceToHost, s));
+    YSP_CHECK(hipStreamSynchronize(s));
+    std::vector<uint32_t> longs;
+    for (uint32_t r = 0; r < nbr; ++r)
+      if (rp[r + 1] - rp[r] > long_row) longs.push_back(r);
+    if (!longs.empty()) {
+      long_rows = static_cast<uint32_t*>(scratch(66, sizeof(uint32_t) * longs.size()));
+      if (!long_rows) return YSP_ERR_HIP;
+      YSP_CHECK(hipMemcpyAsync(long_rows, longs.data(), sizeof(uint32_t) * longs.size(),
+                               hipMemcpyHostToDevice, s));
+      YSP_CHECK(hipStreamSynchronize(s));
+      num_long_rows = static_cast<uint32_t>(longs.size());
+    }
+  }
   auto product = [&](const double* x, double* y, double* quad, const double* stop) -> int {
     if (nbr == 0) return YSP_OK;
     hipLaunchKernelGGL(full3_product_kernel, dim3(grid_for(nbr, kFullRows)), dim3(kFullThreads), 0,
-                       s, row_ptr, col, values, stride, nbr, x, y, quad, stop);
+                       s, row_ptr, col, values, stride, nbr, long_row, x, y, quad, stop);
+    if (num_long_rows)
+      hipLaunchKernelGGL(full3_long_row_kernel, dim3(num_long_rows), dim3(kFullThreads), 0, s,
+                         row_ptr, col, values, stride, long_rows, x, y, quad, stop);
     YSP_CHECK_LAUNCH();
     return YSP_OK;
   };

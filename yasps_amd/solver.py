@@ -109,7 +109,7 @@ class jacobianPCGSolver:
             f = h.split
             rc = lib.ysp_pcg_solve_full(
                 int(maxIterations), float(tolerance), _MODES[self.threshold_mode],
-                dev.ptr(f.row_ptr), dev.ptr(f.col), dev.ptr(f.v64), f.nnz,
+                dev.ptr(f.row_ptr), dev.ptr(f.col), dev.ptr(f.v64), f.nnz, 0,
                 dev.ptr(h.diagonal_blocks_inverse), _lib.u32_array(starts), _lib.u32_array(counts),
                 _lib.u32_array(sizes), _lib.u32_array(h.gradient_segments_start_cpu[:-1]),
                 len(h.wrt), dev.ptr(right_hand_side.value), n, dev.ptr(self.__solution),
