@@ -81,7 +81,8 @@ def test_contact_scene_matches_oracle_and_solves():
 
 def test_split_operator_long_rows_match_short_rows():
     """ysp_pcg_solve_full with a long-row threshold of 4 (nearly every block row handled
-    by a workgroup of its own) gives the solve of the default threshold."""
+    by a workgroup of its own) follows the default threshold's iterates (five iterations
+    agree to rounding) and reaches the requested residual of the assembled system."""
     import ctypes as C
     from yasps_amd import _lib, device as dev
     cs = contact_scene(**SCENE)
@@ -92,24 +93,29 @@ def test_split_operator_long_rows_match_short_rows():
     n = hs.rows
     g, H = contact.assemble(cs, 2, golden_terms()[0])
     lib = _lib.load()
-    out = []
-    for threshold in (0, 4):
-        print(f"long-row threshold {threshold}", flush=True)
+
+    def solve(threshold, max_it, tol):
         x = dev.zeros(n, "float64")
         work = dev.zeros(5 * n, "float64")
         scalars = dev.zeros(12288, "float64")
         st = _lib.PcgStats()
         rc = lib.ysp_pcg_solve_full(
-            20000, 1e-10, 1, dev.ptr(f.row_ptr), dev.ptr(f.col), dev.ptr(f.v64), f.nnz, threshold,
+            max_it, tol, 1, dev.ptr(f.row_ptr), dev.ptr(f.col), dev.ptr(f.v64), f.nnz, threshold,
             dev.ptr(hs.diagonal_blocks_inverse), _lib.u32_array(hs.diagonal_blocks_start_cpu[:-1]),
             _lib.u32_array([w.numInstances for w in hs.wrt]), _lib.u32_array([w.size for w in hs.wrt]),
             _lib.u32_array(hs.gradient_segments_start_cpu[:-1]), len(hs.wrt),
             dev.ptr(hs.gradient.value), n, dev.ptr(x), None, 1, dev.ptr(work), dev.ptr(scalars),
             C.byref(st), dev.stream())
         dev.torch().cuda.synchronize()
+        return rc, dev.to_host(x)
+
+    early = []
+    for threshold in (0, 4):
+        print(f"long-row threshold {threshold}", flush=True)
+        rc, x5 = solve(threshold, 5, 1e-30)
+        assert rc == -4
+        early.append(x5)
+        rc, x = solve(threshold, 20000, 1e-10)
         assert rc >= 0
-        xs = dev.to_host(x)
-        assert np.linalg.norm(H @ xs - g) <= 2e-10 * np.linalg.norm(g)
-        out.append((st.iterations, xs))
-    assert abs(out[0][0] - out[1][0]) <= 1
-    assert np.linalg.norm(out[0][1] - out[1][1]) <= 1e-8 * np.linalg.norm(out[0][1])
+        assert np.linalg.norm(H @ x - g) <= 2e-10 * np.linalg.norm(g)
+    assert np.linalg.norm(early[0] - early[1]) <= 1e-11 * np.linalg.norm(early[0])
