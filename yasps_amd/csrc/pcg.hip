@@ -345,7 +345,10 @@ int launch_invert(const double* in, double* out, uint32_t count, uint32_t size, 
 
 // y = A x over a mirrored full 3x3 block-CSR (component-major values, `stride` entries
 // per component): one 16-lane group per block row, every row written; with quad, the
-// workgroup's sum of x_r^T y_r goes into its partial-sum slot.
+// workgroup's sum of x_r^T y_r goes into its partial-sum slot. `reverse` walks the rows
+// from the end: alternating directions from one product to the next, an operator larger
+// than the 256 MB Infinity Cache finds the part it streamed last still resident instead
+// of evicting it just before use (LRU over a cyclic stream never hits).
 constexpr unsigned kFullThreads = 256;
 constexpr unsigned kFullLanes = 16;
 constexpr unsigned kFullRows = kFullThreads / kFullLanes;
@@ -356,10 +359,12 @@ constexpr uint32_t kLongRow = 128;
 __global__ void __launch_bounds__(kFullThreads)
     full3_product_kernel(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
                          const double* __restrict__ v, uint32_t stride, uint32_t nbr,
-                         uint32_t long_row, const double* __restrict__ x, double* __restrict__ y,
-                         double* __restrict__ quad, const double* __restrict__ stop) {
+                         uint32_t long_row, int reverse, const double* __restrict__ x,
+                         double* __restrict__ y, double* __restrict__ quad,
+                         const double* __restrict__ stop) {
   if (stop && *stop != 0.0) return;
-  const uint32_t r = blockIdx.x * kFullRows + threadIdx.x / kFullLanes;
+  const uint32_t wg = reverse ? gridDim.x - 1 - blockIdx.x : blockIdx.x;
+  const uint32_t r = wg * kFullRows + threadIdx.x / kFullLanes;
   const int l = threadIdx.x % kFullLanes;
   double y0 = 0.0, y1 = 0.0, y2 = 0.0, xq = 0.0;
   if (r < nbr && row_ptr[r + 1] - row_ptr[r] <= long_row) {
@@ -396,11 +401,11 @@ __global__ void __launch_bounds__(kFullThreads)
 __global__ void __launch_bounds__(kFullThreads)
     full3_long_row_kernel(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
                           const double* __restrict__ v, uint32_t stride,
-                          const uint32_t* __restrict__ long_rows, const double* __restrict__ x,
-                          double* __restrict__ y, double* __restrict__ quad,
-                          const double* __restrict__ stop) {
+                          const uint32_t* __restrict__ long_rows, int reverse,
+                          const double* __restrict__ x, double* __restrict__ y,
+                          double* __restrict__ quad, const double* __restrict__ stop) {
   if (stop && *stop != 0.0) return;
-  const uint32_t r = long_rows[blockIdx.x];
+  const uint32_t r = long_rows[reverse ? gridDim.x - 1 - blockIdx.x : blockIdx.x];
   const uint32_t k0 = row_ptr[r], k1 = row_ptr[r + 1];
   double y0 = 0.0, y1 = 0.0, y2 = 0.0;
   const size_t st = stride;
@@ -637,13 +642,15 @@ extern "C" int ysp_pcg_solve_full(uint32_t max_iterations, double threshold,
       num_long_rows = static_cast<uint32_t>(longs.size());
     }
   }
+  int reverse = 0;                 // direction of the next product's row walk
   auto product = [&](const double* x, double* y, double* quad, const double* stop) -> int {
     if (nbr == 0) return YSP_OK;
     hipLaunchKernelGGL(full3_product_kernel, dim3(grid_for(nbr, kFullRows)), dim3(kFullThreads), 0,
-                       s, row_ptr, col, values, stride, nbr, long_row, x, y, quad, stop);
+                       s, row_ptr, col, values, stride, nbr, long_row, reverse, x, y, quad, stop);
     if (num_long_rows)
       hipLaunchKernelGGL(full3_long_row_kernel, dim3(num_long_rows), dim3(kFullThreads), 0, s,
-                         row_ptr, col, values, stride, long_rows, x, y, quad, stop);
+                         row_ptr, col, values, stride, long_rows, reverse, x, y, quad, stop);
+    reverse ^= 1;
     YSP_CHECK_LAUNCH();
     return YSP_OK;
   };
