@@ -104,6 +104,101 @@ __device__ __forceinline__ void project_packed(double (&H)[N * (N + 1) / 2], uin
     }
 }
 
+// 4-vertex kinds whose energy depends on vertex differences only (barriers, dihedral
+// bending, SNH): their 12x12 Hessians annihilate the three translations, so a failing
+// Hessian is projected in the 9-dimensional complement, H = (U x I3) M (U x I3)^T with U
+// the orthonormal 4x3 basis orthogonal to (1,1,1,1): the same result as the 12x12
+// decomposition up to the rounding-level translation eigenvalues, on a 9x9 Jacobi
+// (0.42 of the rotations' work and 81 instead of 144 eigenvector registers). The PSD
+// test stays on the full 12x12 matrix, as the reference runs it.
+template <int ID>
+__host__ __device__ constexpr bool translation_free() {
+  return ID == YSP_ENERGY_POINT_TRIANGLE || ID == YSP_ENERGY_EDGE_EDGE ||
+         ID == YSP_ENERGY_BENDING || ID == YSP_ENERGY_SNH;
+}
+
+__device__ __forceinline__ double basis4(int a, int k) {
+  // columns of U: (1,-1,0,0)/sqrt2, (1,1,-2,0)/sqrt6, (1,1,1,-3)/sqrt12
+  constexpr double c0 = 0.70710678118654752440, c1 = 0.40824829046386301637,
+                   c2 = 0.28867513459481288225;
+  return k == 0 ? (a == 0 ? c0 : (a == 1 ? -c0 : 0.0))
+       : k == 1 ? (a <= 1 ? c1 : (a == 2 ? -2.0 * c1 : 0.0))
+                : (a <= 2 ? c2 : -3.0 * c2);
+}
+
+__device__ __forceinline__ void project_translation_free(double (&H)[78], uint32_t method) {
+  double L[78];
+#pragma unroll
+  for (int r = 0; r < 12; ++r)
+#pragma unroll
+    for (int c = 0; c <= r; ++c) L[r * (r + 1) / 2 + c] = H[upper_index<12>(c, r)];
+  if (psd_test_packed<12>(L)) return;
+  auto h12 = [&](int r, int c) { return r <= c ? H[upper_index<12>(r, c)] : H[upper_index<12>(c, r)]; };
+  double M[45];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int l = k; l < 3; ++l)
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          if (k == l && j < i) continue;
+          double acc = 0.0;
+#pragma unroll
+          for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+              acc += basis4(a, k) * basis4(b, l) * h12(3 * a + i, 3 * b + j);
+          M[upper_index<9>(3 * k + i, 3 * l + j)] = acc;
+        }
+  double V[9][9];
+  jacobi_eigen_packed<9>(M, V);
+  double lam[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    const double l = M[upper_index<9>(i, i)];
+    lam[i] = (l < 0.0) ? (method == 1 ? -l : 0.0) : l;
+  }
+  double P[45];
+#pragma unroll
+  for (int i = 0; i < 9; ++i)
+#pragma unroll
+    for (int j = i; j < 9; ++j) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) acc += V[i][k] * lam[k] * V[j][k];
+      P[upper_index<9>(i, j)] = acc;
+    }
+  auto p9 = [&](int r, int c) { return r <= c ? P[upper_index<9>(r, c)] : P[upper_index<9>(c, r)]; };
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = a; b < 4; ++b)
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          if (a == b && j < i) continue;
+          double acc = 0.0;
+#pragma unroll
+          for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int l = 0; l < 3; ++l)
+              acc += basis4(a, k) * basis4(b, l) * p9(3 * k + i, 3 * l + j);
+          H[upper_index<12>(3 * a + i, 3 * b + j)] = acc;
+        }
+}
+
+template <int ID>
+__device__ __forceinline__ void project_element(double (&H)[Kind<ID>::N * (Kind<ID>::N + 1) / 2],
+                                                uint32_t method) {
+  if constexpr (translation_free<ID>() && Kind<ID>::A == 4 && Kind<ID>::W == 3)
+    project_translation_free(H, method);
+  else
+    project_packed<Kind<ID>::N>(H, method);
+}
+
 // ----------------------------------------------------------------- slots (any size)
 // Slot groups by |perm| in first-occurrence order (hessianKernelNoProject.pyx:93-163),
 // with every slot's size and local column offset (sum of the previous slot sizes).
@@ -352,7 +447,7 @@ __device__ __forceinline__ void eval_element_general(const UnionView& P, const A
   scene_params<ID>(sp, x, p, s);
   double E, g[T::N], H[T::N * (T::N + 1) / 2];
   T::eval(x, p, s, E, g, H);
-  if (method != 0) project_packed<T::N>(H, method);
+  if (method != 0) project_element<ID>(H, method);
   scatter_generic<KT, T::N>(SA, e, si, r, g, H);
 }
 
@@ -533,7 +628,7 @@ __global__ void __launch_bounds__(kEB)
   scene_params<ID>(sp, x, p, s);
   double E, g[T::N], H[T::N * (T::N + 1) / 2];
   T::eval(x, p, s, E, g, H);
-  if (method != 0) project_packed<T::N>(H, method);
+  if (method != 0) project_element<ID>(H, method);
   scatter_vertex<T::A, T::W, T::N>(SA, e, vs, g, H);
 }
 
