@@ -501,6 +501,10 @@ __device__ __forceinline__ void load_vslots(const ScatterArgs& S, uint32_t e, VS
     const uint32_t ii = S.seg_idx[o];
     const uint32_t sz = S.seg_size[o];
     const int pm = S.perm[o];
+    // a union's padding slot (index 0) holds no DoFs, whatever its size (the size makes
+    // a vertex of a small child as wide as the union's widest child, e.g. next to
+    // affine bodies)
+    if (ii == 0) continue;
     int g = -1;
     if (pm != 0) {
       const int p = (pm < 0 ? -pm : pm) - 1;
@@ -745,16 +749,14 @@ int launch_eval(const UnionView& U, const AffineView& AF, const uint32_t* e2v, u
   const dim3 g(grid_for(n, kEB)), b(kEB);
   auto run = [&](auto kt) {
     constexpr int KT = decltype(kt)::value;
-    if (AF.mask) {
-      hipLaunchKernelGGL((energy_eval_kernel<ID, KT>), g, b, 0, s, U, AF, e2v, n, ep, sp, method,
-                         SA, flags, nullptr);
-      return;
-    }
+    // vertex-layout elements in the fast kernel; the others (affine-driven vertices, other
+    // slot layouts) are queued for the general kernel: a small grid-stride launch when
+    // the queue is usually empty, a full grid when the union has affine children
     hipLaunchKernelGGL((energy_eval_vertex_kernel<ID, KT>), g, b, 0, s, U, e2v, n, ep, sp, method,
                        SA, flags, queue);
-    // the queued elements (usually none): a small grid-stride launch
-    hipLaunchKernelGGL((energy_eval_kernel<ID, KT>), dim3(std::min<unsigned>(g.x, 256)), b, 0, s,
-                       U, AF, e2v, n, ep, sp, method, SA, flags, queue);
+    const unsigned qg = AF.mask ? g.x : std::min<unsigned>(g.x, 256);
+    hipLaunchKernelGGL((energy_eval_kernel<ID, KT>), dim3(qg), b, 0, s, U, AF, e2v, n, ep, sp,
+                       method, SA, flags, queue);
   };
   if (SA.K <= 8) run(std::integral_constant<int, 8>());
   else run(std::integral_constant<int, kSlotsMax>());
