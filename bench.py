@@ -327,7 +327,7 @@ def dynamic_leg(args):
 
 
 def contact_leg(args):
-    """The contact layout of the mixed-DoF configuration on one GPU: 40 voxelised balls
+    """Contact scene with mixed DoFs on one GPU: 40 voxelised balls
     (16^3 cells, radius 0.045, 13,056 tets each) on a jittered 5 x 8 grid, every other one a
     12-DoF affine body (x = t + A r through JOIN/UNION), the others free vertices, resting
     U(0.2, 2) dhat above a 256 x 256 cloth (StVK membrane + dihedral bending, border
@@ -335,9 +335,9 @@ def contact_leg(args):
     pairs (a uniform sample of --contact-pairs when more are active), inertia and affine
     orthogonality. One Newton linear solve per step: re-index of the dynamic pair terms,
     element evaluation, PSD projection, assembly (9x9 / 9x3 / 3x9 / 3x3 blocks), the split
-    3x3 operator, preconditioner setup and PCG to a true relative residual of --tol. The
-    configuration names MAS for this solve; block-Jacobi runs here (9x9 blocks for the
-    affine matrices), since the device MAS takes 3-DoF vertex nodes only."""
+    3x3 operator, preconditioner setup and PCG to a true relative residual of --tol, with
+    block-Jacobi (9x9 blocks for the affine matrices): the device MAS takes 3-DoF vertex
+    nodes only."""
     from yasps_amd import device as dev
     from yasps_amd.scenes import contact_scene, build_contact_scene
     t0 = time.perf_counter()

@@ -350,7 +350,7 @@ int ysp_pcg_solve_full(uint32_t max_iterations, double threshold, int32_t thresh
  * aggregation maps) is built on the host once per static structure
  * (solver/mas/hierarchy.py:150-271); these entry points do the per-solve numeric work
  * and the apply on the device. All nodes are 3-vectors; domains are padded to 32
- * scalars (MAS leaf size of the benchmark configuration). */
+ * scalars. */
 
 /* values (level l, AoS 9 doubles per block) = sum of the level-(l-1) blocks listed in
  * items[offsets[e] .. offsets[e+1]) (identity-aggregation Galerkin product); the previous

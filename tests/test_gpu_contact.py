@@ -1,4 +1,4 @@
-"""Contact scene (the config-2 layout at test size): one affine and one free voxelised ball
+"""Contact scene at test size: one affine and one free voxelised ball
 resting on a cloth with a pinned border, SNH on the balls, StVK membrane and dihedral
 bending on the cloth, inertia, affine orthogonality, and dynamic IPC point-triangle /
 edge-edge barrier pairs through the [fixed, affine, moving] vertex union. Gradient and

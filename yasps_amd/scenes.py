@@ -464,7 +464,7 @@ def contact_scene(bodies_x: int = 5, bodies_z: int = 8, cells: int = 16, radius:
                   cloth_m: int = 256, dhat: float = 1e-3, kappa: float = 1e-2,
                   max_pairs: int = 1_500_000, young: float = 1e5, poisson: float = 0.4,
                   density: float = 1000.0, dt: float = 1e-2, seed: int = 0) -> ContactScene:
-    """Config-2 style contact scene: bodies_x x bodies_z voxelised balls on a jittered grid
+    """Contact scene with mixed DoFs: bodies_x x bodies_z voxelised balls on a jittered grid
     above a cloth_m x cloth_m cloth (StVK membrane + dihedral bending, border pinned), every
     other ball a 12-DoF affine body, the others free vertices. Every ball's lowest point
     sits U(0.2, 2) * dhat above the cloth. Barrier pairs (squared-distance convention of
