@@ -81,3 +81,25 @@ def test_dynamic_contact_term_reindexed_and_solved():
         assert _rel(hs.dense(), A) < REL
         x = dev.to_host(s0.minimizeEnergy(tolerance=1e-10)[0])
         assert np.linalg.norm(g - A @ x) <= 1.5e-10 * np.linalg.norm(g)
+
+
+def test_mas_with_dynamic_term_solves():
+    """MAS PCG on the cube plus the dynamic point-point term: blocks present in both parts
+    are summed into the domain matrices; the solve reaches the requested residual."""
+    from yasps_amd import device as dev
+    from yasps_amd.energies import point_point
+    print("mas with a dynamic term", flush=True)
+    cs = cube_scene(3, seed=3)
+    s0, h = build_yasps_scene(cs, solver_name="mas", threshold_mode="relative_residual")
+    m = h["mesh"]
+    pairs = _pairs(cs, 40, seed=1)
+    contacts = m.addPrimitive("contacts", numInstances=len(pairs), isDynamic=True)
+    c2v = contacts.addConnectivity("c2v", m.vertices, pairs, 2)
+    cp = contacts.addAttribute("positions", through=c2v, source=m.vertices["position"])
+    pp = contacts.addAttribute("pp", computed_attribute=point_point(cp, DHAT, KAPPA))
+    s0.addEnergy(pp, projection_method=2, dynamic_instances=True)
+    x = dev.to_host(s0.minimizeEnergy(tolerance=1e-10)[0])
+    g, A = _oracle(cs, pairs)
+    st = s0.minimizer.linearSolver.statistics
+    assert st["converged"] and st["inverse_status"] == 0
+    assert np.linalg.norm(g - A @ x) <= 1.5e-10 * np.linalg.norm(g)

@@ -115,6 +115,11 @@ def test_mas_operator_is_spd_and_inverts_domains(cube):
 def emulate_device_apply(layout, block_values, r, weights):
     """FP64 restatement of the device kernels over the layout tables."""
     D = DOMAIN_SCALARS
+    b0 = layout.blocks[0]
+    if b0.coarsen_offsets is not None:          # repeated (row, col) entries summed first
+        block_values = np.stack([block_values[b0.coarsen_items[b0.coarsen_offsets[e]:
+                                                               b0.coarsen_offsets[e + 1]]].sum(0)
+                                 for e in range(b0.nnz)])
     vals = [block_values]
     for b in layout.blocks[1:]:
         prev = vals[-1]
@@ -168,6 +173,28 @@ def test_device_tables_reproduce_the_oracle_apply(cube, levels, coarsest):
         ref = omas.apply(h, invs, r, w)
         got = emulate_device_apply(layout, A.data, r, w)
         assert np.abs(got - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+def test_repeated_entries_are_summed(cube):
+    """An operator whose block-CSR repeats (row, col) pairs (a block held by both the
+    static and the dynamic part) gives the MAS of the summed operator: every block of the
+    cube is entered twice, as 0.3 B and 0.7 B."""
+    _, A, _ = cube
+    h = _hierarchy(A)
+    nnz = A.indices.size
+    rows = np.repeat(np.arange(A.shape[0] // 3), np.diff(A.indptr))
+    order = np.argsort(np.concatenate([rows, rows]), kind="stable")
+    col2 = np.concatenate([A.indices, A.indices])[order]
+    data2 = np.concatenate([0.3 * A.data, 0.7 * A.data])[order]
+    indptr2 = 2 * A.indptr
+    layout = build_layout(h, indptr2, col2)
+    assert layout.blocks[0].coarsen_offsets is not None and layout.blocks[0].nnz == nnz
+    w = level_weights(h.num_levels, 2.5).astype(np.float64)
+    invs = omas.domain_inverses(h, omas.level_matrices(h, A.toarray()))
+    r = np.random.default_rng(11).standard_normal(A.shape[0])
+    ref = omas.apply(h, invs, r, w)
+    got = emulate_device_apply(layout, data2, r, w)
+    assert np.abs(got - ref).max() <= 1e-10 * np.abs(ref).max()
 
 
 def test_layout_destinations_stay_inside_their_domain(cube):

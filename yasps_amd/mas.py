@@ -35,7 +35,8 @@ NO_DEST = np.uint32(0xFFFFFFFF)
 @dataclass
 class LevelBlocks:
     """Blocks of one level: (row, col) node pairs and, for coarse levels, the CSR of the
-    finer level's blocks summed into each block."""
+    finer level's blocks summed into each block (at level 0, when the operator repeats a
+    (row, col) pair, the CSR of its entries summed into each unique block)."""
     rows: np.ndarray
     cols: np.ndarray
     coarsen_offsets: np.ndarray = None
@@ -98,12 +99,13 @@ def level_weights(num_levels: int, coarsest_level_weight: float = 2.5,
                      f"hierarchy level ({num_levels})")
 
 
-def static_pairs(row_ptr: np.ndarray, col: np.ndarray, src: np.ndarray) -> np.ndarray:
+def static_pairs(row_ptr: np.ndarray, col: np.ndarray, src: np.ndarray,
+                 dynamic_bit: int = 30) -> np.ndarray:
     """Off-diagonal node pairs of the static blocks of a full block-CSR (src bit 30 marks
-    dynamic blocks)."""
+    dynamic blocks; bit 43 in the split block-CSR of affine-body layouts)."""
     nbr = row_ptr.size - 1
     rows = np.repeat(np.arange(nbr, dtype=np.int64), np.diff(row_ptr.astype(np.int64)))
-    keep = ((src.astype(np.int64) >> 30) & 1) == 0
+    keep = ((src.astype(np.int64) >> dynamic_bit) & 1) == 0
     keep &= rows != col
     return np.stack([rows[keep], col.astype(np.int64)[keep]], 1)
 
@@ -155,7 +157,17 @@ def build_layout(h: Hierarchy, row_ptr: np.ndarray, col: np.ndarray) -> MasLayou
 
     # blocks of every level and their destinations in the domain matrices
     rows0 = np.repeat(np.arange(nv, dtype=np.int64), np.diff(row_ptr.astype(np.int64)))
-    blocks = [LevelBlocks(rows0, col.astype(np.int64))]
+    cols0 = col.astype(np.int64)
+    key0 = rows0 * nv + cols0
+    uniq0, inv0 = np.unique(key0, return_inverse=True)
+    if uniq0.size < key0.size:
+        # a block present in the static and the dynamic part appears twice in the full
+        # block-CSR: the domain matrices take the sum, so level 0 is first summed into its
+        # unique blocks (the domain scatter stores each block once)
+        off0, items0 = _csr_by_key(inv0.reshape(-1), uniq0.size)
+        blocks = [LevelBlocks(uniq0 // nv, uniq0 % nv, off0, items0)]
+    else:
+        blocks = [LevelBlocks(rows0, cols0)]
     for l in range(1, h.num_levels):
         prev = blocks[-1]
         m = h.levels[l - 1].fine_to_parent
@@ -182,9 +194,11 @@ class _DeviceMas:
         self.layout = layout
         u32 = lambda a: dev.to_device(np.asarray(a, dtype=np.uint32), "uint32")   # noqa: E731
         self.dest = [u32(b.dest) for b in layout.blocks]
-        self.coarsen = [(u32(b.coarsen_offsets), u32(b.coarsen_items)) if l else (None, None)
-                        for l, b in enumerate(layout.blocks)]
-        self.values = [None] + [dev.empty(9 * b.nnz, "float64") for b in layout.blocks[1:]]
+        self.coarsen = [(u32(b.coarsen_offsets), u32(b.coarsen_items))
+                        if b.coarsen_offsets is not None else (None, None) for b in layout.blocks]
+        self.merged0 = layout.blocks[0].coarsen_offsets is not None
+        self.values = ([dev.empty(9 * layout.blocks[0].nnz, "float64") if self.merged0 else None]
+                       + [dev.empty(9 * b.nnz, "float64") for b in layout.blocks[1:]])
         nc = int(layout.level_offsets[-1])
         self.restrict_offsets = u32(layout.restrict_offsets)
         self.restrict_items = u32(layout.restrict_items)
@@ -233,13 +247,22 @@ class _DeviceMas:
         self.domains.zero_()
         self.status.zero_()
         blocks = self.layout.blocks
-        _lib.check(lib.ysp_mas_domain_scatter(dev.ptr(full.v64), 1, full.nnz, dev.ptr(self.dest[0]),
-                                              dev.ptr(self.domains), s), "ysp_mas_domain_scatter")
+        if self.merged0:                 # repeated (row, col) entries summed first
+            off, items = self.coarsen[0]
+            _lib.check(lib.ysp_mas_coarsen(dev.ptr(full.v64), 1, full.nnz, dev.ptr(off), dev.ptr(items),
+                                           blocks[0].nnz, dev.ptr(self.values[0]), s), "ysp_mas_coarsen")
+            _lib.check(lib.ysp_mas_domain_scatter(dev.ptr(self.values[0]), 0, blocks[0].nnz,
+                                                  dev.ptr(self.dest[0]), dev.ptr(self.domains), s),
+                       "ysp_mas_domain_scatter")
+        else:
+            _lib.check(lib.ysp_mas_domain_scatter(dev.ptr(full.v64), 1, full.nnz, dev.ptr(self.dest[0]),
+                                                  dev.ptr(self.domains), s), "ysp_mas_domain_scatter")
         for l in range(1, len(blocks)):
-            prev_vals = full.v64 if l == 1 else self.values[l - 1]
-            prev_nnz = full.nnz if l == 1 else blocks[l - 1].nnz
+            from_full = l == 1 and not self.merged0
+            prev_vals = full.v64 if from_full else self.values[l - 1]
+            prev_nnz = full.nnz if from_full else blocks[l - 1].nnz
             off, items = self.coarsen[l]
-            _lib.check(lib.ysp_mas_coarsen(dev.ptr(prev_vals), int(l == 1), prev_nnz, dev.ptr(off),
+            _lib.check(lib.ysp_mas_coarsen(dev.ptr(prev_vals), int(from_full), prev_nnz, dev.ptr(off),
                                            dev.ptr(items), blocks[l].nnz, dev.ptr(self.values[l]), s),
                        "ysp_mas_coarsen")
             _lib.check(lib.ysp_mas_domain_scatter(dev.ptr(self.values[l]), 0, blocks[l].nnz,
@@ -336,19 +359,24 @@ class masSolver:
         return self._device
 
     def _ensure(self, h, n):
-        f = h.full
+        # every block 3x3, or the split 3x3 sub-blocks of an affine-body layout (an affine
+        # body's 12 DoFs are then four 3-DoF nodes of the hierarchy)
+        f = h.full if h.full is not None else h.split
         if f is None:
-            raise NotImplementedError("masSolver: the device MAS path needs 3 x 3 variable blocks")
+            raise NotImplementedError("masSolver: the device MAS path needs block dimensions "
+                                      "that are multiples of 3")
         skey = f.key[:2]
         rebuilt = False
         if self._hierarchy is None or self._hierarchy_key != skey or self._device_key != f.key:
             row_ptr = dev.to_host(f.row_ptr, "uint32").astype(np.int64)
             col = dev.to_host(f.col[:f.nnz], "uint32").astype(np.int64)
             if self._hierarchy is None or self._hierarchy_key != skey:
-                src = dev.to_host(f.src[:f.nnz], "uint32")
+                split = f is h.split
+                src = dev.to_host(f.src[:f.nnz], "int64" if split else "uint32")
                 t0 = time.perf_counter()
                 self._hierarchy = build_hierarchy(
-                    np.full(f.nbr, 3), static_pairs(row_ptr, col, src), self.max_domain_dofs,
+                    np.full(f.nbr, 3), static_pairs(row_ptr, col, src, 43 if split else 30),
+                    self.max_domain_dofs,
                     self.max_levels, self.target_nodes_per_partition, self.global_coarse_domain)
                 self._hierarchy_key = skey
                 self.hierarchy_build_count += 1
@@ -387,11 +415,11 @@ class masSolver:
         rebuilt = self._ensure(h, n)
         t1 = time.perf_counter()
         m = self._device
-        m.update(h.full)
+        f = h.full if h.full is not None else h.split
+        m.update(f)
         t2 = time.perf_counter()
         lib = _lib.load()
         stats = _lib.PcgStats()
-        f = h.full
         if self.precision == "mixed":
             return self._solve_mixed(h, rhs, n, tolerance, maxIterations, rebuilt, t0, t1, t2)
         rc = lib.ysp_pcg3_mas_solve(
