@@ -9,6 +9,8 @@ stage runs on the device through the C ABI.
 """
 from __future__ import annotations
 
+import itertools
+
 import ctypes as C
 from dataclasses import dataclass, field
 from typing import List, Optional
@@ -179,6 +181,9 @@ def build_part(terms: List[TermIndex], stream) -> Part:
     return part
 
 
+_STRUCTURE_VERSIONS = itertools.count(1)
+
+
 @dataclass
 class FullBsr:
     """Mirrored full 3x3 block-CSR of the current layout (the solve loop's operator)."""
@@ -191,6 +196,20 @@ class FullBsr:
     v64: object = None
     v32: object = None
     v32_valid: bool = False      # v32 holds the current values
+    # changes only when the structure does (a rebuild onto the same pairs keeps it)
+    version: int = field(default_factory=lambda: next(_STRUCTURE_VERSIONS))
+
+
+def _same_structure(old: Optional[FullBsr], new: FullBsr) -> bool:
+    """The rebuilt block-CSR equals the previous one (a dynamic term re-indexed onto the
+    same pairs): the previous object, and with it its structure version, is kept, so
+    structures derived from it (the MAS device layout) are not rebuilt."""
+    if old is None or old.nnz != new.nnz or old.nbr != new.nbr:
+        return False
+    t = dev.torch()
+    k = new.nnz
+    return bool(t.equal(old.row_ptr, new.row_ptr) and t.equal(old.col[:k], new.col[:k])
+                and t.equal(old.src[:k], new.src[:k]))
 
 
 class hessian:
@@ -461,7 +480,11 @@ class hessian:
                                                 self.rows, dev.ptr(row_ptr), dev.ptr(col),
                                                 dev.ptr(src), cap, C.byref(nnz), s),
                        "ysp_bsr3_build_split")
-            self.split = FullBsr(key, nbr, row_ptr, col, src, nnz.value)
+            new = FullBsr(key, nbr, row_ptr, col, src, nnz.value)
+            if _same_structure(self.split, new):
+                self.split.key = key
+            else:
+                self.split = new
         f = self.split
         stride = (f.nnz + 63) // 64 * 64
         if f.v64 is None or f.v64.numel() != 9 * stride:
@@ -491,7 +514,11 @@ class hessian:
                                           dev.ptr(self.dynamic.positions), self.dynamic.num_blocks,
                                           self.rows, dev.ptr(row_ptr), dev.ptr(col), dev.ptr(src),
                                           cap, C.byref(nnz), s), "ysp_bsr3_build")
-            self.full = FullBsr(key, nbr, row_ptr, col, src, nnz.value)
+            new = FullBsr(key, nbr, row_ptr, col, src, nnz.value)
+            if _same_structure(self.full, new):
+                self.full.key = key
+            else:
+                self.full = new
         f = self.full
         stride = (f.nnz + 63) // 64 * 64
         if f.v64 is None or f.v64.numel() != 9 * stride:

@@ -367,7 +367,8 @@ class masSolver:
                                       "that are multiples of 3")
         skey = f.key[:2]
         rebuilt = False
-        if self._hierarchy is None or self._hierarchy_key != skey or self._device_key != f.key:
+        if (self._hierarchy is None or self._hierarchy_key != skey
+                or self._device_key != (id(f), f.version)):
             row_ptr = dev.to_host(f.row_ptr, "uint32").astype(np.int64)
             col = dev.to_host(f.col[:f.nnz], "uint32").astype(np.int64)
             if self._hierarchy is None or self._hierarchy_key != skey:
@@ -384,7 +385,7 @@ class masSolver:
             layout = build_layout(self._hierarchy, row_ptr, col)
             w = level_weights(self._hierarchy.num_levels, self.coarsest_level_weight, self.level_weights)
             self._device = _DeviceMas(layout, w, n, self.coarse_bank)
-            self._device_key = f.key
+            self._device_key = (id(f), f.version)
             rebuilt = True
         if self._solution is None or self._solution.numel() != n:
             self._solution = dev.zeros(n, "float64")
