@@ -17,6 +17,7 @@
 
 #include <hipcub/hipcub.hpp>
 #include <algorithm>
+#include <vector>
 
 namespace ysp {
 namespace {
@@ -120,6 +121,9 @@ constexpr int kG = 16;  // lanes per block row
 // Block rows per workgroup of the fused PCG product: about 2048 workgroups, in multiples
 // of the 16 rows one pass of a workgroup covers, at most kMaxRowsPerWg.
 constexpr uint32_t kMaxRowsPerWg = 64;
+// Block rows with more entries than this get a workgroup of their own in the PCG product.
+constexpr uint32_t kLongRow3 = 128;
+constexpr uint32_t kNoLongSlot = 0xffffffffu;
 static_assert(3 * kMaxRowsPerWg <= kB, "one thread per scalar of a workgroup's rows");
 inline uint32_t spmv_rows_per_wg(uint32_t nv) {
   constexpr uint32_t kSpmvWgs = 2048, kPass = kB / kG;
@@ -348,7 +352,9 @@ __device__ __forceinline__ void pcg3_spmv_body(const uint32_t* __restrict__ row_
                                                double* __restrict__ p, double* __restrict__ q,
                                                double* __restrict__ st, int k,
                                                uint32_t rows_per_wg, uint32_t blk, uint32_t sb,
-                                               bool lead_wg) {
+                                               bool lead_wg,
+                                               const uint32_t* __restrict__ long_slot = nullptr,
+                                               const double* __restrict__ ylong = nullptr) {
   if (stopped(st)) return;
   const int cur = k % 3, prev = (k + 2) % 3, prev2 = (k + 1) % 3;
   __shared__ double lds[kB / 64];
@@ -391,7 +397,14 @@ __device__ __forceinline__ void pcg3_spmv_body(const uint32_t* __restrict__ row_
   const int l = threadIdx.x % kG;
   for (uint32_t r = rbase + threadIdx.x / kG; r < r_end; r += kB / kG) {
     double y0, y1, y2;
-    row_product<VT>(row_ptr, col, v, stride, z, r, l, y0, y1, y2);
+    const uint32_t ls = long_slot ? long_slot[r] : kNoLongSlot;
+    if (ls != kNoLongSlot) {           // a long row: its product came from pcg3_long_rows_kernel
+      y0 = ylong[3 * static_cast<size_t>(ls)];
+      y1 = ylong[3 * static_cast<size_t>(ls) + 1];
+      y2 = ylong[3 * static_cast<size_t>(ls) + 2];
+    } else {
+      row_product<VT>(row_ptr, col, v, stride, z, r, l, y0, y1, y2);
+    }
     if (l < 3) ys[3 * (r - rbase) + l] = l == 0 ? y0 : (l == 1 ? y1 : y2);
   }
   __syncthreads();
@@ -416,9 +429,46 @@ __global__ void __launch_bounds__(kB)
                      const VT* __restrict__ v, uint32_t stride, uint32_t nbr,
                      const double* __restrict__ z, double* __restrict__ p,
                      double* __restrict__ q, double* __restrict__ st, int k,
-                     uint32_t rows_per_wg) {
+                     uint32_t rows_per_wg, const uint32_t* __restrict__ long_slot,
+                     const double* __restrict__ ylong) {
   pcg3_spmv_body<VT>(row_ptr, col, v, stride, nbr, z, p, q, st, k, rows_per_wg,
-                     xcd_swizzle(blockIdx.x, gridDim.x), blockIdx.x, blockIdx.x == 0);
+                     xcd_swizzle(blockIdx.x, gridDim.x), blockIdx.x, blockIdx.x == 0, long_slot,
+                     ylong);
+}
+
+// The block rows longer than kLongRow3 entries (the coupling rows of affine bodies, the
+// cloth rows under a ball): one workgroup per row computes (A z)_r into ylong ahead of
+// the product kernel, which reads it instead of walking the row with 16 lanes.
+template <typename VT>
+__global__ void __launch_bounds__(kB)
+    pcg3_long_rows_kernel(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
+                          const VT* __restrict__ v, uint32_t stride,
+                          const uint32_t* __restrict__ long_rows, const double* __restrict__ z,
+                          double* __restrict__ ylong, const double* __restrict__ st) {
+  if (stopped(st)) return;
+  const uint32_t r = long_rows[blockIdx.x];
+  const uint32_t k0 = row_ptr[r], k1 = row_ptr[r + 1];
+  double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+  const size_t sd = stride;
+  for (uint32_t k = k0 + threadIdx.x; k < k1; k += kB) {
+    const uint32_t c = 3 * col[k];
+    const double x0 = z[c], x1 = z[c + 1], x2 = z[c + 2];
+    y0 += static_cast<double>(v[k]) * x0 + static_cast<double>(v[sd + k]) * x1 +
+          static_cast<double>(v[2 * sd + k]) * x2;
+    y1 += static_cast<double>(v[3 * sd + k]) * x0 + static_cast<double>(v[4 * sd + k]) * x1 +
+          static_cast<double>(v[5 * sd + k]) * x2;
+    y2 += static_cast<double>(v[6 * sd + k]) * x0 + static_cast<double>(v[7 * sd + k]) * x1 +
+          static_cast<double>(v[8 * sd + k]) * x2;
+  }
+  __shared__ double red[kB / 64];
+  y0 = block_sum<kB>(y0, red);
+  y1 = block_sum<kB>(y1, red);
+  y2 = block_sum<kB>(y2, red);
+  if (threadIdx.x == 0) {
+    ylong[3 * static_cast<size_t>(blockIdx.x)] = y0;
+    ylong[3 * static_cast<size_t>(blockIdx.x) + 1] = y1;
+    ylong[3 * static_cast<size_t>(blockIdx.x) + 2] = y2;
+  }
 }
 
 // Iteration k, kernel 2: alpha, x += alpha p, r -= alpha q, z = M^-1 r, r.z and r.r.
@@ -901,6 +951,48 @@ int pcg3_run(uint32_t max_iterations, double threshold, int32_t mode, const uint
   }
   // product grid: contiguous chunks of block rows, a multiple of 16 rows per workgroup
   const uint32_t rows_per_wg = spmv_rows_per_wg(nv);
+  // rows longer than kLongRow3 (none in vertex meshes), listed when the operator's
+  // (row_ptr, rows, entries) differ from the previous solve's. Any list of rows gives the
+  // same product (a listed row is computed whole by its workgroup), so a list kept for an
+  // operator rebuilt in place costs speed at most, never correctness.
+  static struct { const uint32_t* row_ptr; uint32_t nv, nnz, num_long; } cache{nullptr, 0, 0, 0};
+  uint32_t num_long = 0;
+  uint32_t* long_rows = nullptr;
+  uint32_t* long_slot = nullptr;
+  double* ylong = nullptr;
+  if (cache.row_ptr == row_ptr && cache.nv == nv && cache.nnz == nnz) {
+    num_long = cache.num_long;
+    if (num_long) {
+      long_rows = static_cast<uint32_t*>(scratch(67, sizeof(uint32_t) * num_long));
+      long_slot = static_cast<uint32_t*>(scratch(68, sizeof(uint32_t) * nv));
+      ylong = static_cast<double*>(scratch(69, sizeof(double) * 3 * num_long));
+      if (!long_rows || !long_slot || !ylong) return YSP_ERR_HIP;
+    }
+  } else {
+    cache = {nullptr, 0, 0, 0};
+    std::vector<uint32_t> rp(static_cast<size_t>(nv) + 1);
+    YSP_CHECK(hipMemcpyAsync(rp.data(), row_ptr, sizeof(uint32_t) * rp.size(),
+                             hipMemcpyDeviceToHost, s));
+    YSP_CHECK(hipStreamSynchronize(s));
+    std::vector<uint32_t> longs;
+    for (uint32_t r = 0; r < nv; ++r)
+      if (rp[r + 1] - rp[r] > kLongRow3) longs.push_back(r);
+    if (!longs.empty()) {
+      num_long = static_cast<uint32_t>(longs.size());
+      std::vector<uint32_t> slot(nv, kNoLongSlot);
+      for (uint32_t i = 0; i < num_long; ++i) slot[longs[i]] = i;
+      long_rows = static_cast<uint32_t*>(scratch(67, sizeof(uint32_t) * num_long));
+      long_slot = static_cast<uint32_t*>(scratch(68, sizeof(uint32_t) * nv));
+      ylong = static_cast<double*>(scratch(69, sizeof(double) * 3 * num_long));
+      if (!long_rows || !long_slot || !ylong) return YSP_ERR_HIP;
+      YSP_CHECK(hipMemcpyAsync(long_rows, longs.data(), sizeof(uint32_t) * num_long,
+                               hipMemcpyHostToDevice, s));
+      YSP_CHECK(hipMemcpyAsync(long_slot, slot.data(), sizeof(uint32_t) * nv,
+                               hipMemcpyHostToDevice, s));
+      YSP_CHECK(hipStreamSynchronize(s));
+    }
+    cache = {row_ptr, nv, nnz, num_long};
+  }
   static_assert(kStateDoubles <= YSP_PCG3_SCALARS, "cg_scalars too small");
   const dim3 vg(grid_for(nv, kB)), sg(grid_for(nv, rows_per_wg)), ug(grid_for(n, kUB)), bl(kB);
   hipLaunchKernelGGL(pcg3_init_kernel<MT>, vg, bl, 0, s, minv, rhs, x0, ax0, x, r, z, nv, st);
@@ -939,8 +1031,11 @@ int pcg3_run(uint32_t max_iterations, double threshold, int32_t mode, const uint
   auto enqueue = [&](uint32_t size) -> int {
     const uint32_t last = std::min<uint32_t>(end, next + size - 1);
     for (uint32_t it = next; it <= last; ++it) {
+      if (num_long)
+        hipLaunchKernelGGL(pcg3_long_rows_kernel<VT>, dim3(num_long), bl, 0, s, row_ptr, col, v,
+                           stride, long_rows, z, ylong, st);
       hipLaunchKernelGGL(pcg3_spmv_kernel<VT>, sg, bl, 0, s, row_ptr, col, v, stride, nv, z, pa, q,
-                         st, static_cast<int>(it), rows_per_wg);
+                         st, static_cast<int>(it), rows_per_wg, long_slot, ylong);
       if (it <= max_iterations) {
         hipLaunchKernelGGL(pcg3_update_kernel<MT>, ug, dim3(kUB), 0, s, minv, x, r, pa, q, z, n,
                            st, static_cast<int>(it));
