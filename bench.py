@@ -320,8 +320,10 @@ def dynamic_leg(args):
         hs._reindex_dynamic()
     torch.cuda.synchronize()
     reindex_ms = 1e3 * (time.perf_counter() - t0) / reps
-    # the same solves with MAS (blocks present in both parts summed into its domains)
-    s0.minimizer.setSolver("mas", threshold_mode="relative_residual")
+    # the same solves with MAS as the forms run it (4 levels; blocks present in both parts
+    # summed into its domains)
+    s0.minimizer.setSolver("mas", threshold_mode="relative_residual", max_levels=4,
+                           coarsest_level_weight=1.0)
     m_total, m_lat, m_iters = timed_solves(s0, max(3, args.steps // 2), 2, args.tol)
     return {"pairs": int(len(pairs)), "dynamic_blocks": int(hs.dynamic.num_blocks),
             "static_blocks": int(hs.static.num_blocks), "grid": n1,
