@@ -608,8 +608,10 @@ def contact_scene(bodies_x: int = 5, bodies_z: int = 8, cells: int = 16, radius:
                     & (d2 < dhat))
             edges_b.append(e[ei[keep]] + base[b])
             edges_c.append(cedge[ci[keep]])
-    pt = np.concatenate([np.concatenate(pts)[:, None], cloth_id[c_tri[np.concatenate(tris)]]], 1)
-    ee = np.concatenate([np.concatenate(edges_b), cloth_id[np.concatenate(edges_c)]], 1)
+    cat = lambda parts, shape: (np.concatenate(parts) if parts            # noqa: E731
+                                else np.zeros(shape, dtype=np.int64))
+    pt = np.concatenate([cat(pts, (0,))[:, None], cloth_id[c_tri[cat(tris, (0,))]]], 1)
+    ee = np.concatenate([cat(edges_b, (0, 2)), cloth_id[cat(edges_c, (0, 2))]], 1)
     total = pt.shape[0] + ee.shape[0]
     if total > max_pairs:
         keep = np.sort(rng.choice(total, size=max_pairs, replace=False))
