@@ -378,7 +378,9 @@ int ysp_mas_invert_domains(double* domains, const uint32_t* domain_sizes, uint32
 
 typedef struct ysp_mas_args {
   const uint32_t* restrict_offsets;     /* [num_coarse_nodes + 1] */
-  const uint32_t* restrict_items;       /* level-0 descendants of every coarse node */
+  const uint32_t* restrict_items;       /* level-0 descendants of the nodes of levels 1
+                                           to 3; level-3 descendants (coarse ids) of the
+                                           nodes of deeper levels */
   uint32_t num_coarse_nodes;            /* coarse levels concatenated */
   uint32_t num_coarse_levels;           /* <= YSP_MAS_MAX_COARSE_LEVELS */
   uint32_t level_offsets[YSP_MAS_MAX_COARSE_LEVELS + 1];  /* coarse ids of level l + 1:

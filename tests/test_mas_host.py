@@ -143,7 +143,7 @@ def emulate_device_apply(layout, block_values, r, weights):
     for l in range(layout.num_coarse_levels):
         for J in range(layout.level_offsets[l], layout.level_offsets[l + 1]):
             kids = layout.restrict_items[layout.restrict_offsets[J]:layout.restrict_offsets[J + 1]]
-            rc[J] = r3[kids].sum(0)
+            rc[J] = r3[kids].sum(0) if l < 3 else rc[kids].sum(0)
     c0 = np.zeros_like(r3)
     cc = np.zeros((nc, 3))
     for d in range(layout.num_domains):
@@ -158,10 +158,12 @@ def emulate_device_apply(layout, block_values, r, weights):
     return z.reshape(-1)
 
 
-@pytest.mark.parametrize("levels,coarsest", [(4, 2.5), (2, 1.0), (1, 2.5)])
-def test_device_tables_reproduce_the_oracle_apply(cube, levels, coarsest):
+@pytest.mark.parametrize("levels,coarsest,target",
+                         [(6, 2.5, 3), (4, 2.5, 8), (2, 1.0, 8), (1, 2.5, 8)])
+def test_device_tables_reproduce_the_oracle_apply(cube, levels, coarsest, target):
     _, A, _ = cube
-    h = _hierarchy(A, max_levels=levels)
+    h = _hierarchy(A, max_levels=levels, target_nodes_per_partition=target)
+    assert h.num_levels == levels
     layout = build_layout(h, A.indptr, A.indices)
     w = level_weights(h.num_levels, coarsest).astype(np.float64)
     assert np.array_equal(w, np.asarray(omas.level_weights(h, coarsest)))
@@ -173,6 +175,25 @@ def test_device_tables_reproduce_the_oracle_apply(cube, levels, coarsest):
         ref = omas.apply(h, invs, r, w)
         got = emulate_device_apply(layout, A.data, r, w)
         assert np.abs(got - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+def test_deep_levels_restrict_from_level_three(cube):
+    """Nodes of level 4 and deeper list level-3 coarse ids, whose level-0 descendants are
+    exactly their own (the hierarchy is nested), so the two-stage sum equals the flat one."""
+    _, A, _ = cube
+    h = _hierarchy(A, max_levels=6, target_nodes_per_partition=3)
+    assert h.num_levels >= 5
+    layout = build_layout(h, A.indptr, A.indices)
+    n2, n3 = layout.level_offsets[2], layout.level_offsets[3]
+    off, items = layout.restrict_offsets, layout.restrict_items
+    assert np.all(items[:off[n3]] < A.shape[0] // 3)
+    deep = items[off[n3]:]
+    assert deep.size and np.all((n2 <= deep) & (deep < n3))
+    for l in range(4, h.num_levels):
+        for J in range(layout.level_offsets[l - 1], layout.level_offsets[l]):
+            kids = items[off[J]:off[J + 1]] - n2
+            flat = np.flatnonzero(h.composed[l] == J - layout.level_offsets[l - 1])
+            assert np.array_equal(np.sort(np.flatnonzero(np.isin(h.composed[3], kids))), flat)
 
 
 def test_repeated_entries_are_summed(cube):

@@ -12,8 +12,9 @@
 //     solver/mas/cuda/local_inverse.cu:244-308). A pivot below 1e-12 * max|diag| marks the
 //     domain as failed.
 // Per application z = sum_l w_l P_l B_l^-1 R_l r:
-//   restrict (every coarse node of every level sums the residual of its level-0
-//   descendants, one launch), one 32-lane group per domain multiplies its inverse with the
+//   restrict (a node of levels 1 to 3 sums the residual of its level-0 descendants, a
+//   node of a deeper level the level-3 residuals of its level-3 descendants: a second
+//   launch), one 32-lane group per domain multiplies its inverse with the
 //   local residual, prolong adds the weighted corrections of every level and accumulates
 //   r.z.
 #include "mas.hpp"
@@ -182,15 +183,13 @@ __device__ __forceinline__ void descendant_sum(const double* __restrict__ r,
   }
 }
 
-// All coarse residuals in one launch, straight from the level-0 residual: workgroups
-// [0, b1) give 8 lanes to each level-1 node, [b1, b1 + b2) a wavefront to each level-2
-// node, the rest a whole workgroup to each node of the deeper levels (few nodes with
-// many descendants).
+// Residuals of levels 1 to 3, straight from the level-0 residual: workgroups [0, b1)
+// give 8 lanes to each level-1 node, [b1, b1 + b2) a wavefront to each level-2 node, the
+// rest a whole workgroup to each level-3 node (few nodes with many descendants).
 __global__ void __launch_bounds__(kB)
     restrict_kernel(const double* __restrict__ r, const uint32_t* __restrict__ offsets,
-                    const uint32_t* __restrict__ items, uint32_t n1, uint32_t n2, uint32_t nc,
-                    uint32_t b1, uint32_t b2, double* __restrict__ rc,
-                    const double* __restrict__ stop) {
+                    const uint32_t* __restrict__ items, uint32_t n1, uint32_t n2, uint32_t b1,
+                    uint32_t b2, double* __restrict__ rc, const double* __restrict__ stop) {
   if (mas_stopped(stop)) return;
   double sum[3];
   uint32_t J;
@@ -206,7 +205,7 @@ __global__ void __launch_bounds__(kB)
     descendant_sum<64>(r, offsets, items, J, threadIdx.x % 64, sum);
     write = threadIdx.x % 64 == 0;
   } else {
-    J = n2 + (blockIdx.x - b1 - b2);           // grid sized so that J < nc
+    J = n2 + (blockIdx.x - b1 - b2);           // grid sized so that J < n3
     descendant_sum<kB>(r, offsets, items, J, threadIdx.x, sum);
     __shared__ double part[3][kB / 64];
     if (threadIdx.x % 64 == 0)
@@ -224,7 +223,25 @@ __global__ void __launch_bounds__(kB)
     rc[3 * static_cast<size_t>(J) + 1] = sum[1];
     rc[3 * static_cast<size_t>(J) + 2] = sum[2];
   }
-  (void)nc;
+}
+
+// Residuals of levels 4 and deeper, a wavefront per node, from the level-3 residuals of
+// its level-3 descendants (coarse ids in [n2, n3), written by restrict_kernel; the nodes
+// written here, [n3, nc), are never read in this launch).
+__global__ void __launch_bounds__(kB)
+    restrict_deep_kernel(const uint32_t* __restrict__ offsets,
+                         const uint32_t* __restrict__ items, uint32_t n3, uint32_t nc,
+                         double* rc, const double* __restrict__ stop) {
+  if (mas_stopped(stop)) return;
+  const uint32_t J = n3 + blockIdx.x * (kB / 64) + threadIdx.x / 64;
+  if (J >= nc) return;                         // whole wavefronts leave together
+  double sum[3];
+  descendant_sum<64>(rc, offsets, items, J, threadIdx.x % 64, sum);
+  if (threadIdx.x % 64 == 0) {
+    rc[3 * static_cast<size_t>(J)] = sum[0];
+    rc[3 * static_cast<size_t>(J) + 1] = sum[1];
+    rc[3 * static_cast<size_t>(J) + 2] = sum[2];
+  }
 }
 
 // One 32-lane group per domain: c_loc = B^-1 r_loc. Lane i reads row i of the inverse,
@@ -407,11 +424,15 @@ int launch_mas_apply(const MasApply& m, const double* r, uint32_t n, double* z,
     const uint32_t nc = m.num_coarse_nodes;
     const uint32_t n1 = m.level_offsets[1];
     const uint32_t n2 = m.level_offsets[m.num_coarse_levels >= 2 ? 2 : 1];
+    const uint32_t n3 = m.level_offsets[m.num_coarse_levels >= 3 ? 3 : m.num_coarse_levels];
     const uint32_t b1 = grid_for(n1, kB / 8), b2 = grid_for(n2 - n1, kB / 64);
-    const uint32_t blocks = b1 + b2 + (nc - n2);
+    const uint32_t blocks = b1 + b2 + (n3 - n2);
     if (blocks)
       hipLaunchKernelGGL(restrict_kernel, dim3(blocks), dim3(kB), 0, s, r, m.restrict_offsets,
-                         m.restrict_items, n1, n2, nc, b1, b2, m.coarse_residual, stop);
+                         m.restrict_items, n1, n2, b1, b2, m.coarse_residual, stop);
+    if (nc > n3)
+      hipLaunchKernelGGL(restrict_deep_kernel, dim3(grid_for(nc - n3, kB / 64)), dim3(kB), 0, s,
+                         m.restrict_offsets, m.restrict_items, n3, nc, m.coarse_residual, stop);
   }
   if (m.num_domains) {
     const uint32_t first = m.inverses_bf16 ? m.bf16_first_domain : m.num_domains;

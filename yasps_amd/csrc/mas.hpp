@@ -10,7 +10,8 @@ constexpr int kMasMaxCoarseLevels = 8;
 
 struct MasApply {
   const uint32_t* restrict_offsets;   // [num_coarse_nodes + 1]
-  const uint32_t* restrict_items;     // level-0 descendants of every coarse node
+  const uint32_t* restrict_items;     // level-0 descendants of the nodes of levels 1 to
+                                      // 3; level-3 descendants (coarse ids) of deeper ones
   uint32_t num_coarse_nodes;          // all coarse levels, concatenated ids
   uint32_t num_coarse_levels;
   uint32_t level_offsets[kMasMaxCoarseLevels + 1];  // coarse id range of every level

@@ -124,10 +124,20 @@ def build_layout(h: Hierarchy, row_ptr: np.ndarray, col: np.ndarray) -> MasLayou
     bases[1:] = np.concatenate([[0], np.cumsum(sizes[1:])[:-1]]) if h.num_levels > 1 else []
     level_offsets = np.concatenate([[0], np.cumsum(sizes[1:])]).astype(np.int64)
 
-    # restriction: level-0 descendants of every coarse node
+    # restriction: level-0 descendants of the nodes of levels 1 to 3; for deeper levels
+    # the level-3 descendants as coarse ids (the hierarchy is nested, so their residuals
+    # sum to the same level-0 sum, read from a few entries instead of every vertex)
     r_off, r_items = [np.zeros(1, dtype=np.int64)], []
     for l in range(1, h.num_levels):
-        off, items = _csr_by_key(h.composed[l], sizes[l])
+        if l <= 3:
+            off, items = _csr_by_key(h.composed[l], sizes[l])
+        else:
+            anc = np.full(sizes[3], -1, dtype=np.int64)
+            anc[h.composed[3]] = h.composed[l]
+            if np.any(anc < 0) or not np.array_equal(anc[h.composed[3]], h.composed[l]):
+                raise ValueError(f"MAS layout: level {l} is not nested over level 3")
+            off, items = _csr_by_key(anc, sizes[l])
+            items = items + bases[3]
         r_off.append(off[1:] + r_off[-1][-1])
         r_items.append(items)
     restrict_offsets = np.concatenate(r_off)
