@@ -27,12 +27,15 @@ def _oracle_matrix(cs):
     return newton.bsr_matrix(st, asm.values), asm.gradient
 
 
-@pytest.mark.parametrize("levels,coarsest", [(4, 2.5), (3, 1.0)])
-def test_mas_apply_and_inverses_match_oracle(levels, coarsest):
+@pytest.mark.parametrize("levels,coarsest,target", [(4, 2.5, 8), (3, 1.0, 8), (6, 2.5, 3)])
+def test_mas_apply_and_inverses_match_oracle(levels, coarsest, target):
+    """(6 levels, 3 nodes per partition: levels 4 and 5 restrict from level 3 in a launch
+    of their own.)"""
     from yasps_amd import device as dev
     from yasps_amd.mas import DOMAIN_SCALARS
-    print(f"mas apply n=5 levels={levels}", flush=True)
-    cs, s0, hs = _mas_scene(5, seed=1, max_levels=levels, coarsest_level_weight=coarsest)
+    print(f"mas apply n=5 levels={levels} target={target}", flush=True)
+    cs, s0, hs = _mas_scene(5, seed=1, max_levels=levels, coarsest_level_weight=coarsest,
+                            target_nodes_per_partition=target)
     slv = s0.minimizer.linearSolver.implementation
     slv._ensure(hs, hs.rows)
     m = slv.device_tables
@@ -40,7 +43,7 @@ def test_mas_apply_and_inverses_match_oracle(levels, coarsest):
     dev.torch().cuda.synchronize()
     assert int(m.status.item()) == 0
     h = slv.hierarchy
-    assert h.num_levels >= 2
+    assert h.num_levels >= (5 if levels == 6 else 2)
     A, _ = _oracle_matrix(cs)
     mats = omas.level_matrices(h, A.toarray())
     invs = omas.domain_inverses(h, mats)
