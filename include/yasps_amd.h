@@ -229,6 +229,13 @@ int ysp_snh_theta_adjoint(const ysp_union_f64* positions, const uint32_t* tet2v,
 #define YSP_ENERGY_AFFINE_ORTHOGONALITY 7
 #define YSP_ENERGY_SNH 8       /* p [Dm^-1 (9), vol], s [mu, lam, dt] (helpers.py:42-67) */
 #define YSP_ENERGY_INERTIA 9   /* p [x_prev (3), v (3), m], s [dt] (helpers.py:69-71) */
+/* QUADRATIC_FORM: 4 vertices (12 DoFs), y = J x + c, E = y^T Q y / 2 + l^T y; the map is
+ * linear, so the projection acts on the inner Hessian Q and H = J^T spd(Q) J
+ * (differentiator.pyx:795-801), as the energy of
+ * tests/differentiator/test_hessian_sparsity_permutation_gpu.py:17-38 is projected there.
+ * Scene params [J (6 x 12, row-major), Q (6 x 6), l (6), c (6)]: a map with m < 6 rows
+ * leaves rows m.. of J, l and c zero and Q identity there. */
+#define YSP_ENERGY_QUADRATIC_FORM 10
 
 /* Union children whose vertices follow an affine body, x = t + A r
  * (examples/one_bunny_partial_abd/one_bunny_partial_abd.py:127-144): bit c of mask marks
@@ -300,6 +307,17 @@ int ysp_block_spmv(const ysp_block_matrix* part, const ysp_block_matrix* dynamic
 int ysp_invert_diagonal_blocks(const double* diagonal_blocks, double* inverse,
                                const uint32_t* starts, const uint32_t* counts,
                                const uint32_t* sizes, int32_t num_attributes, void* stream);
+
+/* Scalar diagonal of the assembled Hessian from its diagonal blocks:
+ * diagonal[gradient_segments_start[a] + i * sizes[a] + k] = block (a, i) entry (k, k).
+ * Replaces the `diagonal` output of the FullProject element kernels
+ * (kernel/Hessian/hessianKernelFullProject.pyx:187-192), read through hessian.diagonal
+ * (matrixAndVector/matrix/hessian.pyx:188) and scene.diagonal (scene/scene.pyx:186-188).
+ * starts (doubles), counts, sizes and gradient_segments_start are host arrays. */
+int ysp_diagonal_from_blocks(const double* diagonal_blocks, double* diagonal,
+                             const uint32_t* starts, const uint32_t* counts,
+                             const uint32_t* sizes, const uint32_t* gradient_segments_start,
+                             int32_t num_attributes, void* stream);
 
 typedef struct ysp_pcg_stats {
   int32_t iterations;
@@ -397,9 +415,14 @@ typedef struct ysp_mas_args {
   float* coarse_correction;             /* work [3 x num_coarse_nodes] */
   const uint16_t* inverses_bf16;        /* NULL, or the BF16 bank of domains >= */
   uint32_t bf16_first_domain;           /*   bf16_first_domain (ysp_mas_invert_domains) */
+  const float* domain_scale;            /* NULL, or a factor per domain on its correction:
+                                           duplicate_level_weight for a domain that repeats
+                                           the previous level's (preconditioner_specialized.cu
+                                           :415-440), 1 otherwise */
 } ysp_mas_args;
 
-/* z = sum_l w_l P_l B_l^-1 R_l r (solver/mas/preconditioner.py:128-154). */
+/* z = sum_l w_l P_l S_l B_l^-1 R_l r (solver/mas/preconditioner.py:128-154; S_l the
+ * per-domain factors of domain_scale, the identity when it is NULL). */
 int ysp_mas_apply(const ysp_mas_args* args, const double* r, uint32_t n, double* z,
                   void* stream);
 

@@ -7,11 +7,16 @@ the fine blocks (``solver/mas/numeric_assembly.py:64-83``), domain matrices = pr
 submatrices over a domain's nodes (``:47-61``), explicit FP64 inverses
 (``solver/mas/local_inverse.py:40-78``), and the apply
 z = sum_l w_l P_l B_l^-1 R_l r with level weights (1, ..., 1, coarsest)
-(``solver/mas/preconditioner.py:26-154``).
+(``solver/mas/preconditioner.py:26-154``), optionally with the GPU path's topology
+weight per domain (``duplicate_domains``: a domain that only repeats a domain of the
+previous level, ``solver/mas/cuda_runtime.py:1285-1340``, weighted by
+``duplicate_level_weight`` in the collect, ``cuda/preconditioner_specialized.cu:415-440``).
 
-Pinned by invariants only (the reference's METIS partition is not reproducible offline):
+Pinned by invariants (the reference's METIS partition is not reproducible offline):
 R_l = P_l^T, every B_l^-1 inverts its domain matrix, the applied operator is symmetric
-positive definite, and PCG with it converges to the stated residual.
+positive definite, and PCG with it converges to the stated residual; and by closed forms
+on one-domain hierarchies (z = w A^-1 r; a second level of one coarse node adds
+w R^T (R A R^T)^-1 R r) and a hand-built hierarchy for the duplicate rule.
 """
 from __future__ import annotations
 
@@ -58,18 +63,51 @@ def level_weights(h, coarsest_weight=2.5):
     return [1.0] * (h.num_levels - 1) + [coarsest_weight]
 
 
-def apply(h, invs, r, weights):
+def apply(h, invs, r, weights, scales=None):
+    """``scales``: optional per-level lists of a factor per domain (topology weights)."""
     r = np.asarray(r, dtype=np.float64)
     z = np.zeros_like(r)
     for l, lv in enumerate(h.levels):
         m = scalar_map(h, l)
         rl = np.bincount(m, weights=r, minlength=lv.num_dofs)
         cl = np.zeros(lv.num_dofs)
-        for d, inv in zip(lv.domains, invs[l]):
+        for k, (d, inv) in enumerate(zip(lv.domains, invs[l])):
             s = domain_scalars(lv, d)
-            cl[s] = inv @ rl[s]
+            cl[s] = inv @ rl[s] * (1.0 if scales is None else scales[l][k])
         z += weights[l] * cl[m]
     return z
+
+
+def duplicate_domains(h):
+    """Per level, per domain: the domain repeats one domain of the previous level (every
+    node has one child, and the children are exactly that domain's nodes). Plain loops
+    over the hierarchy's maps (``solver/mas/cuda_runtime.py:1301-1337``)."""
+    out = [[False] * len(h.levels[0].domains)]
+    for l in range(1, len(h.levels)):
+        prev, cur = h.levels[l - 1], h.levels[l]
+        children = [[] for _ in range(cur.num_nodes)]
+        for child, parent in enumerate(prev.fine_to_parent):
+            children[int(parent)].append(child)
+        owner = {}
+        for k, d in enumerate(prev.domains):
+            for node in d:
+                owner[int(node)] = k
+        flags = []
+        for d in cur.domains:
+            kids = []
+            ok = True
+            for parent in d:
+                if len(children[int(parent)]) != 1:
+                    ok = False
+                    break
+                kids.append(children[int(parent)][0])
+            if ok:
+                src = owner[kids[0]]
+                ok = all(owner[c] == src for c in kids) and \
+                    set(kids) == {int(v) for v in prev.domains[src]}
+            flags.append(ok)
+        out.append(flags)
+    return out
 
 
 def dense_operator(h, invs, weights):

@@ -17,7 +17,8 @@ def _rel(a, b):
     return np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(b).max(), 1e-300)
 
 
-@pytest.mark.parametrize("n,layers,bodies,method", [(2, 1, 2, 2), (3, 2, 2, 2), (3, 2, 3, 0)])
+@pytest.mark.parametrize("n,layers,bodies,method", [(2, 1, 2, 2), (3, 2, 2, 2), (3, 2, 3, 0),
+                                                    (3, 2, 2, 1)])
 def test_abd_gradient_hessian_match_oracle(n, layers, bodies, method):
     from yasps_amd import device as dev
     print(f"abd n={n} layers={layers} bodies={bodies} method={method}", flush=True)

@@ -1,17 +1,18 @@
 """Material adjoint: dL/dtheta from the adjoint solve and the per-tet dg/dtheta kernel
 against central finite differences of the loss (equilibrium re-solved, h = 1e-6), the
 torch.autograd.Function backward equal to it, and the mixed-precision adjoint close to
-the FP64 one."""
+the FP64 one; equilibrium and adjoint solved with MAS (the default) and block-Jacobi."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module")
-def setup():
+@pytest.fixture(scope="module", params=["mas", "jacobian"])
+def setup(request):
     from yasps_amd.adjoint import MaterialProblem
-    prob = MaterialProblem(n=3)
+    print(f"adjoint setup, {request.param}", flush=True)
+    prob = MaterialProblem(n=3, solver=request.param)
     rng = np.random.default_rng(0)
     theta = np.log(1e5) + 0.3 * rng.standard_normal(prob.num_tets)
     theta_t = np.log(1e5) + 0.3 * np.random.default_rng(1).standard_normal(prob.num_tets)
@@ -55,7 +56,7 @@ def test_mixed_precision_adjoint_close_to_fp64(setup):
     from yasps_amd.adjoint import MaterialProblem
     prob, theta, target, x = setup
     g64 = prob.vjp(x, prob.loss_grad_x(x, target))
-    pm = MaterialProblem(n=3, precision="mixed")
+    pm = MaterialProblem(n=3, precision="mixed", solver=prob.scene.minimizer.solverName)
     xm = pm.equilibrium(theta)
     assert np.linalg.norm(xm - x) <= 1e-9 * np.linalg.norm(x)
     g32 = pm.vjp(xm, pm.loss_grad_x(xm, target))

@@ -18,20 +18,20 @@ def _rel_err(a, b):
     return np.abs(np.asarray(a) - np.asarray(b)).max() / scale
 
 
-def _gpu(n, seed=0, noise=0.02):
+def _gpu(n, seed=0, noise=0.02, method=2):
     from yasps_amd import device as dev
     cs = cube_scene(n, seed=seed, noise=noise)
-    s0, h = build_yasps_scene(cs)
+    s0, h = build_yasps_scene(cs, projection_method=method)
     hs = s0.minimizer.computeNumericValue()
     dev.torch().cuda.synchronize()
     return cs, s0, hs
 
 
-def _oracle_general(cs):
+def _oracle_general(cs, method=2):
     t_tet, t_in, layout = newton.general_structure(cs)
     dinv, vol = energies.snh_rest_data(cs.rest[cs.tets])
     g_t, H_t = energies.snh_gradient_hessian(cs.position[cs.tets], dinv, vol, cs.mu, cs.lam, cs.dt)
-    H_t = projection.project_batch(H_t, 2)
+    H_t = projection.project_batch(H_t, method)
     gs, ds = assembly.diagonal_layout([3], [cs.n_moving])
     grad = np.zeros(cs.n_dof)
     vals = np.zeros(layout.total_size)
@@ -77,26 +77,38 @@ def test_index_pipeline_bit_exact(n):
         np.testing.assert_array_equal(dev.to_host(ti.lookups, "uint32"), lk)
 
 
+@pytest.mark.parametrize("method", [1, 2])
 @pytest.mark.parametrize("n,seed", [(2, 0), (3, 1), (3, 5)])
-def test_gradient_hessian_parity(n, seed):
+def test_gradient_hessian_parity(n, seed, method):
+    """Projection method 1 (|lambda|, the reference's addEnergy default) and 2 (max)."""
     from yasps_amd import device as dev
-    cs, s0, hs = _gpu(n, seed)
-    *_, grad, vals, diag = _oracle_general(cs)
+    print(f"cube parity n={n} seed={seed} method={method}", flush=True)
+    cs, s0, hs = _gpu(n, seed, method=method)
+    *_, grad, vals, diag = _oracle_general(cs, method)
     assert _rel_err(dev.to_host(hs.gradient.value), grad) < REL
     assert _rel_err(dev.to_host(hs.blocks_flattened), vals) < REL
     assert _rel_err(dev.to_host(hs.diagonal_blocks), diag) < REL
+    # scene.diagonal (scene.pyx:186-188): the scalar diagonal of the assembled Hessian
+    d = dev.to_host(s0.diagonal)
+    assert _rel_err(d, np.diagonal(diag.reshape(-1, 3, 3), axis1=1, axis2=2).reshape(-1)) < REL
 
 
-def test_large_noise_exercises_projection():
-    """With 10x the noise most tets are indefinite: the projected blocks still match."""
+@pytest.mark.parametrize("method", [1, 2])
+def test_large_noise_exercises_projection(method):
+    """With 10x the noise most tets are indefinite: the projected blocks still match, for
+    |lambda| (method 1) and max(lambda, 0) (method 2), which differ on every such tet."""
     from yasps_amd import device as dev
-    cs, s0, hs = _gpu(3, seed=3, noise=0.2)
+    print(f"cube large noise method={method}", flush=True)
+    cs, s0, hs = _gpu(3, seed=3, noise=0.2, method=method)
     dinv, vol = energies.snh_rest_data(cs.rest[cs.tets])
     _, H_t = energies.snh_gradient_hessian(cs.position[cs.tets], dinv, vol, cs.mu, cs.lam, cs.dt)
     assert (~projection.is_psd_batch(H_t)).sum() > len(H_t) // 4
-    *_, grad, vals, diag = _oracle_general(cs)
+    *_, grad, vals, diag = _oracle_general(cs, method)
+    *_, vals_other, _ = _oracle_general(cs, 3 - method)
+    assert _rel_err(vals_other, vals) > 1e-3          # the two rules do differ here
     assert _rel_err(dev.to_host(hs.blocks_flattened), vals) < REL
     assert _rel_err(dev.to_host(hs.gradient.value), grad) < REL
+    assert _rel_err(dev.to_host(hs.diagonal_blocks), diag) < REL
 
 
 def test_cube_n8_fast_structure_parity():

@@ -137,3 +137,40 @@ def test_long_rows_in_the_product(solver):
     g, A = _oracle_with(cs, pairs, 2.0, 1e-3)
     assert s0.minimizer.linearSolver.statistics["converged"]
     assert np.linalg.norm(g - A @ x) <= 1.5e-10 * np.linalg.norm(g)
+
+
+def test_dynamic_term_emptied_and_refilled():
+    """The dynamic pair set shrinks to zero instances and grows back (the reference's
+    sequence with an empty active set, test_hessian_sparsity_permutation_gpu.py:256-271):
+    with no pairs the Hessian is the static part alone; the solve runs on it; refilled, the
+    dense oracle matches again."""
+    from yasps_amd import device as dev
+    from yasps_amd.energies import point_point
+    print("dynamic term emptied and refilled", flush=True)
+    cs = cube_scene(3, seed=3)
+    s0, h = build_yasps_scene(cs, threshold_mode="relative_residual")
+    m = h["mesh"]
+    p1 = _pairs(cs, 40, seed=1)
+    contacts = m.addPrimitive("contacts", numInstances=len(p1), isDynamic=True)
+    c2v = contacts.addConnectivity("c2v", m.vertices, p1, 2)
+    cp = contacts.addAttribute("positions", through=c2v, source=m.vertices["position"])
+    pp = contacts.addAttribute("pp", computed_attribute=point_point(cp, DHAT, KAPPA))
+    s0.addEnergy(pp, projection_method=2, dynamic_instances=True)
+    empty = np.zeros((0, 2), dtype=np.uint32)
+    for step, pairs in enumerate((p1, empty, _pairs(cs, 31, seed=5), empty)):
+        print(f"  pair set {step}: {len(pairs)} pairs", flush=True)
+        contacts.updateNumInstances(len(pairs))
+        if len(pairs):
+            c2v.updateConnectivity(pairs)
+        for solver in ("jacobian", "mas"):
+            s0.minimizer.setSolver(solver, threshold_mode="relative_residual")
+            hs = s0.minimizer.computeNumericValue()
+            dev.torch().cuda.synchronize()
+            g, A = _oracle(cs, pairs)
+            if len(pairs) == 0:
+                assert hs.dynamic.num_blocks == 0 and sum(hs.block_counts_dynamic) == 0
+            assert _rel(dev.to_host(hs.gradient.value), g) < REL
+            assert _rel(hs.dense(), A) < REL
+            x = dev.to_host(s0.minimizeEnergy(tolerance=1e-10)[0])
+            assert s0.minimizer.linearSolver.statistics["converged"], solver
+            assert np.linalg.norm(g - A @ x) <= 1.5e-10 * np.linalg.norm(g), solver

@@ -83,3 +83,50 @@ def test_snh_element_repeating_a_vertex_is_rejected():
     s0, _ = build_yasps_scene(cs)
     with pytest.raises(RuntimeError, match="repeated vertex"):
         s0.minimizer.computeNumericValue()
+
+
+@pytest.mark.parametrize("size", [1, 2, 3, 4, 6, 9, 12])
+def test_jacobi_pseudo_inverse_entrywise(size):
+    """ysp_invert_diagonal_blocks against oracle.projection.jacobi_block_inverse entry by
+    entry, on blocks whose spectra hold negative eigenvalues, eigenvalues with |l| < 1e-6
+    (kept as |l|, not inverted: diagonalBlockInverseKernel.pyx:69,91) and ordinary ones."""
+    import ctypes as C
+    from yasps_amd import _lib, device as dev
+    print(f"jacobi pseudo-inverse size {size}", flush=True)
+    rng = np.random.default_rng(100 + size)
+    spectra = []
+    for kind in range(6):
+        if kind == 0:
+            w = rng.uniform(0.5, 4.0, size)                       # SPD
+        elif kind == 1:
+            w = rng.uniform(-4.0, -0.5, size)                     # negative definite
+        elif kind == 2:
+            w = rng.uniform(0.5, 4.0, size)
+            w[0] = 3e-7                                           # |l| < 1e-6: kept as l
+        elif kind == 3:
+            w = rng.uniform(-3.0, 3.0, size)
+            w[-1] = -4e-7                                         # kept as |l|
+        elif kind == 4:
+            w = np.full(size, 2.5e-7)                             # every eigenvalue tiny
+        else:
+            w = rng.uniform(-3.0, 3.0, size)                      # indefinite
+        spectra.append(w)
+    blocks = []
+    for w in spectra:
+        V, _ = np.linalg.qr(rng.standard_normal((size, size)))
+        B = (V * w) @ V.T
+        blocks.append(0.5 * (B + B.T))
+    blocks = np.asarray(blocks)
+    ref = projection.jacobi_block_inverse(blocks)
+    count = len(blocks)
+    src = dev.to_device(blocks.reshape(-1), "float64")
+    out = dev.zeros(blocks.size, "float64")
+    lib = _lib.load()
+    rc = lib.ysp_invert_diagonal_blocks(dev.ptr(src), dev.ptr(out), _lib.u32_array([0]),
+                                        _lib.u32_array([count]), _lib.u32_array([size]), 1,
+                                        dev.stream())
+    assert rc == 0
+    got = dev.to_host(out).reshape(count, size, size)
+    for b in range(count):
+        scale = np.abs(ref[b]).max()
+        assert np.abs(got[b] - ref[b]).max() <= 1e-9 * scale, (b, spectra[b])

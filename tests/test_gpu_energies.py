@@ -99,7 +99,7 @@ def _oracle_dense(G, method, uid, nfixed, fixed):
 
 
 @pytest.mark.parametrize("kind", VERTEX_KINDS)
-@pytest.mark.parametrize("method", [0, 2])
+@pytest.mark.parametrize("method", [0, 1, 2])
 def test_vertex_energy_matches_golden(kind, method):
     from yasps_amd import device as dev
     print(f"energy {kind} method {method}", flush=True)
@@ -117,7 +117,7 @@ def test_vertex_energy_matches_golden(kind, method):
     assert _rel(hs.dense(), D) < REL
 
 
-@pytest.mark.parametrize("method", [0, 2])
+@pytest.mark.parametrize("method", [0, 1, 2])
 def test_affine_orthogonality_matches_golden(method):
     from yasps_amd import scene, device as dev
     from yasps_amd.energies import affine_energy
@@ -139,8 +139,11 @@ def test_affine_orthogonality_matches_golden(method):
     assert _rel(dev.to_host(att.compute().value), G["E"]) < REL
     assert _rel(dev.to_host(hs.gradient.value), G["g"].reshape(-1)) < REL
     assert _rel(hs.dense(), D) < REL
-    # the 9x9 blocks live in their own (9, 9) category; the solve runs on the generic path
+    # the 9x9 blocks live in their own (9, 9) category; block-Jacobi runs on the generic
+    # path, MAS (the default solver, SPD operators only) on the split 3x3 operator
     assert hs.static.dims == [9, 9] and hs.full is None
-    sol = s0.minimizeEnergy(tolerance=1e-20)
-    x = dev.to_host(sol[0])
-    assert np.linalg.norm(D @ x - G["g"].reshape(-1)) <= 1e-8 * np.linalg.norm(G["g"])
+    for name in ("jacobian", "mas") if method else ("jacobian",):
+        s0.minimizer.setSolver(name)
+        sol = s0.minimizeEnergy(tolerance=1e-20)
+        x = dev.to_host(sol[0])
+        assert np.linalg.norm(D @ x - G["g"].reshape(-1)) <= 1e-8 * np.linalg.norm(G["g"]), name

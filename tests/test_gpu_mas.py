@@ -57,6 +57,8 @@ def test_mas_apply_and_inverses_match_oracle(levels, coarsest, target):
             assert np.abs(got - ref).max() <= 1e-5 * np.abs(ref).max(), (l, k)
             k += 1
     w = np.asarray(omas.level_weights(h, coarsest))
+    dup = omas.duplicate_domains(h)
+    scales = [[slv.duplicate_level_weight if f else 1.0 for f in lvl] for lvl in dup]
     rng = np.random.default_rng(11)
     for _ in range(3):
         r = rng.standard_normal(hs.rows)
@@ -64,7 +66,7 @@ def test_mas_apply_and_inverses_match_oracle(levels, coarsest, target):
         z = dev.empty(hs.rows, "float64")
         m.apply(rt, z)
         got = dev.to_host(z)
-        ref = omas.apply(h, invs, r, w)
+        ref = omas.apply(h, invs, r, w, scales)
         assert np.linalg.norm(got - ref) <= APPLY_TOL * np.linalg.norm(ref)
 
 
@@ -172,3 +174,30 @@ def test_mas_pcg_initial_guess():
     x = dev.to_host(slv.solution)
     assert slv.statistics["iterations"] < it0
     assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) < 1e-8
+
+
+@pytest.mark.parametrize("levels", [1, 2])
+def test_mas_single_domain_closed_form(levels):
+    """n=1 cube, one level-0 domain: the device apply equals w A^-1 r (one level) or
+    A^-1 r + w R^T (R A R^T)^-1 R r (a second level of one coarse node, R = [I I I I]),
+    within the FP32 bank's rounding."""
+    from yasps_amd import device as dev
+    print(f"mas closed form levels={levels}", flush=True)
+    cs, s0, hs = _mas_scene(1, seed=2, max_levels=levels, coarsest_level_weight=2.5)
+    slv = s0.minimizer.linearSolver.implementation
+    slv._ensure(hs, hs.rows)
+    m = slv.device_tables
+    m.update(hs.full)
+    assert slv.hierarchy.num_levels == levels
+    A, _ = _oracle_matrix(cs)
+    Ad = A.toarray()
+    r = np.random.default_rng(3).standard_normal(hs.rows)
+    if levels == 1:
+        expect = 2.5 * np.linalg.solve(Ad, r)
+    else:
+        R = np.tile(np.eye(3), (1, hs.rows // 3))
+        expect = np.linalg.solve(Ad, r) + 2.5 * R.T @ np.linalg.solve(R @ Ad @ R.T, R @ r)
+    z = dev.empty(hs.rows, "float64")
+    m.apply(dev.to_device(r, "float64"), z)
+    got = dev.to_host(z)
+    assert np.linalg.norm(got - expect) <= 1e-5 * np.linalg.norm(expect)

@@ -27,8 +27,6 @@ def _oracle_step(cs, st, max_newton=30, max_halvings=8):
     for _ in range(max_newton):
         asm = newton.evaluate(cs, st)
         energies.append(asm.energy)
-        if np.abs(asm.gradient).max() < GRAD_TOL:
-            break
         d = spla.spsolve(newton.bsr_matrix(st, asm.values).tocsc(), asm.gradient)
         x0 = cs.position[nf:].copy()
         step = 1.0
@@ -37,6 +35,8 @@ def _oracle_step(cs, st, max_newton=30, max_halvings=8):
             if newton.evaluate(cs, st).energy <= asm.energy:
                 break
             step *= 0.5
+        if np.abs(asm.gradient).max() < GRAD_TOL:   # tested after the step, as the example
+            break
     cs.velocity[nf:] = (cs.position[nf:] - last[nf:]) / cs.dt
     return energies
 

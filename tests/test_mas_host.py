@@ -112,7 +112,7 @@ def test_mas_operator_is_spd_and_inverts_domains(cube):
     assert np.linalg.eigvalsh(0.5 * (M + M.T)).min() > 0
 
 
-def emulate_device_apply(layout, block_values, r, weights):
+def emulate_device_apply(layout, block_values, r, weights, duplicate_weight=1.0):
     """FP64 restatement of the device kernels over the layout tables."""
     D = DOMAIN_SCALARS
     b0 = layout.blocks[0]
@@ -151,7 +151,8 @@ def emulate_device_apply(layout, block_values, r, weights):
         src, dst = (r3, c0) if d < layout.level0_domains else (rc, cc)
         loc = np.zeros(D)
         loc[:3 * nodes.size] = src[nodes].reshape(-1)
-        dst[nodes] = (inv[d] @ loc)[:3 * nodes.size].reshape(-1, 3)
+        scale = duplicate_weight if layout.duplicate[d] else 1.0
+        dst[nodes] = scale * (inv[d] @ loc)[:3 * nodes.size].reshape(-1, 3)
     z = weights[0] * c0
     for l in range(layout.num_coarse_levels):
         z = z + weights[l + 1] * cc[layout.prolong_maps[l]]
@@ -249,3 +250,70 @@ def test_mas_pcg_oracle_converges(cube):
     st_m, x_m, it_m, _ = pcg.pcg(lambda v: A @ v, lambda v: omas.apply(h, invs, v, w), b, 1e-12, 500)
     assert st_m == it_m > 0                         # reference code: iterations on success
     assert np.linalg.norm(A @ x_m - b) <= 1e-5 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("levels", [1, 2])
+def test_mas_single_domain_closed_form(levels):
+    """n=1 cube (4 moving vertices, 12 scalars): one level-0 domain holding everything
+    gives z = w A^-1 r with w the coarsest weight (one level), and with a second level the
+    four nodes collapse into one coarse node: z = A^-1 r + w R^T (R A R^T)^-1 R r with
+    R = [I I I I]. A mis-weighted level or a wrong restriction fails this."""
+    sc = cube_scene(n=1, noise=0.05, seed=2)
+    st = newton.fast_structure(sc)
+    asm = newton.evaluate(sc, st, 2)
+    A = newton.bsr_matrix(st, asm.values)
+    h = _hierarchy(A, max_levels=levels)
+    assert h.num_levels == levels and len(h.levels[0].domains) == 1
+    Ad = A.toarray()
+    w = level_weights(h.num_levels, 2.5).astype(np.float64)
+    r = np.random.default_rng(3).standard_normal(Ad.shape[0])
+    if levels == 1:
+        expect = 2.5 * np.linalg.solve(Ad, r)
+    else:
+        R = np.tile(np.eye(3), (1, Ad.shape[0] // 3))
+        expect = np.linalg.solve(Ad, r) + 2.5 * R.T @ np.linalg.solve(R @ Ad @ R.T, R @ r)
+    invs = omas.domain_inverses(h, omas.level_matrices(h, Ad))
+    got = omas.apply(h, invs, r, w)
+    assert np.abs(got - expect).max() <= 1e-10 * np.abs(expect).max()
+    layout = build_layout(h, A.indptr, A.indices)
+    emu = emulate_device_apply(layout, A.data, r, w)
+    assert np.abs(emu - expect).max() <= 1e-10 * np.abs(expect).max()
+
+
+def test_duplicate_domain_rule_on_a_built_hierarchy():
+    """Level 0: domains {0, 1} and {2}; level 1: node 0 = {0, 1}, node 1 = {2}, each its
+    own domain. Level-1 domain {1} repeats level-0 domain {2} (one child, the whole
+    domain); {0} has two children. Level 2 merges both: not a duplicate."""
+    L0 = mh.Level(np.full(3, 3), mh.offsets_of(np.full(3, 3)), np.zeros(4, np.int64),
+                  np.zeros(0, np.int64), [np.array([0, 1]), np.array([2])],
+                  fine_to_parent=np.array([0, 0, 1]))
+    L1 = mh.Level(np.full(2, 3), mh.offsets_of(np.full(2, 3)), np.zeros(3, np.int64),
+                  np.zeros(0, np.int64), [np.array([0]), np.array([1])],
+                  fine_to_parent=np.array([0, 0]))
+    L2 = mh.Level(np.full(1, 3), mh.offsets_of(np.full(1, 3)), np.zeros(2, np.int64),
+                  np.zeros(0, np.int64), [np.array([0])])
+    h = mh.Hierarchy([L0, L1, L2], [np.arange(3), np.array([0, 0, 1]), np.zeros(3, np.int64)])
+    from yasps_amd.mas import duplicate_domains
+    expect = [[False, False], [False, True], [False]]
+    assert [list(map(bool, d)) for d in duplicate_domains(h)] == expect
+    assert omas.duplicate_domains(h) == expect
+
+
+@pytest.mark.parametrize("levels,target", [(6, 3), (4, 8)])
+def test_duplicate_rule_matches_oracle_and_weights_apply(cube, levels, target):
+    """The product's duplicate flags equal the oracle's restatement on cube hierarchies,
+    and the device tables with a 0.5 topology weight reproduce the oracle apply."""
+    from yasps_amd.mas import duplicate_domains
+    _, A, _ = cube
+    h = _hierarchy(A, max_levels=levels, target_nodes_per_partition=target)
+    prod = [list(map(bool, d)) for d in duplicate_domains(h)]
+    assert prod == omas.duplicate_domains(h)
+    layout = build_layout(h, A.indptr, A.indices)
+    w = level_weights(h.num_levels, 2.5).astype(np.float64)
+    mats = omas.level_matrices(h, A.toarray())
+    invs = omas.domain_inverses(h, mats)
+    scales = [[0.5 if f else 1.0 for f in lvl] for lvl in prod]
+    r = np.random.default_rng(9).standard_normal(A.shape[0])
+    ref = omas.apply(h, invs, r, w, scales)
+    got = emulate_device_apply(layout, A.data, r, w, duplicate_weight=0.5)
+    assert np.abs(got - ref).max() <= 1e-10 * np.abs(ref).max()

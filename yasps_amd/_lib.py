@@ -77,7 +77,8 @@ class MasArgs(C.Structure):
                 ("num_domains", C.c_uint32), ("level0_domains", C.c_uint32),
                 ("weights", C.c_void_p), ("coarse_residual", C.c_void_p),
                 ("fine_correction", C.c_void_p), ("coarse_correction", C.c_void_p),
-                ("inverses_bf16", C.c_void_p), ("bf16_first_domain", C.c_uint32)]
+                ("inverses_bf16", C.c_void_p), ("bf16_first_domain", C.c_uint32),
+                ("domain_scale", C.c_void_p)]
 
 
 P = C.c_void_p
@@ -116,6 +117,8 @@ SIGNATURES = {
                                  C.c_int, C.c_int, P]),
     "ysp_invert_diagonal_blocks": (C.c_int, [P, P, C.POINTER(U32), C.POINTER(U32),
                                              C.POINTER(U32), I32, P]),
+    "ysp_diagonal_from_blocks": (C.c_int, [P, P, C.POINTER(U32), C.POINTER(U32), C.POINTER(U32),
+                                           C.POINTER(U32), I32, P]),
     "ysp_pcg_solve": (C.c_int, [U32, D, I32, C.POINTER(BlockMatrix), C.POINTER(BlockMatrix), P,
                                 C.POINTER(U32), C.POINTER(U32), C.POINTER(U32), C.POINTER(U32),
                                 I32, P, U32, P, P, I32, P, P, C.POINTER(PcgStats), P]),

@@ -3,7 +3,8 @@ per-tet parameter theta_e = log E_e, and its adjoint.
 
 Equilibrium: minimise Pi(x) = sum_e vol_e psi_SNH(F_e; mu(theta_e), lam(theta_e))
 + sum_v m_v g y_v over the free vertices (bottom face fixed) by Newton with a
-backtracking line search, each step solving H d = grad Pi with the block-Jacobi PCG.
+backtracking line search, each step solving H d = grad Pi with the PCG of ``solver``
+(MAS by default, as the reference's minimizer; or block-Jacobi).
 
 Adjoint: for a loss L(x*(theta)), H(x*) lambda = dL/dx (the same PCG, FP64 or the mixed
 FP32/FP64 refinement) and dL/dtheta_e = -lambda_e . dg_e/dtheta_e, evaluated per tet by
@@ -25,7 +26,7 @@ from .scenes import cube_scene, lame_parameters
 class MaterialProblem:
     def __init__(self, n: int = 8, young: float = 1e5, poisson: float = 0.4,
                  density: float = 1000.0, gravity: float = 9.8, precision: str = "fp64",
-                 projection_method: int = 2):
+                 projection_method: int = 2, solver: str = "mas", **solver_options):
         from . import scene, stable_neo_hookean
         cs = cube_scene(n, young=young, poisson=poisson, density=density, dt=1.0, noise=0.0)
         self.cs = cs
@@ -57,7 +58,8 @@ class MaterialProblem:
         snh_att = tets.addAttribute("stable_neo_hookean", computed_attribute=self.snh)
         s0.addEnergy(snh_att, projection_method=projection_method)
         s0.addMinimizeTarget([self.x_att])
-        s0.minimizer.setSolver("jacobian", threshold_mode="relative_residual", precision=precision)
+        s0.minimizer.setSolver(solver, threshold_mode="relative_residual", precision=precision,
+                               **solver_options)
         self.scene = s0
         self.snh_att = snh_att
         # gravity potential sum_v m_v g y_v: constant gradient m_v g e_y

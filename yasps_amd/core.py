@@ -388,6 +388,7 @@ class scene(_AttributeOwner):
     def __init__(self, name: str):
         super().__init__(name, None)
         self.meshes: Dict[str, mesh] = {}
+        self._pre_targets = set()
         from .minimizer import minimizer
         self.minimizer = minimizer()
 
@@ -397,20 +398,52 @@ class scene(_AttributeOwner):
         setattr(self, name, m)
         return m
 
-    def addEnergy(self, energy: attribute, targets=(), projection_method: int = 1,
+    def addEnergy(self, e: attribute, targets=(), projection_method: int = 1,
+                  save_intermediate: bool = False, gradient_only: bool = False,
                   dynamic_instances: bool = False, separate_hessian_jacobian: bool = False,
                   grouped_add: bool = False, lto: bool = False):
-        """``scene.pyx:155``. ``separate_hessian_jacobian``, ``grouped_add`` and ``lto``
-        select reference code-generation variants; every energy here is a hand-written
-        kernel, so they are accepted and have no effect on the result."""
+        """``scene.pyx:155`` (same argument order). ``save_intermediate``,
+        ``separate_hessian_jacobian``, ``grouped_add`` and ``lto`` select reference
+        code-generation variants; every energy here is a hand-written kernel, so they are
+        accepted and do not change the result. ``gradient_only=True`` raises
+        NotImplementedError, as the reference minimizer does (``minimizer.pyx:188-189``).
+        Every attribute in ``targets`` must later be passed to ``addMinimizeTarget``
+        (``scene.pyx:165-166,190-198``)."""
+        if e.name == "":
+            raise ValueError("scene.addEnergy: energy attribute must have a name.")
+        if not isinstance(grouped_add, bool):
+            raise TypeError("scene.addEnergy: grouped_add must be bool.")
+        if not isinstance(lto, bool):
+            raise TypeError("scene.addEnergy: lto must be bool.")
         if projection_method not in (-1, 0, 1, 2):
             raise ValueError("addEnergy: projection_method must be -1, 0, 1 or 2")
-        if energy.operator != COMPUTED:
+        if e.operator != COMPUTED:
             raise TypeError("addEnergy: expects an energy attribute built from yasps_amd.energies")
-        self.minimizer.addEnergy(energy, list(targets), projection_method,
-                                 dynamic_instances or energy.isDynamic)
+        if gradient_only:
+            raise NotImplementedError("minimizer.addEnergy: gradient_only is not supported in the "
+                                      "Hessian-based minimizer yet.")
+        for t in targets:
+            self._pre_targets.add(t.fullName)
+        self.minimizer.addEnergy(e, list(targets), projection_method,
+                                 dynamic_instances or e.isDynamic)
 
     def addMinimizeTarget(self, wrt: List[attribute]):
+        """``scene.pyx:190-198``. Every target's size must be a multiple of 3: the device
+        energies act on 3-vector vertices and on the 9 + 3 segments of affine bodies, so an
+        element never has a 1-scalar compressed Hessian and the reference's N == 1
+        projection rule (``hessianKernelHeader.pyx:149-152``) has no kernel here."""
+        for t in wrt:
+            if t.size % 3:
+                raise NotImplementedError(
+                    f"scene.addMinimizeTarget: {t.fullName} has {t.size} scalars per instance; "
+                    "targets of 3-vectors (vertices) and 9 / 3 (affine bodies) are supported")
+        names = {t.fullName for t in wrt}
+        if len(names) != len(wrt):
+            raise ValueError("scene.addMinimizeTarget: target contains duplicate attributes.")
+        if not self._pre_targets.issubset(names):
+            missing = self._pre_targets - names
+            raise ValueError(f"scene.addMinimizeTarget: target is missing attributes {missing} "
+                             "that are required by the energies added.")
         self.minimizer.addWrt(wrt)
 
     def minimizeEnergy(self, tolerance: float = 1e-3, maxIterations: int = 20000):
@@ -425,6 +458,12 @@ class scene(_AttributeOwner):
     @property
     def gradientSegments(self):
         return self.minimizer.gradientSegments
+
+    @property
+    def diagonal(self):
+        """Scalar diagonal of the assembled Hessian (``scene.pyx:186-188``; the reference
+        fills it in its FullProject element kernels, ``hessianKernelFullProject.pyx:187-192``)."""
+        return self.minimizer.diagonal
 
     @property
     def hessian(self):

@@ -17,6 +17,7 @@
 
 #include <hipcub/hipcub.hpp>
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 namespace ysp {
@@ -600,11 +601,11 @@ extern "C" int ysp_bsr3_build(const uint32_t* pos_static, uint32_t nb_static,
     ysp_set_error(__FILE__, __LINE__, "ysp_bsr3_build: too many blocks");
     return YSP_ERR_ARG;
   }
-  uint64_t* keys = static_cast<uint64_t*>(scratch(40, sizeof(uint64_t) * cap));
-  uint32_t* vals = static_cast<uint32_t*>(scratch(41, sizeof(uint32_t) * cap));
-  uint64_t* keys2 = static_cast<uint64_t*>(scratch(42, sizeof(uint64_t) * cap));
-  uint32_t* count = static_cast<uint32_t*>(scratch(43, sizeof(uint32_t)));
-  uint32_t* vals2 = static_cast<uint32_t*>(scratch(45, sizeof(uint32_t) * cap));
+  uint64_t* keys = static_cast<uint64_t*>(ysp::scratch(40, sizeof(uint64_t) * cap));
+  uint32_t* vals = static_cast<uint32_t*>(ysp::scratch(41, sizeof(uint32_t) * cap));
+  uint64_t* keys2 = static_cast<uint64_t*>(ysp::scratch(42, sizeof(uint64_t) * cap));
+  uint32_t* count = static_cast<uint32_t*>(ysp::scratch(43, sizeof(uint32_t)));
+  uint32_t* vals2 = static_cast<uint32_t*>(ysp::scratch(45, sizeof(uint32_t) * cap));
   if (!keys || !vals || !keys2 || !count || !vals2) return YSP_ERR_HIP;
   if (nb_static)
     hipLaunchKernelGGL(emit_entries_kernel, dim3(grid_for(nb_static, kB)), dim3(kB), 0, s,
@@ -623,7 +624,7 @@ extern "C" int ysp_bsr3_build(const uint32_t* pos_static, uint32_t nb_static,
   const uint32_t total = static_cast<uint32_t>(cap);
   size_t temp = 0;
   YSP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, temp, kb, vb, total, 0, end_bit, s));
-  void* tmp = scratch(44, temp);
+  void* tmp = ysp::scratch(44, temp);
   if (!tmp) return YSP_ERR_HIP;
   YSP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, temp, kb, vb, total, 0, end_bit, s));
   hipLaunchKernelGGL(count_entries_kernel, dim3(1), dim3(1), 0, s, kb.Current(), total, sentinel,
@@ -749,11 +750,11 @@ extern "C" int ysp_bsr3_build_split(const ysp_block_matrix* part, const ysp_bloc
     ysp_set_error(__FILE__, __LINE__, "ysp_bsr3_build_split: too many blocks");
     return YSP_ERR_ARG;
   }
-  uint64_t* keys = static_cast<uint64_t*>(scratch(60, sizeof(uint64_t) * cap));
-  uint64_t* vals = static_cast<uint64_t*>(scratch(61, sizeof(uint64_t) * cap));
-  uint64_t* keys2 = static_cast<uint64_t*>(scratch(62, sizeof(uint64_t) * cap));
-  uint64_t* vals2 = static_cast<uint64_t*>(scratch(63, sizeof(uint64_t) * cap));
-  uint32_t* count = static_cast<uint32_t*>(scratch(64, sizeof(uint32_t)));
+  uint64_t* keys = static_cast<uint64_t*>(ysp::scratch(60, sizeof(uint64_t) * cap));
+  uint64_t* vals = static_cast<uint64_t*>(ysp::scratch(61, sizeof(uint64_t) * cap));
+  uint64_t* keys2 = static_cast<uint64_t*>(ysp::scratch(62, sizeof(uint64_t) * cap));
+  uint64_t* vals2 = static_cast<uint64_t*>(ysp::scratch(63, sizeof(uint64_t) * cap));
+  uint32_t* count = static_cast<uint32_t*>(ysp::scratch(64, sizeof(uint32_t)));
   if (!keys || !vals || !keys2 || !vals2 || !count) return YSP_ERR_HIP;
   size_t base = 0;
   for (int which = 0; which < 2; ++which) {
@@ -781,7 +782,7 @@ extern "C" int ysp_bsr3_build_split(const ysp_block_matrix* part, const ysp_bloc
   const uint32_t total = static_cast<uint32_t>(cap);
   size_t temp = 0;
   YSP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, temp, kb, vb, total, 0, end_bit, s));
-  void* tmp = scratch(65, temp);
+  void* tmp = ysp::scratch(65, temp);
   if (!tmp) return YSP_ERR_HIP;
   YSP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, temp, kb, vb, total, 0, end_bit, s));
   hipLaunchKernelGGL(count_entries_kernel, dim3(1), dim3(1), 0, s, kb.Current(), total, sentinel,
@@ -901,7 +902,7 @@ extern "C" int ysp_bsr3_refine(const uint32_t* row_ptr, const uint32_t* col, con
   }
   hipStream_t s = as_stream(stream);
   const size_t nsum = 2 * kSlots * kSpace;
-  double* sums = static_cast<double*>(scratch(33, nsum * sizeof(double)));
+  double* sums = static_cast<double*>(ysp::scratch(33, nsum * sizeof(double)));
   if (!sums) return YSP_ERR_HIP;
   YSP_CHECK(hipMemsetAsync(sums, 0, nsum * sizeof(double), s));
   if (correction)
@@ -932,18 +933,31 @@ extern "C" int ysp_bsr3_refine(const uint32_t* row_ptr, const uint32_t* col, con
 }
 
 namespace {
+__global__ void __launch_bounds__(kB) fill_zero_kernel(double* __restrict__ p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+// no_alloc: the variant behind ysp_pcg3_solve_noalloc. It allocates nothing (no pinned
+// status ring, no events, no long-row tables: every row goes through the 16-lane product)
+// and clears the scalar state with a kernel; the host waits for each chunk's status with
+// a stream synchronisation.
 template <typename VT, typename MT>
 int pcg3_run(uint32_t max_iterations, double threshold, int32_t mode, const uint32_t* row_ptr,
              const uint32_t* col, const VT* v, uint32_t nnz, const MT* minv, const MasApply* mas,
              const double* rhs, uint32_t n, double* x, const double* x0, double* work,
-             double* st, ysp_pcg_stats* stats, hipStream_t s) {
+             double* st, ysp_pcg_stats* stats, hipStream_t s, bool no_alloc = false) {
   const uint32_t nv = n / 3;
   double* r = work;
   double* z = work + n;
   double* pa = work + 2 * static_cast<size_t>(n);
   double* q = work + 4 * static_cast<size_t>(n);
   const uint32_t stride = pad64(nnz);
-  YSP_CHECK(hipMemsetAsync(st, 0, kStateDoubles * sizeof(double), s));
+  if (no_alloc)
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(grid_for(kStateDoubles, kB)), dim3(kB), 0, s, st,
+                       static_cast<uint32_t>(kStateDoubles));
+  else
+    YSP_CHECK(hipMemsetAsync(st, 0, kStateDoubles * sizeof(double), s));
   const double* ax0 = nullptr;
   if (x0) {
     launch_full_spmv(row_ptr, col, v, nnz, nv, x0, q, s);
@@ -960,12 +974,14 @@ int pcg3_run(uint32_t max_iterations, double threshold, int32_t mode, const uint
   uint32_t* long_rows = nullptr;
   uint32_t* long_slot = nullptr;
   double* ylong = nullptr;
-  if (cache.row_ptr == row_ptr && cache.nv == nv && cache.nnz == nnz) {
+  if (no_alloc) {
+    // every row through the product kernel
+  } else if (cache.row_ptr == row_ptr && cache.nv == nv && cache.nnz == nnz) {
     num_long = cache.num_long;
     if (num_long) {
-      long_rows = static_cast<uint32_t*>(scratch(67, sizeof(uint32_t) * num_long));
-      long_slot = static_cast<uint32_t*>(scratch(68, sizeof(uint32_t) * nv));
-      ylong = static_cast<double*>(scratch(69, sizeof(double) * 3 * num_long));
+      long_rows = static_cast<uint32_t*>(ysp::scratch(67, sizeof(uint32_t) * num_long));
+      long_slot = static_cast<uint32_t*>(ysp::scratch(68, sizeof(uint32_t) * nv));
+      ylong = static_cast<double*>(ysp::scratch(69, sizeof(double) * 3 * num_long));
       if (!long_rows || !long_slot || !ylong) return YSP_ERR_HIP;
     }
   } else {
@@ -981,9 +997,9 @@ int pcg3_run(uint32_t max_iterations, double threshold, int32_t mode, const uint
       num_long = static_cast<uint32_t>(longs.size());
       std::vector<uint32_t> slot(nv, kNoLongSlot);
       for (uint32_t i = 0; i < num_long; ++i) slot[longs[i]] = i;
-      long_rows = static_cast<uint32_t*>(scratch(67, sizeof(uint32_t) * num_long));
-      long_slot = static_cast<uint32_t*>(scratch(68, sizeof(uint32_t) * nv));
-      ylong = static_cast<double*>(scratch(69, sizeof(double) * 3 * num_long));
+      long_rows = static_cast<uint32_t*>(ysp::scratch(67, sizeof(uint32_t) * num_long));
+      long_slot = static_cast<uint32_t*>(ysp::scratch(68, sizeof(uint32_t) * nv));
+      ylong = static_cast<double*>(ysp::scratch(69, sizeof(double) * 3 * num_long));
       if (!long_rows || !long_slot || !ylong) return YSP_ERR_HIP;
       YSP_CHECK(hipMemcpyAsync(long_rows, longs.data(), sizeof(uint32_t) * num_long,
                                hipMemcpyHostToDevice, s));
@@ -1007,6 +1023,71 @@ int pcg3_run(uint32_t max_iterations, double threshold, int32_t mode, const uint
   hipLaunchKernelGGL(pcg3_init_finish_kernel, dim3(1), dim3(64), 0, s, st, threshold, mode,
                      (mas && !x0) ? 1 : 0);
   YSP_CHECK_LAUNCH();
+  const uint32_t end = max_iterations + 1;
+  auto launch_pass = [&](uint32_t it) -> int {
+    if (num_long)
+      hipLaunchKernelGGL(pcg3_long_rows_kernel<VT>, dim3(num_long), bl, 0, s, row_ptr, col, v,
+                         stride, long_rows, z, ylong, st);
+    hipLaunchKernelGGL(pcg3_spmv_kernel<VT>, sg, bl, 0, s, row_ptr, col, v, stride, nv, z, pa, q,
+                       st, static_cast<int>(it), rows_per_wg, long_slot, ylong);
+    if (it <= max_iterations) {
+      hipLaunchKernelGGL(pcg3_update_kernel<MT>, ug, dim3(kUB), 0, s, minv, x, r, pa, q, z, n, st,
+                         static_cast<int>(it));
+      if (mas) {
+        if (int rc = launch_mas_apply(*mas, r, n, z, st + slots_base(SET_RZ + (it % 3)),
+                                      st + S_STATUS, s))
+          return rc;
+      }
+    }
+    return YSP_OK;
+  };
+  auto finish = [&]() -> int {
+    double fin[S_HEADER];
+    YSP_CHECK(hipMemcpyAsync(fin, st, sizeof(fin), hipMemcpyDeviceToHost, s));
+    YSP_CHECK(hipStreamSynchronize(s));
+    int code;
+    const int iters = static_cast<int>(fin[S_ITER]);
+    if (fin[S_STATUS] == 1.0) code = iters;
+    else if (fin[S_STATUS] == -1.0) code = -iters - 4;
+    else if (fin[S_STATUS] == -2.0) code = -5;
+    else code = -4;
+    if (stats) {
+      stats->iterations = fin[S_STATUS] == 0.0 ? static_cast<int>(max_iterations) : iters;
+      stats->status = code;
+      stats->delta0 = fin[S_RZ0];
+      stats->delta = fin[S_FINAL_RZ];
+      stats->rr0 = fin[S_BB];
+      stats->rr = fin[S_FINAL_RR];
+    }
+    return code;
+  };
+  if (no_alloc) {
+    // chunks of passes, each followed by a synchronous read of the header; the chunk
+    // size follows the observed convergence rate as in the pipelined loop below
+    double hb[kBoxDoubles];
+    uint32_t next = 1;
+    double it_a = -1.0, val_a = 0.0;
+    uint32_t chunk = 8;
+    while (next <= end) {
+      const uint32_t last = std::min<uint32_t>(end, next + chunk - 1);
+      for (uint32_t it = next; it <= last; ++it)
+        if (int rc = launch_pass(it)) return rc;
+      YSP_CHECK_LAUNCH();
+      next = last + 1;
+      YSP_CHECK(hipMemcpyAsync(hb, st, sizeof(hb), hipMemcpyDeviceToHost, s));
+      YSP_CHECK(hipStreamSynchronize(s));
+      if (hb[S_STATUS] != 0.0) break;
+      const double it_b = hb[S_CUR_ITER];
+      const double val_b = hb[S_MODE] == 0.0 ? hb[S_FINAL_RZ] : hb[S_FINAL_RR];
+      const long more = predict_more(it_a, val_a, it_b, val_b, hb[S_THR], next - 1);
+      if (it_b > it_a) {
+        it_a = it_b;
+        val_a = val_b;
+      }
+      chunk = more < 0 ? 8u : static_cast<uint32_t>(std::min<long>(32, std::max<long>(1, more)));
+    }
+    return finish();
+  }
   double* box = status_box();
   if (!box) {
     ysp_set_error(__FILE__, __LINE__, "pinned status allocation failed");
@@ -1025,27 +1106,12 @@ int pcg3_run(uint32_t max_iterations, double threshold, int32_t mode, const uint
   // the device never waits for the host; chunk sizes shrink as the observed convergence
   // rate predicts the last iteration, which bounds the no-op passes after convergence.
   // The extra spmv pass max+1 records convergence of iteration max.
-  const uint32_t end = max_iterations + 1;
   uint32_t next = 1;
   int head = 0, pending = 0;
   auto enqueue = [&](uint32_t size) -> int {
     const uint32_t last = std::min<uint32_t>(end, next + size - 1);
-    for (uint32_t it = next; it <= last; ++it) {
-      if (num_long)
-        hipLaunchKernelGGL(pcg3_long_rows_kernel<VT>, dim3(num_long), bl, 0, s, row_ptr, col, v,
-                           stride, long_rows, z, ylong, st);
-      hipLaunchKernelGGL(pcg3_spmv_kernel<VT>, sg, bl, 0, s, row_ptr, col, v, stride, nv, z, pa, q,
-                         st, static_cast<int>(it), rows_per_wg, long_slot, ylong);
-      if (it <= max_iterations) {
-        hipLaunchKernelGGL(pcg3_update_kernel<MT>, ug, dim3(kUB), 0, s, minv, x, r, pa, q, z, n,
-                           st, static_cast<int>(it));
-        if (mas) {
-          if (int rc = launch_mas_apply(*mas, r, n, z, st + slots_base(SET_RZ + (it % 3)),
-                                        st + S_STATUS, s))
-            return rc;
-        }
-      }
-    }
+    for (uint32_t it = next; it <= last; ++it)
+      if (int rc = launch_pass(it)) return rc;
     YSP_CHECK_LAUNCH();
     const int slot = (head + pending) % kRing;
     YSP_CHECK(hipMemcpyAsync(box + slot * kBoxDoubles, st, kBoxDoubles * sizeof(double),
@@ -1085,24 +1151,7 @@ int pcg3_run(uint32_t max_iterations, double threshold, int32_t mode, const uint
     if (more > 16 && pending < 2 && next <= end)
       if (int rc = enqueue(static_cast<uint32_t>(std::min<long>(16, more - 16)))) return rc;
   }
-  double fin[S_HEADER];
-  YSP_CHECK(hipMemcpyAsync(fin, st, sizeof(fin), hipMemcpyDeviceToHost, s));
-  YSP_CHECK(hipStreamSynchronize(s));
-  int code;
-  const int iters = static_cast<int>(fin[S_ITER]);
-  if (fin[S_STATUS] == 1.0) code = iters;
-  else if (fin[S_STATUS] == -1.0) code = -iters - 4;
-  else if (fin[S_STATUS] == -2.0) code = -5;
-  else code = -4;
-  if (stats) {
-    stats->iterations = fin[S_STATUS] == 0.0 ? static_cast<int>(max_iterations) : iters;
-    stats->status = code;
-    stats->delta0 = fin[S_RZ0];
-    stats->delta = fin[S_FINAL_RZ];
-    stats->rr0 = fin[S_BB];
-    stats->rr = fin[S_FINAL_RR];
-  }
-  return code;
+  return finish();
 }
 }  // namespace
 
@@ -1267,10 +1316,10 @@ int batch_run(ysp_pcg3_system* systems, uint32_t count, uint32_t passes, hipStre
     for (uint32_t k = 0, nb = grid_for(nv, b.rows_per_wg); k < nb; ++k) wsp.push_back({i, k});
     for (uint32_t k = 0, nb = grid_for(n, kUB); k < nb; ++k) wup.push_back({i, k});
   }
-  BatchSystem* dsys = static_cast<BatchSystem*>(scratch(35, sizeof(BatchSystem) * count));
-  BatchWork* dsp = static_cast<BatchWork*>(scratch(36, sizeof(BatchWork) * wsp.size()));
-  BatchWork* dup = static_cast<BatchWork*>(scratch(37, sizeof(BatchWork) * wup.size()));
-  double* dout = static_cast<double*>(scratch(38, sizeof(double) * kBatchOut * count));
+  BatchSystem* dsys = static_cast<BatchSystem*>(ysp::scratch(35, sizeof(BatchSystem) * count));
+  BatchWork* dsp = static_cast<BatchWork*>(ysp::scratch(36, sizeof(BatchWork) * wsp.size()));
+  BatchWork* dup = static_cast<BatchWork*>(ysp::scratch(37, sizeof(BatchWork) * wup.size()));
+  double* dout = static_cast<double*>(ysp::scratch(38, sizeof(double) * kBatchOut * count));
   if (!dsys || !dsp || !dup || !dout) return YSP_ERR_HIP;
   YSP_CHECK(hipMemcpyAsync(dsys, host.data(), sizeof(BatchSystem) * count, hipMemcpyHostToDevice, s));
   YSP_CHECK(hipMemcpyAsync(dsp, wsp.data(), sizeof(BatchWork) * wsp.size(), hipMemcpyHostToDevice, s));
@@ -1355,3 +1404,4 @@ extern "C" int ysp_pcg3_batch_run(ysp_pcg3_system* systems, uint32_t count, uint
   return inverse_f32 ? batch_run<double, float>(systems, count, passes, s)
                      : batch_run<double, double>(systems, count, passes, s);
 }
+

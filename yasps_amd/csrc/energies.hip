@@ -52,6 +52,78 @@ YSP_ENERGY_KIND(YSP_ENERGY_SNH, snh, 4, 3, 10, 3)
 YSP_ENERGY_KIND(YSP_ENERGY_INERTIA, inertia, 1, 3, 7, 1)
 #undef YSP_ENERGY_KIND
 
+// Quadratic form of a linear map of the element's vertex data, y = J x + c with J 6 x 12:
+// E = y^T Q y / 2 + l^T y, g = J^T (Q y + l), H = J^T spd(Q) J (the projection acts on the
+// inner Hessian: the map is linear). Scene parameters are read from memory in place
+// (element_eval), not copied per element.
+constexpr int kQfRows = 6;
+template <int N>
+__device__ __forceinline__ void project_packed(double (&H)[N * (N + 1) / 2], uint32_t method);
+
+template <>
+struct Kind<YSP_ENERGY_QUADRATIC_FORM> {
+  static constexpr int A = 4, W = 3, N = 12, P = 0, S = kQfRows * 12 + kQfRows * kQfRows + 2 * kQfRows;
+};
+
+__device__ __forceinline__ void quadratic_form_eval(const double (&x)[12], const double* __restrict__ sp,
+                                                    uint32_t method, double& E, double (&g)[12],
+                                                    double (&H)[78]) {
+  const double* J = sp;
+  const double* Q = sp + kQfRows * 12;
+  const double* lin = Q + kQfRows * kQfRows;
+  const double* off = lin + kQfRows;
+  double y[kQfRows], qy[kQfRows];
+#pragma unroll
+  for (int i = 0; i < kQfRows; ++i) {
+    double acc = off[i];
+#pragma unroll
+    for (int j = 0; j < 12; ++j) acc += J[12 * i + j] * x[j];
+    y[i] = acc;
+  }
+  E = 0.0;
+#pragma unroll
+  for (int i = 0; i < kQfRows; ++i) {
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < kQfRows; ++k) acc += Q[kQfRows * i + k] * y[k];
+    qy[i] = acc + lin[i];
+    E += y[i] * (0.5 * acc + lin[i]);
+  }
+#pragma unroll
+  for (int j = 0; j < 12; ++j) {
+    double acc = 0.0;
+#pragma unroll
+    for (int i = 0; i < kQfRows; ++i) acc += J[12 * i + j] * qy[i];
+    g[j] = acc;
+  }
+  double Qp[kQfRows * (kQfRows + 1) / 2];
+#pragma unroll
+  for (int i = 0; i < kQfRows; ++i)
+#pragma unroll
+    for (int k = i; k < kQfRows; ++k) Qp[upper_index<kQfRows>(i, k)] = Q[kQfRows * i + k];
+  if (method != 0) project_packed<kQfRows>(Qp, method);
+  double JQ[kQfRows][12];                       // spd(Q) J
+#pragma unroll
+  for (int i = 0; i < kQfRows; ++i)
+#pragma unroll
+    for (int j = 0; j < 12; ++j) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < kQfRows; ++k)
+        acc += Qp[i <= k ? upper_index<kQfRows>(i, k) : upper_index<kQfRows>(k, i)] * J[12 * k + j];
+      JQ[i][j] = acc;
+    }
+#pragma unroll
+  for (int a = 0; a < 12; ++a)
+#pragma unroll
+    for (int b = a; b < 12; ++b) {
+      double acc = 0.0;
+#pragma unroll
+      for (int i = 0; i < kQfRows; ++i) acc += J[12 * i + a] * JQ[i][b];
+      H[upper_index<12>(a, b)] = acc;
+    }
+}
+
 struct AffineView {
   uint32_t mask;                                  // bit c: union child c is affine
   const uint32_t* body[YSP_MAX_UNION_CHILDREN];   // vertex -> body
@@ -395,6 +467,22 @@ __device__ __forceinline__ void scene_params(const double* sp, const double* x, 
   }
 }
 
+// Energy, gradient and (projected) packed Hessian of one element.
+template <int ID>
+__device__ __forceinline__ void element_eval(const double (&x)[Kind<ID>::N], const double* p,
+                                             const double* sp, uint32_t method, double& E,
+                                             double (&g)[Kind<ID>::N],
+                                             double (&H)[Kind<ID>::N * (Kind<ID>::N + 1) / 2]) {
+  if constexpr (ID == YSP_ENERGY_QUADRATIC_FORM) {
+    quadratic_form_eval(x, sp, method, E, g, H);
+  } else {
+    double s[Kind<ID>::S];
+    scene_params<ID>(sp, x, p, s);
+    Kind<ID>::eval(x, p, s, E, g, H);
+    if (method != 0) project_element<ID>(H, method);
+  }
+}
+
 // One element through the general slot bookkeeping (any slot sizes, affine vertices).
 template <int ID, int KT>
 __device__ __forceinline__ void eval_element_general(const UnionView& P, const AffineView& AF,
@@ -443,11 +531,8 @@ __device__ __forceinline__ void eval_element_general(const UnionView& P, const A
   double x[T::N];
   gather<ID>(P, e2v, e, x);
   const double* p = ep + static_cast<size_t>(e) * T::P;
-  double s[T::S];
-  scene_params<ID>(sp, x, p, s);
   double E, g[T::N], H[T::N * (T::N + 1) / 2];
-  T::eval(x, p, s, E, g, H);
-  if (method != 0) project_element<ID>(H, method);
+  element_eval<ID>(x, p, sp, method, E, g, H);
   scatter_generic<KT, T::N>(SA, e, si, r, g, H);
 }
 
@@ -628,11 +713,8 @@ __global__ void __launch_bounds__(kEB)
   double x[T::N];
   gather<ID>(P, e2v, e, x);
   const double* p = ep + static_cast<size_t>(e) * T::P;
-  double s[T::S];
-  scene_params<ID>(sp, x, p, s);
   double E, g[T::N], H[T::N * (T::N + 1) / 2];
-  T::eval(x, p, s, E, g, H);
-  if (method != 0) project_element<ID>(H, method);
+  element_eval<ID>(x, p, sp, method, E, g, H);
   scatter_vertex<T::A, T::W, T::N>(SA, e, vs, g, H);
 }
 
@@ -647,10 +729,8 @@ __global__ void __launch_bounds__(kEB)
   double x[T::N];
   gather<ID>(P, e2v, e, x);
   const double* p = ep + static_cast<size_t>(e) * T::P;
-  double s[T::S];
-  scene_params<ID>(sp, x, p, s);
   double E, g[T::N], H[T::N * (T::N + 1) / 2];
-  T::eval(x, p, s, E, g, H);
+  element_eval<ID>(x, p, sp, 0, E, g, H);
   energy[e] = E;
 }
 
@@ -799,6 +879,8 @@ int dispatch_kind(int32_t kind, Fn&& fn) {
       return fn(std::integral_constant<int, YSP_ENERGY_AFFINE_ORTHOGONALITY>());
     case YSP_ENERGY_SNH: return fn(std::integral_constant<int, YSP_ENERGY_SNH>());
     case YSP_ENERGY_INERTIA: return fn(std::integral_constant<int, YSP_ENERGY_INERTIA>());
+    case YSP_ENERGY_QUADRATIC_FORM:
+      return fn(std::integral_constant<int, YSP_ENERGY_QUADRATIC_FORM>());
     default: return -1;
   }
 }

@@ -253,7 +253,8 @@ __device__ __forceinline__ void local_solve_domain(
     const void* __restrict__ bank, const uint32_t* __restrict__ dom_nodes,
     const uint32_t* __restrict__ dom_node_off, uint32_t d, size_t row,
     const double* __restrict__ r0, const double* __restrict__ rc, uint32_t level0_domains,
-    float* __restrict__ c0, float* __restrict__ cc, float* __restrict__ rs, int lane) {
+    const float* __restrict__ dscale, float* __restrict__ c0, float* __restrict__ cc,
+    float* __restrict__ rs, int lane) {
   const uint32_t n0 = dom_node_off[d], n1 = dom_node_off[d + 1];
   const uint32_t P = 3 * (n1 - n0);
   float rl = 0.0f;
@@ -293,6 +294,7 @@ __device__ __forceinline__ void local_solve_domain(
       acc = fmaf(b.w, rq.w, acc);
     }
   }
+  if (dscale) acc *= dscale[d];
   if (static_cast<uint32_t>(lane) < P) {
     const uint32_t s = 3 * node + lane % 3;
     if (d < level0_domains) c0[s] = acc;
@@ -307,8 +309,9 @@ __global__ void __launch_bounds__(kB)
                        uint32_t bf16_first, const uint32_t* __restrict__ dom_nodes,
                        const uint32_t* __restrict__ dom_node_off, uint32_t ndom,
                        const double* __restrict__ r0, const double* __restrict__ rc,
-                       uint32_t level0_domains, float* __restrict__ c0,
-                       float* __restrict__ cc, const double* __restrict__ stop) {
+                       uint32_t level0_domains, const float* __restrict__ dscale,
+                       float* __restrict__ c0, float* __restrict__ cc,
+                       const double* __restrict__ stop) {
   if (mas_stopped(stop)) return;
   constexpr uint32_t kPer = kB / kD;
   __shared__ __align__(16) float rs[kPer][kD];
@@ -318,13 +321,13 @@ __global__ void __launch_bounds__(kB)
     const uint32_t d = blockIdx.x * kPer + g;
     if (d >= bf16_first) return;
     local_solve_domain<false>(inv, dom_nodes, dom_node_off, d, static_cast<size_t>(d) * kD + lane,
-                              r0, rc, level0_domains, c0, cc, rs[g], lane);
+                              r0, rc, level0_domains, dscale, c0, cc, rs[g], lane);
   } else {
     const uint32_t d = bf16_first + (blockIdx.x - nb32) * kPer + g;
     if (d >= ndom) return;
     local_solve_domain<true>(inv_bf16, dom_nodes, dom_node_off, d,
                              static_cast<size_t>(d - bf16_first) * kD + lane, r0, rc,
-                             level0_domains, c0, cc, rs[g], lane);
+                             level0_domains, dscale, c0, cc, rs[g], lane);
   }
 }
 
@@ -440,7 +443,8 @@ int launch_mas_apply(const MasApply& m, const double* r, uint32_t n, double* z,
                        dim3(grid_for(first, kB / kD) + grid_for(m.num_domains - first, kB / kD)),
                        dim3(kB), 0, s, m.inverses, m.inverses_bf16, first, m.domain_nodes,
                        m.domain_node_offsets, m.num_domains, r, m.coarse_residual,
-                       m.level0_domains, m.fine_correction, m.coarse_correction, stop);
+                       m.level0_domains, m.domain_scale, m.fine_correction, m.coarse_correction,
+                       stop);
   }
   hipLaunchKernelGGL(prolong_kernel, dim3(grid_for(nv, kB)), dim3(kB), 0, s, m.fine_correction,
                      m.coarse_correction, m.prolong_maps, nv, m.num_coarse_levels, m.weights, r, z,

@@ -27,9 +27,11 @@ struct MasApply {
   float* coarse_correction;           // [3 x num_coarse_nodes]
   const uint16_t* inverses_bf16;      // nullptr, or BF16 inverses of domains >= bf16_first
   uint32_t bf16_first_domain;
+  const float* domain_scale;          // nullptr, or a factor per domain on its correction
 };
 static_assert(sizeof(MasApply) == sizeof(ysp_mas_args) &&
-                  offsetof(MasApply, bf16_first_domain) == offsetof(ysp_mas_args, bf16_first_domain),
+                  offsetof(MasApply, bf16_first_domain) == offsetof(ysp_mas_args, bf16_first_domain) &&
+                  offsetof(MasApply, domain_scale) == offsetof(ysp_mas_args, domain_scale),
               "MasApply mirrors ysp_mas_args");
 
 // BF16 (round to nearest even) and back.

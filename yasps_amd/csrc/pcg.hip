@@ -446,6 +446,37 @@ double* status_mailbox() {
 
 using namespace ysp;
 
+namespace ysp {
+namespace {
+// diagonal[g0 + i * sz + k] = blocks[d0 + i * sz * sz + k * (sz + 1)] for one target.
+__global__ void __launch_bounds__(256)
+    block_diagonal_kernel(const double* __restrict__ blocks, double* __restrict__ diagonal,
+                          uint32_t count, uint32_t size) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count * size) return;
+  const uint32_t i = t / size, k = t % size;
+  diagonal[t] = blocks[static_cast<size_t>(i) * size * size + k * (size + 1)];
+}
+}  // namespace
+}  // namespace ysp
+
+extern "C" int ysp_diagonal_from_blocks(const double* diagonal_blocks, double* diagonal,
+                                        const uint32_t* starts, const uint32_t* counts,
+                                        const uint32_t* sizes,
+                                        const uint32_t* gradient_segments_start,
+                                        int32_t num_attributes, void* stream) {
+  hipStream_t s = as_stream(stream);
+  for (int a = 0; a < num_attributes; ++a) {
+    const uint32_t total = counts[a] * sizes[a];
+    if (total == 0) continue;
+    hipLaunchKernelGGL(ysp::block_diagonal_kernel, dim3(grid_for(total, 256)), dim3(256), 0, s,
+                       diagonal_blocks + starts[a], diagonal + gradient_segments_start[a],
+                       counts[a], sizes[a]);
+  }
+  YSP_CHECK_LAUNCH();
+  return YSP_OK;
+}
+
 extern "C" int ysp_invert_diagonal_blocks(const double* diagonal_blocks, double* inverse,
                                           const uint32_t* starts, const uint32_t* counts,
                                           const uint32_t* sizes, int32_t num_attributes,

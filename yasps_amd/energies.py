@@ -332,7 +332,7 @@ def inertia(x_before, vel, dt, x, mass) -> Inertia:
 # ---------------------------------------------------------------- generated energies
 KIND_IDS = {"point_point": 0, "point_edge": 1, "point_triangle": 2, "edge_edge": 3,
             "bending": 4, "baraff_witkin": 5, "stvk_membrane": 6, "affine_orthogonality": 7,
-            "snh": 8, "inertia": 9}
+            "snh": 8, "inertia": 9, "quadratic_form": 10}
 
 
 class ElementEnergy(EnergyExpr):
@@ -523,3 +523,35 @@ def stvk_membrane(x_init, x, mu, lam, thickness, dt) -> ElementEnergy:
 def affine_energy(rotation, stiffness=1.0) -> AffineOrthogonality:
     """``examples/one_bunny_partial_abd/helpers.py:132-139``."""
     return AffineOrthogonality(rotation, stiffness)
+
+
+QUADRATIC_FORM_ROWS = 6
+
+
+def quadratic_form(position, jacobian, inner_hessian, linear=None, offset=None) -> ElementEnergy:
+    """E = y^T Q y / 2 + l^T y on y = J x + c, x the 12 coordinates of a 4-vertex element:
+    the energy the reference's sparsity tests build symbolically from a linear map and a
+    quadratic (``tests/differentiator/test_hessian_sparsity_permutation_gpu.py:17-38``).
+    The projection acts on Q (the map is linear: ``differentiator.pyx:795-801``), so the
+    element Hessian is J^T spd(Q) J. J is m x 12 with 1 <= m <= 6."""
+    J = np.asarray(jacobian, dtype=np.float64)
+    if J.ndim != 2 or J.shape[1] != 12 or not 1 <= J.shape[0] <= QUADRATIC_FORM_ROWS:
+        raise ValueError("quadratic_form: jacobian must be m x 12 with 1 <= m <= 6")
+    m = J.shape[0]
+    Q = np.asarray(inner_hessian, dtype=np.float64)
+    if Q.shape != (m, m) or not np.allclose(Q, Q.T, rtol=0, atol=0):
+        raise ValueError("quadratic_form: inner_hessian must be a symmetric m x m matrix")
+    lin = np.zeros(m) if linear is None else np.asarray(linear, dtype=np.float64).reshape(-1)
+    off = np.zeros(m) if offset is None else np.asarray(offset, dtype=np.float64).reshape(-1)
+    if lin.size != m or off.size != m:
+        raise ValueError("quadratic_form: linear and offset must have m entries")
+    R = QUADRATIC_FORM_ROWS
+    Jp = np.zeros((R, 12))
+    Jp[:m] = J
+    Qp = np.eye(R)
+    Qp[:m, :m] = Q
+    lp, cp = np.zeros(R), np.zeros(R)
+    lp[:m], cp[:m] = lin, off
+    params = np.concatenate([Jp.ravel(), Qp.ravel(), lp, cp])
+    return ElementEnergy("quadratic_form", position, [float(v) for v in params])
+

@@ -374,6 +374,23 @@ class hessian:
     def block_dimensions_dynamic(self):
         return self.dynamic.dims
 
+    @property
+    def diagonal(self):
+        """Scalar diagonal of the assembled Hessian (``hessian.pyx:188``), taken from the
+        diagonal blocks every assembly fills."""
+        if not self._setup:
+            return dev.zeros(0, "float64")
+        out = dev.zeros(self.rows, "float64")
+        lib = _lib.load()
+        sizes = [w.size for w in self.wrt]
+        counts = [w.numInstances for w in self.wrt]
+        _lib.check(lib.ysp_diagonal_from_blocks(
+            dev.ptr(self.diagonal_blocks), dev.ptr(out),
+            _lib.u32_array(self.diagonal_blocks_start_cpu[:-1]), _lib.u32_array(counts),
+            _lib.u32_array(sizes), _lib.u32_array(self.gradient_segments_start_cpu[:-1]),
+            len(self.wrt), dev.stream()), "ysp_diagonal_from_blocks")
+        return out
+
     # --------------------------------------------------------------- numeric
     def _scatter_args(self, ti: TermIndex, part: Part) -> _lib.ScatterArgs:
         return _lib.ScatterArgs(dev.ptr(ti.indices), dev.ptr(ti.sizes), dev.ptr(ti.permutations),

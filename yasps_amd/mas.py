@@ -61,6 +61,8 @@ class MasLayout:
     domain_sizes: np.ndarray             # scalars per domain
     level0_domains: int
     weights: np.ndarray = field(default=None)
+    duplicate: np.ndarray = field(default=None)   # per domain: repeats a previous-level domain
+    weights_base: np.ndarray = field(default=None)   # configured level weights
 
     @property
     def num_domains(self) -> int:
@@ -108,6 +110,34 @@ def static_pairs(row_ptr: np.ndarray, col: np.ndarray, src: np.ndarray,
     keep = ((src.astype(np.int64) >> dynamic_bit) & 1) == 0
     keep &= rows != col
     return np.stack([rows[keep], col.astype(np.int64)[keep]], 1)
+
+
+def duplicate_domains(h: Hierarchy) -> List[np.ndarray]:
+    """Per level, per domain: True when the domain only repeats a domain of the previous
+    level, i.e. each of its nodes has exactly one child and the children are exactly the
+    nodes of one previous-level domain (the topology rule of
+    ``solver/mas/cuda_runtime.py:1285-1340``; such a domain's correction is weighted by
+    ``duplicate_level_weight``). Level 0 has none."""
+    out = [np.zeros(len(h.levels[0].domains), dtype=bool)]
+    for l in range(1, h.num_levels):
+        prev, cur = h.levels[l - 1], h.levels[l]
+        mapping = np.asarray(prev.fine_to_parent, dtype=np.int64)
+        nchild = np.bincount(mapping, minlength=cur.num_nodes)
+        only_child = np.full(cur.num_nodes, -1, dtype=np.int64)
+        only_child[mapping] = np.arange(mapping.size)
+        prev_dom = np.full(prev.num_nodes, -1, dtype=np.int64)
+        for k, d in enumerate(prev.domains):
+            prev_dom[d] = k
+        dup = np.zeros(len(cur.domains), dtype=bool)
+        for k, d in enumerate(cur.domains):
+            if np.any(nchild[d] != 1):
+                continue
+            kids = only_child[d]
+            src = prev_dom[kids[0]]
+            if np.all(prev_dom[kids] == src) and set(kids.tolist()) == set(prev.domains[src].tolist()):
+                dup[k] = True
+        out.append(dup)
+    return out
 
 
 def build_layout(h: Hierarchy, row_ptr: np.ndarray, col: np.ndarray) -> MasLayout:
@@ -194,13 +224,15 @@ def build_layout(h: Hierarchy, row_ptr: np.ndarray, col: np.ndarray) -> MasLayou
         b.dest = np.where(same, dest, np.int64(NO_DEST)).astype(np.uint32)
     return MasLayout(h, blocks, level_offsets, restrict_offsets, restrict_items, prolong,
                      np.concatenate(dom_nodes), np.asarray(dom_off, dtype=np.int64),
-                     np.asarray(dom_sizes, dtype=np.int64), len(h.levels[0].domains))
+                     np.asarray(dom_sizes, dtype=np.int64), len(h.levels[0].domains),
+                     duplicate=np.concatenate(duplicate_domains(h)))
 
 
 class _DeviceMas:
     """Device copies of a layout plus the per-solve numeric buffers."""
 
-    def __init__(self, layout: MasLayout, weights: np.ndarray, n: int, coarse_bank: str = "fp32"):
+    def __init__(self, layout: MasLayout, weights: np.ndarray, n: int, coarse_bank: str = "fp32",
+                 duplicate_level_weight: float = 1.0):
         self.layout = layout
         u32 = lambda a: dev.to_device(np.asarray(a, dtype=np.uint32), "uint32")   # noqa: E731
         self.dest = [u32(b.dest) for b in layout.blocks]
@@ -225,6 +257,11 @@ class _DeviceMas:
         self.inverses_bf16 = dev.empty(nb16 * D * D, "int16") if nb16 else None
         self.status = dev.zeros(1, "int32")
         self.weights = dev.to_device(weights.astype(np.float32), "float32")
+        self.weights_host = weights.astype(np.float32).copy()
+        self.domain_scale = None
+        if duplicate_level_weight != 1.0 and layout.duplicate is not None and layout.duplicate.any():
+            scale = np.where(layout.duplicate, duplicate_level_weight, 1.0).astype(np.float32)
+            self.domain_scale = dev.to_device(scale, "float32")
         self.coarse_residual = dev.empty(max(1, 3 * nc), "float64")
         self.coarse_correction = dev.empty(max(1, 3 * nc), "float32")
         self.fine_correction = dev.empty(n, "float32")
@@ -247,7 +284,15 @@ class _DeviceMas:
         a.coarse_correction = dev.ptr(self.coarse_correction)
         a.inverses_bf16 = dev.ptr(self.inverses_bf16)
         a.bf16_first_domain = self.bf16_first
+        a.domain_scale = dev.ptr(self.domain_scale)
         self.args = a
+
+    def set_fine_weight(self, w0: float):
+        """Level-0 weight of the apply (the adaptive fine weight changes with the dynamic
+        block count)."""
+        if self.weights_host[0] != np.float32(w0):
+            self.weights_host[0] = np.float32(w0)
+            self.weights.copy_(dev.to_device(self.weights_host, "float32"))
 
     def update(self, full):
         """Coarse operators, domain matrices and their inverses from the full block-CSR's
@@ -291,51 +336,125 @@ class _DeviceMas:
 class masSolver:
     """MAS-preconditioned CG on the device (``solver/masSolver.py:11-133``).
 
-    Options (reference names): ``max_domain_dofs`` (<= 32), ``max_levels``,
-    ``target_nodes_per_partition``, ``coarsest_level_weight``, ``level_weights``,
-    ``global_coarse_domain``; extensions ``threshold_mode`` as for ``jacobianPCGSolver`` and
-    ``precision="mixed"`` (FP32 block values in the Krylov loop, FP64 iterative refinement
-    on the true relative residual) and ``coarse_bank="bf16"`` (the coarse levels' domain
-    inverses stored BF16, level 0 FP32; halves the coarse banks' bytes in every apply).
-    Returns 0 when converged and -4 otherwise, as the reference.
+    Takes the options of the reference's ``MASSolver`` (``solver/mas/solver.py:89-120``)
+    by name, with its validation:
+
+    * hierarchy: ``max_domain_dofs`` (at most 32 scalars on this device path),
+      ``max_levels``, ``target_nodes_per_partition``, ``minimum_domains_for_next_level``,
+      ``global_coarse_domain``, ``domain_dof_schedule``, ``target_node_schedule``;
+    * weights: ``coarsest_level_weight``, ``level_weights``, ``duplicate_level_weight`` (a
+      domain that repeats a previous-level domain), ``adaptive_fine_level_weight`` (default:
+      on unless ``level_weights`` is given), ``dense_collision_fine_level_weight`` and
+      ``dense_collision_block_ratio`` (the fine weight is raised to the former when the
+      dynamic blocks per fine node reach the latter);
+    * numeric updates: ``numeric_rebuild_interval`` / ``dynamic_numeric_rebuild_interval``
+      (the domain inverses are rebuilt every k-th solve of an unchanged structure, lagged
+      in between as ``solver/mas/solver.py:526-568``);
+    * ``mixed_spmv=True`` selects the mixed-precision solve below;
+    * the CUDA launch tuning (``cuda_threads_per_block``, ``cuda_fixed_inverse_bucket_size``,
+      ``cuda_enable_inverse_fallback``) and ``fine_correction_iterations`` (used only by the
+      reduced-then-fine mode) are accepted and have no effect on the device kernels here;
+    * ``solve_mode="reduced_then_fine"``, ``dynamic_edge_domains``,
+      ``collision_aware_reorder``, ``collision_merge_across_types``, another
+      ``inverse_backend`` and ``allow_cpu_fallback`` raise NotImplementedError.
+
+    Extensions: ``threshold_mode`` as for ``jacobianPCGSolver``, ``precision="mixed"`` (FP32
+    block values in the Krylov loop, FP64 iterative refinement on the true relative
+    residual) and ``coarse_bank="bf16"`` (the coarse levels' domain inverses stored BF16,
+    level 0 FP32). Returns 0 when converged and -4 otherwise, as the reference.
     """
 
     def __init__(self, max_domain_dofs=None, max_levels=None, target_nodes_per_partition=None,
+                 fine_correction_iterations: int = 8, solve_mode: str = "mas_pcg",
+                 inverse_backend: str = "cooperative_gauss_jordan", *,
+                 minimum_domains_for_next_level: int = 0, cuda_threads_per_block: int = 96,
+                 cuda_fixed_inverse_bucket_size=None, dynamic_edge_domains: bool = False,
+                 dynamic_edge_block_ratio: float = 0.0, collision_aware_reorder: bool = False,
+                 collision_merge_across_types: bool = False,
+                 duplicate_level_weight: float = 0.5, adaptive_fine_level_weight=None,
+                 dense_collision_fine_level_weight: float = 3.0,
+                 dense_collision_block_ratio: float = 0.1,
                  coarsest_level_weight: float = 2.5, level_weights=None,
-                 global_coarse_domain: bool = True, threshold_mode: str = "reference",
-                 inverse_backend: str = "cooperative_gauss_jordan",
-                 allow_cpu_fallback: bool = False, precision: str = "fp64",
-                 coarse_bank: str = "fp32"):
-        if any(v is not None and v <= 0 for v in (max_domain_dofs, max_levels,
-                                                  target_nodes_per_partition)):
+                 cuda_enable_inverse_fallback: bool = True, mixed_spmv: bool = False,
+                 mixed_spmv_tolerance_factor: float = 1.0, global_coarse_domain: bool = True,
+                 domain_dof_schedule=None, target_node_schedule=None,
+                 numeric_rebuild_interval: int = 1, dynamic_numeric_rebuild_interval: int = 1,
+                 allow_cpu_fallback: bool = False, threshold_mode: str = "reference",
+                 precision: str = "fp64", coarse_bank: str = "fp32"):
+        if (any(v is not None and v <= 0 for v in (max_domain_dofs, max_levels,
+                                                   target_nodes_per_partition))
+                or fine_correction_iterations < 0):
             raise ValueError("domain capacity/levels must be positive and correction iterations non-negative")
-        if max_domain_dofs is not None and max_domain_dofs > DOMAIN_SCALARS:
-            raise ValueError(f"max_domain_dofs is limited to {DOMAIN_SCALARS} scalars on this device path")
-        if max_levels is not None and max_levels > _lib.MAS_MAX_COARSE_LEVELS + 1:
-            raise ValueError(f"max_levels is limited to {_lib.MAS_MAX_COARSE_LEVELS + 1}")
+        if int(minimum_domains_for_next_level) < 0:
+            raise ValueError("minimum_domains_for_next_level must be non-negative")
+        if cuda_fixed_inverse_bucket_size is not None and cuda_fixed_inverse_bucket_size <= 0:
+            raise ValueError("cuda_fixed_inverse_bucket_size must be positive")
+        if solve_mode not in ("mas_pcg", "reduced_then_fine"):
+            raise ValueError("solve_mode must be 'mas_pcg' or 'reduced_then_fine'")
+        if not np.isfinite(dynamic_edge_block_ratio) or dynamic_edge_block_ratio < 0.0:
+            raise ValueError("dynamic_edge_block_ratio must be finite and non-negative")
+        if not np.isfinite(duplicate_level_weight) or not 0.0 <= duplicate_level_weight <= 1.0:
+            raise ValueError("duplicate_level_weight must be between zero and one")
+        if not np.isfinite(dense_collision_fine_level_weight) or dense_collision_fine_level_weight <= 0:
+            raise ValueError("dense collision fine-level weight must be positive")
+        if not np.isfinite(dense_collision_block_ratio) or dense_collision_block_ratio <= 0:
+            raise ValueError("dense collision block ratio must be positive")
         if not np.isfinite(coarsest_level_weight) or coarsest_level_weight <= 0:
             raise ValueError("coarsest_level_weight must be finite and positive")
         if level_weights is not None and (not level_weights or any(
                 not np.isfinite(w) or w <= 0 for w in level_weights)):
             raise ValueError("level_weights must contain finite positive values")
+        if not np.isfinite(mixed_spmv_tolerance_factor) or not 0.0 < mixed_spmv_tolerance_factor <= 1.0:
+            raise ValueError("mixed_spmv_tolerance_factor must be in (0, 1]")
+        for sched, what in ((domain_dof_schedule, "domain DOF"), (target_node_schedule, "target-node")):
+            if sched is not None and (not sched or any(int(v) <= 0 for v in sched)):
+                raise ValueError(f"{what} schedule must contain positive values")
+        if int(numeric_rebuild_interval) <= 0:
+            raise ValueError("numeric rebuild interval must be positive")
+        if int(dynamic_numeric_rebuild_interval) <= 0:
+            raise ValueError("dynamic numeric rebuild interval must be positive")
         if threshold_mode not in _MODES:
             raise ValueError(f"threshold_mode must be one of {sorted(_MODES)}")
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         if coarse_bank not in ("fp32", "bf16"):
             raise ValueError("coarse_bank must be 'fp32' or 'bf16'")
-        if inverse_backend != "cooperative_gauss_jordan":
-            raise ValueError("only the cooperative_gauss_jordan inverse backend exists on this device path")
-        if allow_cpu_fallback:
-            raise ValueError("there is no CPU fallback on this device path")
+        # device-path limits, named
+        if max_domain_dofs is not None and max_domain_dofs > DOMAIN_SCALARS:
+            raise ValueError(f"max_domain_dofs is limited to {DOMAIN_SCALARS} scalars on this device path")
+        if domain_dof_schedule is not None and max(int(v) for v in domain_dof_schedule) > DOMAIN_SCALARS:
+            raise ValueError(f"domain_dof_schedule entries are limited to {DOMAIN_SCALARS} scalars")
+        if max_levels is not None and max_levels > _lib.MAS_MAX_COARSE_LEVELS + 1:
+            raise ValueError(f"max_levels is limited to {_lib.MAS_MAX_COARSE_LEVELS + 1}")
+        unsupported = [name for name, bad in (
+            ("solve_mode='reduced_then_fine'", solve_mode != "mas_pcg"),
+            ("dynamic_edge_domains", bool(dynamic_edge_domains)),
+            ("collision_aware_reorder", bool(collision_aware_reorder)),
+            ("collision_merge_across_types", bool(collision_merge_across_types)),
+            (f"inverse_backend={inverse_backend!r}", inverse_backend != "cooperative_gauss_jordan"),
+            ("allow_cpu_fallback (there is no CPU fallback)", bool(allow_cpu_fallback))) if bad]
+        if unsupported:
+            raise NotImplementedError("masSolver: not on this device path: " + ", ".join(unsupported))
         self.max_domain_dofs = int(max_domain_dofs or DOMAIN_SCALARS)
         self.max_levels = int(max_levels or 6)
         self.target_nodes_per_partition = int(target_nodes_per_partition or 8)
+        self.minimum_domains_for_next_level = int(minimum_domains_for_next_level)
+        self.domain_dof_schedule = None if domain_dof_schedule is None else tuple(int(v) for v in domain_dof_schedule)
+        self.target_node_schedule = None if target_node_schedule is None else tuple(int(v) for v in target_node_schedule)
         self.coarsest_level_weight = float(coarsest_level_weight)
         self.level_weights = None if level_weights is None else tuple(float(w) for w in level_weights)
+        self.duplicate_level_weight = float(duplicate_level_weight)
+        self.adaptive_fine_level_weight = (level_weights is None if adaptive_fine_level_weight is None
+                                           else bool(adaptive_fine_level_weight))
+        self.dense_collision_fine_level_weight = float(dense_collision_fine_level_weight)
+        self.dense_collision_block_ratio = float(dense_collision_block_ratio)
         self.global_coarse_domain = bool(global_coarse_domain)
+        self.numeric_rebuild_interval = int(numeric_rebuild_interval)
+        self.dynamic_numeric_rebuild_interval = int(dynamic_numeric_rebuild_interval)
+        self.fine_correction_iterations = int(fine_correction_iterations)
+        self.mixed_spmv_tolerance_factor = float(mixed_spmv_tolerance_factor)
         self.threshold_mode = threshold_mode
-        self.precision = precision
+        self.precision = "mixed" if mixed_spmv else precision
         self.coarse_bank = coarse_bank
         self._refine = RefinementBuffers()
         self.reset()
@@ -351,6 +470,8 @@ class masSolver:
         self._statistics = {}
         self.hierarchy_build_count = 0
         self.iterations = 0
+        self._numeric_age = None
+        self._numeric_dyn = 0
 
     @property
     def solution(self):
@@ -388,14 +509,19 @@ class masSolver:
                 self._hierarchy = build_hierarchy(
                     np.full(f.nbr, 3), static_pairs(row_ptr, col, src, 43 if split else 30),
                     self.max_domain_dofs,
-                    self.max_levels, self.target_nodes_per_partition, self.global_coarse_domain)
+                    self.max_levels, self.target_nodes_per_partition, self.global_coarse_domain,
+                    minimum_domains_for_next_level=self.minimum_domains_for_next_level,
+                    domain_dof_schedule=self.domain_dof_schedule,
+                    target_node_schedule=self.target_node_schedule)
                 self._hierarchy_key = skey
                 self.hierarchy_build_count += 1
                 self._hierarchy_seconds = time.perf_counter() - t0
             layout = build_layout(self._hierarchy, row_ptr, col)
             w = level_weights(self._hierarchy.num_levels, self.coarsest_level_weight, self.level_weights)
-            self._device = _DeviceMas(layout, w, n, self.coarse_bank)
+            layout.weights_base = w.copy()
+            self._device = _DeviceMas(layout, w, n, self.coarse_bank, self.duplicate_level_weight)
             self._device_key = (id(f), f.version)
+            self._numeric_age = None          # the numeric update must run
             rebuilt = True
         if self._solution is None or self._solution.numel() != n:
             self._solution = dev.zeros(n, "float64")
@@ -427,7 +553,25 @@ class masSolver:
         t1 = time.perf_counter()
         m = self._device
         f = h.full if h.full is not None else h.split
-        m.update(f)
+        dyn = int(sum(h.block_counts_dynamic))
+        # adaptive fine weight (solver/mas/cuda_runtime.py:2716-2731)
+        w0 = float(m.layout.weights_base[0]) if m.layout.weights_base is not None else 1.0
+        if self.adaptive_fine_level_weight and dyn / max(1, f.nbr) >= self.dense_collision_block_ratio:
+            w0 = max(w0, self.dense_collision_fine_level_weight)
+        m.set_fine_weight(w0)
+        # lagged numeric updates (solver/mas/solver.py:520-568): the inverses of an earlier
+        # SPD operator stay a valid preconditioner; rebuild when contact turns on or off or
+        # the interval expires
+        interval = self.dynamic_numeric_rebuild_interval if dyn else self.numeric_rebuild_interval
+        transition = (dyn > 0) != (self._numeric_dyn > 0)
+        if self._numeric_age is None or transition or self._numeric_age + 1 >= interval:
+            m.update(f)
+            self._numeric_age = 0
+            self._numeric_dyn = dyn
+            numeric_rebuilt = True
+        else:
+            self._numeric_age += 1
+            numeric_rebuilt = False
         t2 = time.perf_counter()
         lib = _lib.load()
         stats = _lib.PcgStats()
@@ -450,7 +594,9 @@ class masSolver:
             "delta0": stats.delta0, "delta": stats.delta, "rr0": stats.rr0, "rr": stats.rr,
             "hierarchy_build_count": self.hierarchy_build_count,
             "hierarchy_build_seconds": getattr(self, "_hierarchy_seconds", 0.0),
-            "layout_rebuilt": rebuilt,
+            "layout_rebuilt": rebuilt, "numeric_rebuilt": numeric_rebuilt,
+            "fine_level_weight": float(m.weights_host[0]),
+            "duplicate_domains": int(lay.duplicate.sum()) if lay.duplicate is not None else 0,
             "nodes_per_level": [lv.num_nodes for lv in lay.hierarchy.levels],
             "domains_per_level": [len(lv.domains) for lv in lay.hierarchy.levels],
             "inverse_status": inverse_status,

@@ -154,21 +154,40 @@ def offsets_of(dims):
 
 def build_hierarchy(dims, pairs, max_domain_dofs: int = 32, max_levels: int = 4,
                     target_nodes_per_partition: int = 10,
-                    global_coarse_domain: bool = True) -> Hierarchy:
+                    global_coarse_domain: bool = True,
+                    minimum_domains_for_next_level: int = 0,
+                    domain_dof_schedule=None, target_node_schedule=None) -> Hierarchy:
     """``dims``: dimension of every level-0 node; ``pairs``: off-diagonal node pairs of
-    the static blocks."""
+    the static blocks. The level loop and its stopping rules follow
+    ``solver/mas/hierarchy.py:193-270``: per-level domain sizes and partition targets from
+    the schedules (the last entry repeats), and no further level once the level before
+    the last one (from level 3 on) has fewer than ``minimum_domains_for_next_level``
+    domains."""
+    if minimum_domains_for_next_level < 0:
+        raise ValueError("minimum_domains_for_next_level must be non-negative")
+    for sched, what in ((domain_dof_schedule, "domain DOF"), (target_node_schedule, "target-node")):
+        if sched is not None and (not sched or any(int(v) <= 0 for v in sched)):
+            raise ValueError(f"{what} schedule must contain positive values")
+
+    def at(sched, default, level):
+        return default if sched is None else int(sched[min(level, len(sched) - 1)])
+
     dims = np.asarray(dims, dtype=np.int64)
     n = dims.size
     xadj, adjncy = graph_from_pairs(n, pairs)
     levels: List[Level] = []
     composed = [np.arange(n, dtype=np.int64)]
     while len(levels) < max_levels:
-        groups = bfs_groups(xadj, adjncy, target_nodes_per_partition)
-        domains = pack_domain_groups(groups, dims, max_domain_dofs)
+        li = len(levels)
+        groups = bfs_groups(xadj, adjncy, at(target_node_schedule, target_nodes_per_partition, li))
+        domains = pack_domain_groups(groups, dims, at(domain_dof_schedule, max_domain_dofs, li))
         lvl = Level(dims.copy(), offsets_of(dims), xadj, adjncy,
                     [np.asarray(d, dtype=np.int64) for d in domains])
         levels.append(lvl)
         if len(levels) >= max_levels or dims.size <= 1 or adjncy.size == 0:
+            break
+        if (len(levels) >= 3 and minimum_domains_for_next_level
+                and len(levels[-2].domains) < minimum_domains_for_next_level):
             break
         mapping, cdims = collapse_compatible(xadj, adjncy, domains, dims)
         if cdims.size == dims.size:
@@ -180,6 +199,7 @@ def build_hierarchy(dims, pairs, max_domain_dofs: int = 32, max_levels: int = 4,
         xadj, adjncy = graph_from_pairs(cdims.size, cp)
         dims = cdims
     last = levels[-1]
-    if global_coarse_domain and len(last.domains) > 1 and last.num_dofs <= max_domain_dofs:
+    cap = at(domain_dof_schedule, max_domain_dofs, len(levels) - 1)
+    if global_coarse_domain and len(last.domains) > 1 and last.num_dofs <= cap:
         last.domains = [np.arange(last.num_nodes, dtype=np.int64)]
     return Hierarchy(levels, composed)

@@ -19,7 +19,7 @@ class minimizer:
         self.__requests = []             # (energy attribute, targets, method, dynamic)
         self.__wrt = []
         self.__hessian: Optional[hessian] = None
-        self.__solver = solver("jacobian")
+        self.__solver = solver("mas")            # the reference default (minimizer.pyx:101)
         self.__segments: List = []
         self.__ignored = set()
 
@@ -62,7 +62,7 @@ class minimizer:
     def solverName(self):
         return getattr(self.__solver, "solverName", type(self.__solver).__name__)
 
-    def setSolver(self, solver_name="jacobian", **options):
+    def setSolver(self, solver_name="mas", **options):
         self.linearSolver = solver(solver_name, **options)
         return self.__solver
 
@@ -93,6 +93,13 @@ class minimizer:
         if self.__hessian is None or not self.__hessian._setup:
             return dev.zeros(0, "float64")
         return self.__hessian.gradient.value
+
+    @property
+    def diagonal(self):
+        """``minimizer.pyx:162-165``: the active Hessian's scalar diagonal."""
+        if self.__hessian is None or not self.__hessian._setup:
+            return dev.zeros(0, "float64")
+        return self.__hessian.diagonal
 
     @property
     def gradientSegments(self):

@@ -181,9 +181,10 @@ class jacobianPCGSolver:
 
 
 class solver:
-    """Solver dispatch (``solver/solver.py:9-68``)."""
+    """Solver dispatch (``solver/solver.py:9-68``); MAS by default, as the reference
+    (``solver/solver.py:22``)."""
 
-    def __init__(self, solver="jacobian", **options):
+    def __init__(self, solver="mas", **options):
         self.__name = ""
         self.__impl = None
         self.setSolver(solver, **options)
@@ -204,7 +205,7 @@ class solver:
     def statistics(self) -> dict:
         return self.__impl.statistics
 
-    def setSolver(self, solver="jacobian", **options):
+    def setSolver(self, solver="mas", **options):
         if not isinstance(solver, str):
             raise TypeError("solver must be either 'jacobian' or 'mas'")
         name = solver.strip().lower()
