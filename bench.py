@@ -35,6 +35,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--no-pmc", action="store_true",
+                    help="skip the hardware-counter traffic passes (rocprofv3 --pmc)")
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the mixed-precision and MAS legs")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving stream")
@@ -51,20 +53,68 @@ def parse():
     ap.add_argument("--adjoint-fd", type=int, default=64)
     ap.add_argument("--serving-requests", type=int, default=512)
     ap.add_argument("--kernels-only", action="store_true",
-                    help="one solve, the SpMV roofline launches and a calibration copy of a "
+                    help="assembly, the PCG product launches and a calibration copy of a "
                          "known size, no JSON line (for hardware-counter runs)")
     return ap.parse_args()
 
 
-# HBM bytes per launch of bsr3_full_spmv_kernel<double> measured with hardware counters
-# on the same workload (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, over
-# `bench.py --kernels-only`, each scaled by the counter's reading for a 256 MiB device
-# copy: FETCH x1.9998, WRITE x1.0); summary in profiles/spmv_cube48_fp64_pmc.md.
-MEASURED_TRAFFIC = {48: 133998758}
-
-
-def measured_traffic(n):
-    return MEASURED_TRAFFIC.get(n)
+def measure_traffic(args):
+    """HBM bytes per launch of pcg3_spmv_kernel<double> (the PCG loop's product kernel) from
+    two hardware-counter runs, FETCH_SIZE then WRITE_SIZE, of ``bench.py --kernels-only``
+    under rocprofv3, each counter scaled by its reading for a 256 MiB device copy of known
+    size. Runs as child processes before this process touches the GPU. Returns
+    (bytes, details); bytes is None when the counters could not be collected."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+    rocprof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(rocprof):
+        return None, {"error": "rocprofv3 not found"}
+    tmp = tempfile.mkdtemp(prefix="ysp_pmc_", dir="/tmp")
+    env = dict(os.environ, TMPDIR="/tmp")
+    detail = {"command": ("rocprofv3 --pmc <COUNTER> --output-format csv -- python3 bench.py "
+                          f"--kernels-only --n {args.n} --seed {args.seed} --spmv-reps 5")}
+    total = 0.0
+    try:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            d = os.path.join(tmp, counter)
+            cmd = ["timeout", "-s", "KILL", "180", rocprof, "--pmc", counter, "--output-format",
+                   "csv", "-d", d, "-o", "run", "--", sys.executable, os.path.abspath(__file__),
+                   "--kernels-only", "--n", str(args.n), "--seed", str(args.seed),
+                   "--spmv-reps", "5"]
+            t0 = time.perf_counter()
+            r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True)
+            files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+            print(f"traffic: {counter} pass rc={r.returncode} in {time.perf_counter() - t0:.1f} s",
+                  file=sys.stderr, flush=True)
+            if not files:
+                return None, {**detail, "error": f"{counter}: rocprofv3 exit {r.returncode}, "
+                                                 f"no counter file: {r.stderr[-300:]}"}
+            prod, copy = [], []
+            for fn in files:
+                with open(fn, newline="") as fh:
+                    for row in csv.DictReader(fh):
+                        if row.get("Counter_Name") != counter:
+                            continue
+                        name = row.get("Kernel_Name", "")
+                        if "pcg3_spmv_kernel<double>" in name:
+                            prod.append(float(row["Counter_Value"]))
+                        elif "copyBuffer" in name:
+                            copy.append(float(row["Counter_Value"]))
+            if not prod or not copy:
+                return None, {**detail, "error": f"{counter}: kernels missing in the counter file"}
+            # the calibration copy moves 256 MiB each way: scale = known KiB / counter KiB
+            scale = 262144.0 / float(np.median(copy))
+            kib = float(np.median(prod))
+            total += kib * scale * 1024.0
+            detail[counter] = {"product_kib_per_dispatch": round(kib, 1),
+                               "copy_kib_per_dispatch": round(float(np.median(copy)), 1),
+                               "scale": round(scale, 4), "dispatches": len(prod)}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return int(round(total)), detail
 
 
 def calibration_copy(mib=256):
@@ -78,21 +128,32 @@ def calibration_copy(mib=256):
     torch.cuda.synchronize()
 
 
-def spmv_roofline(hs, reps):
-    """Average duration of the full-BSR 3x3 SpMV kernel (the operator of the PCG loop),
-    timed with HIP events on the stream it is launched on, and its algorithmic bytes
-    per launch (symmetric upper storage, each stored block read once)."""
+def product_roofline(hs, reps, traffic=None, traffic_detail=None, cold_reps=20):
+    """Roofline of pcg3_spmv_kernel<double>, the kernel that dominates the timed PCG loop
+    (one launch per iteration: q = A z + beta q, p = z + beta p and the partial sums of
+    p.q over the mirrored 3x3 block-CSR), launched through ysp_pcg3_product_pass with the
+    loop's own grid and timed by HIP events on the stream it runs on.
+
+    Algorithmic bytes per launch: the upper 3x3 blocks read once (72 B values + 8 B (row,
+    col)) plus five vector passes (z read, p read and written, q read and written). Two
+    regimes: back-to-back launches as in the loop (the n=48 operator, 134 MB, then stays in
+    the 256 MB Infinity Cache) and launches after a 1 GiB read has evicted L2 and the
+    Infinity Cache, so that each streams its operands from HBM; the HBM figure is the
+    headline."""
     from yasps_amd import _lib, device as dev
     torch = dev.torch()
     lib = _lib.load()
     f = hs.full
-    x = dev.to_device(np.linspace(-1.0, 1.0, hs.rows), "float64")
-    y = dev.zeros(hs.rows, "float64")
+    n = hs.rows
+    work = dev.zeros(5 * n, "float64")
+    scal = dev.zeros(12288, "float64")                 # YSP_PCG3_SCALARS
     stream = torch.cuda.current_stream()
 
-    def launch():
-        lib.ysp_bsr3_spmv(dev.ptr(f.row_ptr), dev.ptr(f.col), dev.ptr(f.v64), 0, f.nnz, hs.rows,
-                          dev.ptr(x), dev.ptr(y), stream.cuda_stream)
+    def launch(prepare=0):
+        _lib.check(lib.ysp_pcg3_product_pass(dev.ptr(f.row_ptr), dev.ptr(f.col), dev.ptr(f.v64), 0,
+                                             f.nnz, n, dev.ptr(work), dev.ptr(scal), prepare,
+                                             stream.cuda_stream), "ysp_pcg3_product_pass")
+    launch(1)
     for _ in range(3):
         launch()
     start = torch.cuda.Event(enable_timing=True)
@@ -102,40 +163,45 @@ def spmv_roofline(hs, reps):
         launch()
     end.record(stream)
     end.synchronize()
-    ms = start.elapsed_time(end) / reps
-    # cold: reading 1 GiB between launches evicts L2 and the 256 MB Infinity Cache (with
-    # clean lines: a write here would leave dirty lines whose write-back competes with the
-    # next launch), so each timed launch streams its operands from HBM
-    flush = torch.ones(1 << 27, dtype=torch.float64, device=dev.device())
-    cold = []
-    for _ in range(min(reps, 20)):
-        flush.sum()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        launch()
-        e1.record(stream)
-        cold.append((e0, e1))
-    torch.cuda.synchronize()
-    ms_cold = float(np.mean([a.elapsed_time(b) for a, b in cold]))
-    del flush
+    ms_warm = start.elapsed_time(end) / reps
+    # a read (clean lines: a write would leave dirty lines whose write-back competes with
+    # the timed launch) of 1 GiB between launches evicts L2 and the Infinity Cache
+    ms_cold = ms_warm
+    if cold_reps:
+        flush = torch.ones(1 << 27, dtype=torch.float64, device=dev.device())
+        cold = []
+        for _ in range(cold_reps):
+            flush.sum()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            launch()
+            e1.record(stream)
+            cold.append((e0, e1))
+        torch.cuda.synchronize()
+        ms_cold = float(np.mean([a.elapsed_time(b) for a, b in cold]))
+        del flush
+    del work, scal
     nb = hs.static.num_blocks
-    n = hs.rows
-    # algorithmic: 72 B values + 8 B (row, col) per stored upper block, x read, y written
-    bytes_alg = nb * (9 * 8 + 8) + n * 16
-    # what the mirrored full block-CSR kernel actually streams
-    bytes_full = f.nnz * (9 * 8 + 4) + (n // 3 + 1) * 4 + n * 16
-    gbs = bytes_alg / (ms * 1e-3) / 1e9
-    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(hs_n(hs)),
-            "kernel": "bsr3_full_spmv_kernel<double>", "avg_us": round(ms * 1e3, 2),
+    bytes_alg = nb * (9 * 8 + 8) + 5 * n * 8
+    bytes_full = f.nnz * (9 * 8 + 4) + (n // 3 + 1) * 4 + 5 * n * 8
+    gbs_cold = bytes_alg / (ms_cold * 1e-3) / 1e9
+    gbs_warm = bytes_alg / (ms_warm * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(gbs_cold, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs_cold / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": "pcg3_spmv_kernel<double>",
+            "regime": ("HBM: each timed launch follows a 1 GiB read that evicts L2 and the "
+                       "Infinity Cache; single-launch event pairs"),
+            "avg_us": round(ms_cold * 1e3, 2), "launches": cold_reps,
             "bytes_per_launch": int(bytes_alg), "full_bsr_bytes_per_launch": int(bytes_full),
-            "full_bsr_GBs": round(bytes_full / (ms * 1e-3) / 1e9, 1),
-            "cold": {"avg_us": round(ms_cold * 1e3, 2),
-                     "achieved": round(bytes_alg / (ms_cold * 1e-3) / 1e9, 1),
-                     "frac": round(bytes_alg / (ms_cold * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                     "full_bsr_GBs": round(bytes_full / (ms_cold * 1e-3) / 1e9, 1),
-                     "note": "1 GiB read between launches (L2 and MALL evicted); single-launch event pairs"}}
+            "full_bsr_GBs": round(bytes_full / (ms_cold * 1e-3) / 1e9, 1),
+            "in_loop": {"avg_us": round(ms_warm * 1e3, 2), "launches": reps,
+                        "achieved": round(gbs_warm, 1),
+                        "frac": round(gbs_warm / HBM_PEAK_GBS, 4),
+                        "full_bsr_GBs": round(bytes_full / (ms_warm * 1e-3) / 1e9, 1),
+                        "note": ("back-to-back launches as in the PCG loop: the operator "
+                                 "(134 MB at n=48) stays in the 256 MB Infinity Cache")},
+            "traffic_detail": traffic_detail}
 
 
 def hs_n(hs):
@@ -197,10 +263,11 @@ def serving_leg(args):
     """Serving stream: requests with n log-uniform in [6, 48] and E log-uniform in
     [1e4, 1e6], each carrying its positions (pinned host payload), first all at once
     (capacity), then Poisson arrivals at 80% of that capacity; p50/p99 latency from
-    arrival to solution. The default server overlaps request loading (a loader thread
-    and stream) with the batched PCG chunks; the same stream through the sequential
-    server (loads between chunks: larger batches in bursts, longer waits) is reported
-    beside it."""
+    arrival to solution. The leg's own figures are the mixed FP32/FP64 PCG + MAS server
+    (the serving configuration's solver); FP64 block-Jacobi is reported beside it, with
+    the default server (a loader thread and stream overlap request loading with the
+    batched PCG chunks) and the sequential one (loads between chunks: larger batches in
+    bursts, longer waits)."""
     from yasps_amd.serving import (SolveServer, attach_payloads, serving_stream,
                                    poisson_arrivals, summary)
     base = attach_payloads(serving_stream(count=args.serving_requests, seed=args.seed))
@@ -209,8 +276,9 @@ def serving_leg(args):
         return [type(r)(id=r.id, n=r.n, young=r.young, seed=r.seed, positions=r.positions,
                         arrival=r.arrival if arrivals else 0.0) for r in base]
 
-    def run(overlap):
-        srv = SolveServer(max_active=32, passes=8, tolerance=args.tol, overlap_loads=overlap)
+    def run(overlap, solver="jacobian", precision="fp64"):
+        srv = SolveServer(max_active=32, passes=8, tolerance=args.tol, overlap_loads=overlap,
+                          solver=solver, precision=precision)
         t0 = time.perf_counter()
         srv.warm([r.n for r in base])
         # the first closed-loop pass grows the slot pool to the concurrency it needs (static
@@ -222,7 +290,10 @@ def serving_leg(args):
         reqs = poisson_arrivals(fresh(), rate, seed=args.seed + 1)
         srv.batch_sizes.clear()
         st = summary(srv.serve(reqs))
-        return {"capacity_solves_per_s": round(cap["solves_per_s"], 2),
+        done = [r for r in reqs if r.status is not None]
+        res = [r.rel_residual for r in done if r.rel_residual is not None]
+        return {"solver": solver, "precision": precision,
+                "capacity_solves_per_s": round(cap["solves_per_s"], 2),
                 "arrival_rate_per_s": round(rate, 2), "value": round(st["solves_per_s"], 2),
                 "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
                 "admission_wait_p50_ms": round(st["wait_p50_ms"], 3),
@@ -230,11 +301,19 @@ def serving_leg(args):
                 "converged": st["converged"], "mean_pcg_iterations": round(st["mean_iterations"], 1),
                 "mean_batch": round(float(np.mean(srv.batch_sizes)), 2),
                 "max_batch": int(max(srv.batch_sizes)), "warm_seconds": round(warm_s, 2),
-                "slots": int(sum(len(v) for v in srv.pool.values()))}
+                "slots": int(sum(len(v) for v in srv.pool.values())),
+                "max_true_fp64_rel_residual": (float(max(res)) if res else None),
+                "mean_refinement_steps": (round(float(np.mean([r.outer_iterations for r in done])), 2)
+                                          if precision == "mixed" else None)}
 
-    out = {"requests": len(base), **run(True),
-           "solver": "batched block-Jacobi PCG, FP64, ||g-Hx||/||g|| <= %g" % args.tol}
-    out["sequential_loads"] = run(False)
+    # config 4's solver: mixed FP32/FP64 PCG + MAS (FP32 values and banks in the batched
+    # Krylov loop, FP64 refinement to the true residual); FP64 block-Jacobi beside it
+    out = {"requests": len(base), **run(True, "mas", "mixed"),
+           "solver_note": ("batched PCG + MAS (4 levels, 32-DoF domains), FP32 values and "
+                           "MAS banks in the Krylov loop, FP64 vectors and dots, FP64 "
+                           "iterative refinement to ||g-Hx||/||g|| <= %g" % args.tol)}
+    out["fp64_jacobi"] = run(True)
+    out["fp64_jacobi_sequential_loads"] = run(False)
     return out
 
 
@@ -371,7 +450,9 @@ def contact_leg(args):
             legs[name]["nodes_per_level"] = st.get("nodes_per_level")
     hs = s0.minimizer.activeHessian
     dev.torch().cuda.synchronize()
-    return {"bodies": int(cs.n_bodies + cs.n_free_bodies), "affine_bodies": int(cs.n_bodies),
+    sets = side_leg(lambda _: contact_sets_timing(cs), None)
+    return {"contact_sets": sets,
+            "bodies": int(cs.n_bodies + cs.n_free_bodies), "affine_bodies": int(cs.n_bodies),
             "tets": int(cs.tets.shape[0]), "cloth_triangles": int(cs.triangles.shape[0]),
             "hinges": int(cs.hinges.shape[0]), "point_triangle_pairs": int(cs.pt.shape[0]),
             "edge_edge_pairs": int(cs.ee.shape[0]), "unknowns": int(hs.rows),
@@ -380,6 +461,65 @@ def contact_leg(args):
             "split_3x3_entries": int(hs.split.nnz) if hs.split is not None else None,
             "value": legs["mas"]["value"], "unit": "Newton solves/s", "solvers": legs,
             "scene_generation_s": round(scene_s, 2)}
+
+
+def side_leg(fn, args):
+    """A leg beside the headline solve: its error is reported in its own JSON field (and
+    its traceback on stderr) so that the headline line is still printed."""
+    try:
+        return fn(args)
+    except Exception as e:  # noqa: BLE001
+        import traceback
+        traceback.print_exc()
+        return {"error": f"{type(e).__name__}: {e}"}
+
+
+def contact_sets_timing(cs, reps=5):
+    """The collision module (yasps_amd.ccd, csrc/ccd.hip) on the contact scene's surfaces:
+    the balls' boundary triangles and the cloth (each ball its own mesh id, the cloth
+    another, so no pair lies within one body). Times a proximity detection at dhat
+    (cd: BVHs, queries, distance types), a sweep of the balls falling by 0.01 (ccd) and its
+    additive-CCD step bound, each a full synchronous call, median of reps."""
+    from yasps_amd import device as dev
+    from yasps_amd.ccd import CCD
+    from yasps_amd.scenes import boundary_faces
+    x = cs.positions()
+    nb = cs.n_bodies + cs.n_free_bodies
+    per = cs.tets.shape[0] // nb
+    mesh = np.ones(x.shape[0], dtype=np.uint32)
+    mesh[cs.tets.reshape(-1)] = 2 + np.repeat(np.arange(cs.tets.shape[0]) // per, 4)
+    faces = np.concatenate([boundary_faces(cs.tets), cs.triangles])
+    e = np.sort(np.concatenate([faces[:, [0, 1]], faces[:, [1, 2]], faces[:, [0, 2]]]), axis=1)
+    edges = np.unique(e, axis=0)
+    surface = np.unique(faces)
+    # reach sqrt(dhat) ~ 0.03 spans several cloth cells, so the proximity sets hold tens of
+    # millions of pairs: capacities up to the int32 limit (buffers grow only to what is found)
+    c = CCD(len(surface), x.shape[0], max_cd_pairs=2_000_000_000, max_ccd_pairs=2_000_000_000,
+            mesh_indices=mesh, print_timings=False)
+    xd = dev.to_device(x.reshape(-1), "float64")
+    c.init_faces(xd, faces.reshape(-1), surface, len(faces))
+    c.init_edges(xd, xd, edges.reshape(-1), len(edges))
+    d = np.zeros_like(x)
+    d[mesh > 1, 1] = 0.01                        # x - t d: every ball falls by 0.01
+    dd = dev.to_device(d.reshape(-1), "float64")
+    t_cd, t_ccd, t_step = [], [], []
+    for _ in range(reps + 1):
+        t0 = time.perf_counter()
+        counts = c.cd(xd, cs.dhat)
+        t1 = time.perf_counter()
+        n = c.ccd(xd, cs.dhat, dd, 1.0)
+        t2 = time.perf_counter()
+        step = c.compute_largest_step_size(0.8, xd, dd)
+        t3 = time.perf_counter()
+        t_cd.append(t1 - t0)
+        t_ccd.append(t2 - t1)
+        t_step.append(t3 - t2)
+    med = lambda v: round(1e3 * float(np.median(v[1:])), 3)  # noqa: E731
+    return {"faces": int(len(faces)), "edges": int(len(edges)), "surface_vertices": int(len(surface)),
+            "proximity_pairs": {"pp": counts[0], "pe": counts[1], "pt": counts[2], "ee": counts[3]},
+            "swept_candidates": int(n), "step_bound": float(step),
+            "cd_ms": med(t_cd), "ccd_ms": med(t_ccd), "step_bound_ms": med(t_step),
+            "note": "synchronous calls (BVH builds for faces and edges, queries, narrow phase)"}
 
 
 def _energy_compute_ms(s0, reps=10):
@@ -467,56 +607,132 @@ def element_leg(args):
     return out
 
 
-def cpu_baseline(cs, values_host, st, tol):
-    """The oracle on the host: SNH + inertia assembly timed on 1/8 of the tets (scaled x8)
-    and block-Jacobi CG (scipy, one thread) timed in full on the same matrix."""
-    from oracle import energies, newton, projection
-    import scipy.sparse.linalg as spla
+class CpuPath:
+    """The oracle's CPU path of one Newton linear solve of the tet cube: SNH gradients and
+    Hessians with their PSD projection (method 2) on chunks of tets spread over `threads`
+    threads (numpy releases the GIL in its array loops; every BLAS/LAPACK call is limited
+    to one thread), inertia, scatter into the 3x3 upper block storage, the scipy BSR
+    matrix of the full operator, then block-Jacobi CG (scipy, single-threaded) to
+    ||g - Hx|| / ||g|| <= tol. The static layout (rest data, block lookups, BSR order) is
+    built before timing, as on the GPU."""
+
+    def __init__(self, cs, threads):
+        from oracle import energies, newton
+        self.cs = cs
+        self.threads = threads
+        st = newton.fast_structure(cs)
+        self.st = st
+        T = cs.tets.shape[0]
+        self.rest = energies.snh_rest_data(cs.rest[cs.tets])
+        nchunk = 4 * threads
+        self.chunks = [slice(i * T // nchunk, (i + 1) * T // nchunk) for i in range(nchunk)]
+        sel = st.tet_lookup_index >= 0                          # [T, 4, 4]
+        per_tet = sel.sum(axis=(1, 2))
+        self.blk_start = np.concatenate([[0], np.cumsum([per_tet[c].sum() for c in self.chunks])])
+        self.sel = sel
+        self.dst = (st.tet_lookup_index[sel][:, None] + np.arange(9)).reshape(-1)
+        nf = cs.n_fixed
+        self.gidx = np.where(cs.tets >= nf, 3 * (cs.tets - nf), -1)
+        self.gmask = self.gidx >= 0
+        self.grows = (self.gidx[:, :, None] + np.arange(3))[self.gmask].reshape(-1)
+        # BSR of the full operator from the upper storage: fixed block order
+        pos = st.layout.block_positions.astype(np.int64).reshape(-1, 2) // 3
+        self.off = pos[:, 0] != pos[:, 1]
+        rows = np.concatenate([pos[:, 0], pos[self.off, 1]])
+        cols = np.concatenate([pos[:, 1], pos[self.off, 0]])
+        self.order = np.lexsort((cols, rows))
+        self.cols = cols[self.order]
+        nb = st.n_dof // 3
+        indptr = np.zeros(nb + 1, dtype=np.int64)
+        np.add.at(indptr, rows + 1, 1)
+        self.indptr = np.cumsum(indptr)
+        self.H = np.empty((T, 12, 12))
+        self.G = np.empty((T, 12))
+        self.BLK = np.empty((int(self.blk_start[-1]), 9))
+
+    def _chunk(self, i):
+        from oracle import energies, projection
+        cs, c = self.cs, self.chunks[i]
+        g, H = energies.snh_gradient_hessian(cs.position[cs.tets[c]], self.rest[0][c],
+                                             self.rest[1][c], cs.mu, cs.lam, cs.dt)
+        H = projection.project_batch(H, 2)
+        self.G[c] = g
+        Hb = H.reshape(-1, 4, 3, 4, 3).transpose(0, 1, 3, 2, 4)    # [t, a, b, 3, 3]
+        sel = self.sel[c]
+        blk = Hb[sel]
+        tr = self.st.tet_transpose[c][sel]
+        blk = np.where(tr[:, None, None], np.transpose(blk, (0, 2, 1)), blk)
+        self.BLK[self.blk_start[i]:self.blk_start[i + 1]] = blk.reshape(-1, 9)
+
+    def solve(self, tol):
+        from concurrent.futures import ThreadPoolExecutor
+        from oracle import energies, projection
+        import scipy.sparse as sp
+        import scipy.sparse.linalg as spla
+        cs, st = self.cs, self.st
+        nf = cs.n_fixed
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(self.threads) as ex:
+            list(ex.map(self._chunk, range(len(self.chunks))))
+        t1 = time.perf_counter()
+        grad = np.zeros(st.n_dof)
+        np.add.at(grad, self.grows, self.G.reshape(-1, 4, 3)[self.gmask].reshape(-1))
+        values = np.zeros(st.layout.total_size)
+        np.add.at(values, self.dst, self.BLK.reshape(-1))
+        gi, Hi = energies.inertia_gradient_hessian(cs.position[nf:], cs.last_position[nf:],
+                                                   cs.velocity[nf:], cs.mass[nf:], cs.dt)
+        grad += gi.reshape(-1)
+        dsti = (st.vert_lookup[:, None] + np.arange(9)).reshape(-1)
+        np.add.at(values, dsti, Hi.reshape(-1))
+        B = values.reshape(-1, 3, 3)
+        data = np.concatenate([B, np.transpose(B[self.off], (0, 2, 1))])[self.order]
+        A = sp.bsr_matrix((data, self.cols, self.indptr), shape=(st.n_dof, st.n_dof))
+        t2 = time.perf_counter()
+        inv = projection.jacobi_block_inverse(values[dsti].reshape(-1, 3, 3))
+        M = spla.LinearOperator(A.shape, matvec=lambda v: np.einsum(
+            "bij,bj->bi", inv, v.reshape(-1, 3)).reshape(-1), dtype=np.float64)
+        it = [0]
+
+        def count(_):
+            it[0] += 1
+        x, info = spla.cg(A, grad, rtol=tol, atol=0.0, maxiter=20000, M=M, callback=count)
+        t3 = time.perf_counter()
+        rel = float(np.linalg.norm(grad - A @ x) / np.linalg.norm(grad))
+        return {"seconds": t3 - t0, "eval_project_s": t1 - t0, "scatter_bsr_s": t2 - t1,
+                "cg_s": t3 - t2, "cg_iterations": it[0], "info": int(info), "rel_residual": rel}
+
+
+def cpu_baseline(cs, tol, threads):
+    """One full Newton linear solve of the benchmark scene on the host (CpuPath), timed in
+    full; threadpoolctl keeps every BLAS/LAPACK call on one thread so that the thread
+    count is the one stated."""
     import contextlib
     try:
         from threadpoolctl import threadpool_limits
         limiter = threadpool_limits(limits=1)
-        threads = 1
-    except Exception:  # threadpoolctl absent: numpy keeps its default thread count
+    except Exception:  # threadpoolctl absent: numpy keeps its default BLAS threads
         limiter = contextlib.nullcontext()
-        threads = os.cpu_count() or 1
     with limiter:
-        T = cs.tets.shape[0]
-        frac = 8
-        sub = slice(0, T // frac)
-        t0 = time.perf_counter()
-        rd = energies.snh_rest_data(cs.rest[cs.tets[sub]])
-        g, H = energies.snh_gradient_hessian(cs.position[cs.tets[sub]], rd[0], rd[1], cs.mu,
-                                             cs.lam, cs.dt)
-        H = projection.project_batch(H, 2)
-        t_asm = (time.perf_counter() - t0) * frac
-        t0 = time.perf_counter()
-        A = newton.bsr_matrix(st, values_host)
-        diag = values_host[(st.vert_lookup[:, None] + np.arange(9)).reshape(-1)].reshape(-1, 3, 3)
-        inv = projection.jacobi_block_inverse(diag)
-        b = np.asarray(cs.extras["gradient"])
-        M = spla.LinearOperator(A.shape, matvec=lambda v: np.einsum(
-            "bij,bj->bi", inv, v.reshape(-1, 3)).reshape(-1), dtype=np.float64)
-        x, info = spla.cg(A, b, rtol=tol, atol=0.0, maxiter=20000, M=M)
-        t_cg = time.perf_counter() - t0
-        rel = float(np.linalg.norm(b - A @ x) / np.linalg.norm(b))
-    total = t_asm + t_cg
+        path = CpuPath(cs, threads)
+        r = path.solve(tol)
     model = ""
     try:
         with open("/proc/cpuinfo") as f:
             model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
     except OSError:
         pass
-    return {"value": round(1.0 / total, 4), "unit": "Newton solves/s", "cores": threads,
+    return {"value": round(1.0 / r["seconds"], 4), "unit": "Newton solves/s", "cores": threads,
             "kind": "port", "host_cores_available": len(os.sched_getaffinity(0)),
             "cpu_model": model, "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
-            "threads_note": "numpy/BLAS limited to 1 thread (threadpoolctl); scipy CG and "
-                            "SpMV are single-threaded",
-            "sample": (f"n={cs.n} cube: SNH assembly+projection on 1/{frac} of the {T} tets "
-                       f"timed ({t_asm / frac:.2f}s) and scaled x{frac}; block-Jacobi CG "
-                       f"(scipy, tol {tol:g}) timed in full ({t_cg:.2f}s, info={info}, "
-                       f"rel.res {rel:.2e}) on the GPU-assembled matrix"),
-            "seconds_per_solve": round(total, 3)}
+            "threads_note": (f"element evaluation and projection on {threads} threads (numpy, "
+                             "one BLAS/LAPACK thread each); scatter, scipy BSR build and "
+                             "scipy CG on one thread"),
+            "sample": (f"one full Newton linear solve of the n={cs.n} cube ({cs.tets.shape[0]} "
+                       f"tets, {cs.n_dof} unknowns): evaluation + projection "
+                       f"{r['eval_project_s']:.2f} s, scatter + BSR {r['scatter_bsr_s']:.2f} s, "
+                       f"block-Jacobi CG {r['cg_s']:.2f} s ({r['cg_iterations']} iterations, "
+                       f"info={r['info']}, rel.res {r['rel_residual']:.2e})"),
+            "seconds_per_solve": round(r["seconds"], 3)}
 
 
 def main():
@@ -524,6 +740,10 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; rerun with --gpus 1"}))
         sys.exit(2)
+    # hardware-counter passes run as child processes before this one touches the GPU
+    traffic, traffic_detail = None, None
+    if not args.kernels_only and not args.no_pmc:
+        traffic, traffic_detail = measure_traffic(args)
     from yasps_amd.scenes import cube_scene, build_yasps_scene
     from yasps_amd import device as dev
     from yasps_amd.hessian import vector
@@ -531,13 +751,16 @@ def main():
 
     cs = cube_scene(args.n, seed=args.seed)
     s0, hnd = build_yasps_scene(cs, threshold_mode="relative_residual")
+    if args.kernels_only:
+        # assembly only (the operator of the loop), then the product launches and the
+        # calibration copy: the only pcg3_spmv_kernel launches are the measured ones
+        s0.minimizer.computeNumericValue()
+        product_roofline(s0.minimizer.activeHessian, args.spmv_reps, cold_reps=0)
+        calibration_copy()
+        return
     # setup: static sparsity layout, rest data, first solve
     s0.minimizeEnergy(tolerance=args.tol)
     torch.cuda.synchronize()
-    if args.kernels_only:
-        spmv_roofline(s0.minimizer.activeHessian, args.spmv_reps)
-        calibration_copy()
-        return
     elapsed, lat, iters = timed_solves(s0, args.steps, args.warmup, args.tol)
 
     hs = s0.minimizer.activeHessian
@@ -546,7 +769,7 @@ def main():
     r = hs.spmv(vector(hs.rows, sol.clone())).value - g
     true_rel = float(torch.linalg.norm(r) / torch.linalg.norm(g))
 
-    roof = spmv_roofline(hs, args.spmv_reps)
+    roof = product_roofline(hs, args.spmv_reps, traffic, traffic_detail)
     value = args.steps / elapsed
     forms = None
     if not args.no_forms:
@@ -585,18 +808,15 @@ def main():
         "true_rel_residual": true_rel,
         "roofline": roof,
         "forms": forms,
-        "serving": None if args.no_serving else serving_leg(args),
-        "adjoint": None if args.no_adjoint else adjoint_leg(args),
-        "dynamic_contacts": None if args.no_dynamic else dynamic_leg(args),
-        "element_kernels": None if args.no_elements else element_leg(args),
-        "contact_mixed_dof": None if args.no_contact else contact_leg(args),
+        "serving": None if args.no_serving else side_leg(serving_leg, args),
+        "adjoint": None if args.no_adjoint else side_leg(adjoint_leg, args),
+        "dynamic_contacts": None if args.no_dynamic else side_leg(dynamic_leg, args),
+        "element_kernels": None if args.no_elements else side_leg(element_leg, args),
+        "contact_mixed_dof": None if args.no_contact else side_leg(contact_leg, args),
     }
     if not args.no_cpu:
-        from oracle import newton
-        st = newton.fast_structure(cs)
-        vals = dev.to_host(hs.blocks_flattened)
-        cs.extras["gradient"] = dev.to_host(g)
-        out["cpu_baseline"] = cpu_baseline(cs, vals, st, args.tol)
+        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+        out["cpu_baseline"] = cpu_baseline(cs, args.tol, threads)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

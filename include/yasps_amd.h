@@ -535,12 +535,27 @@ int ysp_pcg3_mas_solve(uint32_t max_iterations, double threshold, int32_t thresh
                        const double* initial_guess, int32_t zero_initial_guess, double* work,
                        double* cg_scalars, ysp_pcg_stats* stats, void* stream);
 
+/* One product pass of the PCG loop on caller state, no reference counterpart: the kernel
+ * that dominates every block-Jacobi and MAS iteration of ysp_pcg3_solve /
+ * ysp_pcg3_mas_solve (q = A z + beta q, p = z + beta p, partial sums of p.q), exposed so
+ * that a benchmark can time the loop's own kernel with events on its stream. With
+ * prepare != 0 the scalar state is first set to a running iteration (beta = 1, never
+ * converged) and z, p, q to nonzero data; later passes keep updating p and q. Every block
+ * row goes through the product kernel (no long-row list). work: 5 n doubles laid out as
+ * ysp_pcg3_solve's, cg_scalars: YSP_PCG3_SCALARS doubles. Asynchronous. */
+int ysp_pcg3_product_pass(const uint32_t* row_ptr, const uint32_t* col, const void* values,
+                          int32_t values_f32, uint32_t nnz, uint32_t n, double* work,
+                          double* cg_scalars, int32_t prepare, void* stream);
+
 /* ------------------------------------------------------------ batched PCG (serving) */
 /* No reference counterpart: several independent 3x3 systems (e.g. the Newton solves of
  * different requests) advance through the same launches; every workgroup owns a chunk
  * of one system and reductions are segmented per system (each system's own scalar
- * state). Block-Jacobi preconditioner, zero initial guess, same recurrence, stopping
- * rules and return codes as ysp_pcg3_solve. */
+ * state). Each system is preconditioned by block-Jacobi (diagonal_inverse) or by its own
+ * MAS (mas, the apply of solver/mas/pcg.py:23-102 batched over the systems: restriction,
+ * local solves and prolongation each one launch for all of them). Zero initial guess,
+ * same recurrence, stopping rules and return codes as ysp_pcg3_solve /
+ * ysp_pcg3_mas_solve. */
 #define YSP_PCG3_RUNNING 0x7fffffff   /* status of a system still iterating */
 
 typedef struct ysp_pcg3_system {
@@ -559,6 +574,9 @@ typedef struct ysp_pcg3_system {
   int32_t status;                     /* YSP_PCG3_RUNNING, or the ysp_pcg3_solve code */
   int32_t iterations;
   double delta0, delta, rr;           /* r0^T z0, r^T z, r^T r of the last test */
+  const ysp_mas_args* mas;            /* NULL: block-Jacobi (diagonal_inverse); else the MAS
+                                         preconditioner of this system (current inverses,
+                                         its own coarse and correction buffers) */
 } ysp_pcg3_system;
 
 /* Initial residual, preconditioned residual and threshold of one system (asynchronous). */
@@ -569,6 +587,47 @@ int ysp_pcg3_batch_start(ysp_pcg3_system* system, double threshold, int32_t thre
  * (synchronous). */
 int ysp_pcg3_batch_run(ysp_pcg3_system* systems, uint32_t count, uint32_t passes,
                        int32_t values_f32, int32_t inverse_f32, void* stream);
+
+/* -------------------------------------------------------------- contact sets */
+/* The collision module the reference's examples drive (examples/ccd/ccd.py over
+ * examples/ccd/mlbvh.cu and examples/ccd/ACCD.cu). Positions x and directions d are
+ * [vertices x 3] FP64; faces uint32 [nf x 3] with surface the vertices queried against
+ * them (uint32 [ns]); edges uint32 [ne x 2]; mesh (NULL, or uint32 [vertices]) suppresses
+ * pairs whose four vertices share one nonzero id. dhat is a squared distance. */
+
+/* Candidates: a linear BVH over the faces and one over the edges (boxes of the vertices
+ * at x, and at x - alpha d when d is non-NULL), every surface vertex queried against the
+ * faces and every edge against the edges with partner id above its own, the gap
+ * sqrt(dhat) (mlbvh.cu:1015-1137; lbvh_{f,e}_construct[_full_ccd] +
+ * lbvh_{f,e}_append_{proximity,swept}_candidates, mlbvh.cu:1603-1700,1906-1946). Writes
+ * compact int32 pairs (-(p + 1), face) or (edge i, edge j) into
+ * candidates[2 * capacity]; *count (host) gets the number found, which may exceed
+ * capacity (only the first capacity are stored). Synchronous. */
+int ysp_ccd_candidates(const double* x, const double* d, double alpha, uint32_t num_vertices,
+                       const uint32_t* faces, uint32_t num_faces, const uint32_t* surface,
+                       uint32_t num_surface, const uint32_t* edges, uint32_t num_edges,
+                       const uint32_t* mesh, double dhat, int32_t* candidates,
+                       uint32_t capacity, uint32_t* count, void* stream);
+
+/* Active pairs of the candidates at positions x: the distance type of each (vertex, edge
+ * or face of the triangle; the nine segment cases of two edges), its squared distance and,
+ * below dhat, the pair of that type appended to pp [2 ids], pe [3], pt [4] or ee [4], each
+ * with room for capacity pairs (process_cached_candidates + scatter_packed_cases,
+ * mlbvh.cu:594-725,1139-1300; lbvh_refilter_cached_candidates + lbvh_scatter_packed_cases,
+ * mlbvh.cu:1948-2021). counts[4] (host) get the numbers found (may exceed capacity).
+ * Order within a case is not fixed. Synchronous. */
+int ysp_ccd_active_pairs(const double* x, const uint32_t* faces, const uint32_t* edges,
+                         const int32_t* candidates, uint32_t num_candidates, double dhat,
+                         uint32_t* pp, uint32_t* pe, uint32_t* pt, uint32_t* ee,
+                         uint32_t capacity, uint32_t* counts, void* stream);
+
+/* Largest feasible step along x - t d (self_largestFeasibleStepSizeCompact,
+ * ACCD.cu:676-805): the additive CCD time of every candidate with eta = 1 - slackness
+ * (times[i] when times is non-NULL), their minimum in *step (host), 1 without
+ * candidates. A candidate already at zero distance gets 0. Synchronous. */
+int ysp_ccd_step_bound(const double* x, const double* d, const uint32_t* faces,
+                       const uint32_t* edges, const int32_t* candidates, uint32_t num_candidates,
+                       double slackness, double* times, double* step, void* stream);
 
 #ifdef __cplusplus
 }

@@ -69,3 +69,21 @@ def test_mas_is_the_default_solver():
     assert s0.minimizer.solverName == "mas"
     s0.minimizer.setSolver("jacobian")
     assert s0.minimizer.solverName == "jacobian"
+
+
+def test_quadratic_form_uses_the_symmetric_part_of_q():
+    """The inner matrix may carry rounding asymmetry (B diag(d) B^T); the energy's Hessian
+    is its symmetric part. A matrix that is not symmetric to rounding is refused."""
+    import numpy as np
+    from yasps_amd.energies import quadratic_form, QUADRATIC_FORM_ROWS
+    rng = np.random.default_rng(214)
+    B, _ = np.linalg.qr(rng.normal(size=(5, 5)))
+    Q = (B * np.linspace(-1.7, 3.1, 5)) @ B.T
+    J = rng.normal(size=(5, 12))
+    e = quadratic_form(None, J, Q)
+    R = QUADRATIC_FORM_ROWS
+    Qp = np.asarray(e.scene_params[12 * R:12 * R + R * R]).reshape(R, R)
+    assert np.array_equal(Qp, Qp.T)
+    np.testing.assert_allclose(Qp[:5, :5], 0.5 * (Q + Q.T), rtol=0, atol=0)
+    with pytest.raises(ValueError, match="symmetric"):
+        quadratic_form(None, J, Q + 1e-6 * np.triu(np.ones((5, 5)), 1))

@@ -106,3 +106,72 @@ def test_request_larger_than_the_budget_is_rejected():
         srv.budget = srv.pool[5][0].footprint - 1
         with pytest.raises(ValueError, match="does not fit"):
             srv.serve([SolveRequest(id=0, n=5, young=1e5, seed=0)])
+
+
+def _oracle_system(r):
+    """FP64 operator and right-hand side of a request from the oracle (same positions as
+    the server derives from the request's seed)."""
+    from yasps_amd.scenes import cube_scene
+    from yasps_amd.serving import request_positions
+    from oracle import newton
+    cs = cube_scene(r.n, young=r.young, seed=0)
+    nf = cs.n_fixed
+    cs.position[nf:] = request_positions(cs, r)
+    st = newton.fast_structure(cs)
+    asm = newton.evaluate(cs, st)
+    return newton.bsr_matrix(st, asm.values), asm.gradient
+
+
+def test_batched_mas_matches_single_system_mas():
+    """Every system of a batch carries its own MAS: the batched restriction, local solves
+    and prolongation give each request the single-system MAS solve."""
+    from yasps_amd import device as dev
+    from yasps_amd.scenes import cube_scene, build_yasps_scene
+    from yasps_amd.serving import SolveServer, SolveRequest, request_positions
+    print("batched MAS vs single-system MAS", flush=True)
+    reqs = [SolveRequest(id=i, n=n, young=y, seed=60 + i)
+            for i, (n, y) in enumerate([(5, 1e5), (8, 3e4), (6, 8e5), (7, 2e5)])]
+    srv = SolveServer(max_active=8, keep_solutions=True, overlap_loads=False, solver="mas")
+    done = {r.id: r for r in srv.serve(reqs)}
+    assert max(srv.batch_sizes) == 4
+    for r in reqs:
+        cs = cube_scene(r.n, young=r.young, seed=0)
+        nf = cs.n_fixed
+        cs.position[nf:] = request_positions(cs, r)
+        s0, _ = build_yasps_scene(cs, solver_name="mas", threshold_mode="relative_residual",
+                                  max_levels=4, coarsest_level_weight=1.0)
+        x = dev.to_host(s0.minimizeEnergy(tolerance=1e-6)[0])
+        it = s0.minimizer.linearSolver.statistics["iterations"]
+        d = done[r.id]
+        assert d.status >= 0 and abs(d.iterations - it) <= 1, (d.iterations, it)
+        assert np.linalg.norm(d.solution - x) <= 1e-8 * np.linalg.norm(x)
+
+
+@pytest.mark.parametrize("solver", ["mas", "jacobian"])
+def test_mixed_serving_reaches_fp64_residual_same_alone_and_in_mix(solver):
+    """Mixed precision serving (FP32 values and preconditioner data in the batched Krylov
+    loop, FP64 iterative refinement): every request reaches ||g - H x|| <= 1e-6 ||g|| on
+    the oracle's FP64 operator, requests join while others iterate, and each result is
+    the same alone and in the mix (1e-8 relative: the per-system arithmetic does not
+    depend on the batch)."""
+    from yasps_amd.serving import SolveServer, serving_stream, poisson_arrivals
+    print(f"mixed serving {solver}", flush=True)
+    reqs = serving_stream(count=12, n_min=4, n_max=10, seed=5)
+    srv = SolveServer(max_active=16, keep_solutions=True, passes=4, precision="mixed",
+                      solver=solver)
+    srv.warm([r.n for r in reqs])
+    poisson_arrivals(reqs, rate=300.0, seed=4)
+    done = srv.serve(reqs)
+    assert len(done) == 12 and all(r.status >= 0 for r in done)
+    assert max(srv.batch_sizes) > 1
+    for r in done:
+        assert r.rel_residual <= 1e-6 and r.outer_iterations >= 1
+        A, g = _oracle_system(r)
+        assert np.linalg.norm(g - A @ r.solution) <= 1.01e-6 * np.linalg.norm(g)
+    from yasps_amd.serving import SolveRequest
+    for r in done[:6]:
+        (a,) = SolveServer(max_active=1, keep_solutions=True, precision="mixed",
+                           solver=solver).serve(
+            [SolveRequest(id=r.id, n=r.n, young=r.young, seed=r.seed)])
+        assert a.status >= 0 and abs(a.iterations - r.iterations) <= 1
+        assert np.linalg.norm(a.solution - r.solution) <= 1e-8 * np.linalg.norm(a.solution)

@@ -3,8 +3,8 @@
 total energy from computeTotalEnergy, velocity update), yasps_amd.stepping on the GPU
 against the same loop on the CPU with the oracle (oracle/newton.py: gradient, projected
 Hessian, energy; a sparse direct solve for the direction): two steps of the n=4 cube,
-energies at every Newton iterate within 1e-12, positions and velocities after each step
-within 1e-9 of the largest entry."""
+energies at every Newton iterate within 1e-12, positions after each step within 1e-9 of
+the largest entry, velocities within that bound divided by dt (v = (x - x_prev) / dt)."""
 import copy
 
 import numpy as np
@@ -58,5 +58,12 @@ def test_time_steps_match_oracle():
         np.testing.assert_allclose(out["energies"], e_ref, rtol=1e-12, atol=0)
         x = dev.to_host(h["moving"]["position"].value).reshape(-1, 3)
         v = dev.to_host(h["moving"]["velocity"].value).reshape(-1, 3)
-        assert np.abs(x - ref.position[nf:]).max() <= 1e-9 * np.abs(ref.position[nf:]).max()
-        assert np.abs(v - ref.velocity[nf:]).max() <= 1e-9 * max(np.abs(ref.velocity[nf:]).max(), 1e-300)
+        tol_x = 1e-9 * np.abs(ref.position[nf:]).max()
+        assert np.abs(x - ref.position[nf:]).max() <= tol_x
+        # v = (x - x_prev) / dt carries the position difference divided by dt
+        assert np.abs(v - ref.velocity[nf:]).max() * cs.dt <= tol_x
+        # the next step starts both loops from the same state: near convergence the
+        # backtracking test compares energies equal to rounding, so the last step's length
+        # (and the state it leaves) may differ by that much between the two loops
+        ref.position[nf:] = x
+        ref.velocity[nf:] = v

@@ -43,7 +43,8 @@ class Pcg3System(C.Structure):
                 ("n", C.c_uint32), ("solution", C.c_void_p), ("work", C.c_void_p),
                 ("cg_scalars", C.c_void_p), ("max_iterations", C.c_uint32),
                 ("passes_done", C.c_uint32), ("status", C.c_int32), ("iterations", C.c_int32),
-                ("delta0", C.c_double), ("delta", C.c_double), ("rr", C.c_double)]
+                ("delta0", C.c_double), ("delta", C.c_double), ("rr", C.c_double),
+                ("mas", C.c_void_p)]
 
 
 PCG3_RUNNING = 0x7FFFFFFF
@@ -140,8 +141,13 @@ SIGNATURES = {
                                   P]),
     "ysp_pcg3_solve": (C.c_int, [U32, D, I32, P, P, P, I32, U32, P, I32, P, U32, P, P, I32, P, P,
                                  C.POINTER(PcgStats), P]),
+    "ysp_pcg3_product_pass": (C.c_int, [P, P, P, I32, U32, U32, P, P, I32, P]),
     "ysp_pcg3_batch_start": (C.c_int, [C.POINTER(Pcg3System), D, I32, I32, P]),
     "ysp_pcg3_batch_run": (C.c_int, [C.POINTER(Pcg3System), U32, U32, I32, I32, P]),
+    "ysp_ccd_candidates": (C.c_int, [P, P, D, U32, P, U32, P, U32, P, U32, P, D, P, U32,
+                                     C.POINTER(U32), P]),
+    "ysp_ccd_active_pairs": (C.c_int, [P, P, P, P, U32, D, P, P, P, P, U32, C.POINTER(U32), P]),
+    "ysp_ccd_step_bound": (C.c_int, [P, P, P, P, P, U32, D, P, C.POINTER(D), P]),
     "ysp_mas_coarsen": (C.c_int, [P, I32, U32, P, P, U32, P, P]),
     "ysp_mas_domain_scatter": (C.c_int, [P, I32, U32, P, P, P]),
     "ysp_mas_invert_domains": (C.c_int, [P, P, U32, P, P, U32, P, P]),

@@ -1155,6 +1155,35 @@ int pcg3_run(uint32_t max_iterations, double threshold, int32_t mode, const uint
 }
 }  // namespace
 
+namespace {
+// Dispatch on the value and inverse types (FP64 / FP32).
+int pcg3_dispatch(uint32_t max_iterations, double threshold, int32_t threshold_mode,
+                  const uint32_t* row_ptr, const uint32_t* col, const void* values,
+                  int32_t values_f32, uint32_t nnz, const void* diagonal_inverse,
+                  int32_t inverse_f32, const double* rhs, uint32_t n, double* solution,
+                  const double* x0, double* work, double* cg_scalars, ysp_pcg_stats* stats,
+                  hipStream_t s, bool no_alloc) {
+  if (values_f32) {
+    const float* v = static_cast<const float*>(values);
+    if (inverse_f32)
+      return pcg3_run(max_iterations, threshold, threshold_mode, row_ptr, col, v, nnz,
+                      static_cast<const float*>(diagonal_inverse), nullptr, rhs, n, solution, x0,
+                      work, cg_scalars, stats, s, no_alloc);
+    return pcg3_run(max_iterations, threshold, threshold_mode, row_ptr, col, v, nnz,
+                    static_cast<const double*>(diagonal_inverse), nullptr, rhs, n, solution, x0,
+                    work, cg_scalars, stats, s, no_alloc);
+  }
+  const double* v = static_cast<const double*>(values);
+  if (inverse_f32)
+    return pcg3_run(max_iterations, threshold, threshold_mode, row_ptr, col, v, nnz,
+                    static_cast<const float*>(diagonal_inverse), nullptr, rhs, n, solution, x0,
+                    work, cg_scalars, stats, s, no_alloc);
+  return pcg3_run(max_iterations, threshold, threshold_mode, row_ptr, col, v, nnz,
+                  static_cast<const double*>(diagonal_inverse), nullptr, rhs, n, solution, x0,
+                  work, cg_scalars, stats, s, no_alloc);
+}
+}  // namespace
+
 extern "C" int ysp_pcg3_solve(uint32_t max_iterations, double threshold, int32_t threshold_mode,
                               const uint32_t* row_ptr, const uint32_t* col, const void* values,
                               int32_t values_f32, uint32_t nnz, const void* diagonal_inverse,
@@ -1168,26 +1197,66 @@ extern "C" int ysp_pcg3_solve(uint32_t max_iterations, double threshold, int32_t
     ysp_set_error(__FILE__, __LINE__, "ysp_pcg3_solve: bad arguments");
     return YSP_ERR_ARG;
   }
-  hipStream_t s = as_stream(stream);
   const double* x0 = (!zero_initial_guess && initial_guess) ? initial_guess : nullptr;
-  if (values_f32) {
-    const float* v = static_cast<const float*>(values);
-    if (inverse_f32)
-      return pcg3_run(max_iterations, threshold, threshold_mode, row_ptr, col, v, nnz,
-                      static_cast<const float*>(diagonal_inverse), nullptr, rhs, n, solution, x0, work,
-                      cg_scalars, stats, s);
-    return pcg3_run(max_iterations, threshold, threshold_mode, row_ptr, col, v, nnz,
-                    static_cast<const double*>(diagonal_inverse), nullptr, rhs, n, solution, x0, work,
-                    cg_scalars, stats, s);
+  return pcg3_dispatch(max_iterations, threshold, threshold_mode, row_ptr, col, values,
+                       values_f32, nnz, diagonal_inverse, inverse_f32, rhs, n, solution, x0, work,
+                       cg_scalars, stats, as_stream(stream), false);
+}
+
+// ------------------------------------------------------------ one product pass
+namespace {
+// A running iteration k = 2 of the loop: r.z of the two previous iterations and r.r equal
+// to 1 (beta = 1), the reference stopping rule with threshold 0 (never met); z, p, q set
+// to nonzero data.
+__global__ void __launch_bounds__(kB)
+    product_state_kernel(double* __restrict__ st, double* __restrict__ z,
+                         double* __restrict__ p, double* __restrict__ q, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    st[slots_base(SET_RZ + 1)] = 1.0;
+    st[slots_base(SET_RR + 1)] = 1.0;
+    st[slots_base(SET_RZ + 0)] = 1.0;
   }
-  const double* v = static_cast<const double*>(values);
-  if (inverse_f32)
-    return pcg3_run(max_iterations, threshold, threshold_mode, row_ptr, col, v, nnz,
-                    static_cast<const float*>(diagonal_inverse), nullptr, rhs, n, solution, x0, work,
-                    cg_scalars, stats, s);
-  return pcg3_run(max_iterations, threshold, threshold_mode, row_ptr, col, v, nnz,
-                  static_cast<const double*>(diagonal_inverse), nullptr, rhs, n, solution, x0, work,
-                  cg_scalars, stats, s);
+  if (i < n) {
+    z[i] = 1.0 + 0.01 * static_cast<double>(i % 7);
+    p[i] = 0.5;
+    q[i] = 0.25;
+  }
+}
+}  // namespace
+
+extern "C" int ysp_pcg3_product_pass(const uint32_t* row_ptr, const uint32_t* col,
+                                     const void* values, int32_t values_f32, uint32_t nnz,
+                                     uint32_t n, double* work, double* cg_scalars,
+                                     int32_t prepare, void* stream) {
+  if (n == 0) return YSP_OK;
+  if (n % 3 || !row_ptr || !col || !values || !work || !cg_scalars) {
+    ysp_set_error(__FILE__, __LINE__, "ysp_pcg3_product_pass: bad arguments");
+    return YSP_ERR_ARG;
+  }
+  hipStream_t s = as_stream(stream);
+  const uint32_t nv = n / 3;
+  double* z = work + n;
+  double* pa = work + 2 * static_cast<size_t>(n);
+  double* q = work + 4 * static_cast<size_t>(n);
+  if (prepare) {
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(grid_for(kStateDoubles, kB)), dim3(kB), 0, s,
+                       cg_scalars, static_cast<uint32_t>(kStateDoubles));
+    hipLaunchKernelGGL(product_state_kernel, dim3(grid_for(n, kB)), dim3(kB), 0, s, cg_scalars,
+                       z, pa, q, n);
+  }
+  const uint32_t rows_per_wg = spmv_rows_per_wg(nv);
+  const dim3 sg(grid_for(nv, rows_per_wg)), bl(kB);
+  if (values_f32)
+    hipLaunchKernelGGL(pcg3_spmv_kernel<float>, sg, bl, 0, s, row_ptr, col,
+                       static_cast<const float*>(values), pad64(nnz), nv, z, pa, q, cg_scalars, 2,
+                       rows_per_wg, nullptr, nullptr);
+  else
+    hipLaunchKernelGGL(pcg3_spmv_kernel<double>, sg, bl, 0, s, row_ptr, col,
+                       static_cast<const double*>(values), pad64(nnz), nv, z, pa, q, cg_scalars,
+                       2, rows_per_wg, nullptr, nullptr);
+  YSP_CHECK_LAUNCH();
+  return YSP_OK;
 }
 
 extern "C" int ysp_pcg3_mas_solve(uint32_t max_iterations, double threshold,
@@ -1300,7 +1369,7 @@ int batch_run(ysp_pcg3_system* systems, uint32_t count, uint32_t passes, hipStre
     b.row_ptr = y.row_ptr;
     b.col = y.col;
     b.values = y.values;
-    b.minv = y.diagonal_inverse;
+    b.minv = y.mas ? nullptr : y.diagonal_inverse;   // MAS: the update leaves z to the apply
     b.x = y.solution;
     b.r = y.work;
     b.z = y.work + n;
@@ -1324,12 +1393,53 @@ int batch_run(ysp_pcg3_system* systems, uint32_t count, uint32_t passes, hipStre
   YSP_CHECK(hipMemcpyAsync(dsys, host.data(), sizeof(BatchSystem) * count, hipMemcpyHostToDevice, s));
   YSP_CHECK(hipMemcpyAsync(dsp, wsp.data(), sizeof(BatchWork) * wsp.size(), hipMemcpyHostToDevice, s));
   YSP_CHECK(hipMemcpyAsync(dup, wup.data(), sizeof(BatchWork) * wup.size(), hipMemcpyHostToDevice, s));
+  // systems preconditioned by MAS: one item each and the work tables of the apply
+  std::vector<MasBatchItem> items;
+  MasBatchTables mt;
+  for (uint32_t i = 0; i < count; ++i) {
+    if (!systems[i].mas) continue;
+    const BatchSystem& b = host[i];
+    MasBatchItem it{};
+    it.m = *reinterpret_cast<const MasApply*>(systems[i].mas);
+    it.r = b.r;
+    it.z = b.z;
+    it.st = b.st;
+    it.n = b.n;
+    it.k0 = b.k0;
+    it.kmax = b.kmax;
+    it.status_offset = S_STATUS;
+    it.rz_offset = slots_base(SET_RZ);
+    it.rz_stride = kSet;
+    mas_batch_geometry(it, static_cast<uint32_t>(items.size()), mt);
+    items.push_back(it);
+  }
+  const MasBatchWork* dtab[4] = {nullptr, nullptr, nullptr, nullptr};
+  uint32_t ntab[4] = {0, 0, 0, 0};
+  MasBatchItem* ditems = nullptr;
+  if (!items.empty()) {
+    ditems = static_cast<MasBatchItem*>(ysp::scratch(70, sizeof(MasBatchItem) * items.size()));
+    if (!ditems) return YSP_ERR_HIP;
+    YSP_CHECK(hipMemcpyAsync(ditems, items.data(), sizeof(MasBatchItem) * items.size(),
+                             hipMemcpyHostToDevice, s));
+    const std::vector<MasBatchWork>* src[4] = {&mt.restrict_, &mt.deep, &mt.solve, &mt.prolong};
+    for (int t = 0; t < 4; ++t) {
+      ntab[t] = static_cast<uint32_t>(src[t]->size());
+      if (!ntab[t]) continue;
+      MasBatchWork* d = static_cast<MasBatchWork*>(ysp::scratch(71 + t, sizeof(MasBatchWork) * ntab[t]));
+      if (!d) return YSP_ERR_HIP;
+      YSP_CHECK(hipMemcpyAsync(d, src[t]->data(), sizeof(MasBatchWork) * ntab[t],
+                               hipMemcpyHostToDevice, s));
+      dtab[t] = d;
+    }
+  }
   const uint32_t nsp = static_cast<uint32_t>(wsp.size()), nup = static_cast<uint32_t>(wup.size());
   for (uint32_t pass = 0; pass < passes; ++pass) {
     hipLaunchKernelGGL(batch_spmv_kernel<VT>, dim3(nsp), dim3(kB), 0, s, dsys, dsp,
                        static_cast<int>(pass));
     hipLaunchKernelGGL(batch_update_kernel<MT>, dim3(nup), dim3(kUB), 0, s, dsys, dup,
                        static_cast<int>(pass));
+    if (ditems)
+      if (int rc = launch_mas_apply_batch(ditems, dtab, ntab, static_cast<int>(pass), s)) return rc;
   }
   hipLaunchKernelGGL(batch_status_kernel, dim3(grid_for(count, 64)), dim3(64), 0, s, dsys, count,
                      dout);
@@ -1373,6 +1483,24 @@ extern "C" int ysp_pcg3_batch_start(ysp_pcg3_system* y, double threshold, int32_
   const uint32_t n = y->n, nv = n / 3;
   YSP_CHECK(hipMemsetAsync(y->cg_scalars, 0, kStateDoubles * sizeof(double), s));
   const dim3 vg(grid_for(nv, kB)), bl(kB);
+  if (y->mas) {
+    // r = b, x = 0, then z = M r and r.z from the MAS apply (the single-system start)
+    const MasApply& m = *reinterpret_cast<const MasApply*>(y->mas);
+    if (check_mas_args(m, n) != YSP_OK) {
+      ysp_set_error(__FILE__, __LINE__, "ysp_pcg3_batch_start: inconsistent MAS arguments");
+      return YSP_ERR_ARG;
+    }
+    const float* no_minv = nullptr;
+    hipLaunchKernelGGL(pcg3_init_kernel<float>, vg, bl, 0, s, no_minv, y->rhs, nullptr, nullptr,
+                       y->solution, y->work, y->work + n, nv, y->cg_scalars);
+    if (int rc = launch_mas_apply(m, y->work, n, y->work + n, y->cg_scalars + slots_base(SET_RZ + 0),
+                                  nullptr, s))
+      return rc;
+    hipLaunchKernelGGL(pcg3_init_finish_kernel, dim3(1), dim3(64), 0, s, y->cg_scalars, threshold,
+                       threshold_mode, 1);
+    YSP_CHECK_LAUNCH();
+    return YSP_OK;
+  }
   if (inverse_f32)
     hipLaunchKernelGGL(pcg3_init_kernel<float>, vg, bl, 0, s,
                        static_cast<const float*>(y->diagonal_inverse), y->rhs, nullptr, nullptr,
@@ -1392,7 +1520,8 @@ extern "C" int ysp_pcg3_batch_run(ysp_pcg3_system* systems, uint32_t count, uint
   if (count == 0 || passes == 0) return YSP_OK;
   if (!systems) return YSP_ERR_ARG;
   for (uint32_t i = 0; i < count; ++i)
-    if (systems[i].n == 0 || systems[i].n % 3 || !systems[i].diagonal_inverse) {
+    if (systems[i].n == 0 || systems[i].n % 3 ||
+        (!systems[i].diagonal_inverse && !systems[i].mas)) {
       ysp_set_error(__FILE__, __LINE__, "ysp_pcg3_batch_run: bad system");
       return YSP_ERR_ARG;
     }

@@ -281,6 +281,7 @@ struct GSlots {
   uint8_t col[KT];
   uint8_t xbase[KT];   // first position entry of the slot's vertex
   uint8_t amode[KT];   // 1: affine-matrix slot of an affine vertex (x_j, coefficient r_l)
+  uint8_t aff[KT];     // 1: a slot of an affine vertex (its A or its t)
   int8_t group_first[KT];
   uint32_t group_mask[KT];
   int ng;
@@ -303,6 +304,7 @@ __device__ __forceinline__ void load_gslots(const ScatterArgs& S, uint32_t e, in
     si.col[i] = 0;
     si.xbase[i] = static_cast<uint8_t>(col);
     si.amode[i] = 0;
+    si.aff[i] = 0;
     if (static_cast<uint32_t>(i) >= K) continue;
     const size_t o = static_cast<size_t>(e) * K + i;
     const uint32_t ii = S.seg_idx[o];
@@ -353,11 +355,69 @@ __device__ __forceinline__ int xmap(const GSlots<KT>& si, const double (*r)[3], 
   return si.xbase[i] + k;
 }
 
+// Per-workgroup sums of the contended destinations of the general kernel (atomic mode):
+// every element of an affine body adds into the same (A, A), (A, t), (t, t) blocks and the
+// same 12 gradient entries of that body, so a device-scope FP64 atomic per element and
+// entry serialises on a few addresses. Blocks coupling two affine groups and gradient
+// segments of affine groups are summed in LDS instead (LDS atomics, a few slots keyed by
+// the destination's offset) and leave with one global atomic per entry and workgroup; a
+// destination that finds no free slot takes the global atomic directly.
+constexpr uint32_t kHotEmpty = 0xffffffffu;
+constexpr int kHotBlocks = 8, kHotBlockMax = 81, kHotGrad = 4, kHotGradMax = 9;
+struct HotTable {
+  uint32_t bkey[kHotBlocks];    // value offset of the block in S.blocks
+  uint32_t bsize[kHotBlocks];   // h * w
+  uint32_t bdiag[kHotBlocks];   // offset in S.diag_blocks, kHotEmpty without one
+  uint32_t gkey[kHotGrad];      // gradient offset
+  uint32_t gsize[kHotGrad];
+  double bval[kHotBlocks][kHotBlockMax];
+  double gval[kHotGrad][kHotGradMax];
+};
+
+__device__ __forceinline__ int hot_slot(uint32_t* keys, int cap, uint32_t key) {
+  for (int j = 0; j < cap; ++j) {
+    const uint32_t cur = __hip_atomic_load(keys + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (cur == key) return j;
+    if (cur == kHotEmpty) {
+      const uint32_t old = atomicCAS(keys + j, kHotEmpty, key);
+      if (old == kHotEmpty || old == key) return j;
+    }
+  }
+  return -1;
+}
+
+__device__ __forceinline__ void hot_init(HotTable& t) {
+  for (uint32_t i = threadIdx.x; i < kHotBlocks * kHotBlockMax; i += blockDim.x)
+    t.bval[i / kHotBlockMax][i % kHotBlockMax] = 0.0;
+  for (uint32_t i = threadIdx.x; i < kHotGrad * kHotGradMax; i += blockDim.x)
+    t.gval[i / kHotGradMax][i % kHotGradMax] = 0.0;
+  if (threadIdx.x < kHotBlocks) t.bkey[threadIdx.x] = kHotEmpty;
+  if (threadIdx.x < kHotGrad) t.gkey[threadIdx.x] = kHotEmpty;
+  __syncthreads();
+}
+
+__device__ __forceinline__ void hot_flush(HotTable& t, const ScatterArgs& S) {
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < kHotBlocks * kHotBlockMax; i += blockDim.x) {
+    const uint32_t j = i / kHotBlockMax, o = i % kHotBlockMax;
+    if (t.bkey[j] == kHotEmpty || o >= t.bsize[j]) continue;
+    const double v = t.bval[j][o];
+    atomic_add_f64(S.blocks + t.bkey[j] + o, v);
+    if (t.bdiag[j] != kHotEmpty) atomic_add_f64(S.diag_blocks + t.bdiag[j] + o, v);
+  }
+  for (uint32_t i = threadIdx.x; i < kHotGrad * kHotGradMax; i += blockDim.x) {
+    const uint32_t j = i / kHotGradMax, o = i % kHotGradMax;
+    if (t.gkey[j] == kHotEmpty || o >= t.gsize[j]) continue;
+    atomic_add_f64(S.gradient + t.gkey[j] + o, t.gval[j][o]);
+  }
+}
+
 template <int KT, int N>
 __device__ __forceinline__ void scatter_generic(const ScatterArgs& S, uint32_t e,
                                                 const GSlots<KT>& si, const double (*r)[3],
                                                 const double (&g)[N],
-                                                const double (&H)[N * (N + 1) / 2]) {
+                                                const double (&H)[N * (N + 1) / 2],
+                                                HotTable* ht) {
   // gradient
   if (S.grad_staging) {
     double* out = S.grad_staging + static_cast<size_t>(e) * S.grad_local;
@@ -373,10 +433,16 @@ __device__ __forceinline__ void scatter_generic(const ScatterArgs& S, uint32_t e
     for (int i = 0; i < KT; ++i) {
       if (si.idx[i] < 2) continue;
       const uint32_t p = si.idx[i] - 2;
+      int gs = -1;
+      if (ht && si.aff[i] && si.size[i] <= kHotGradMax) {
+        gs = hot_slot(ht->gkey, kHotGrad, p);
+        if (gs >= 0) ht->gsize[gs] = si.size[i];
+      }
       for (int k = 0; k < si.size[i]; ++k) {
         double c;
         const int x = xmap(si, r, i, k, c);
-        atomic_add_f64(S.gradient + p + k, c * g[x]);
+        if (gs >= 0) atomicAdd(&ht->gval[gs][k], c * g[x]);
+        else atomic_add_f64(S.gradient + p + k, c * g[x]);
       }
     }
   }
@@ -402,6 +468,14 @@ __device__ __forceinline__ void scatter_generic(const ScatterArgs& S, uint32_t e
                                     : S.blocks + S.lookups[coord];
       double* dd = nullptr;
       if (!S.block_staging && gi == gj && S.diag_blocks) dd = S.diag_blocks + diag_offset(S, ii - 2, h);
+      int hs = -1;
+      if (ht && !S.block_staging && si.aff[fi] && si.aff[fj] && h * w <= kHotBlockMax) {
+        hs = hot_slot(ht->bkey, kHotBlocks, S.lookups[coord]);
+        if (hs >= 0) {
+          ht->bsize[hs] = h * w;
+          ht->bdiag[hs] = dd ? static_cast<uint32_t>(dd - S.diag_blocks) : kHotEmpty;
+        }
+      }
       const bool tr = !(ii < jj);
       for (int k = 0; k < h; ++k)
         for (int l = 0; l < w; ++l) {
@@ -423,6 +497,8 @@ __device__ __forceinline__ void scatter_generic(const ScatterArgs& S, uint32_t e
           const int o = tr ? l * h + k : k * w + l;
           if (S.block_staging) {
             dst[o] = v;
+          } else if (hs >= 0) {
+            atomicAdd(&ht->bval[hs][o], v);
           } else {
             atomic_add_f64(dst + o, v);
             if (dd) atomic_add_f64(dd + o, v);
@@ -490,7 +566,8 @@ __device__ __forceinline__ void eval_element_general(const UnionView& P, const A
                                                      const double* __restrict__ ep,
                                                      const double* __restrict__ sp, uint32_t method,
                                                      const ScatterArgs& SA,
-                                                     int* __restrict__ error_flag) {
+                                                     int* __restrict__ error_flag,
+                                                     HotTable* ht) {
   using T = Kind<ID>;
   GSlots<KT> si;
   load_gslots<KT>(SA, e, AF.mask ? -1 : T::N, si);
@@ -518,6 +595,7 @@ __device__ __forceinline__ void eval_element_general(const UnionView& P, const A
         if (i >= KT) continue;
         si.xbase[i] = static_cast<uint8_t>(3 * a);
         si.amode[i] = aff && q == 0;
+        si.aff[i] = aff;
         const int want = aff ? (q == 0 ? 9 : (q == 1 ? 3 : -1)) : (q == 0 ? 3 : -1);
         if (want > 0 && si.size[i] != want) si.ok = false;
         if (want < 0 && si.idx[i] >= 2) si.ok = false;
@@ -533,20 +611,27 @@ __device__ __forceinline__ void eval_element_general(const UnionView& P, const A
   const double* p = ep + static_cast<size_t>(e) * T::P;
   double E, g[T::N], H[T::N * (T::N + 1) / 2];
   element_eval<ID>(x, p, sp, method, E, g, H);
-  scatter_generic<KT, T::N>(SA, e, si, r, g, H);
+  scatter_generic<KT, T::N>(SA, e, si, r, g, H, ht);
 }
 
 // General kernel: every element (queue == nullptr) or the elements queued by the vertex
-// kernel (count in flags[1]), in a grid-stride loop.
+// kernel (count in flags[1]), in a grid-stride loop. With affine children and atomic
+// assembly, the affine destinations of the workgroup's elements are summed in LDS first.
+constexpr unsigned kGB = 256;
 template <int ID, int KT>
-__global__ void __launch_bounds__(kEB)
+__global__ void __launch_bounds__(kGB)
     energy_eval_kernel(UnionView P, AffineView AF, const uint32_t* __restrict__ e2v, uint32_t n,
                        const double* __restrict__ ep, const double* __restrict__ sp,
                        uint32_t method, ScatterArgs SA, int* __restrict__ flags,
                        const uint32_t* __restrict__ queue) {
+  __shared__ HotTable table;
+  const bool hot = AF.mask && !SA.block_staging;   // uniform over the grid
+  if (hot) hot_init(table);
   const uint32_t count = queue ? static_cast<uint32_t>(flags[1]) : n;
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < count; t += gridDim.x * blockDim.x)
-    eval_element_general<ID, KT>(P, AF, e2v, queue ? queue[t] : t, ep, sp, method, SA, flags);
+    eval_element_general<ID, KT>(P, AF, e2v, queue ? queue[t] : t, ep, sp, method, SA, flags,
+                                 hot ? &table : nullptr);
+  if (hot) hot_flush(table, SA);
 }
 
 // ----------------------------------------------------------------- vertex fast path
@@ -834,9 +919,10 @@ int launch_eval(const UnionView& U, const AffineView& AF, const uint32_t* e2v, u
     // the queue is usually empty, a full grid when the union has affine children
     hipLaunchKernelGGL((energy_eval_vertex_kernel<ID, KT>), g, b, 0, s, U, e2v, n, ep, sp, method,
                        SA, flags, queue);
-    const unsigned qg = AF.mask ? g.x : std::min<unsigned>(g.x, 256);
-    hipLaunchKernelGGL((energy_eval_kernel<ID, KT>), dim3(qg), b, 0, s, U, AF, e2v, n, ep, sp,
-                       method, SA, flags, queue);
+    const unsigned gg = grid_for(n, kGB);
+    const unsigned qg = AF.mask ? gg : std::min<unsigned>(gg, 64);
+    hipLaunchKernelGGL((energy_eval_kernel<ID, KT>), dim3(qg), dim3(kGB), 0, s, U, AF, e2v, n, ep,
+                       sp, method, SA, flags, queue);
   };
   if (SA.K <= 8) run(std::integral_constant<int, 8>());
   else run(std::integral_constant<int, kSlotsMax>());

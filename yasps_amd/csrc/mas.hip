@@ -186,26 +186,26 @@ __device__ __forceinline__ void descendant_sum(const double* __restrict__ r,
 // Residuals of levels 1 to 3, straight from the level-0 residual: workgroups [0, b1)
 // give 8 lanes to each level-1 node, [b1, b1 + b2) a wavefront to each level-2 node, the
 // rest a whole workgroup to each level-3 node (few nodes with many descendants).
-__global__ void __launch_bounds__(kB)
-    restrict_kernel(const double* __restrict__ r, const uint32_t* __restrict__ offsets,
-                    const uint32_t* __restrict__ items, uint32_t n1, uint32_t n2, uint32_t b1,
-                    uint32_t b2, double* __restrict__ rc, const double* __restrict__ stop) {
-  if (mas_stopped(stop)) return;
+__device__ __forceinline__ void restrict_body(const double* __restrict__ r,
+                                              const uint32_t* __restrict__ offsets,
+                                              const uint32_t* __restrict__ items, uint32_t n1,
+                                              uint32_t n2, uint32_t b1, uint32_t b2,
+                                              double* __restrict__ rc, uint32_t bid) {
   double sum[3];
   uint32_t J;
   bool write;
-  if (blockIdx.x < b1) {
-    J = blockIdx.x * (kB / 8) + threadIdx.x / 8;
+  if (bid < b1) {
+    J = bid * (kB / 8) + threadIdx.x / 8;
     if (J >= n1) return;                       // whole 8-lane groups leave together
     descendant_sum<8>(r, offsets, items, J, threadIdx.x % 8, sum);
     write = threadIdx.x % 8 == 0;
-  } else if (blockIdx.x < b1 + b2) {
-    J = n1 + (blockIdx.x - b1) * (kB / 64) + threadIdx.x / 64;
+  } else if (bid < b1 + b2) {
+    J = n1 + (bid - b1) * (kB / 64) + threadIdx.x / 64;
     if (J >= n2) return;                       // whole wavefronts leave together
     descendant_sum<64>(r, offsets, items, J, threadIdx.x % 64, sum);
     write = threadIdx.x % 64 == 0;
   } else {
-    J = n2 + (blockIdx.x - b1 - b2);           // grid sized so that J < n3
+    J = n2 + (bid - b1 - b2);                  // grid sized so that J < n3
     descendant_sum<kB>(r, offsets, items, J, threadIdx.x, sum);
     __shared__ double part[3][kB / 64];
     if (threadIdx.x % 64 == 0)
@@ -225,15 +225,22 @@ __global__ void __launch_bounds__(kB)
   }
 }
 
+__global__ void __launch_bounds__(kB)
+    restrict_kernel(const double* __restrict__ r, const uint32_t* __restrict__ offsets,
+                    const uint32_t* __restrict__ items, uint32_t n1, uint32_t n2, uint32_t b1,
+                    uint32_t b2, double* __restrict__ rc, const double* __restrict__ stop) {
+  if (mas_stopped(stop)) return;
+  restrict_body(r, offsets, items, n1, n2, b1, b2, rc, blockIdx.x);
+}
+
 // Residuals of levels 4 and deeper, a wavefront per node, from the level-3 residuals of
 // its level-3 descendants (coarse ids in [n2, n3), written by restrict_kernel; the nodes
 // written here, [n3, nc), are never read in this launch).
-__global__ void __launch_bounds__(kB)
-    restrict_deep_kernel(const uint32_t* __restrict__ offsets,
-                         const uint32_t* __restrict__ items, uint32_t n3, uint32_t nc,
-                         double* rc, const double* __restrict__ stop) {
-  if (mas_stopped(stop)) return;
-  const uint32_t J = n3 + blockIdx.x * (kB / 64) + threadIdx.x / 64;
+__device__ __forceinline__ void restrict_deep_body(const uint32_t* __restrict__ offsets,
+                                                   const uint32_t* __restrict__ items,
+                                                   uint32_t n3, uint32_t nc, double* rc,
+                                                   uint32_t bid) {
+  const uint32_t J = n3 + bid * (kB / 64) + threadIdx.x / 64;
   if (J >= nc) return;                         // whole wavefronts leave together
   double sum[3];
   descendant_sum<64>(rc, offsets, items, J, threadIdx.x % 64, sum);
@@ -242,6 +249,14 @@ __global__ void __launch_bounds__(kB)
     rc[3 * static_cast<size_t>(J) + 1] = sum[1];
     rc[3 * static_cast<size_t>(J) + 2] = sum[2];
   }
+}
+
+__global__ void __launch_bounds__(kB)
+    restrict_deep_kernel(const uint32_t* __restrict__ offsets,
+                         const uint32_t* __restrict__ items, uint32_t n3, uint32_t nc,
+                         double* rc, const double* __restrict__ stop) {
+  if (mas_stopped(stop)) return;
+  restrict_deep_body(offsets, items, n3, nc, rc, blockIdx.x);
 }
 
 // One 32-lane group per domain: c_loc = B^-1 r_loc. Lane i reads row i of the inverse,
@@ -304,6 +319,30 @@ __device__ __forceinline__ void local_solve_domain(
 
 // Workgroups [0, nb32) hold the FP32-bank domains [0, bf16_first), the rest the BF16-bank
 // domains [bf16_first, ndom): the bank type is uniform per workgroup.
+__device__ __forceinline__ void local_solve_body(
+    const float* __restrict__ inv, const uint16_t* __restrict__ inv_bf16, uint32_t bf16_first,
+    const uint32_t* __restrict__ dom_nodes, const uint32_t* __restrict__ dom_node_off,
+    uint32_t ndom, const double* __restrict__ r0, const double* __restrict__ rc,
+    uint32_t level0_domains, const float* __restrict__ dscale, float* __restrict__ c0,
+    float* __restrict__ cc, uint32_t bid) {
+  constexpr uint32_t kPer = kB / kD;
+  __shared__ __align__(16) float rs[kPer][kD];
+  const int g = threadIdx.x / kD, lane = threadIdx.x % kD;
+  const uint32_t nb32 = (bf16_first + kPer - 1) / kPer;
+  if (bid < nb32) {
+    const uint32_t d = bid * kPer + g;
+    if (d >= bf16_first) return;
+    local_solve_domain<false>(inv, dom_nodes, dom_node_off, d, static_cast<size_t>(d) * kD + lane,
+                              r0, rc, level0_domains, dscale, c0, cc, rs[g], lane);
+  } else {
+    const uint32_t d = bf16_first + (bid - nb32) * kPer + g;
+    if (d >= ndom) return;
+    local_solve_domain<true>(inv_bf16, dom_nodes, dom_node_off, d,
+                             static_cast<size_t>(d - bf16_first) * kD + lane, r0, rc,
+                             level0_domains, dscale, c0, cc, rs[g], lane);
+  }
+}
+
 __global__ void __launch_bounds__(kB)
     local_solve_kernel(const float* __restrict__ inv, const uint16_t* __restrict__ inv_bf16,
                        uint32_t bf16_first, const uint32_t* __restrict__ dom_nodes,
@@ -313,33 +352,20 @@ __global__ void __launch_bounds__(kB)
                        float* __restrict__ c0, float* __restrict__ cc,
                        const double* __restrict__ stop) {
   if (mas_stopped(stop)) return;
-  constexpr uint32_t kPer = kB / kD;
-  __shared__ __align__(16) float rs[kPer][kD];
-  const int g = threadIdx.x / kD, lane = threadIdx.x % kD;
-  const uint32_t nb32 = (bf16_first + kPer - 1) / kPer;
-  if (blockIdx.x < nb32) {
-    const uint32_t d = blockIdx.x * kPer + g;
-    if (d >= bf16_first) return;
-    local_solve_domain<false>(inv, dom_nodes, dom_node_off, d, static_cast<size_t>(d) * kD + lane,
-                              r0, rc, level0_domains, dscale, c0, cc, rs[g], lane);
-  } else {
-    const uint32_t d = bf16_first + (blockIdx.x - nb32) * kPer + g;
-    if (d >= ndom) return;
-    local_solve_domain<true>(inv_bf16, dom_nodes, dom_node_off, d,
-                             static_cast<size_t>(d - bf16_first) * kD + lane, r0, rc,
-                             level0_domains, dscale, c0, cc, rs[g], lane);
-  }
+  local_solve_body(inv, inv_bf16, bf16_first, dom_nodes, dom_node_off, ndom, r0, rc,
+                   level0_domains, dscale, c0, cc, blockIdx.x);
 }
 
 // z = w0 c0 + sum_l w_l c_l[map_l(node)], and r.z
-__global__ void __launch_bounds__(kB)
-    prolong_kernel(const float* __restrict__ c0, const float* __restrict__ cc,
-                   const uint32_t* __restrict__ maps, uint32_t nv, uint32_t ncoarse_levels,
-                   const float* __restrict__ weights, const double* __restrict__ r,
-                   double* __restrict__ z, double* __restrict__ rz_slot,
-                   const double* __restrict__ stop) {
-  if (mas_stopped(stop)) return;
-  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void prolong_body(const float* __restrict__ c0,
+                                             const float* __restrict__ cc,
+                                             const uint32_t* __restrict__ maps, uint32_t nv,
+                                             uint32_t ncoarse_levels,
+                                             const float* __restrict__ weights,
+                                             const double* __restrict__ r,
+                                             double* __restrict__ z, double* __restrict__ rz_slot,
+                                             uint32_t bid) {
+  const uint32_t v = bid * kB + threadIdx.x;
   double rz = 0.0;
   if (v < nv) {
     float s0 = weights[0] * c0[3 * v], s1 = weights[0] * c0[3 * v + 1],
@@ -360,8 +386,71 @@ __global__ void __launch_bounds__(kB)
   if (rz_slot) {
     __shared__ double lds[kB / 64];
     rz = block_sum<kB>(rz, lds);
-    if (threadIdx.x == 0) atomic_add_f64(rz_slot + (blockIdx.x % 64) * 16, rz);
+    if (threadIdx.x == 0) atomic_add_f64(rz_slot + (bid % 64) * 16, rz);
   }
+}
+
+__global__ void __launch_bounds__(kB)
+    prolong_kernel(const float* __restrict__ c0, const float* __restrict__ cc,
+                   const uint32_t* __restrict__ maps, uint32_t nv, uint32_t ncoarse_levels,
+                   const float* __restrict__ weights, const double* __restrict__ r,
+                   double* __restrict__ z, double* __restrict__ rz_slot,
+                   const double* __restrict__ stop) {
+  if (mas_stopped(stop)) return;
+  prolong_body(c0, cc, maps, nv, ncoarse_levels, weights, r, z, rz_slot, blockIdx.x);
+}
+
+// ------------------------------------------------------------------ batched apply
+// The same four steps for several systems per launch: a workgroup reads from a work table
+// which system's item it serves and which of that system's workgroups it is, so each
+// system computes exactly what its own launch would. Finished systems (status set by the
+// product pass) and passes beyond a system's last iteration do nothing.
+__device__ __forceinline__ bool batch_idle(const MasBatchItem& it, int pass) {
+  return it.k0 + pass > it.kmax || it.st[it.status_offset] != 0.0;
+}
+
+__global__ void __launch_bounds__(kB)
+    batch_restrict_kernel(const MasBatchItem* __restrict__ I, const MasBatchWork* __restrict__ W,
+                          int pass) {
+  const MasBatchWork w = W[blockIdx.x];
+  const MasBatchItem& it = I[w.item];
+  if (batch_idle(it, pass)) return;
+  restrict_body(it.r, it.m.restrict_offsets, it.m.restrict_items, it.n1, it.n2, it.b1, it.b2,
+                it.m.coarse_residual, w.blk);
+}
+
+__global__ void __launch_bounds__(kB)
+    batch_restrict_deep_kernel(const MasBatchItem* __restrict__ I,
+                               const MasBatchWork* __restrict__ W, int pass) {
+  const MasBatchWork w = W[blockIdx.x];
+  const MasBatchItem& it = I[w.item];
+  if (batch_idle(it, pass)) return;
+  restrict_deep_body(it.m.restrict_offsets, it.m.restrict_items, it.n3, it.m.num_coarse_nodes,
+                     it.m.coarse_residual, w.blk);
+}
+
+__global__ void __launch_bounds__(kB)
+    batch_local_solve_kernel(const MasBatchItem* __restrict__ I,
+                             const MasBatchWork* __restrict__ W, int pass) {
+  const MasBatchWork w = W[blockIdx.x];
+  const MasBatchItem& it = I[w.item];
+  if (batch_idle(it, pass)) return;
+  local_solve_body(it.m.inverses, it.m.inverses_bf16, it.first, it.m.domain_nodes,
+                   it.m.domain_node_offsets, it.m.num_domains, it.r, it.m.coarse_residual,
+                   it.m.level0_domains, it.m.domain_scale, it.m.fine_correction,
+                   it.m.coarse_correction, w.blk);
+}
+
+__global__ void __launch_bounds__(kB)
+    batch_prolong_kernel(const MasBatchItem* __restrict__ I, const MasBatchWork* __restrict__ W,
+                         int pass) {
+  const MasBatchWork w = W[blockIdx.x];
+  const MasBatchItem& it = I[w.item];
+  if (batch_idle(it, pass)) return;
+  const int k = it.k0 + pass;
+  prolong_body(it.m.fine_correction, it.m.coarse_correction, it.m.prolong_maps, it.nv,
+               it.m.num_coarse_levels, it.m.weights, it.r, it.z,
+               it.st + it.rz_offset + static_cast<size_t>(k % 3) * it.rz_stride, w.blk);
 }
 
 }  // namespace
@@ -449,6 +538,48 @@ int launch_mas_apply(const MasApply& m, const double* r, uint32_t n, double* z,
   hipLaunchKernelGGL(prolong_kernel, dim3(grid_for(nv, kB)), dim3(kB), 0, s, m.fine_correction,
                      m.coarse_correction, m.prolong_maps, nv, m.num_coarse_levels, m.weights, r, z,
                      rz_slots, stop);
+  YSP_CHECK_LAUNCH();
+  return YSP_OK;
+}
+
+void mas_batch_geometry(MasBatchItem& it, uint32_t item, MasBatchTables& t) {
+  const MasApply& m = it.m;
+  it.nv = it.n / 3;
+  it.n1 = it.n2 = it.n3 = it.b1 = it.b2 = 0;
+  if (m.num_coarse_levels) {
+    it.n1 = m.level_offsets[1];
+    it.n2 = m.level_offsets[m.num_coarse_levels >= 2 ? 2 : 1];
+    it.n3 = m.level_offsets[m.num_coarse_levels >= 3 ? 3 : m.num_coarse_levels];
+    it.b1 = grid_for(it.n1, kB / 8);
+    it.b2 = grid_for(it.n2 - it.n1, kB / 64);
+    for (uint32_t b = 0, nb = it.b1 + it.b2 + (it.n3 - it.n2); b < nb; ++b)
+      t.restrict_.push_back({item, b});
+    if (m.num_coarse_nodes > it.n3)
+      for (uint32_t b = 0, nb = grid_for(m.num_coarse_nodes - it.n3, kB / 64); b < nb; ++b)
+        t.deep.push_back({item, b});
+  }
+  it.first = m.inverses_bf16 ? m.bf16_first_domain : m.num_domains;
+  if (m.num_domains)
+    for (uint32_t b = 0, nb = grid_for(it.first, kB / kD) + grid_for(m.num_domains - it.first, kB / kD);
+         b < nb; ++b)
+      t.solve.push_back({item, b});
+  for (uint32_t b = 0, nb = grid_for(it.nv, kB); b < nb; ++b) t.prolong.push_back({item, b});
+}
+
+int launch_mas_apply_batch(const MasBatchItem* items, const MasBatchWork* const tables[4],
+                           const uint32_t counts[4], int pass, hipStream_t s) {
+  if (counts[0])
+    hipLaunchKernelGGL(batch_restrict_kernel, dim3(counts[0]), dim3(kB), 0, s, items, tables[0],
+                       pass);
+  if (counts[1])
+    hipLaunchKernelGGL(batch_restrict_deep_kernel, dim3(counts[1]), dim3(kB), 0, s, items,
+                       tables[1], pass);
+  if (counts[2])
+    hipLaunchKernelGGL(batch_local_solve_kernel, dim3(counts[2]), dim3(kB), 0, s, items,
+                       tables[2], pass);
+  if (counts[3])
+    hipLaunchKernelGGL(batch_prolong_kernel, dim3(counts[3]), dim3(kB), 0, s, items, tables[3],
+                       pass);
   YSP_CHECK_LAUNCH();
   return YSP_OK;
 }

@@ -3,6 +3,7 @@
 #include "common.hpp"
 
 #include <cstddef>
+#include <vector>
 
 namespace ysp {
 
@@ -46,6 +47,36 @@ __device__ __forceinline__ uint16_t to_bf16(float f) {
 // kernels do nothing (a converged Krylov loop).
 int launch_mas_apply(const MasApply& m, const double* r, uint32_t n, double* z,
                      double* rz_slots, const double* stop, hipStream_t s);
+
+// One system of a batched apply: its argument block, the PCG vectors it reads (r) and
+// writes (z), its scalar state (status word and the r.z slots of iteration k at
+// st + rz_offset + (k % 3) * rz_stride), the iteration number of pass 0 and the last
+// iteration; the launch geometry is filled by mas_batch_geometry.
+struct MasBatchItem {
+  MasApply m;
+  const double* r;
+  double* z;
+  double* st;
+  uint32_t n;
+  int32_t k0, kmax;
+  uint32_t status_offset, rz_offset, rz_stride;
+  uint32_t nv, n1, n2, n3, b1, b2, first;
+};
+struct MasBatchWork {
+  uint32_t item;   // index into the item table
+  uint32_t blk;    // workgroup index within that system's launch
+};
+struct MasBatchTables {
+  std::vector<MasBatchWork> restrict_, deep, solve, prolong;
+};
+
+// Fills the geometry of item `item` and appends its workgroups to the four work tables.
+void mas_batch_geometry(MasBatchItem& it, uint32_t item, MasBatchTables& t);
+
+// The four launches of one batched apply (tables: restrict, deep restrict, local solve,
+// prolong, device arrays of counts[i] entries; empty tables launch nothing).
+int launch_mas_apply_batch(const MasBatchItem* items, const MasBatchWork* const tables[4],
+                           const uint32_t counts[4], int pass, hipStream_t s);
 
 // YSP_OK when the argument block is consistent (sizes, level ranges, pointers).
 int check_mas_args(const MasApply& m, uint32_t n);

@@ -533,14 +533,18 @@ def quadratic_form(position, jacobian, inner_hessian, linear=None, offset=None) 
     the energy the reference's sparsity tests build symbolically from a linear map and a
     quadratic (``tests/differentiator/test_hessian_sparsity_permutation_gpu.py:17-38``).
     The projection acts on Q (the map is linear: ``differentiator.pyx:795-801``), so the
-    element Hessian is J^T spd(Q) J. J is m x 12 with 1 <= m <= 6."""
+    element Hessian is J^T spd(Q) J. J is m x 12 with 1 <= m <= 6. Q must be symmetric to
+    rounding (a product such as B diag(d) B^T); its symmetric part (Q + Q^T) / 2, the
+    Hessian of y^T Q y / 2, is used."""
     J = np.asarray(jacobian, dtype=np.float64)
     if J.ndim != 2 or J.shape[1] != 12 or not 1 <= J.shape[0] <= QUADRATIC_FORM_ROWS:
         raise ValueError("quadratic_form: jacobian must be m x 12 with 1 <= m <= 6")
     m = J.shape[0]
     Q = np.asarray(inner_hessian, dtype=np.float64)
-    if Q.shape != (m, m) or not np.allclose(Q, Q.T, rtol=0, atol=0):
+    if Q.shape != (m, m) or not np.all(np.isfinite(Q)) or \
+            np.abs(Q - Q.T).max() > 1e-12 * max(np.abs(Q).max(), 1e-300):
         raise ValueError("quadratic_form: inner_hessian must be a symmetric m x m matrix")
+    Q = 0.5 * (Q + Q.T)
     lin = np.zeros(m) if linear is None else np.asarray(linear, dtype=np.float64).reshape(-1)
     off = np.zeros(m) if offset is None else np.asarray(offset, dtype=np.float64).reshape(-1)
     if lin.size != m or off.size != m:

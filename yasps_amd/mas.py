@@ -27,6 +27,13 @@ from .hessian import vector
 from .mas_hierarchy import Hierarchy, build_hierarchy
 from .refine import RefinementBuffers, refine_solve
 
+
+class LocalInverseError(np.linalg.LinAlgError):
+    """A MAS domain matrix could not be inverted (``solver/mas/local_inverse.py:15``): a
+    Gauss-Jordan pivot at or below 1e-12 of the domain's largest diagonal entry, as for a
+    singular (PSD but not definite) or indefinite operator. The reference raises it from its
+    numeric update (``solver/mas/cuda_runtime.py:2904-2920``)."""
+
 _MODES = {"reference": 0, "relative_residual": 1}
 DOMAIN_SCALARS = 32          # padded domain size of the device kernels
 NO_DEST = np.uint32(0xFFFFFFFF)
@@ -584,6 +591,7 @@ class masSolver:
             dev.ptr(self._work), dev.ptr(self._scalars), C.byref(stats), dev.stream())
         _lib.check(rc, "ysp_pcg3_mas_solve")
         inverse_status = int(m.status.item())
+        self._check_inverses(inverse_status)
         t3 = time.perf_counter()
         self.iterations = max(0, int(stats.iterations))
         lay = m.layout
@@ -605,6 +613,14 @@ class masSolver:
         }
         return 0 if converged else -4
 
+    def _check_inverses(self, inverse_status):
+        """Raise when the last numeric update flagged a domain; the next solve rebuilds."""
+        if inverse_status & 1:
+            self._numeric_age = None
+            raise LocalInverseError(
+                "MAS local inversion failed: a domain matrix has a pivot at or below 1e-12 of "
+                "its largest diagonal entry (the operator is singular or indefinite)")
+
     def _solve_mixed(self, h, rhs, n, tolerance, max_iterations, rebuilt, t0, t1, t2):
         lib = _lib.load()
         f = h.full
@@ -624,6 +640,7 @@ class masSolver:
         out = refine_solve(h, rhs, self._solution, float(tolerance), int(max_iterations), inner,
                            self._refine)
         inverse_status = int(m.status.item())
+        self._check_inverses(inverse_status)
         t3 = time.perf_counter()
         self.iterations = out["iterations"]
         lay = m.layout

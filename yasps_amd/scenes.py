@@ -712,3 +712,129 @@ def build_contact_scene(cs: ContactScene, projection_method: int = 2, solver_nam
     s0.addMinimizeTarget([mv["position"], ab["affine_matrix"], ab["translation"]])
     s0.minimizer.setSolver(solver_name, **solver_options)
     return s0, handles
+
+
+# --------------------------------------------------------------- drop onto a floor
+@dataclass
+class DropScene:
+    """A soft tet cube, turned and tilted, its lowest corner `height` above a fixed
+    triangulated floor (the plane y = 0; gravity acts along -y in the inertia term). Vertices: the floor's first (fixed), then the cube's. Contact
+    barriers come from the collision module each trial step: pp / pe / pt / ee pairs below
+    dhat (squared distance) with stiffness kappa."""
+    rest: np.ndarray
+    position: np.ndarray
+    tets: np.ndarray              # union ids of the cube's tets
+    mass: np.ndarray              # moving (cube) vertices
+    n_fixed: int
+    faces: np.ndarray             # floor triangles then the cube's boundary (union ids)
+    edges: np.ndarray             # unique edges of those faces
+    surface: np.ndarray           # every floor vertex and the cube's boundary vertices
+    mesh_ids: np.ndarray          # 1 floor, 2 cube: no self-contact
+    mu: float
+    lam: float
+    dt: float
+    dhat: float
+    kappa: float
+
+    @property
+    def n_moving(self) -> int:
+        return self.rest.shape[0] - self.n_fixed
+
+
+def drop_scene(n: int = 3, side: float = 0.2, height: float = 0.02, floor_n: int = 8,
+               floor_size: float = 0.8, young: float = 1e5, poisson: float = 0.4,
+               density: float = 1000.0, dt: float = 1e-2, dhat: float = 1e-4,
+               kappa: float = 10.0) -> DropScene:
+    g = np.linspace(-(floor_size - side) / 2, (floor_size + side) / 2, floor_n)
+    fx, fz = np.meshgrid(g, g, indexing="ij")
+    floor = np.stack([fx.ravel(), np.zeros(fx.size), fz.ravel()], axis=1)
+    ff = []
+    for i in range(floor_n - 1):
+        for j in range(floor_n - 1):
+            a = i * floor_n + j
+            ff += [[a, a + 1, a + floor_n + 1], [a, a + floor_n + 1, a + floor_n]]
+    ff = np.array(ff, dtype=np.int64)
+    pos, tets = cube_tet_mesh(n, side)
+    pos = pos[:, [0, 2, 1]]                       # the cube's third axis becomes y
+    # the axis swap mirrors the mesh: swap two vertices of every tet back to det > 0
+    tets = tets[:, [0, 1, 3, 2]]
+    # turned about y and tilted about x, so that no cube edge runs parallel to a floor edge
+    # (edge-edge distances of near-parallel lines are ill-conditioned without a mollifier)
+    a, b = np.radians(30.0), np.radians(12.0)
+    ry = np.array([[np.cos(a), 0, np.sin(a)], [0, 1, 0], [-np.sin(a), 0, np.cos(a)]])
+    rx = np.array([[1, 0, 0], [0, np.cos(b), -np.sin(b)], [0, np.sin(b), np.cos(b)]])
+    c = pos.mean(axis=0)
+    pos = (pos - c) @ (rx @ ry).T + [side / 2, 0.0, side / 2]
+    pos[:, 1] += height - pos[:, 1].min()
+    nf = floor.shape[0]
+    rest = np.concatenate([floor, pos])
+    tets = tets + nf
+    bf = boundary_faces(tets)
+    faces = np.concatenate([ff, bf])
+    edges = np.array(sorted({tuple(sorted((int(t[a]), int(t[b])))) for t in faces
+                             for a, b in ((0, 1), (1, 2), (2, 0))}), dtype=np.int64)
+    surface = np.concatenate([np.arange(nf), np.unique(bf)])
+    p = rest[tets]
+    vol = np.linalg.det(np.stack([p[:, 1] - p[:, 0], p[:, 2] - p[:, 0],
+                                  p[:, 3] - p[:, 0]], axis=2)) / 6.0
+    assert (vol > 0).all()
+    mass = np.zeros(rest.shape[0])
+    np.add.at(mass, tets.ravel(), np.repeat(density * vol / 4.0, 4))
+    mu, lam = lame_parameters(young, poisson)
+    mesh_ids = np.r_[np.ones(nf), 2 * np.ones(pos.shape[0])].astype(np.uint32)
+    return DropScene(rest=rest, position=rest.copy(), tets=tets, mass=mass[nf:], n_fixed=nf,
+                     faces=faces, edges=edges, surface=surface, mesh_ids=mesh_ids, mu=mu, lam=lam,
+                     dt=dt, dhat=dhat, kappa=kappa)
+
+
+def build_drop_scene(ds: DropScene, projection_method: int = 2, solver_name: str = "jacobian",
+                     **solver_options):
+    """The drop scene through the drop-in API as ``examples/one_bunny/one_bunny.py`` lays
+    out its contact scene: fixed/moving vertex union, SNH on the tets, inertia (with
+    gravity) on the moving vertices and four dynamic barrier primitives pp / pe / pt / ee
+    over the union (empty until the collision module fills them). Returns (scene, handles)."""
+    from . import scene, stable_neo_hookean, inertia
+    from . import energies as en
+    s0 = scene("drop")
+    dt = s0.addConstant("dt", rows=1, cols=1)
+    dt.updateValue([ds.dt])
+    m = s0.addMesh("drop")
+    nf = ds.n_fixed
+    fv = m.addPrimitive("fixed_vertices", numInstances=nf)
+    mv = m.addPrimitive("moving_vertices", numInstances=ds.n_moving)
+    m.addPrimitiveUnion("vertices", [fv, mv])
+    fv.addAttribute("position", rows=3, cols=1).updateValue(ds.position[:nf])
+    fv.addConstant("rest_position", rows=3, cols=1).updateValue(ds.rest[:nf])
+    mv.addAttribute("position", rows=3, cols=1).updateValue(ds.position[nf:])
+    mv.addConstant("rest_position", rows=3, cols=1).updateValue(ds.rest[nf:])
+    mv.addConstant("last_position", rows=3, cols=1).updateValue(ds.position[nf:])
+    mv.addConstant("velocity", rows=3, cols=1).updateValue(np.zeros((ds.n_moving, 3)))
+    mv.addConstant("mass", rows=1, cols=1).updateValue(ds.mass)
+    m.vertices.addAttribute("position")
+    m.vertices.addAttribute("rest_position")
+    mu = m.addConstant("mu", rows=1, cols=1)
+    mu.updateValue([ds.mu])
+    lam = m.addConstant("lam", rows=1, cols=1)
+    lam.updateValue([ds.lam])
+    tets = m.addPrimitive("tets", numInstances=ds.tets.shape[0])
+    t2v = tets.addConnectivity("tet2v", m.vertices, ds.tets, 4)
+    tp = tets.addAttribute("positions", through=t2v, source=m.vertices["position"])
+    trp = tets.addAttribute("rest_positions", through=t2v, source=m.vertices["rest_position"])
+    handles = {"mesh": m, "moving": mv, "fixed": fv}
+    handles["snh"] = tets.addAttribute("snh", computed_attribute=stable_neo_hookean(trp, tp, mu, lam, dt))
+    handles["inertia"] = mv.addAttribute("inertia", computed_attribute=inertia(
+        mv["last_position"], mv["velocity"], dt, mv["position"], mv["mass"]))
+    s0.addEnergy(handles["snh"], projection_method=projection_method)
+    s0.addEnergy(handles["inertia"], projection_method=-1)
+    for name, arity, fn in (("pp", 2, en.point_point), ("pe", 3, en.point_edge),
+                            ("pt", 4, en.point_triangle), ("ee", 4, en.edge_edge)):
+        prim = m.addPrimitive(name, numInstances=0, isDynamic=True)
+        conn = prim.addConnectivity(name + "2v", m.vertices, np.zeros((0, arity), dtype=np.int64), arity)
+        pos = prim.addAttribute("positions", through=conn, source=m.vertices["position"])
+        handles[name] = prim.addAttribute(name + "_barrier", computed_attribute=fn(pos, ds.dhat, ds.kappa))
+        handles[name + "_primitive"] = prim
+        handles[name + "_connectivity"] = conn
+        s0.addEnergy(handles[name], projection_method=projection_method, dynamic_instances=True)
+    s0.addMinimizeTarget([mv["position"]])
+    s0.minimizer.setSolver(solver_name, **solver_options)
+    return s0, handles

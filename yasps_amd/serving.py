@@ -9,10 +9,20 @@ system and every request reduces into its own scalar state. Requests that arrive
 others iterate join at the next chunk; finished ones leave and free their slot.
 
 Meshes of one size share a pool of slots whose static sparsity structure (index
-pipeline, block layout, full block-CSR) is built once; a request only reloads positions
-and material, evaluates the elements and assembles (``hessian.compute``). By default a
-loader thread does that on a second stream, so the element kernels of arriving requests
-overlap the batched PCG chunks of the admitted ones.
+pipeline, block layout, full block-CSR, and with ``solver="mas"`` the MAS hierarchy and its
+device tables) is built once; a request only reloads positions and material, evaluates
+the elements and assembles (``hessian.compute``), then inverts its block-Jacobi blocks or
+its MAS domains. By default a loader thread does that on a second stream, so the element
+kernels of arriving requests overlap the batched PCG chunks of the admitted ones.
+
+Preconditioners: block-Jacobi or MAS (``solver/mas/pcg.py:23-102``: every system of a
+batch carries its own MAS, applied in batched restriction / local-solve / prolongation
+launches). Precision: ``"fp64"`` runs the Krylov loop on the FP64 operator to the
+tolerance; ``"mixed"`` runs it on FP32 block values (and FP32 / BF16 MAS banks, FP32
+Jacobi blocks) with FP64 vectors and dots, and an FP64 iterative refinement around it: when
+a system's inner solve ends, the true residual b - A x is formed with the FP64 values, and
+if it is above the tolerance the system restarts on A d = r (correction) within the
+running batch, until ||b - A x|| <= tol ||b||.
 """
 from __future__ import annotations
 
@@ -47,7 +57,9 @@ class SolveRequest:
     admitted: Optional[float] = None
     finished: Optional[float] = None
     status: Optional[int] = None
-    iterations: int = 0
+    iterations: int = 0               # inner PCG iterations (all refinement steps)
+    outer_iterations: int = 0         # FP64 refinement steps (mixed precision)
+    rel_residual: Optional[float] = None   # final true FP64 residual (mixed precision)
     solution: Optional[np.ndarray] = None
 
     @property
@@ -58,9 +70,10 @@ class SolveRequest:
 class _Slot:
     """One device-resident problem of mesh size n (reused across requests)."""
 
-    def __init__(self, n: int, precision: str):
+    def __init__(self, n: int, precision: str, solver: str = "jacobian", coarse_bank: str = "fp32"):
         self.n = n
         self.precision = precision
+        self.solver = solver
         cs = cube_scene(n)
         self.scene, self.h = build_yasps_scene(cs, threshold_mode="relative_residual")
         self.cs = cs
@@ -82,9 +95,32 @@ class _Slot:
             + (4 * 9 * f.nnz if precision == "mixed" else 0)
             + 4 * (2 * f.nnz + m // 3)
             + (staging.numel() * staging.element_size() if staging is not None else 0))
+        self.mas = None
+        if solver == "mas":
+            from .mas import masSolver
+            # 4 levels, coarsest weight 1 (the configuration the benchmark legs run)
+            self.mas = masSolver(max_levels=4, coarsest_level_weight=1.0,
+                                 threshold_mode="relative_residual", coarse_bank=coarse_bank)
+            shared = _MAS_HIERARCHIES.get(n)
+            if shared is not None:                        # slots of one mesh size share it
+                self.mas._hierarchy = shared
+                self.mas._hierarchy_key = hs.full.key[:2]
+            self.mas._ensure(hs, m)                       # hierarchy and device tables
+            _MAS_HIERARCHIES[n] = self.mas._hierarchy
+            self.footprint += _device_bytes(self.mas._device)
+        elif solver != "jacobian":
+            raise ValueError("solver must be 'jacobian' or 'mas'")
+        if precision == "mixed":
+            self.x = dev.zeros(m, "float64")             # the refined solution
+            self.r = dev.zeros(m, "float64")             # FP64 residual b - A x
+            self.footprint += 16 * m
         self.system = _lib.Pcg3System()
         self.pos_dev = dev.empty(3 * (cs.rest.shape[0] - cs.n_fixed), "float64")
         self.request: Optional[SolveRequest] = None
+        self.outer = 0
+        self.inner_iterations = 0
+        self.rel_residual = None
+        self.failed = None
 
     def load(self, req: SolveRequest, tolerance: float, max_iterations: int):
         """Positions and material of the request, element evaluation and assembly, block
@@ -101,34 +137,112 @@ class _Slot:
         hs = self.scene.minimizer.computeNumericValue()
         lib = _lib.load()
         s = dev.stream()
-        counts = [w.numInstances for w in hs.wrt]
-        sizes = [w.size for w in hs.wrt]
-        _lib.check(lib.ysp_invert_diagonal_blocks(
-            dev.ptr(hs.diagonal_blocks), dev.ptr(hs.diagonal_blocks_inverse),
-            _lib.u32_array(hs.diagonal_blocks_start_cpu[:-1]), _lib.u32_array(counts),
-            _lib.u32_array(sizes), len(hs.wrt), s), "ysp_invert_diagonal_blocks")
         f = hs.full
         y = self.system
         mixed = self.precision == "mixed"
-        if mixed:
-            values = hs.full_values_f32()
-            nd = hs.diagonal_blocks_inverse.numel()
-            _lib.check(lib.ysp_cast_f32(dev.ptr(hs.diagonal_blocks_inverse), dev.ptr(self.minv32), nd, s),
-                       "ysp_cast_f32")
-            minv = self.minv32
+        minv = None
+        if self.mas is not None:
+            self.mas._device.update(f)                 # coarse operators, domain inverses
+            y.mas = C.addressof(self.mas._device.args)
         else:
-            values = f.v64
+            counts = [w.numInstances for w in hs.wrt]
+            sizes = [w.size for w in hs.wrt]
+            _lib.check(lib.ysp_invert_diagonal_blocks(
+                dev.ptr(hs.diagonal_blocks), dev.ptr(hs.diagonal_blocks_inverse),
+                _lib.u32_array(hs.diagonal_blocks_start_cpu[:-1]), _lib.u32_array(counts),
+                _lib.u32_array(sizes), len(hs.wrt), s), "ysp_invert_diagonal_blocks")
             minv = hs.diagonal_blocks_inverse
+            if mixed:
+                nd = hs.diagonal_blocks_inverse.numel()
+                _lib.check(lib.ysp_cast_f32(dev.ptr(hs.diagonal_blocks_inverse), dev.ptr(self.minv32),
+                                            nd, s), "ysp_cast_f32")
+                minv = self.minv32
+            y.mas = None
+        values = hs.full_values_f32() if mixed else f.v64
         self._keep = (values, minv)
         y.row_ptr, y.col, y.values, y.nnz = dev.ptr(f.row_ptr), dev.ptr(f.col), dev.ptr(values), f.nnz
         y.diagonal_inverse = dev.ptr(minv)
-        y.rhs = dev.ptr(hs.gradient.value)
         y.n = hs.rows
-        y.solution, y.work, y.cg_scalars = dev.ptr(self.solution), dev.ptr(self.work), dev.ptr(self.scalars)
+        y.work, y.cg_scalars = dev.ptr(self.work), dev.ptr(self.scalars)
         y.max_iterations = int(max_iterations)
-        _lib.check(lib.ysp_pcg3_batch_start(C.byref(y), float(tolerance), 1, int(mixed), s),
-                   "ysp_pcg3_batch_start")
         self.request = req
+        self.tolerance = float(tolerance)
+        self.max_iterations = int(max_iterations)
+        self.outer = 0
+        self.inner_iterations = 0
+        self.rel_residual = None
+        self.failed = None
+        if mixed:
+            # first inner solve on b itself (r = b, x = 0), to a slightly tighter tolerance
+            self.x.zero_()
+            self._start(hs.gradient.value, self.x, _SAFETY * self.tolerance)
+        else:
+            self._start(hs.gradient.value, self.solution, self.tolerance)
+
+    def _start(self, rhs, solution, tol_in):
+        y = self.system
+        y.rhs = dev.ptr(rhs)
+        y.solution = dev.ptr(solution)
+        y.max_iterations = max(1, self.max_iterations - self.inner_iterations)
+        mixed = self.precision == "mixed"
+        _lib.check(_lib.load().ysp_pcg3_batch_start(C.byref(y), float(tol_in), 1,
+                                                    int(mixed and self.mas is None), dev.stream()),
+                   "ysp_pcg3_batch_start")
+
+    def finish_inner(self) -> bool:
+        """The inner solve of this slot ended (status in self.system). Returns True when the
+        request is done; otherwise a correction solve has been started (mixed precision)."""
+        y = self.system
+        self.inner_iterations += max(0, int(y.iterations))
+        if self.mas is not None and int(self.mas._device.status.item()) & 1:
+            self.failed = "MAS local inversion failed (singular or indefinite domain)"
+            return True
+        if self.precision != "mixed":
+            return True             # status >= 0: ||b - A x|| <= tol ||b|| (FP64 recurrence)
+        lib = _lib.load()
+        hs = self.hs
+        f = hs.full
+        rr, bb = C.c_double(0.0), C.c_double(0.0)
+        first = self.outer == 0
+        # x += d (after a correction solve), r = b - A x with the FP64 values
+        _lib.check(lib.ysp_bsr3_refine(dev.ptr(f.row_ptr), dev.ptr(f.col), dev.ptr(f.v64), 0, f.nnz,
+                                       hs.rows, dev.ptr(hs.gradient.value), dev.ptr(self.x),
+                                       None if first else dev.ptr(self.solution), dev.ptr(self.r),
+                                       C.byref(rr), C.byref(bb), dev.stream()), "ysp_bsr3_refine")
+        self.outer += 1
+        rel = math.sqrt(rr.value / bb.value) if bb.value > 0 else 0.0
+        self.rel_residual = rel
+        if (rel <= self.tolerance or not math.isfinite(rel) or self.outer > _MAX_OUTER
+                or self.inner_iterations >= self.max_iterations or (y.status < 0 and y.status != -4)):
+            return True
+        self._start(self.r, self.solution, min(0.5, max(1e-14, _SAFETY * self.tolerance / rel)))
+        return False
+
+    @property
+    def result(self):
+        """The solution of the request (refined x in mixed precision)."""
+        return self.x if self.precision == "mixed" else self.solution
+
+
+# MAS hierarchy of the cube mesh of each size (a function of the sparsity structure only)
+_MAS_HIERARCHIES: Dict[int, object] = {}
+_SAFETY = 0.7        # inner tolerance factor of the refinement (as refine.refine_solve)
+_MAX_OUTER = 8
+
+
+def _device_bytes(obj) -> int:
+    """Bytes of the device tensors an object holds (attributes and lists of tensors)."""
+    total = 0
+    for v in vars(obj).values():
+        for t in (v if isinstance(v, (list, tuple)) else [v]):
+            if isinstance(t, (list, tuple)):
+                items = t
+            else:
+                items = [t]
+            for u in items:
+                if hasattr(u, "is_cuda") and u.is_cuda:
+                    total += int(u.numel() * u.element_size())
+    return total
 
 
 class SolveServer:
@@ -136,9 +250,16 @@ class SolveServer:
 
     def __init__(self, hbm_budget_bytes: int = 64 << 30, max_active: int = 32, passes: int = 8,
                  tolerance: float = 1e-6, max_iterations: int = 20000, precision: str = "fp64",
-                 keep_solutions: bool = False, overlap_loads: bool = True):
+                 keep_solutions: bool = False, overlap_loads: bool = True,
+                 solver: str = "jacobian", coarse_bank: str = "fp32"):
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
+        if solver not in ("jacobian", "mas"):
+            raise ValueError("solver must be 'jacobian' or 'mas'")
+        if coarse_bank not in ("fp32", "bf16"):
+            raise ValueError("coarse_bank must be 'fp32' or 'bf16'")
+        self.solver = solver
+        self.coarse_bank = coarse_bank
         self.overlap_loads = bool(overlap_loads)
         self._lock = threading.Lock()           # the slot pool, shared with the loader
         self.budget = int(hbm_budget_bytes)
@@ -153,6 +274,9 @@ class SolveServer:
         self.batch_sizes: List[int] = []
 
     # ------------------------------------------------------------ slots
+    def _new_slot(self, n: int) -> _Slot:
+        return _Slot(n, self.precision, self.solver, self.coarse_bank)
+
     def _in_use(self) -> int:
         return sum(s.footprint for slots in self.pool.values() for s in slots if s.request is not None)
 
@@ -160,14 +284,14 @@ class SolveServer:
         """Build one idle slot per mesh size (static structures) before serving."""
         for n in sorted(set(int(x) for x in sizes)):
             if not self.pool.get(n):
-                self.pool.setdefault(n, []).append(_Slot(n, self.precision))
+                self.pool.setdefault(n, []).append(self._new_slot(n))
         dev.torch().cuda.synchronize()
 
     def _acquire(self, n: int) -> Optional[_Slot]:
         for s in self.pool.get(n, []):
             if s.request is None:
                 return s
-        slot = _Slot(n, self.precision)
+        slot = self._new_slot(n)
         self.pool.setdefault(n, []).append(slot)
         return slot
 
@@ -196,14 +320,19 @@ class SolveServer:
         keep = []
         for sl, y in zip(active, arr):
             sl.system = y
-            if y.status != _lib.PCG3_RUNNING:
+            if y.status != _lib.PCG3_RUNNING and sl.finish_inner():
                 req = sl.request
                 req.finished = t_end
-                req.status = int(y.status)
-                req.iterations = int(y.iterations)
+                req.status = -6 if sl.failed else int(y.status)
+                req.iterations = sl.inner_iterations
+                req.outer_iterations = sl.outer
+                req.rel_residual = sl.rel_residual
+                if self.precision == "mixed" and sl.rel_residual is not None and \
+                        not sl.rel_residual <= self.tolerance and req.status >= 0:
+                    req.status = -4          # the refinement did not reach the tolerance
                 if self.keep_solutions:
                     torch.cuda.synchronize()
-                    req.solution = dev.to_host(sl.solution).copy()
+                    req.solution = dev.to_host(sl.result).copy()
                 with self._lock:
                     sl.request = None
                 done.append(req)
@@ -249,7 +378,7 @@ class SolveServer:
                             if slot is not None:
                                 slot.request = req
                         if slot is None:              # a new slot, built outside the lock
-                            slot = _Slot(req.n, self.precision)
+                            slot = self._new_slot(req.n)
                             slot.request = req
                             with self._lock:
                                 self.pool.setdefault(req.n, []).append(slot)
