@@ -547,6 +547,25 @@ int ysp_pcg3_product_pass(const uint32_t* row_ptr, const uint32_t* col, const vo
                           int32_t values_f32, uint32_t nnz, uint32_t n, double* work,
                           double* cg_scalars, int32_t prepare, void* stream);
 
+/* No reference counterpart. ysp_pcg3_solve for callers that cannot allocate memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * work vectors and the scalar state live in a device array of the library, one per GPU
+ * (held until the process ends) of YSP_PCG3_SCALARS + 5 * YSP_PCG3_NOALLOC_MAX_N doubles
+ * (15.8 MB), looked up on the first call on each device (the stream's device must be the
+ * current one). n above YSP_PCG3_NOALLOC_MAX_N is rejected with YSP_ERR_ARG. No device,
+ * pinned or library allocation, no events: every block row goes through the one product
+ * kernel and the host reads the status with stream synchronisations. All calls on one
+ * GPU share the array, so calls that overlap on one GPU (other streams or threads) race.
+ * Same recurrence, stopping rules and return codes as ysp_pcg3_solve. Synchronous. */
+#define YSP_PCG3_NOALLOC_MAX_N 393216
+
+int ysp_pcg3_solve_noalloc(uint32_t max_iterations, double threshold, int32_t threshold_mode,
+                           const uint32_t* row_ptr, const uint32_t* col, const void* values,
+                           int32_t values_f32, uint32_t nnz, const void* diagonal_inverse,
+                           int32_t inverse_f32, const double* rhs, uint32_t n, double* solution,
+                           const double* initial_guess, int32_t zero_initial_guess,
+                           ysp_pcg_stats* stats, void* stream);
+
 /* ------------------------------------------------------------ batched PCG (serving) */
 /* No reference counterpart: several independent 3x3 systems (e.g. the Newton solves of
  * different requests) advance through the same launches; every workgroup owns a chunk

@@ -142,6 +142,8 @@ SIGNATURES = {
     "ysp_pcg3_solve": (C.c_int, [U32, D, I32, P, P, P, I32, U32, P, I32, P, U32, P, P, I32, P, P,
                                  C.POINTER(PcgStats), P]),
     "ysp_pcg3_product_pass": (C.c_int, [P, P, P, I32, U32, U32, P, P, I32, P]),
+    "ysp_pcg3_solve_noalloc": (C.c_int, [U32, D, I32, P, P, P, I32, U32, P, I32, P, U32, P, P,
+                                         I32, C.POINTER(PcgStats), P]),
     "ysp_pcg3_batch_start": (C.c_int, [C.POINTER(Pcg3System), D, I32, I32, P]),
     "ysp_pcg3_batch_run": (C.c_int, [C.POINTER(Pcg3System), U32, U32, I32, I32, P]),
     "ysp_ccd_candidates": (C.c_int, [P, P, D, U32, P, U32, P, U32, P, U32, P, D, P, U32,

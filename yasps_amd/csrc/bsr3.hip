@@ -1259,6 +1259,72 @@ extern "C" int ysp_pcg3_product_pass(const uint32_t* row_ptr, const uint32_t* co
   return YSP_OK;
 }
 
+// ---------------------------------------------------------- PCG without allocations
+// Scalar state (first YSP_PCG3_SCALARS doubles) and work vectors (5 n doubles) of
+// ysp_pcg3_solve_noalloc. An array of the code object: every GPU the library runs on
+// holds its own copy, at its own address, until the process ends.
+static __device__ __align__(256) double
+    scratch[YSP_PCG3_SCALARS + 5 * static_cast<size_t>(YSP_PCG3_NOALLOC_MAX_N)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mutex;
+double* g_noalloc_base[kMaxDevices] = {};
+
+// Device address of the current GPU's copy of the array, looked up once per device. In
+// host code the array's name stands for a host-side shadow, not for device memory: only
+// the looked-up address may reach a kernel or a copy.
+int noalloc_base(double** out) {
+  int dev = -1;
+  YSP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    ysp_set_error(__FILE__, __LINE__, "ysp_pcg3_solve_noalloc: device ordinal out of range");
+    return YSP_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+  if (!g_noalloc_base[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(::scratch));
+    if (e != hipSuccess || !p) {
+      ysp_set_error(__FILE__, __LINE__, e != hipSuccess ? hipGetErrorString(e)
+                                                        : "hipGetSymbolAddress returned NULL");
+      return YSP_ERR_HIP;
+    }
+    g_noalloc_base[dev] = static_cast<double*>(p);
+  }
+  *out = g_noalloc_base[dev];
+  return YSP_OK;
+}
+}  // namespace
+
+extern "C" int ysp_pcg3_solve_noalloc(uint32_t max_iterations, double threshold,
+                                      int32_t threshold_mode, const uint32_t* row_ptr,
+                                      const uint32_t* col, const void* values, int32_t values_f32,
+                                      uint32_t nnz, const void* diagonal_inverse,
+                                      int32_t inverse_f32, const double* rhs, uint32_t n,
+                                      double* solution, const double* initial_guess,
+                                      int32_t zero_initial_guess, ysp_pcg_stats* stats,
+                                      void* stream) {
+  if (stats) *stats = ysp_pcg_stats{0, 0, 0, 0, 0, 0};
+  if (n == 0) return 0;
+  if (n % 3 || (threshold_mode != 0 && threshold_mode != 1)) {
+    ysp_set_error(__FILE__, __LINE__, "ysp_pcg3_solve_noalloc: bad arguments");
+    return YSP_ERR_ARG;
+  }
+  if (n > YSP_PCG3_NOALLOC_MAX_N) {
+    ysp_set_error(__FILE__, __LINE__,
+                  "ysp_pcg3_solve_noalloc: n exceeds YSP_PCG3_NOALLOC_MAX_N (393216 unknowns); "
+                  "use ysp_pcg3_solve with caller-provided work buffers");
+    return YSP_ERR_ARG;
+  }
+  double* base = nullptr;
+  if (int rc = noalloc_base(&base)) return rc;
+  const double* x0 = (!zero_initial_guess && initial_guess) ? initial_guess : nullptr;
+  return pcg3_dispatch(max_iterations, threshold, threshold_mode, row_ptr, col, values,
+                       values_f32, nnz, diagonal_inverse, inverse_f32, rhs, n, solution, x0,
+                       base + YSP_PCG3_SCALARS, base, stats, as_stream(stream), true);
+}
+
 extern "C" int ysp_pcg3_mas_solve(uint32_t max_iterations, double threshold,
                                   int32_t threshold_mode, const uint32_t* row_ptr,
                                   const uint32_t* col, const void* values, int32_t values_f32,
