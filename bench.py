@@ -35,6 +35,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--cold-reps", type=int, default=20,
+                    help="rotations over the operator copies in the HBM-regime product timing")
     ap.add_argument("--no-pmc", action="store_true",
                     help="skip the hardware-counter traffic passes (rocprofv3 --pmc)")
     ap.add_argument("--no-forms", action="store_true",
@@ -75,7 +77,7 @@ def measure_traffic(args):
     tmp = tempfile.mkdtemp(prefix="ysp_pmc_", dir="/tmp")
     env = dict(os.environ, TMPDIR="/tmp")
     detail = {"command": ("rocprofv3 --pmc <COUNTER> --output-format csv -- python3 bench.py "
-                          f"--kernels-only --n {args.n} --seed {args.seed} --spmv-reps 5")}
+                          f"--kernels-only --n {args.n} --seed {args.seed} --spmv-reps 5 --cold-reps 0")}
     total = 0.0
     try:
         for counter in ("FETCH_SIZE", "WRITE_SIZE"):
@@ -83,7 +85,7 @@ def measure_traffic(args):
             cmd = ["timeout", "-s", "KILL", "180", rocprof, "--pmc", counter, "--output-format",
                    "csv", "-d", d, "-o", "run", "--", sys.executable, os.path.abspath(__file__),
                    "--kernels-only", "--n", str(args.n), "--seed", str(args.seed),
-                   "--spmv-reps", "5"]
+                   "--spmv-reps", "5", "--cold-reps", "0"]
             t0 = time.perf_counter()
             r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True)
             files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
@@ -105,7 +107,9 @@ def measure_traffic(args):
                             copy.append(float(row["Counter_Value"]))
             if not prod or not copy:
                 return None, {**detail, "error": f"{counter}: kernels missing in the counter file"}
-            # the calibration copy moves 256 MiB each way: scale = known KiB / counter KiB
+            # the calibration copies (256 MiB each way) are the largest copy dispatches;
+            # smaller ones are uploads of the set-up
+            copy = [c for c in copy if c >= 0.5 * max(copy)]
             scale = 262144.0 / float(np.median(copy))
             kib = float(np.median(prod))
             total += kib * scale * 1024.0
@@ -128,7 +132,7 @@ def calibration_copy(mib=256):
     torch.cuda.synchronize()
 
 
-def product_roofline(hs, reps, traffic=None, traffic_detail=None, cold_reps=20):
+def product_roofline(hs, reps, traffic=None, traffic_detail=None, cold_reps=20, replicas=4):
     """Roofline of pcg3_spmv_kernel<double>, the kernel that dominates the timed PCG loop
     (one launch per iteration: q = A z + beta q, p = z + beta p and the partial sums of
     p.q over the mirrored 3x3 block-CSR), launched through ysp_pcg3_product_pass with the
@@ -136,52 +140,56 @@ def product_roofline(hs, reps, traffic=None, traffic_detail=None, cold_reps=20):
 
     Algorithmic bytes per launch: the upper 3x3 blocks read once (72 B values + 8 B (row,
     col)) plus five vector passes (z read, p read and written, q read and written). Two
-    regimes: back-to-back launches as in the loop (the n=48 operator, 134 MB, then stays in
-    the 256 MB Infinity Cache) and launches after a 1 GiB read has evicted L2 and the
-    Infinity Cache, so that each streams its operands from HBM; the HBM figure is the
-    headline."""
+    regimes, each a run of back-to-back launches between one pair of events:
+    * in the loop: launches on one operator, which (134 MB at n=48) then stays in the
+      256 MB Infinity Cache;
+    * HBM (the headline): launches rotating over ``replicas`` copies of the operator and of
+      the vectors (4 x 155 MB at n=48), so that each launch finds its operands evicted by
+      the launches on the other copies and streams them from HBM."""
     from yasps_amd import _lib, device as dev
     torch = dev.torch()
     lib = _lib.load()
     f = hs.full
     n = hs.rows
-    work = dev.zeros(5 * n, "float64")
-    scal = dev.zeros(12288, "float64")                 # YSP_PCG3_SCALARS
     stream = torch.cuda.current_stream()
 
-    def launch(prepare=0):
-        _lib.check(lib.ysp_pcg3_product_pass(dev.ptr(f.row_ptr), dev.ptr(f.col), dev.ptr(f.v64), 0,
-                                             f.nnz, n, dev.ptr(work), dev.ptr(scal), prepare,
+    def make(copy):
+        if not copy:
+            return (f.row_ptr, f.col, f.v64, dev.zeros(5 * n, "float64"), dev.zeros(12288, "float64"))
+        return (f.row_ptr.clone(), f.col.clone(), f.v64.clone(), dev.zeros(5 * n, "float64"),
+                dev.zeros(12288, "float64"))           # 12288 = YSP_PCG3_SCALARS
+
+    def launch(op, prepare=0):
+        rp, col, v, work, scal = op
+        _lib.check(lib.ysp_pcg3_product_pass(dev.ptr(rp), dev.ptr(col), dev.ptr(v), 0, f.nnz, n,
+                                             dev.ptr(work), dev.ptr(scal), prepare,
                                              stream.cuda_stream), "ysp_pcg3_product_pass")
-    launch(1)
+
+    def timed(seq):
+        start = torch.cuda.Event(enable_timing=True)
+        end = torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        for op in seq:
+            launch(op)
+        end.record(stream)
+        end.synchronize()
+        return start.elapsed_time(end) / len(seq)
+
+    ops = [make(False)]
+    launch(ops[0], 1)
     for _ in range(3):
-        launch()
-    start = torch.cuda.Event(enable_timing=True)
-    end = torch.cuda.Event(enable_timing=True)
-    start.record(stream)
-    for _ in range(reps):
-        launch()
-    end.record(stream)
-    end.synchronize()
-    ms_warm = start.elapsed_time(end) / reps
-    # a read (clean lines: a write would leave dirty lines whose write-back competes with
-    # the timed launch) of 1 GiB between launches evicts L2 and the Infinity Cache
+        launch(ops[0])
+    ms_warm = timed([ops[0]] * reps)
     ms_cold = ms_warm
     if cold_reps:
-        flush = torch.ones(1 << 27, dtype=torch.float64, device=dev.device())
-        cold = []
-        for _ in range(cold_reps):
-            flush.sum()
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-            launch()
-            e1.record(stream)
-            cold.append((e0, e1))
-        torch.cuda.synchronize()
-        ms_cold = float(np.mean([a.elapsed_time(b) for a, b in cold]))
-        del flush
-    del work, scal
+        ops += [make(True) for _ in range(replicas - 1)]
+        for op in ops[1:]:
+            launch(op, 1)
+        seq = [ops[i % replicas] for i in range(cold_reps * replicas)]
+        for op in ops:                                  # one untimed rotation
+            launch(op)
+        ms_cold = timed(seq)
+    del ops
     nb = hs.static.num_blocks
     bytes_alg = nb * (9 * 8 + 8) + 5 * n * 8
     bytes_full = f.nnz * (9 * 8 + 4) + (n // 3 + 1) * 4 + 5 * n * 8
@@ -190,17 +198,18 @@ def product_roofline(hs, reps, traffic=None, traffic_detail=None, cold_reps=20):
     return {"bound": "hbm", "achieved": round(gbs_cold, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs_cold / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": "pcg3_spmv_kernel<double>",
-            "regime": ("HBM: each timed launch follows a 1 GiB read that evicts L2 and the "
-                       "Infinity Cache; single-launch event pairs"),
-            "avg_us": round(ms_cold * 1e3, 2), "launches": cold_reps,
+            "regime": (f"HBM: back-to-back launches rotating over {replicas} copies of the "
+                       "operator and vectors (each launch streams its operands from HBM)"),
+            "avg_us": round(ms_cold * 1e3, 2), "launches": cold_reps * replicas if cold_reps else 0,
             "bytes_per_launch": int(bytes_alg), "full_bsr_bytes_per_launch": int(bytes_full),
             "full_bsr_GBs": round(bytes_full / (ms_cold * 1e-3) / 1e9, 1),
             "in_loop": {"avg_us": round(ms_warm * 1e3, 2), "launches": reps,
                         "achieved": round(gbs_warm, 1),
                         "frac": round(gbs_warm / HBM_PEAK_GBS, 4),
                         "full_bsr_GBs": round(bytes_full / (ms_warm * 1e-3) / 1e9, 1),
-                        "note": ("back-to-back launches as in the PCG loop: the operator "
-                                 "(134 MB at n=48) stays in the 256 MB Infinity Cache")},
+                        "note": ("back-to-back launches on one operator as in the PCG loop: "
+                                 "the operator (134 MB at n=48) stays in the 256 MB Infinity "
+                                 "Cache")},
             "traffic_detail": traffic_detail}
 
 
@@ -752,10 +761,14 @@ def main():
     cs = cube_scene(args.n, seed=args.seed)
     s0, hnd = build_yasps_scene(cs, threshold_mode="relative_residual")
     if args.kernels_only:
-        # assembly only (the operator of the loop), then the product launches and the
-        # calibration copy: the only pcg3_spmv_kernel launches are the measured ones
+        # assembly only (the operator of the loop), then the product launches (back to back,
+        # then each after a cache flush) and the calibration copy: the only pcg3_spmv_kernel
+        # launches are the measured ones
         s0.minimizer.computeNumericValue()
-        product_roofline(s0.minimizer.activeHessian, args.spmv_reps, cold_reps=0)
+        roof = product_roofline(s0.minimizer.activeHessian, args.spmv_reps,
+                                cold_reps=args.cold_reps)
+        print(json.dumps({"kernels_only": {k: roof[k] for k in ("avg_us", "launches", "in_loop")}}),
+              flush=True)
         calibration_copy()
         return
     # setup: static sparsity layout, rest data, first solve
@@ -769,7 +782,7 @@ def main():
     r = hs.spmv(vector(hs.rows, sol.clone())).value - g
     true_rel = float(torch.linalg.norm(r) / torch.linalg.norm(g))
 
-    roof = product_roofline(hs, args.spmv_reps, traffic, traffic_detail)
+    roof = product_roofline(hs, args.spmv_reps, traffic, traffic_detail, args.cold_reps)
     value = args.steps / elapsed
     forms = None
     if not args.no_forms:

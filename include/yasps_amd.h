@@ -280,6 +280,42 @@ int ysp_element_energy_values(int32_t kind, const ysp_union_f64* positions,
                               const uint32_t* elem2v, uint32_t n, const double* element_params,
                               const double* scene_params, double* energy, void* stream);
 
+/* ------------------------------------------- mixed second derivatives d g / d theta */
+/* Element blocks of a rectangular second-order Jacobian (secondOrderJacobian.compute,
+ * matrixAndVector/matrix/secondOrderJacobian.pyx:531-559, with the generated kernels of
+ * kernel/Jacobian/secondOrderJacobianKernelNoProject.pyx): rows = the element's DoF
+ * (arity x width, connectivity order), w columns of one column target theta. For each
+ * element, B = (d g / d p) dparams + (d g / d s) dscene, with dparams [P x w] (the
+ * derivatives of the element parameters w.r.t. theta; inertia only, null = zero) and
+ * dscene [S x w] (of the scene parameters; null = zero), each *_stride doubles apart per
+ * element (0: shared); scene_params likewise scene_stride apart (per-element Lamé
+ * coefficients, for instance). blocks: [n, arity x width, w]. No projection. Vertex
+ * data as in ysp_eval_element_energy (no affine-body children). Asynchronous. */
+int ysp_element_param_jacobian(int32_t kind, const ysp_union_f64* positions,
+                               const uint32_t* element_vertices, uint32_t n,
+                               const double* element_params, const double* scene_params,
+                               uint32_t scene_stride, const double* dparams,
+                               uint32_t dparams_stride, const double* dscene,
+                               uint32_t dscene_stride, uint32_t w, double* blocks, void* stream);
+
+/* Global row of each DoF of each element from the get_indices output of the row targets
+ * (secondOrderJacobianIndicesKernel.pyx:23-299, row stream): rows[n, dofs], -1 for DoF
+ * outside the targets. YSP_ERR_ARG when an element's segments do not cover its dofs.
+ * Synchronous. */
+int ysp_jacobian_rows(const uint32_t* segment_indices, const uint16_t* segment_sizes,
+                      uint32_t max_num_indices, uint32_t n, uint32_t dofs, int32_t* rows,
+                      void* stream);
+
+/* y += B x (transpose = 0: x over the w columns of each element's block, y over rows) or
+ * y += B^T x (transpose = 1) for blocks of ysp_element_param_jacobian (the matrix spmv of
+ * matrixAndVector/matrix/matrix.pyx:21-86 with transpose). Columns start at e w for
+ * element e, or at 0 for every element when shared_columns != 0 (theta with one
+ * instance). Accumulates into y with FP64 atomics where elements share an output.
+ * Asynchronous. */
+int ysp_jacobian_spmv(const double* blocks, uint32_t n, uint32_t dofs, uint32_t w,
+                      const int32_t* rows, int32_t shared_columns, const double* x, double* y,
+                      int32_t transpose, void* stream);
+
 /* --------------------------------------------------------- block storage */
 /* Symmetric upper block storage of one Hessian part (hessian.pyx:783-808,
  * matrix.pyx:138-282). Host arrays describe the (h, w) categories. */
@@ -350,10 +386,12 @@ int ysp_pcg_solve(uint32_t max_iterations, double threshold, int32_t threshold_m
  * ysp_bsr3_build_split / ysp_bsr3_gather_split (gathers instead of FP64 atomics; same
  * block-Jacobi preconditioner over the diagonal blocks of every target, recurrence,
  * stopping rules and return codes). n must be a multiple of 3. Block rows with more than
- * long_row_threshold entries (0: 128) get a workgroup each. */
+ * long_row_threshold entries (0: 128) get a workgroup each. values FP64 or, with
+ * values_f32, FP32 (the inner solve of mixed precision; vectors, dots and the diagonal
+ * block inverses stay FP64). */
 int ysp_pcg_solve_full(uint32_t max_iterations, double threshold, int32_t threshold_mode,
-                       const uint32_t* row_ptr, const uint32_t* col, const double* values,
-                       uint32_t nnz, uint32_t long_row_threshold,
+                       const uint32_t* row_ptr, const uint32_t* col, const void* values,
+                       int32_t values_f32, uint32_t nnz, uint32_t long_row_threshold,
                        const double* diagonal_blocks_inverse,
                        const uint32_t* diagonal_blocks_start,
                        const uint32_t* diagonal_blocks_count,

@@ -74,6 +74,7 @@ def snh_gradient_hessian(tet_positions, dm_inv, vol, mu, lam, dt):
     """
     F, ic, J, cof, alpha = _snh_terms(tet_positions, dm_inv, mu, lam)
     T = F.shape[0]
+    lam_t = np.broadcast_to(np.asarray(lam, dtype=np.float64), (T,))   # scalar or per tet
     s = dt * dt * vol
     a1 = mu * (1.0 - 1.0 / (ic + 1.0))
     a2 = 2.0 * mu / (ic + 1.0) ** 2
@@ -90,7 +91,7 @@ def snh_gradient_hessian(tet_positions, dm_inv, vol, mu, lam, dt):
         K[:, j, :, j, :] += a1[:, None, None] * eye
         for n in range(3):
             K[:, j, :, n, :] += a2[:, None, None] * f[:, j, :, None] * f[:, n, None, :]
-            K[:, j, :, n, :] += lam * c[:, j, :, None] * c[:, n, None, :]
+            K[:, j, :, n, :] += lam_t[:, None, None] * c[:, j, :, None] * c[:, n, None, :]
     # d2J: block (0,1) = -[f2]x, (0,2) = [f1]x, (1,2) = -[f0]x, transposes below
     cx = [_cross_matrix(f[:, k]) for k in range(3)]
     hj = {(0, 1): -cx[2], (0, 2): cx[1], (1, 2): -cx[0]}

@@ -46,6 +46,30 @@ def test_abd_solve_matches_dense():
     assert [v.numel() for v in sol] == [3 * sc.n_free, 9 * sc.n_bodies, 3 * sc.n_bodies]
 
 
+@pytest.mark.parametrize("solver,options", [("jacobian", {}),
+                                            ("mas", {"max_levels": 3, "coarsest_level_weight": 1.0})])
+def test_abd_mixed_precision_reaches_fp64_residual(solver, options):
+    """Mixed precision on the split operator of an affine-body layout (FP32 sub-block
+    values in the Krylov loop, FP64 refinement): the true residual on the oracle's FP64
+    Hessian reaches the tolerance and the solution matches the dense solve."""
+    from yasps_amd import device as dev
+    print(f"abd mixed {solver}", flush=True)
+    sc = abd_cube_scene(n=3, abd_layers=2, bodies=2, seed=7)
+    s0, _ = build_abd_scene(sc, projection_method=2, solver_name=solver,
+                            threshold_mode="relative_residual", precision="mixed", **options)
+    tol = 1e-9
+    sol = s0.minimizeEnergy(tolerance=tol)
+    st = s0.minimizer.linearSolver.statistics
+    assert st["converged"] and st["precision"] == "mixed"
+    x = np.concatenate([dev.to_host(v) for v in sol])
+    g, H = abd.assemble(sc, method=2)
+    rel = np.linalg.norm(H @ x - g) / np.linalg.norm(g)
+    assert rel <= 1.02 * tol, (rel, st["outer_history"])
+    xs = np.linalg.solve(H, g)
+    assert np.linalg.norm(x - xs) <= 1e-5 * np.linalg.norm(xs)
+    assert st["outer_iterations"] >= 2          # beyond FP32 resolution
+
+
 def test_abd_energy_values():
     from yasps_amd import device as dev
     from oracle import energies

@@ -100,7 +100,7 @@ def test_split_operator_long_rows_match_short_rows():
         scalars = dev.zeros(12288, "float64")
         st = _lib.PcgStats()
         rc = lib.ysp_pcg_solve_full(
-            max_it, tol, 1, dev.ptr(f.row_ptr), dev.ptr(f.col), dev.ptr(f.v64), f.nnz, threshold,
+            max_it, tol, 1, dev.ptr(f.row_ptr), dev.ptr(f.col), dev.ptr(f.v64), 0, f.nnz, threshold,
             dev.ptr(hs.diagonal_blocks_inverse), _lib.u32_array(hs.diagonal_blocks_start_cpu[:-1]),
             _lib.u32_array([w.numInstances for w in hs.wrt]), _lib.u32_array([w.size for w in hs.wrt]),
             _lib.u32_array(hs.gradient_segments_start_cpu[:-1]), len(hs.wrt),

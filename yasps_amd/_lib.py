@@ -114,6 +114,10 @@ SIGNATURES = {
                                           U32, P, P, U32, C.POINTER(ScatterArgs), P]),
     "ysp_affine_positions": (C.c_int, [P, P, P, P, U32, P, P]),
     "ysp_element_energy_values": (C.c_int, [I32, C.POINTER(UnionF64), P, U32, P, P, P, P]),
+    "ysp_element_param_jacobian": (C.c_int, [I32, C.POINTER(UnionF64), P, U32, P, P, U32, P, U32,
+                                             P, U32, U32, P, P]),
+    "ysp_jacobian_rows": (C.c_int, [P, P, U32, U32, U32, P, P]),
+    "ysp_jacobian_spmv": (C.c_int, [P, U32, U32, U32, P, I32, P, P, I32, P]),
     "ysp_block_spmv": (C.c_int, [C.POINTER(BlockMatrix), C.POINTER(BlockMatrix), P, P, U32,
                                  C.c_int, C.c_int, P]),
     "ysp_invert_diagonal_blocks": (C.c_int, [P, P, C.POINTER(U32), C.POINTER(U32),
@@ -123,7 +127,7 @@ SIGNATURES = {
     "ysp_pcg_solve": (C.c_int, [U32, D, I32, C.POINTER(BlockMatrix), C.POINTER(BlockMatrix), P,
                                 C.POINTER(U32), C.POINTER(U32), C.POINTER(U32), C.POINTER(U32),
                                 I32, P, U32, P, P, I32, P, P, C.POINTER(PcgStats), P]),
-    "ysp_pcg_solve_full": (C.c_int, [U32, D, I32, P, P, P, U32, U32, P, C.POINTER(U32),
+    "ysp_pcg_solve_full": (C.c_int, [U32, D, I32, P, P, P, I32, U32, U32, P, C.POINTER(U32),
                                      C.POINTER(U32), C.POINTER(U32), C.POINTER(U32), I32, P, U32,
                                      P, P, I32, P, P, C.POINTER(PcgStats), P]),
     "ysp_bsr3_build_split": (C.c_int, [C.POINTER(BlockMatrix), C.POINTER(BlockMatrix), U32, P, P,

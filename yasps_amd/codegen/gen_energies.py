@@ -1,7 +1,8 @@
 """Generate the element energy functions of ``csrc/generated_energies.hpp`` with SymPy.
 
 Run ``python -m yasps_amd.codegen.gen_energies`` (SymPy needed only here; the generated
-header is committed and the build does not run this). Each energy is written once as a
+headers are committed and the build does not run this); ``--mixed`` writes
+``csrc/generated_mixed.hpp``, the derivatives of the gradients w.r.t. the parameters. Each energy is written once as a
 SymPy expression of the element's vertex positions x (arity x 3, row-major), per-element
 parameters p and scene parameters s; the script differentiates it exactly, shares common
 subexpressions (sympy.cse) and emits
@@ -192,6 +193,67 @@ def generate(name):
     return "\n".join(lines), len(repl)
 
 
+# kinds whose element parameters p are energy arguments (not rest-shape data): their mixed
+# derivatives d g / d p are generated as well as d g / d s
+MIXED_P = {"inertia"}
+OUT_MIXED = OUT.with_name("generated_mixed.hpp")
+
+
+def generate_mixed(name):
+    """d g / d s (n x S, row-major) of every kind and d g / d p (n x P) of the kinds in
+    MIXED_P: the mixed second derivatives of the energy w.r.t. its DoF and its parameters,
+    the element blocks of a rectangular Jacobian d g / d theta once multiplied by the
+    parameters' own derivatives d (p, s) / d theta."""
+    fn, n, npar, nsc = ENERGIES[name]
+    x = sp.symbols(f"x0:{n}", real=True)
+    p = sp.symbols(f"p0:{max(npar, 1)}", real=True)
+    s = sp.symbols(f"s0:{nsc}", real=True)
+    E = fn(x, p, s)
+    g = [sp.diff(E, xi) for xi in x]
+    wp = npar if name in MIXED_P else 0
+    Gp = [sp.diff(g[i], p[j]) for i in range(n) for j in range(wp)]
+    Gs = [sp.diff(g[i], s[j]) for i in range(n) for j in range(nsc)]
+    repl, red = sp.cse(Gp + Gs, symbols=sp.numbered_symbols("t"), optimizations="basic")
+    subs = {xi: sp.Symbol(f"x[{i}]") for i, xi in enumerate(x)}
+    subs.update({pi: sp.Symbol(f"p[{i}]") for i, pi in enumerate(p)})
+    subs.update({si: sp.Symbol(f"s[{i}]") for i, si in enumerate(s)})
+    lines = [f"// {name}: d g / d p ({n} x {wp}), d g / d s ({n} x {nsc})",
+             f"constexpr int kMixed_{name}_P = {wp};",
+             f"__device__ __forceinline__ void mixed_{name}(const double* __restrict__ x, "
+             "const double* __restrict__ p, const double* __restrict__ s, "
+             "double* __restrict__ Gp, double* __restrict__ Gs) {"]
+    for sym, expr in repl:
+        lines.append(f"  const double {sym} = {sp.ccode(expr.xreplace(subs))};")
+    for k in range(len(Gp)):
+        lines.append(f"  Gp[{k}] = {sp.ccode(red[k].xreplace(subs))};")
+    for k in range(len(Gs)):
+        lines.append(f"  Gs[{k}] = {sp.ccode(red[len(Gp) + k].xreplace(subs))};")
+    if not Gp:
+        lines.append("  (void)Gp;")
+    lines.append("}")
+    return "\n".join(lines), len(repl)
+
+
+def main_mixed(names=None):
+    names = names or list(ENERGIES)
+    parts = ["// Generated by yasps_amd/codegen/gen_energies.py (mixed derivatives of the energies",
+             "// of generated_energies.hpp w.r.t. their parameters); do not edit by hand.",
+             "#pragma once",
+             "#include <cmath>",
+             "",
+             "namespace ysp {",
+             "namespace gen {",
+             ""]
+    for name in names:
+        code, nrep = generate_mixed(name)
+        print(f"{name} (mixed): {nrep} shared subexpressions", file=sys.stderr)
+        parts.append(code)
+        parts.append("")
+    parts += ["}  // namespace gen", "}  // namespace ysp", ""]
+    OUT_MIXED.write_text("\n".join(parts))
+    print(f"wrote {OUT_MIXED}", file=sys.stderr)
+
+
 def main(names=None):
     names = names or list(ENERGIES)
     parts = ["// Generated by yasps_amd/codegen/gen_energies.py from the SymPy energy definitions",
@@ -213,4 +275,7 @@ def main(names=None):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1:] or None)
+    if sys.argv[1:2] == ["--mixed"]:
+        main_mixed(sys.argv[2:] or None)
+    else:
+        main(sys.argv[1:] or None)

@@ -7,8 +7,11 @@ Mirrors the reference's user-facing classes (``yasps/yasps/scene/scene.pyx``,
 linear-solve path uses: DATA / CONSTANT attributes with per-instance values, JOIN
 attributes (``addAttribute(name, through=conn, source=attr)``), UNION attributes over
 stacked child primitives, and computed energy attributes built from the energy helpers
-in ``yasps_amd.energies``. There is no symbolic algebra: energies are hand-written
-kernels selected by the helper that built them.
+in ``yasps_amd.energies``. There is no symbolic differentiator: energies are hand-written
+kernels selected by the helper that built them. Arithmetic on scalar and 3-vector
+attributes builds elementwise expressions (``yasps_amd.expr``) that energies take as
+parameters (Lamé coefficients from Young's modulus and Poisson's ratio, a velocity from
+two previous positions).
 
 Attribute values live on the device (torch tensors, flat float64 [instances * size]);
 a value set before any device work is kept on the host and uploaded on first use.
@@ -92,15 +95,48 @@ class attribute:
         return self
 
     def __mul__(self, other):
-        from .affine import MatrixVector
-        return MatrixVector(self, other)
+        # a 3 x 3 matrix times a 3-vector attribute: the affine-body vertex map; scalars
+        # and 3-vectors otherwise combine elementwise (yasps_amd.expr)
+        if self.size == 9 and isinstance(other, attribute) and other.size == 3:
+            from .affine import MatrixVector
+            return MatrixVector(self, other)
+        from .expr import as_expr
+        return as_expr(self) * other
 
     def __add__(self, other):
         from .affine import AffineMap, MatrixVector
         if isinstance(other, MatrixVector):
             return AffineMap(self, other.matrix, other.vector)
-        raise NotImplementedError("attribute algebra: only translation + matrix * rest_position "
-                                  "(an affine-body vertex map) is supported")
+        from .expr import as_expr
+        return as_expr(self) + other
+
+    def __radd__(self, other):
+        from .expr import as_expr
+        return as_expr(other) + self
+
+    def __sub__(self, other):
+        from .expr import as_expr
+        return as_expr(self) - other
+
+    def __rsub__(self, other):
+        from .expr import as_expr
+        return as_expr(other) - self
+
+    def __rmul__(self, other):
+        from .expr import as_expr
+        return as_expr(other) * self
+
+    def __truediv__(self, other):
+        from .expr import as_expr
+        return as_expr(self) / other
+
+    def __rtruediv__(self, other):
+        from .expr import as_expr
+        return as_expr(other) / self
+
+    def __neg__(self):
+        from .expr import as_expr
+        return -as_expr(self)
 
     # ------------------------------------------------------------ values
     def updateValue(self, values, deepCopy: bool = False):

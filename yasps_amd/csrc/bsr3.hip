@@ -938,6 +938,10 @@ __global__ void __launch_bounds__(kB) fill_zero_kernel(double* __restrict__ p, u
   if (i < n) p[i] = 0.0;
 }
 
+// The long-row list cache, scratch slots 67-69 and the pinned status ring below are
+// process-wide: single-system solves issued from several host threads run one at a time.
+std::mutex g_pcg3_solve_mutex;
+
 // no_alloc: the variant behind ysp_pcg3_solve_noalloc. It allocates nothing (no pinned
 // status ring, no events, no long-row tables: every row goes through the 16-lane product)
 // and clears the scalar state with a kernel; the host waits for each chunk's status with
@@ -947,6 +951,8 @@ int pcg3_run(uint32_t max_iterations, double threshold, int32_t mode, const uint
              const uint32_t* col, const VT* v, uint32_t nnz, const MT* minv, const MasApply* mas,
              const double* rhs, uint32_t n, double* x, const double* x0, double* work,
              double* st, ysp_pcg_stats* stats, hipStream_t s, bool no_alloc = false) {
+  std::unique_lock<std::mutex> solve_lock(g_pcg3_solve_mutex, std::defer_lock);
+  if (!no_alloc) solve_lock.lock();   // the no-alloc variant touches none of that state
   const uint32_t nv = n / 3;
   double* r = work;
   double* z = work + n;

@@ -58,6 +58,23 @@ inline ScatterArgs make_scatter(const ysp_scatter_args* a) {
                      a->gradient_staging, a->gradient_local_size};
 }
 
+// Baraff-Witkin stretch coefficients of one element from the current deformation
+// gradient F = [x1 - x0, x2 - x0] Minv (helpers.py:252-253: 0.01 below unit stretch,
+// else 1); x: the 3 vertices (9 doubles), p: Minv (4, row-major) first.
+__device__ __forceinline__ void bw_stretch_coefficients(const double* x, const double* p,
+                                                        double& cu, double& cv) {
+  double fu2 = 0.0, fv2 = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const double d1 = x[3 + i] - x[i], d2 = x[6 + i] - x[i];
+    const double fu = d1 * p[0] + d2 * p[2], fv = d1 * p[1] + d2 * p[3];
+    fu2 += fu * fu;
+    fv2 += fv * fv;
+  }
+  cu = sqrt(fu2) < 1.0 ? 0.01 : 1.0;
+  cv = sqrt(fv2) < 1.0 ? 0.01 : 1.0;
+}
+
 __device__ __forceinline__ uint32_t diag_offset(const ScatterArgs& S, uint32_t seg, uint32_t size) {
   uint32_t attr = 0;
   while (attr < S.num_attr && seg >= S.grad_start[attr + 1]) ++attr;

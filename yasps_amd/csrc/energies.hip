@@ -529,18 +529,7 @@ __device__ __forceinline__ void scene_params(const double* sp, const double* x, 
                                              double (&s)[Kind<ID>::S]) {
 #pragma unroll
   for (int i = 0; i < Kind<ID>::S; ++i) s[i] = sp[i];
-  if constexpr (ID == YSP_ENERGY_BARAFF_WITKIN) {
-    double fu2 = 0.0, fv2 = 0.0;
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      const double d1 = x[3 + i] - x[i], d2 = x[6 + i] - x[i];
-      const double fu = d1 * p[0] + d2 * p[2], fv = d1 * p[1] + d2 * p[3];
-      fu2 += fu * fu;
-      fv2 += fv * fv;
-    }
-    s[3] = sqrt(fu2) < 1.0 ? 0.01 : 1.0;
-    s[4] = sqrt(fv2) < 1.0 ? 0.01 : 1.0;
-  }
+  if constexpr (ID == YSP_ENERGY_BARAFF_WITKIN) bw_stretch_coefficients(x, p, s[3], s[4]);
 }
 
 // Energy, gradient and (projected) packed Hessian of one element.

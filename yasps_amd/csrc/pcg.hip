@@ -18,6 +18,7 @@
 #include "spmv.hpp"
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 namespace ysp {
@@ -356,9 +357,10 @@ constexpr unsigned kFullRows = kFullThreads / kFullLanes;
 // workgroup per row) so that a few long rows (the coupling rows of affine bodies) do not
 // set the time.
 constexpr uint32_t kLongRow = 128;
+template <typename VT>
 __global__ void __launch_bounds__(kFullThreads)
     full3_product_kernel(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
-                         const double* __restrict__ v, uint32_t stride, uint32_t nbr,
+                         const VT* __restrict__ v, uint32_t stride, uint32_t nbr,
                          uint32_t long_row, int reverse, const double* __restrict__ x,
                          double* __restrict__ y, double* __restrict__ quad,
                          const double* __restrict__ stop) {
@@ -373,9 +375,12 @@ __global__ void __launch_bounds__(kFullThreads)
       const uint32_t c = 3 * col[k];
       const double x0 = x[c], x1 = x[c + 1], x2 = x[c + 2];
       const size_t st = stride;
-      y0 += v[k] * x0 + v[st + k] * x1 + v[2 * st + k] * x2;
-      y1 += v[3 * st + k] * x0 + v[4 * st + k] * x1 + v[5 * st + k] * x2;
-      y2 += v[6 * st + k] * x0 + v[7 * st + k] * x1 + v[8 * st + k] * x2;
+      y0 += static_cast<double>(v[k]) * x0 + static_cast<double>(v[st + k]) * x1 +
+            static_cast<double>(v[2 * st + k]) * x2;
+      y1 += static_cast<double>(v[3 * st + k]) * x0 + static_cast<double>(v[4 * st + k]) * x1 +
+            static_cast<double>(v[5 * st + k]) * x2;
+      y2 += static_cast<double>(v[6 * st + k]) * x0 + static_cast<double>(v[7 * st + k]) * x1 +
+            static_cast<double>(v[8 * st + k]) * x2;
     }
 #pragma unroll
     for (int off = kFullLanes / 2; off > 0; off >>= 1) {
@@ -398,9 +403,10 @@ __global__ void __launch_bounds__(kFullThreads)
 }
 
 // The rows longer than kLongRow (listed in long_rows): one workgroup per row.
+template <typename VT>
 __global__ void __launch_bounds__(kFullThreads)
     full3_long_row_kernel(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
-                          const double* __restrict__ v, uint32_t stride,
+                          const VT* __restrict__ v, uint32_t stride,
                           const uint32_t* __restrict__ long_rows, int reverse,
                           const double* __restrict__ x, double* __restrict__ y,
                           double* __restrict__ quad, const double* __restrict__ stop) {
@@ -412,9 +418,12 @@ __global__ void __launch_bounds__(kFullThreads)
   for (uint32_t k = k0 + threadIdx.x; k < k1; k += kFullThreads) {
     const uint32_t c = 3 * col[k];
     const double x0 = x[c], x1 = x[c + 1], x2 = x[c + 2];
-    y0 += v[k] * x0 + v[st + k] * x1 + v[2 * st + k] * x2;
-    y1 += v[3 * st + k] * x0 + v[4 * st + k] * x1 + v[5 * st + k] * x2;
-    y2 += v[6 * st + k] * x0 + v[7 * st + k] * x1 + v[8 * st + k] * x2;
+    y0 += static_cast<double>(v[k]) * x0 + static_cast<double>(v[st + k]) * x1 +
+          static_cast<double>(v[2 * st + k]) * x2;
+    y1 += static_cast<double>(v[3 * st + k]) * x0 + static_cast<double>(v[4 * st + k]) * x1 +
+          static_cast<double>(v[5 * st + k]) * x2;
+    y2 += static_cast<double>(v[6 * st + k]) * x0 + static_cast<double>(v[7 * st + k]) * x1 +
+          static_cast<double>(v[8 * st + k]) * x2;
   }
   __shared__ double red[kFullThreads / 64];
   y0 = block_sum<kFullThreads>(y0, red);
@@ -633,7 +642,8 @@ extern "C" int ysp_pcg_solve(uint32_t max_iterations, double threshold, int32_t 
 
 extern "C" int ysp_pcg_solve_full(uint32_t max_iterations, double threshold,
                                   int32_t threshold_mode, const uint32_t* row_ptr,
-                                  const uint32_t* col, const double* values, uint32_t nnz,
+                                  const uint32_t* col, const void* values, int32_t values_f32,
+                                  uint32_t nnz,
                                   uint32_t long_row_threshold,
                                   const double* diagonal_blocks_inverse,
                                   const uint32_t* diagonal_blocks_start,
@@ -674,13 +684,22 @@ extern "C" int ysp_pcg_solve_full(uint32_t max_iterations, double threshold,
     }
   }
   int reverse = 0;                 // direction of the next product's row walk
+  // FP32 values (the mixed-precision inner solve) are widened to FP64 as they are read
+  auto launch = [&](auto vals, const double* x, double* y, double* quad, const double* stop) {
+    using VT = std::remove_const_t<std::remove_pointer_t<decltype(vals)>>;
+    hipLaunchKernelGGL(full3_product_kernel<VT>, dim3(grid_for(nbr, kFullRows)),
+                       dim3(kFullThreads), 0, s, row_ptr, col, vals, stride, nbr, long_row, reverse,
+                       x, y, quad, stop);
+    if (num_long_rows)
+      hipLaunchKernelGGL(full3_long_row_kernel<VT>, dim3(num_long_rows), dim3(kFullThreads), 0, s,
+                         row_ptr, col, vals, stride, long_rows, reverse, x, y, quad, stop);
+  };
   auto product = [&](const double* x, double* y, double* quad, const double* stop) -> int {
     if (nbr == 0) return YSP_OK;
-    hipLaunchKernelGGL(full3_product_kernel, dim3(grid_for(nbr, kFullRows)), dim3(kFullThreads), 0,
-                       s, row_ptr, col, values, stride, nbr, long_row, reverse, x, y, quad, stop);
-    if (num_long_rows)
-      hipLaunchKernelGGL(full3_long_row_kernel, dim3(num_long_rows), dim3(kFullThreads), 0, s,
-                         row_ptr, col, values, stride, long_rows, reverse, x, y, quad, stop);
+    if (values_f32)
+      launch(static_cast<const float*>(values), x, y, quad, stop);
+    else
+      launch(static_cast<const double*>(values), x, y, quad, stop);
     reverse ^= 1;
     YSP_CHECK_LAUNCH();
     return YSP_OK;

@@ -8,6 +8,7 @@ a primitive and ``scene.addEnergy`` evaluates with the matching HIP kernel.
 from __future__ import annotations
 
 import ctypes as C
+from dataclasses import dataclass, field
 from typing import Optional
 
 import numpy as np
@@ -16,6 +17,7 @@ from . import _lib, device as dev
 from .core import attribute, COMPUTED, DATA, CONSTANT, JOIN, UNION, primitiveUnion
 from .program import PLeaf, PJoin, PUnion, PConcat
 from .affine import is_affine
+from .expr import is_expr, values as expr_values
 
 
 def _leaf(att: attribute, wrt_pos: dict) -> Optional[PLeaf]:
@@ -108,9 +110,13 @@ def affine_children(src: attribute):
 
 def _scalar_or_array(att, n_instances: int):
     """(device pointer tensor, stride) of a per-element parameter: a one-instance
-    constant is broadcast (stride 0), a per-instance attribute is read at stride 1."""
+    constant is broadcast (stride 0), a per-instance attribute is read at stride 1; an
+    expression (yasps_amd.expr) is evaluated per instance."""
     if isinstance(att, (int, float)):
         return dev.to_device(np.array([float(att)]), "float64"), 0
+    if is_expr(att):
+        v = expr_values(att, n_instances, 1)
+        return v.reshape(-1), (1 if v.shape[0] == n_instances and n_instances != 1 else 0)
     if att.size != 1:
         raise ValueError(f"{att.fullName}: expected a scalar parameter")
     if att.numInstances == 1:
@@ -123,10 +129,52 @@ def _scalar_or_array(att, n_instances: int):
 def _host_scalar(att) -> float:
     if isinstance(att, (int, float)):
         return float(att)
+    if is_expr(att):
+        if att.instances() != 1:
+            raise ValueError("expected a scalar parameter (an expression of one-instance "
+                             "attributes)")
+        return float(dev.to_host(expr_values(att, 1, 1)).reshape(-1)[0])
     v = att.host_value()
     if v.size != 1:
         raise ValueError(f"{att.fullName}: expected one scalar value")
     return float(v[0])
+
+
+@dataclass
+class MixedInfo:
+    """Inputs of the mixed-derivative kernel (ysp_element_param_jacobian) for one energy:
+    the generated kind, its vertex data and element map, element parameters ep [n, P]
+    (rest-shape data, or the energy's own arguments for the kinds whose d g / d p is
+    generated), scene parameters se (se_stride doubles per element, 0 = shared), the
+    argument each parameter slot comes from (p_args: (argument, rows) in slot order, for
+    the kinds with d g / d p; s_args: one argument per scene slot, None for a slot the
+    kernel fills itself), and the attributes that are DoF sources or rest-shape inputs
+    (a column target among them has no mixed Jacobian here)."""
+    kind: int
+    positions: object
+    elem2v: object
+    n: int
+    dofs: int
+    ep: object
+    se: object
+    se_stride: int
+    p_args: list
+    s_args: list
+    dof_leaves: list
+    rest_leaves: list = field(default_factory=list)
+
+
+def attribute_leaves(src) -> list:
+    """An attribute and, through unions and affine maps, the attributes it is made of."""
+    if src is None:
+        return []
+    out = [src]
+    for k in getattr(src, "children", []) or []:
+        out += attribute_leaves(k)
+    if is_affine(src):
+        m = src.expression
+        out += [m.t.source, m.A.source, m.r]
+    return out
 
 
 class EnergyExpr:
@@ -244,20 +292,50 @@ class StableNeoHookean(EnergyExpr):
         return out
 
 
+    def mixed_info(self) -> MixedInfo:
+        """Generated SNH: element parameters [Dm^-1 (9), vol] (rest shape), scene
+        parameters [mu, lam, dt] per element (Lamé coefficients may be per-tet attributes
+        or expressions)."""
+        if affine_children(self.pos.source) is not None:
+            raise NotImplementedError("mixed Jacobians of SNH on affine-body vertices are not "
+                                      "supported")
+        n = self.num_instances
+        t = dev.torch()
+        dm_inv, vol = self.rest_data()
+        ep = t.cat([dm_inv.reshape(n, 9), vol.reshape(n, 1)], dim=1).contiguous()
+        cols = [expr_values(a, n, 1).expand(n, 1) for a in (self.mu, self.lam, self.dt)]
+        se = t.cat(cols, dim=1).contiguous()
+        return MixedInfo(KIND_IDS["snh"], union_view(self.pos.source), self.conn.value, n, 12,
+                         ep, se, 3, [], [self.mu, self.lam, self.dt],
+                         attribute_leaves(self.pos.source), attribute_leaves(self.rest.source))
+
+
 class Inertia(EnergyExpr):
     """``examples/one_bunny/helpers.py:69-71`` on a vertex primitive."""
     kind = "inertia"
 
     def __init__(self, x_before, vel, dt, x, mass):
-        for a, nm in ((x_before, "x_before"), (vel, "vel"), (x, "x")):
-            if not isinstance(a, attribute) or a.size != 3:
-                raise TypeError(f"inertia: {nm} must be a 3-vector attribute")
-        if not (x_before.correspondance is vel.correspondance is x.correspondance):
-            raise ValueError("inertia: x_before, vel and x must live on one primitive")
+        if not isinstance(x, attribute) or x.size != 3:
+            raise TypeError("inertia: x must be a 3-vector attribute")
+        for a, nm in ((x_before, "x_before"), (vel, "vel")):
+            if is_expr(a):
+                if a.rows != 3:
+                    raise TypeError(f"inertia: {nm} must be a 3-vector expression")
+                if a.owner() not in (None, x.correspondance):
+                    raise ValueError(f"inertia: the attributes of {nm} must live on x's primitive")
+            elif not isinstance(a, attribute) or a.size != 3:
+                raise TypeError(f"inertia: {nm} must be a 3-vector attribute or expression")
+            elif a.correspondance is not x.correspondance:
+                raise ValueError("inertia: x_before, vel and x must live on one primitive")
         if x.operator != DATA and not is_affine(x):
             raise TypeError("inertia: x must be a DATA attribute or an affine-body vertex map")
         self.xp, self.v, self.x, self.mass, self.dt = x_before, vel, x, mass, dt
         self._iota = None
+
+    def _vec(self, a):
+        """[n, 3] contiguous values of x_before / vel (attribute or expression)."""
+        n = self.num_instances
+        return expr_values(a, n, 3).expand(n, 3).contiguous()
 
     def check_owner(self, owner):
         if owner is not self.x.correspondance:
@@ -271,8 +349,8 @@ class Inertia(EnergyExpr):
     def _generic_args(self):
         n = self.num_instances
         t = dev.torch()
-        ep = t.cat([self.xp.value.reshape(n, 3), self.v.value.reshape(n, 3),
-                    self._mass().reshape(n, 1)], dim=1).contiguous()
+        ep = t.cat([self._vec(self.xp), self._vec(self.v), self._mass().reshape(n, 1)],
+                   dim=1).contiguous()
         sp = dev.to_device(np.array([_host_scalar(self.dt)]), "float64")
         if self._iota is None or self._iota.numel() != n:
             self._iota = dev.to_device(np.arange(n, dtype=np.uint32), "uint32")
@@ -282,6 +360,8 @@ class Inertia(EnergyExpr):
         n = self.num_instances
         if isinstance(self.mass, (int, float)):
             return dev.to_device(np.full(n, float(self.mass)), "float64")
+        if is_expr(self.mass):
+            return expr_values(self.mass, n, 1).expand(n, 1).reshape(n).contiguous()
         if self.mass.numInstances == 1 and n != 1:
             return self.mass.value.expand(n).contiguous()
         return self.mass.value
@@ -298,8 +378,9 @@ class Inertia(EnergyExpr):
                 "ysp_eval_element_energy")
             return
         m = self._mass()
+        xp, v = self._vec(self.xp), self._vec(self.v)
         _lib.check(_lib.load().ysp_eval_inertia(
-            dev.ptr(self.x.value), dev.ptr(self.xp.value), dev.ptr(self.v.value), dev.ptr(m),
+            dev.ptr(self.x.value), dev.ptr(xp), dev.ptr(v), dev.ptr(m),
             _host_scalar(self.dt), n, max(0, projection_method), C.byref(scatter), dev.stream()),
             "ysp_eval_inertia")
 
@@ -315,10 +396,22 @@ class Inertia(EnergyExpr):
                 dev.ptr(out), dev.stream()), "ysp_element_energy_values")
             return out
         m = self._mass()
+        xp, v = self._vec(self.xp), self._vec(self.v)
         _lib.check(_lib.load().ysp_energy_inertia(
-            dev.ptr(self.x.value), dev.ptr(self.xp.value), dev.ptr(self.v.value), dev.ptr(m),
+            dev.ptr(self.x.value), dev.ptr(xp), dev.ptr(v), dev.ptr(m),
             _host_scalar(self.dt), n, dev.ptr(out), dev.stream()), "ysp_energy_inertia")
         return out
+
+
+    def mixed_info(self) -> MixedInfo:
+        """Generated inertia: element parameters [x_prev (3), v (3), m] are the energy's own
+        arguments (d g / d p generated), scene parameter [dt]."""
+        if is_affine(self.x):
+            raise NotImplementedError("mixed Jacobians of the inertia of affine-body vertices "
+                                      "are not supported")
+        ep, sp, u, _ = self._generic_args()
+        return MixedInfo(KIND_IDS["inertia"], u, self._iota, self.num_instances, 3, ep, sp, 0,
+                         [(self.xp, 3), (self.v, 3), (self.mass, 1)], [self.dt], [self.x])
 
 
 def stable_neo_hookean(tet_position_rest, tet_position, mu, lam, dt) -> StableNeoHookean:
@@ -432,6 +525,24 @@ class ElementEnergy(EnergyExpr):
         return out
 
 
+    def mixed_info(self) -> MixedInfo:
+        """Generated kinds: element parameters from the rest shape, scene parameters
+        shared by every element (Baraff-Witkin's stretch coefficients are chosen per
+        element by the kernel)."""
+        if self.kind == "quadratic_form":
+            raise NotImplementedError("quadratic_form has no mixed Jacobian")
+        if self.affine() is not None:
+            raise NotImplementedError(f"mixed Jacobians of {self.kind} on affine-body vertices "
+                                      "are not supported")
+        s_args = list(self.scene_params)
+        if self.kind == "baraff_witkin":
+            s_args[3] = s_args[4] = None
+        return MixedInfo(self.kind_id, self.positions(), self.elem2v(), self.num_instances,
+                         self.arity * self.width, self.element_params(), self._scene(), 0, [],
+                         s_args, attribute_leaves(self.pos.source),
+                         attribute_leaves(self.rest.source if self.rest is not None else None))
+
+
 class AffineOrthogonality(ElementEnergy):
     """``examples/one_bunny_partial_abd/helpers.py:132-139`` on the 9-vector affine matrix
     of every body (row-major A), times ``stiffness``."""
@@ -479,6 +590,10 @@ class AffineOrthogonality(ElementEnergy):
         u.width = 9
         u._keep = [counts, self.A.value]
         return u
+
+    def mixed_info(self) -> MixedInfo:
+        return MixedInfo(self.kind_id, self.positions(), self.elem2v(), self.num_instances, 9,
+                         None, self._scene(), 0, [], list(self.scene_params), [self.A])
 
 
 def point_point(position, dHat, kappa) -> ElementEnergy:

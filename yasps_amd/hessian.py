@@ -242,6 +242,17 @@ class hessian:
         self.use_staging = True
         self._grad_offsets = None
 
+    def __add__(self, other: "hessian") -> "hessian":
+        """Sum of two Hessians over the same targets (the adjoint Hessian of
+        ``bunny_material_optimization.py:581-603``, a sum of ``diff2`` terms): a new
+        Hessian that assembles the terms of both."""
+        if not isinstance(other, hessian):
+            raise TypeError(f"hessian.__add__: unsupported operand type(s) for +: 'hessian' and "
+                            f"'{type(other).__name__}'")
+        if len(self.wrt) != len(other.wrt) or any(a is not b for a, b in zip(self.wrt, other.wrt)):
+            raise ValueError("hessian.__add__: wrt mismatch.")
+        return hessian(self.wrt, self.terms + other.terms, self.dynamic_terms + other.dynamic_terms)
+
     # --------------------------------------------------------------- layout
     def _ensure_setup(self):
         if self._setup:
@@ -509,6 +520,7 @@ class hessian:
         _lib.check(lib.ysp_bsr3_gather_split(dev.ptr(self.static.values), dev.ptr(self.dynamic.values),
                                              dev.ptr(f.src), f.nnz, dev.ptr(f.v64), s),
                    "ysp_bsr3_gather_split")
+        f.v32_valid = False                 # FP32 values are cast from these on demand
 
     def _update_full(self):
         if not self.all_3x3:
@@ -548,12 +560,19 @@ class hessian:
                    "ysp_bsr3_gather_values")
         f.v32_valid = bool(self.want_f32)
 
+    @property
+    def operator(self) -> Optional[FullBsr]:
+        """The solve loop's mirrored full 3x3 block-CSR: ``full`` for 3x3 layouts, ``split``
+        for layouts with affine-body segments (None for other layouts)."""
+        return self.full if self.full is not None else self.split
+
     def full_values_f32(self):
-        """FP32 copy of the full block-CSR values (gathered with the FP64 values when
-        ``want_f32`` is set, else cast on demand)."""
-        f = self.full
+        """FP32 copy of the full (or split) block-CSR values (gathered with the FP64 values
+        when ``want_f32`` is set, else cast on demand)."""
+        f = self.operator
         if f is None:
-            raise NotImplementedError("FP32 block values need 3 x 3 variable blocks")
+            raise NotImplementedError("FP32 block values need 3 x 3 blocks or affine-body "
+                                      "segments of multiples of 3")
         stride = (f.nnz + 63) // 64 * 64
         if f.v32 is None or f.v32.numel() != 9 * stride:
             f.v32 = dev.empty(9 * stride, "float32")

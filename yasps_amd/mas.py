@@ -623,7 +623,7 @@ class masSolver:
 
     def _solve_mixed(self, h, rhs, n, tolerance, max_iterations, rebuilt, t0, t1, t2):
         lib = _lib.load()
-        f = h.full
+        f = h.operator
         v32 = h.full_values_f32()
         m = self._device
         stats = _lib.PcgStats()

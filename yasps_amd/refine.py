@@ -30,7 +30,7 @@ def refine_solve(h, rhs, solution, tol, max_iterations, inner, buffers: Refineme
     """``inner(r, d, tol_in, max_it) -> (rc, iterations)`` solves A32 d = r to the relative
     residual tol_in. Returns the outer statistics; ``solution`` holds x."""
     lib = _lib.load()
-    f = h.full
+    f = h.operator
     n = int(rhs.numel())
     buffers.ensure(n)
     r, d = buffers.r, buffers.d

@@ -398,6 +398,7 @@ class SolveServer:
         try:
             while pending or active or reserved:
                 now = clock() - t0
+                refused = False                 # an arrived request exceeded the budget
                 while pending and pending[0].arrival <= now and \
                         len(active) + len(reserved) < self.max_active:
                     n = pending[0].n
@@ -405,6 +406,7 @@ class SolveServer:
                         idle = [sl for sl in self.pool.get(n, []) if sl.request is None]
                         need = idle[0].footprint if idle else self._estimate(n)
                         if self._in_use() + sum(reserved.values()) + need > self.budget:
+                            refused = True
                             break
                     req = pending.pop(0)
                     req.admitted = clock() - t0
@@ -422,7 +424,7 @@ class SolveServer:
                     if reserved:
                         time.sleep(2e-5)
                         continue
-                    if pending and pending[0].arrival <= clock() - t0:
+                    if refused:
                         raise ValueError(f"request {pending[0].id} (n={pending[0].n}) does not "
                                          "fit the HBM budget of an empty server")
                     if pending:
@@ -445,15 +447,18 @@ class SolveServer:
         s = dev.stream()
         while pending or active:
             now = clock() - t0
-            while pending and pending[0].arrival <= now and len(active) < self.max_active \
-                    and self._fits(pending[0].n):
+            refused = False                     # an arrived request exceeded the budget
+            while pending and pending[0].arrival <= now and len(active) < self.max_active:
+                if not self._fits(pending[0].n):
+                    refused = True
+                    break
                 req = pending.pop(0)
                 slot = self._acquire(req.n)
                 slot.load(req, self.tolerance, self.max_iterations)
                 req.admitted = clock() - t0
                 active.append(slot)
             if not active:
-                if pending and pending[0].arrival <= clock() - t0:
+                if refused:
                     raise ValueError(f"request {pending[0].id} (n={pending[0].n}) does not "
                                      "fit the HBM budget of an empty server")
                 if pending:

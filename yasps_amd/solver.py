@@ -109,7 +109,7 @@ class jacobianPCGSolver:
             f = h.split
             rc = lib.ysp_pcg_solve_full(
                 int(maxIterations), float(tolerance), _MODES[self.threshold_mode],
-                dev.ptr(f.row_ptr), dev.ptr(f.col), dev.ptr(f.v64), f.nnz, 0,
+                dev.ptr(f.row_ptr), dev.ptr(f.col), dev.ptr(f.v64), 0, f.nnz, 0,
                 dev.ptr(h.diagonal_blocks_inverse), _lib.u32_array(starts), _lib.u32_array(counts),
                 _lib.u32_array(sizes), _lib.u32_array(h.gradient_segments_start_cpu[:-1]),
                 len(h.wrt), dev.ptr(right_hand_side.value), n, dev.ptr(self.__solution),
@@ -144,26 +144,45 @@ class jacobianPCGSolver:
         return 0
 
     def _solve_mixed(self, h, rhs, n, tolerance, max_iterations, t0, t1, t2):
-        if h.full is None:
-            raise NotImplementedError("precision='mixed' needs 3 x 3 variable blocks")
+        if h.operator is None:
+            raise NotImplementedError("precision='mixed' needs 3 x 3 blocks or affine-body "
+                                      "segments of multiples of 3")
         lib = _lib.load()
         s = dev.stream()
-        f = h.full
+        f = h.operator
         v32 = h.full_values_f32()
-        nd = h.diagonal_blocks_inverse.numel()
-        if self._minv32 is None or self._minv32.numel() != nd:
-            self._minv32 = dev.empty(nd, "float32")
-        _lib.check(lib.ysp_cast_f32(dev.ptr(h.diagonal_blocks_inverse), dev.ptr(self._minv32), nd, s),
-                   "ysp_cast_f32")
         stats = _lib.PcgStats()
+        if h.full is None:
+            # affine-body layout: the split operator in FP32, the reference's block-Jacobi
+            # (9 x 9 inverse blocks for the affine matrices) kept in FP64
+            counts = [w.numInstances for w in h.wrt]
+            sizes = [w.size for w in h.wrt]
+            starts = h.diagonal_blocks_start_cpu[:-1]
 
-        def inner(r, d, tol_in, max_it):
-            rc = lib.ysp_pcg3_solve(int(max_it), float(tol_in), 1, dev.ptr(f.row_ptr), dev.ptr(f.col),
-                                    dev.ptr(v32), 1, f.nnz, dev.ptr(self._minv32), 1, dev.ptr(r), n,
-                                    dev.ptr(d), None, 1, dev.ptr(self.__work), dev.ptr(self.__scalars),
-                                    C.byref(stats), s)
-            _lib.check(rc, "ysp_pcg3_solve")
-            return rc, int(stats.iterations)
+            def inner(r, d, tol_in, max_it):
+                rc = lib.ysp_pcg_solve_full(
+                    int(max_it), float(tol_in), 1, dev.ptr(f.row_ptr), dev.ptr(f.col), dev.ptr(v32), 1,
+                    f.nnz, 0, dev.ptr(h.diagonal_blocks_inverse), _lib.u32_array(starts),
+                    _lib.u32_array(counts), _lib.u32_array(sizes),
+                    _lib.u32_array(h.gradient_segments_start_cpu[:-1]), len(h.wrt), dev.ptr(r), n,
+                    dev.ptr(d), None, 1, dev.ptr(self.__work), dev.ptr(self.__scalars),
+                    C.byref(stats), s)
+                _lib.check(rc, "ysp_pcg_solve_full")
+                return rc, int(stats.iterations)
+        else:
+            nd = h.diagonal_blocks_inverse.numel()
+            if self._minv32 is None or self._minv32.numel() != nd:
+                self._minv32 = dev.empty(nd, "float32")
+            _lib.check(lib.ysp_cast_f32(dev.ptr(h.diagonal_blocks_inverse), dev.ptr(self._minv32),
+                                        nd, s), "ysp_cast_f32")
+
+            def inner(r, d, tol_in, max_it):
+                rc = lib.ysp_pcg3_solve(int(max_it), float(tol_in), 1, dev.ptr(f.row_ptr),
+                                        dev.ptr(f.col), dev.ptr(v32), 1, f.nnz, dev.ptr(self._minv32),
+                                        1, dev.ptr(r), n, dev.ptr(d), None, 1, dev.ptr(self.__work),
+                                        dev.ptr(self.__scalars), C.byref(stats), s)
+                _lib.check(rc, "ysp_pcg3_solve")
+                return rc, int(stats.iterations)
 
         out = refine_solve(h, rhs.value, self.__solution, float(tolerance), int(max_iterations), inner,
                            self._refine)
