@@ -184,7 +184,7 @@ def test_hessian_terms_sum_and_solve_like_the_scene():
     b = dev.to_host(hs.spmv(vector(hs.rows, dev.to_device(v, "float64"))).value)
     assert _rel(a, b) < REL
     assert _rel(dev.to_host(H.gradient.value), dev.to_host(hs.gradient.value)) < REL
-    sol = solver("jacobian")
+    sol = solver("jacobian", threshold_mode="relative_residual")
     rc = sol.computeSolution(H, H.gradient, None, tolerance=1e-10, maxIterations=5000,
                              zero_initial_guess=True)
     assert rc >= 0
