@@ -49,6 +49,8 @@ def parse():
                     help="skip the per-energy assembly leg")
     ap.add_argument("--no-contact", action="store_true",
                     help="skip the contact scene with affine bodies")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the n=64 cube (operator beyond the Infinity Cache)")
     ap.add_argument("--contact-pairs", type=int, default=1_500_000)
     ap.add_argument("--contact-steps", type=int, default=3)
     ap.add_argument("--adjoint-n", type=int, default=32)
@@ -60,8 +62,12 @@ def parse():
     return ap.parse_args()
 
 
+# the PCG product kernel on the mirrored FP64 copy, as rocprofv3 names it
+PRODUCT_KERNEL = "pcg3_spmv_kernel<ysp::Mirrored<double>"
+
+
 def measure_traffic(args):
-    """HBM bytes per launch of pcg3_spmv_kernel<double> (the PCG loop's product kernel) from
+    """HBM bytes per launch of pcg3_spmv_kernel<Mirrored<double>> (the PCG loop's product kernel) from
     two hardware-counter runs, FETCH_SIZE then WRITE_SIZE, of ``bench.py --kernels-only``
     under rocprofv3, each counter scaled by its reading for a 256 MiB device copy of known
     size. Runs as child processes before this process touches the GPU. Returns
@@ -101,7 +107,7 @@ def measure_traffic(args):
                         if row.get("Counter_Name") != counter:
                             continue
                         name = row.get("Kernel_Name", "")
-                        if "pcg3_spmv_kernel<double>" in name:
+                        if PRODUCT_KERNEL in name:
                             prod.append(float(row["Counter_Value"]))
                         elif "copyBuffer" in name:
                             copy.append(float(row["Counter_Value"]))
@@ -133,7 +139,7 @@ def calibration_copy(mib=256):
 
 
 def product_roofline(hs, reps, traffic=None, traffic_detail=None, cold_reps=20, replicas=4):
-    """Roofline of pcg3_spmv_kernel<double>, the kernel that dominates the timed PCG loop
+    """Roofline of pcg3_spmv_kernel<Mirrored<double>>, the kernel that dominates the timed PCG loop
     (one launch per iteration: q = A z + beta q, p = z + beta p and the partial sums of
     p.q over the mirrored 3x3 block-CSR), launched through ysp_pcg3_product_pass with the
     loop's own grid and timed by HIP events on the stream it runs on.
@@ -197,7 +203,7 @@ def product_roofline(hs, reps, traffic=None, traffic_detail=None, cold_reps=20, 
     gbs_warm = bytes_alg / (ms_warm * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(gbs_cold, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs_cold / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": "pcg3_spmv_kernel<double>",
+            "kernel": "pcg3_spmv_kernel<Mirrored<double>>",
             "regime": (f"HBM: back-to-back launches rotating over {replicas} copies of the "
                        "operator and vectors (each launch streams its operands from HBM)"),
             "avg_us": round(ms_cold * 1e3, 2), "launches": cold_reps * replicas if cold_reps else 0,
@@ -470,6 +476,39 @@ def contact_leg(args):
             "split_3x3_entries": int(hs.split.nnz) if hs.split is not None else None,
             "value": legs["mas"]["value"], "unit": "Newton solves/s", "solvers": legs,
             "scene_generation_s": round(scene_s, 2)}
+
+
+def large_operator_leg(args, n=64, steps=5):
+    """The FP64 block-Jacobi Newton solve on a cube whose mirrored block-CSR (285 MB at
+    n=64) does not stay in the 256 MB Infinity Cache, with the solve loop reading the
+    mirrored copy and reading the upper block storage through the entries' source words
+    (the automatic choice for such an operator). Same iterations; ms per solve."""
+    from yasps_amd import device as dev
+    from yasps_amd.scenes import cube_scene, build_yasps_scene
+    torch = dev.torch()
+    cs = cube_scene(n, seed=args.seed)
+    s0, _ = build_yasps_scene(cs, threshold_mode="relative_residual")
+    s0.minimizeEnergy(tolerance=args.tol)
+    hs = s0.minimizer.activeHessian
+    out = {"n": n, "tets": int(cs.tets.shape[0]), "unknowns": int(hs.rows),
+           "mirrored_entries": int(hs.full.nnz),
+           "mirrored_value_MB": round(72 * hs.full.nnz / 1e6, 1)}
+    for source in ("mirrored", "auto"):
+        hs.value_source = source
+        s0.minimizeEnergy(tolerance=args.tol)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s0.minimizeEnergy(tolerance=args.tol)
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / steps
+        st = s0.minimizer.linearSolver.statistics
+        out[source] = {"ms_per_solve": round(ms, 3), "solves_per_s": round(1e3 / ms, 2),
+                       "pcg_iterations": int(st["iterations"]),
+                       "values_read": "upper storage" if hs.solve_values()[1] == 2 else "mirrored copy"}
+    print(f"large operator n={n}: mirrored {out['mirrored']['ms_per_solve']} ms, "
+          f"auto {out['auto']['ms_per_solve']} ms", file=sys.stderr, flush=True)
+    return out
 
 
 def side_leg(fn, args):
@@ -761,9 +800,9 @@ def main():
     cs = cube_scene(args.n, seed=args.seed)
     s0, hnd = build_yasps_scene(cs, threshold_mode="relative_residual")
     if args.kernels_only:
-        # assembly only (the operator of the loop), then the product launches (back to back,
-        # then each after a cache flush) and the calibration copy: the only pcg3_spmv_kernel
-        # launches are the measured ones
+        # assembly only (the operator of the loop), then the product launches (back to back
+        # on one operator, then rotating over its copies) and the calibration copy: the only
+        # pcg3_spmv_kernel launches are the measured ones
         s0.minimizer.computeNumericValue()
         roof = product_roofline(s0.minimizer.activeHessian, args.spmv_reps,
                                 cold_reps=args.cold_reps)
@@ -826,6 +865,7 @@ def main():
         "dynamic_contacts": None if args.no_dynamic else side_leg(dynamic_leg, args),
         "element_kernels": None if args.no_elements else side_leg(element_leg, args),
         "contact_mixed_dof": None if args.no_contact else side_leg(contact_leg, args),
+        "large_operator": None if args.no_large else side_leg(large_operator_leg, args),
     }
     if not args.no_cpu:
         threads = max(1, min(16, len(os.sched_getaffinity(0))))

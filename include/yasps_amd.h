@@ -532,6 +532,26 @@ int ysp_bsr3_build_split(const ysp_block_matrix* part, const ysp_block_matrix* d
 int ysp_bsr3_gather_split(const double* static_values, const double* dynamic_values,
                           const uint64_t* src, uint32_t nnz, double* out64, void* stream);
 
+/* Value sources of the solve loop's operator. Every entry point that takes
+ * (values, values_f32) for the full or split block-CSR reads values_f32 as a format:
+ * YSP_VALUES_F64 / YSP_VALUES_F32: values is the component-major copy
+ * (ysp_bsr3_gather_values / ysp_bsr3_gather_split, or a cast of it);
+ * YSP_VALUES_UPPER: values points to a host ysp_upper_values, and the products read the
+ * upper block storage through each entry's source word (no mirrored copy: every stored
+ * block is read by both of its entries, about half the value bytes of the mirrored copy
+ * for operators too large to stay in the 256 MB Infinity Cache). Same arithmetic, same
+ * results as the FP64 copy. The batched PCG takes the component-major formats only. */
+#define YSP_VALUES_F64 0
+#define YSP_VALUES_F32 1
+#define YSP_VALUES_UPPER 2
+
+typedef struct ysp_upper_values {
+  const void* src;                   /* per entry: uint32 (ysp_bsr3_build) or uint64 (_split) */
+  int32_t src_64;                    /* 1: 64-bit split source words */
+  const double* static_values;       /* upper storage of the static part (blocks_flattened) */
+  const double* dynamic_values;      /* of the dynamic part (NULL: none) */
+} ysp_upper_values;
+
 /* out[i] = (float) in[i] (FP32 copies of FP64 device arrays, e.g. inverse blocks). */
 int ysp_cast_f32(const double* in, float* out, uint64_t n, void* stream);
 

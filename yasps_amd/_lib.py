@@ -61,6 +61,15 @@ class CoordDim(C.Structure):
                 ("w", C.c_uint16), ("placeholder", C.c_uint32)]
 
 
+VALUES_F64, VALUES_F32, VALUES_UPPER = 0, 1, 2
+
+
+class UpperValues(C.Structure):
+    """ysp_upper_values: the upper block storage read through each entry's source word."""
+    _fields_ = [("src", C.c_void_p), ("src_64", C.c_int32), ("static_values", C.c_void_p),
+                ("dynamic_values", C.c_void_p)]
+
+
 class PcgStats(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("status", C.c_int32), ("delta0", C.c_double),
                 ("delta", C.c_double), ("rr0", C.c_double), ("rr", C.c_double)]

@@ -12,6 +12,7 @@
 #include "common.hpp"
 #include "dense.hpp"
 #include "mas.hpp"
+#include "values.hpp"
 
 #include <cmath>
 
@@ -24,10 +25,6 @@ namespace ysp {
 namespace {
 
 constexpr unsigned kB = 256;
-constexpr uint32_t kTrans = 1u << 31;
-constexpr uint32_t kDyn = 1u << 30;
-
-__host__ __device__ __forceinline__ uint32_t pad64(uint32_t n) { return (n + 63u) & ~63u; }
 
 // Blocks per XCD-contiguous chunk: consecutive blockIdx values land on different XCDs
 // (round robin over 8), so remap them so that each XCD walks one contiguous range of
@@ -132,11 +129,11 @@ inline uint32_t spmv_rows_per_wg(uint32_t nv) {
   return std::min<uint32_t>(kMaxRowsPerWg, std::max<uint32_t>(kPass, rows));
 }
 
-// One 16-lane group per block row: (y0, y1, y2) = sum over the row's blocks of B x_c.
-template <typename VT>
+// One 16-lane group per block row: (y0, y1, y2) = sum over the row's blocks of B x_c,
+// the blocks read from the value source (values.hpp).
+template <typename Src>
 __device__ __forceinline__ void row_product(const uint32_t* __restrict__ row_ptr,
-                                            const uint32_t* __restrict__ col,
-                                            const VT* __restrict__ v, uint32_t stride,
+                                            const uint32_t* __restrict__ col, const Src& vals,
                                             const double* __restrict__ x, uint32_t r, int l,
                                             double& y0, double& y1, double& y2) {
   y0 = y1 = y2 = 0.0;
@@ -144,18 +141,9 @@ __device__ __forceinline__ void row_product(const uint32_t* __restrict__ row_ptr
   for (uint32_t k = k0 + l; k < k1; k += kG) {
     const uint32_t c = 3 * col[k];
     const double x0 = x[c], x1 = x[c + 1], x2 = x[c + 2];
-    const double b0 = static_cast<double>(v[0 * static_cast<size_t>(stride) + k]);
-    const double b1 = static_cast<double>(v[1 * static_cast<size_t>(stride) + k]);
-    const double b2 = static_cast<double>(v[2 * static_cast<size_t>(stride) + k]);
-    const double b3 = static_cast<double>(v[3 * static_cast<size_t>(stride) + k]);
-    const double b4 = static_cast<double>(v[4 * static_cast<size_t>(stride) + k]);
-    const double b5 = static_cast<double>(v[5 * static_cast<size_t>(stride) + k]);
-    const double b6 = static_cast<double>(v[6 * static_cast<size_t>(stride) + k]);
-    const double b7 = static_cast<double>(v[7 * static_cast<size_t>(stride) + k]);
-    const double b8 = static_cast<double>(v[8 * static_cast<size_t>(stride) + k]);
-    y0 += b0 * x0 + b1 * x1 + b2 * x2;
-    y1 += b3 * x0 + b4 * x1 + b5 * x2;
-    y2 += b6 * x0 + b7 * x1 + b8 * x2;
+    double b[9];
+    vals.load(k, b);
+    block_fma(b, x0, x1, x2, y0, y1, y2);
   }
 #pragma unroll
   for (int off = kG / 2; off > 0; off >>= 1) {
@@ -165,17 +153,17 @@ __device__ __forceinline__ void row_product(const uint32_t* __restrict__ row_ptr
   }
 }
 
-template <typename VT>
+template <typename Src>
 __global__ void __launch_bounds__(kB)
     bsr3_full_spmv_kernel(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
-                          const VT* __restrict__ v, uint32_t stride, uint32_t nbr,
-                          const double* __restrict__ x, double* __restrict__ y) {
+                          Src vals, uint32_t nbr, const double* __restrict__ x,
+                          double* __restrict__ y) {
   const uint32_t blk = xcd_swizzle(blockIdx.x, gridDim.x);
   const uint32_t r = blk * (kB / kG) + threadIdx.x / kG;
   const int l = threadIdx.x % kG;
   if (r >= nbr) return;
   double y0, y1, y2;
-  row_product<VT>(row_ptr, col, v, stride, x, r, l, y0, y1, y2);
+  row_product(row_ptr, col, vals, x, r, l, y0, y1, y2);
   if (l == 0) {
     y[3 * r] = y0;
     y[3 * r + 1] = y1;
@@ -345,10 +333,9 @@ __global__ void __launch_bounds__(64) pcg3_init_finish_kernel(double* st, double
 // row, so the product costs one pass over the blocks like a plain SpMV.
 // Body of one workgroup of the product kernel: blk is the workgroup's chunk of block rows
 // of this system, sb its partial-sum slot, lead the workgroup that writes the status.
-template <typename VT>
+template <typename Src>
 __device__ __forceinline__ void pcg3_spmv_body(const uint32_t* __restrict__ row_ptr,
-                                               const uint32_t* __restrict__ col,
-                                               const VT* __restrict__ v, uint32_t stride,
+                                               const uint32_t* __restrict__ col, const Src& vals,
                                                uint32_t nbr, const double* __restrict__ z,
                                                double* __restrict__ p, double* __restrict__ q,
                                                double* __restrict__ st, int k,
@@ -404,7 +391,7 @@ __device__ __forceinline__ void pcg3_spmv_body(const uint32_t* __restrict__ row_
       y1 = ylong[3 * static_cast<size_t>(ls) + 1];
       y2 = ylong[3 * static_cast<size_t>(ls) + 2];
     } else {
-      row_product<VT>(row_ptr, col, v, stride, z, r, l, y0, y1, y2);
+      row_product(row_ptr, col, vals, z, r, l, y0, y1, y2);
     }
     if (l < 3) ys[3 * (r - rbase) + l] = l == 0 ? y0 : (l == 1 ? y1 : y2);
   }
@@ -424,15 +411,14 @@ __device__ __forceinline__ void pcg3_spmv_body(const uint32_t* __restrict__ row_
   wg_add(pap, slot_ptr_at(st, SET_PAP + cur, sb), lds);
 }
 
-template <typename VT>
+template <typename Src>
 __global__ void __launch_bounds__(kB)
     pcg3_spmv_kernel(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
-                     const VT* __restrict__ v, uint32_t stride, uint32_t nbr,
-                     const double* __restrict__ z, double* __restrict__ p,
+                     Src vals, uint32_t nbr, const double* __restrict__ z, double* __restrict__ p,
                      double* __restrict__ q, double* __restrict__ st, int k,
                      uint32_t rows_per_wg, const uint32_t* __restrict__ long_slot,
                      const double* __restrict__ ylong) {
-  pcg3_spmv_body<VT>(row_ptr, col, v, stride, nbr, z, p, q, st, k, rows_per_wg,
+  pcg3_spmv_body(row_ptr, col, vals, nbr, z, p, q, st, k, rows_per_wg,
                      xcd_swizzle(blockIdx.x, gridDim.x), blockIdx.x, blockIdx.x == 0, long_slot,
                      ylong);
 }
@@ -440,26 +426,22 @@ __global__ void __launch_bounds__(kB)
 // The block rows longer than kLongRow3 entries (the coupling rows of affine bodies, the
 // cloth rows under a ball): one workgroup per row computes (A z)_r into ylong ahead of
 // the product kernel, which reads it instead of walking the row with 16 lanes.
-template <typename VT>
+template <typename Src>
 __global__ void __launch_bounds__(kB)
     pcg3_long_rows_kernel(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
-                          const VT* __restrict__ v, uint32_t stride,
-                          const uint32_t* __restrict__ long_rows, const double* __restrict__ z,
+                          Src vals, const uint32_t* __restrict__ long_rows,
+                          const double* __restrict__ z,
                           double* __restrict__ ylong, const double* __restrict__ st) {
   if (stopped(st)) return;
   const uint32_t r = long_rows[blockIdx.x];
   const uint32_t k0 = row_ptr[r], k1 = row_ptr[r + 1];
   double y0 = 0.0, y1 = 0.0, y2 = 0.0;
-  const size_t sd = stride;
   for (uint32_t k = k0 + threadIdx.x; k < k1; k += kB) {
     const uint32_t c = 3 * col[k];
     const double x0 = z[c], x1 = z[c + 1], x2 = z[c + 2];
-    y0 += static_cast<double>(v[k]) * x0 + static_cast<double>(v[sd + k]) * x1 +
-          static_cast<double>(v[2 * sd + k]) * x2;
-    y1 += static_cast<double>(v[3 * sd + k]) * x0 + static_cast<double>(v[4 * sd + k]) * x1 +
-          static_cast<double>(v[5 * sd + k]) * x2;
-    y2 += static_cast<double>(v[6 * sd + k]) * x0 + static_cast<double>(v[7 * sd + k]) * x1 +
-          static_cast<double>(v[8 * sd + k]) * x2;
+    double b[9];
+    vals.load(k, b);
+    block_fma(b, x0, x1, x2, y0, y1, y2);
   }
   __shared__ double red[kB / 64];
   y0 = block_sum<kB>(y0, red);
@@ -568,12 +550,12 @@ long predict_more(double it_a, double val_a, double it_b, double val_b, double t
   return static_cast<long>(detect - static_cast<double>(enqueued));
 }
 
-template <typename VT>
-void launch_full_spmv(const uint32_t* row_ptr, const uint32_t* col, const VT* v, uint32_t nnz,
-                      uint32_t nbr, const double* x, double* y, hipStream_t s) {
+template <typename Src>
+void launch_full_spmv(const uint32_t* row_ptr, const uint32_t* col, const Src& vals, uint32_t nbr,
+                      const double* x, double* y, hipStream_t s) {
   const unsigned rows_per_block = kB / kG;
-  hipLaunchKernelGGL(bsr3_full_spmv_kernel<VT>, dim3(grid_for(nbr, rows_per_block)), dim3(kB), 0,
-                     s, row_ptr, col, v, pad64(nnz), nbr, x, y);
+  hipLaunchKernelGGL(bsr3_full_spmv_kernel<Src>, dim3(grid_for(nbr, rows_per_block)), dim3(kB), 0,
+                     s, row_ptr, col, vals, nbr, x, y);
 }
 
 }  // namespace
@@ -670,9 +652,6 @@ extern "C" int ysp_bsr3_gather_values(const double* static_values, const double*
 // element in its part, 40-41 w/3 - 1 (the row stride of the source block), 42 transposed,
 // 43 dynamic part.
 namespace {
-constexpr uint64_t kSplitOffset = (1ull << 40) - 1;
-constexpr uint64_t kSplitTrans = 1ull << 42;
-constexpr uint64_t kSplitDyn = 1ull << 43;
 
 // Entries 2i and 2i + 1 of this category's range for sub-block i = (block, a, b).
 __global__ void emit_split_kernel(const uint32_t* __restrict__ pos, uint32_t nb, uint32_t h,
@@ -824,12 +803,11 @@ extern "C" int ysp_bsr3_spmv(const uint32_t* row_ptr, const uint32_t* col, const
   if (n % 3) return YSP_ERR_ARG;
   if (n == 0) return YSP_OK;
   hipStream_t s = as_stream(stream);
-  if (values_f32)
-    launch_full_spmv(row_ptr, col, static_cast<const float*>(values), nnz, n / 3, x, y, s);
-  else
-    launch_full_spmv(row_ptr, col, static_cast<const double*>(values), nnz, n / 3, x, y, s);
-  YSP_CHECK_LAUNCH();
-  return YSP_OK;
+  return with_values(values, values_f32, nnz, "ysp_bsr3_spmv: bad value source", [&](auto vals) {
+    launch_full_spmv(row_ptr, col, vals, n / 3, x, y, s);
+    YSP_CHECK_LAUNCH();
+    return YSP_OK;
+  });
 }
 
 namespace {
@@ -855,11 +833,11 @@ __global__ void __launch_bounds__(kB)
 }
 
 // r = b - A x over the full block-CSR, with ||r||^2 and ||b||^2 partial sums (sums[0..1]).
-template <typename VT>
+template <typename Src>
 __global__ void __launch_bounds__(kB)
     residual_kernel(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
-                    const VT* __restrict__ v, uint32_t stride, uint32_t nbr,
-                    const double* __restrict__ x, const double* __restrict__ b,
+                    Src vals, uint32_t nbr, const double* __restrict__ x,
+                    const double* __restrict__ b,
                     double* __restrict__ r, double* __restrict__ sums) {
   const uint32_t blk = xcd_swizzle(blockIdx.x, gridDim.x);
   const uint32_t row = blk * (kB / kG) + threadIdx.x / kG;
@@ -867,7 +845,7 @@ __global__ void __launch_bounds__(kB)
   double rr = 0.0, bb = 0.0;
   if (row < nbr) {
     double y0, y1, y2;
-    row_product<VT>(row_ptr, col, v, stride, x, row, l, y0, y1, y2);
+    row_product(row_ptr, col, vals, x, row, l, y0, y1, y2);
     if (l < 3) {
       const size_t d = 3 * static_cast<size_t>(row) + l;
       const double bl = b[d];
@@ -910,14 +888,14 @@ extern "C" int ysp_bsr3_refine(const uint32_t* row_ptr, const uint32_t* col, con
                        static_cast<size_t>(n));
   const uint32_t nbr = n / 3;
   const dim3 g(grid_for(nbr, kB / kG)), b(kB);
-  if (values_f32)
-    hipLaunchKernelGGL(residual_kernel<float>, g, b, 0, s, row_ptr, col,
-                       static_cast<const float*>(values), pad64(nnz), nbr, solution, rhs, residual,
-                       sums);
-  else
-    hipLaunchKernelGGL(residual_kernel<double>, g, b, 0, s, row_ptr, col,
-                       static_cast<const double*>(values), pad64(nnz), nbr, solution, rhs,
-                       residual, sums);
+  if (int rc = with_values(values, values_f32, nnz, "ysp_bsr3_refine: bad value source",
+                           [&](auto vals) {
+                             hipLaunchKernelGGL(residual_kernel<decltype(vals)>, g, b, 0, s,
+                                                row_ptr, col, vals, nbr, solution, rhs, residual,
+                                                sums);
+                             return YSP_OK;
+                           }))
+    return rc;
   YSP_CHECK_LAUNCH();
   double h[nsum];
   YSP_CHECK(hipMemcpyAsync(h, sums, sizeof(h), hipMemcpyDeviceToHost, s));
@@ -946,9 +924,9 @@ std::mutex g_pcg3_solve_mutex;
 // status ring, no events, no long-row tables: every row goes through the 16-lane product)
 // and clears the scalar state with a kernel; the host waits for each chunk's status with
 // a stream synchronisation.
-template <typename VT, typename MT>
+template <typename Src, typename MT>
 int pcg3_run(uint32_t max_iterations, double threshold, int32_t mode, const uint32_t* row_ptr,
-             const uint32_t* col, const VT* v, uint32_t nnz, const MT* minv, const MasApply* mas,
+             const uint32_t* col, const Src& vals, uint32_t nnz, const MT* minv, const MasApply* mas,
              const double* rhs, uint32_t n, double* x, const double* x0, double* work,
              double* st, ysp_pcg_stats* stats, hipStream_t s, bool no_alloc = false) {
   std::unique_lock<std::mutex> solve_lock(g_pcg3_solve_mutex, std::defer_lock);
@@ -958,7 +936,6 @@ int pcg3_run(uint32_t max_iterations, double threshold, int32_t mode, const uint
   double* z = work + n;
   double* pa = work + 2 * static_cast<size_t>(n);
   double* q = work + 4 * static_cast<size_t>(n);
-  const uint32_t stride = pad64(nnz);
   if (no_alloc)
     hipLaunchKernelGGL(fill_zero_kernel, dim3(grid_for(kStateDoubles, kB)), dim3(kB), 0, s, st,
                        static_cast<uint32_t>(kStateDoubles));
@@ -966,7 +943,7 @@ int pcg3_run(uint32_t max_iterations, double threshold, int32_t mode, const uint
     YSP_CHECK(hipMemsetAsync(st, 0, kStateDoubles * sizeof(double), s));
   const double* ax0 = nullptr;
   if (x0) {
-    launch_full_spmv(row_ptr, col, v, nnz, nv, x0, q, s);
+    launch_full_spmv(row_ptr, col, vals, nv, x0, q, s);
     ax0 = q;
   }
   // product grid: contiguous chunks of block rows, a multiple of 16 rows per workgroup
@@ -1032,9 +1009,9 @@ int pcg3_run(uint32_t max_iterations, double threshold, int32_t mode, const uint
   const uint32_t end = max_iterations + 1;
   auto launch_pass = [&](uint32_t it) -> int {
     if (num_long)
-      hipLaunchKernelGGL(pcg3_long_rows_kernel<VT>, dim3(num_long), bl, 0, s, row_ptr, col, v,
-                         stride, long_rows, z, ylong, st);
-    hipLaunchKernelGGL(pcg3_spmv_kernel<VT>, sg, bl, 0, s, row_ptr, col, v, stride, nv, z, pa, q,
+      hipLaunchKernelGGL(pcg3_long_rows_kernel<Src>, dim3(num_long), bl, 0, s, row_ptr, col, vals,
+                         long_rows, z, ylong, st);
+    hipLaunchKernelGGL(pcg3_spmv_kernel<Src>, sg, bl, 0, s, row_ptr, col, vals, nv, z, pa, q,
                        st, static_cast<int>(it), rows_per_wg, long_slot, ylong);
     if (it <= max_iterations) {
       hipLaunchKernelGGL(pcg3_update_kernel<MT>, ug, dim3(kUB), 0, s, minv, x, r, pa, q, z, n, st,
@@ -1162,31 +1139,22 @@ int pcg3_run(uint32_t max_iterations, double threshold, int32_t mode, const uint
 }  // namespace
 
 namespace {
-// Dispatch on the value and inverse types (FP64 / FP32).
+// Dispatch on the value source (values.hpp) and the inverse type (FP64 / FP32).
 int pcg3_dispatch(uint32_t max_iterations, double threshold, int32_t threshold_mode,
                   const uint32_t* row_ptr, const uint32_t* col, const void* values,
-                  int32_t values_f32, uint32_t nnz, const void* diagonal_inverse,
+                  int32_t value_format, uint32_t nnz, const void* diagonal_inverse,
                   int32_t inverse_f32, const double* rhs, uint32_t n, double* solution,
                   const double* x0, double* work, double* cg_scalars, ysp_pcg_stats* stats,
                   hipStream_t s, bool no_alloc) {
-  if (values_f32) {
-    const float* v = static_cast<const float*>(values);
+  return with_values(values, value_format, nnz, "PCG: bad value source", [&](auto vals) {
     if (inverse_f32)
-      return pcg3_run(max_iterations, threshold, threshold_mode, row_ptr, col, v, nnz,
+      return pcg3_run(max_iterations, threshold, threshold_mode, row_ptr, col, vals, nnz,
                       static_cast<const float*>(diagonal_inverse), nullptr, rhs, n, solution, x0,
                       work, cg_scalars, stats, s, no_alloc);
-    return pcg3_run(max_iterations, threshold, threshold_mode, row_ptr, col, v, nnz,
+    return pcg3_run(max_iterations, threshold, threshold_mode, row_ptr, col, vals, nnz,
                     static_cast<const double*>(diagonal_inverse), nullptr, rhs, n, solution, x0,
                     work, cg_scalars, stats, s, no_alloc);
-  }
-  const double* v = static_cast<const double*>(values);
-  if (inverse_f32)
-    return pcg3_run(max_iterations, threshold, threshold_mode, row_ptr, col, v, nnz,
-                    static_cast<const float*>(diagonal_inverse), nullptr, rhs, n, solution, x0,
-                    work, cg_scalars, stats, s, no_alloc);
-  return pcg3_run(max_iterations, threshold, threshold_mode, row_ptr, col, v, nnz,
-                  static_cast<const double*>(diagonal_inverse), nullptr, rhs, n, solution, x0,
-                  work, cg_scalars, stats, s, no_alloc);
+  });
 }
 }  // namespace
 
@@ -1253,16 +1221,14 @@ extern "C" int ysp_pcg3_product_pass(const uint32_t* row_ptr, const uint32_t* co
   }
   const uint32_t rows_per_wg = spmv_rows_per_wg(nv);
   const dim3 sg(grid_for(nv, rows_per_wg)), bl(kB);
-  if (values_f32)
-    hipLaunchKernelGGL(pcg3_spmv_kernel<float>, sg, bl, 0, s, row_ptr, col,
-                       static_cast<const float*>(values), pad64(nnz), nv, z, pa, q, cg_scalars, 2,
-                       rows_per_wg, nullptr, nullptr);
-  else
-    hipLaunchKernelGGL(pcg3_spmv_kernel<double>, sg, bl, 0, s, row_ptr, col,
-                       static_cast<const double*>(values), pad64(nnz), nv, z, pa, q, cg_scalars,
-                       2, rows_per_wg, nullptr, nullptr);
-  YSP_CHECK_LAUNCH();
-  return YSP_OK;
+  return with_values(values, values_f32, nnz, "ysp_pcg3_product_pass: bad value source",
+                     [&](auto vals) {
+                       hipLaunchKernelGGL(pcg3_spmv_kernel<decltype(vals)>, sg, bl, 0, s, row_ptr,
+                                          col, vals, nv, z, pa, q, cg_scalars, 2, rows_per_wg,
+                                          nullptr, nullptr);
+                       YSP_CHECK_LAUNCH();
+                       return YSP_OK;
+                     });
 }
 
 // ---------------------------------------------------------- PCG without allocations
@@ -1352,13 +1318,12 @@ extern "C" int ysp_pcg3_mas_solve(uint32_t max_iterations, double threshold,
     return YSP_ERR_ARG;
   }
   const float* no_minv = nullptr;
-  if (values_f32)
-    return pcg3_run(max_iterations, threshold, threshold_mode, row_ptr, col,
-                    static_cast<const float*>(values), nnz, no_minv, m, rhs, n, solution, x0, work,
-                    cg_scalars, stats, s);
-  return pcg3_run(max_iterations, threshold, threshold_mode, row_ptr, col,
-                  static_cast<const double*>(values), nnz, no_minv, m, rhs, n, solution, x0, work,
-                  cg_scalars, stats, s);
+  return with_values(values, values_f32, nnz, "ysp_pcg3_mas_solve: bad value source",
+                     [&](auto vals) {
+                       return pcg3_run(max_iterations, threshold, threshold_mode, row_ptr, col,
+                                       vals, nnz, no_minv, m, rhs, n, solution, x0, work,
+                                       cg_scalars, stats, s);
+                     });
 }
 
 // ------------------------------------------------------------------ batched PCG
@@ -1401,7 +1366,8 @@ __global__ void __launch_bounds__(kB)
   const BatchSystem& sy = S[w.sys];
   const int k = sy.k0 + pass;
   if (k > sy.kmax + 1) return;
-  pcg3_spmv_body<VT>(sy.row_ptr, sy.col, static_cast<const VT*>(sy.values), sy.stride, sy.nbr,
+  pcg3_spmv_body(sy.row_ptr, sy.col, Mirrored<VT>{static_cast<const VT*>(sy.values), sy.stride},
+                 sy.nbr,
                      sy.z, sy.p, sy.q, sy.st, k, sy.rows_per_wg, w.blk, w.blk, w.blk == 0);
 }
 

@@ -16,9 +16,9 @@
 #include "common.hpp"
 #include "dense.hpp"
 #include "spmv.hpp"
+#include "values.hpp"
 
 #include <algorithm>
-#include <type_traits>
 #include <vector>
 
 namespace ysp {
@@ -357,10 +357,10 @@ constexpr unsigned kFullRows = kFullThreads / kFullLanes;
 // workgroup per row) so that a few long rows (the coupling rows of affine bodies) do not
 // set the time.
 constexpr uint32_t kLongRow = 128;
-template <typename VT>
+template <typename Src>
 __global__ void __launch_bounds__(kFullThreads)
     full3_product_kernel(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
-                         const VT* __restrict__ v, uint32_t stride, uint32_t nbr,
+                         Src vals, uint32_t nbr,
                          uint32_t long_row, int reverse, const double* __restrict__ x,
                          double* __restrict__ y, double* __restrict__ quad,
                          const double* __restrict__ stop) {
@@ -374,13 +374,9 @@ __global__ void __launch_bounds__(kFullThreads)
     for (uint32_t k = k0 + l; k < k1; k += kFullLanes) {
       const uint32_t c = 3 * col[k];
       const double x0 = x[c], x1 = x[c + 1], x2 = x[c + 2];
-      const size_t st = stride;
-      y0 += static_cast<double>(v[k]) * x0 + static_cast<double>(v[st + k]) * x1 +
-            static_cast<double>(v[2 * st + k]) * x2;
-      y1 += static_cast<double>(v[3 * st + k]) * x0 + static_cast<double>(v[4 * st + k]) * x1 +
-            static_cast<double>(v[5 * st + k]) * x2;
-      y2 += static_cast<double>(v[6 * st + k]) * x0 + static_cast<double>(v[7 * st + k]) * x1 +
-            static_cast<double>(v[8 * st + k]) * x2;
+      double b[9];
+      vals.load(k, b);
+      block_fma(b, x0, x1, x2, y0, y1, y2);
     }
 #pragma unroll
     for (int off = kFullLanes / 2; off > 0; off >>= 1) {
@@ -403,10 +399,10 @@ __global__ void __launch_bounds__(kFullThreads)
 }
 
 // The rows longer than kLongRow (listed in long_rows): one workgroup per row.
-template <typename VT>
+template <typename Src>
 __global__ void __launch_bounds__(kFullThreads)
     full3_long_row_kernel(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
-                          const VT* __restrict__ v, uint32_t stride,
+                          Src vals,
                           const uint32_t* __restrict__ long_rows, int reverse,
                           const double* __restrict__ x, double* __restrict__ y,
                           double* __restrict__ quad, const double* __restrict__ stop) {
@@ -414,16 +410,12 @@ __global__ void __launch_bounds__(kFullThreads)
   const uint32_t r = long_rows[reverse ? gridDim.x - 1 - blockIdx.x : blockIdx.x];
   const uint32_t k0 = row_ptr[r], k1 = row_ptr[r + 1];
   double y0 = 0.0, y1 = 0.0, y2 = 0.0;
-  const size_t st = stride;
   for (uint32_t k = k0 + threadIdx.x; k < k1; k += kFullThreads) {
     const uint32_t c = 3 * col[k];
     const double x0 = x[c], x1 = x[c + 1], x2 = x[c + 2];
-    y0 += static_cast<double>(v[k]) * x0 + static_cast<double>(v[st + k]) * x1 +
-          static_cast<double>(v[2 * st + k]) * x2;
-    y1 += static_cast<double>(v[3 * st + k]) * x0 + static_cast<double>(v[4 * st + k]) * x1 +
-          static_cast<double>(v[5 * st + k]) * x2;
-    y2 += static_cast<double>(v[6 * st + k]) * x0 + static_cast<double>(v[7 * st + k]) * x1 +
-          static_cast<double>(v[8 * st + k]) * x2;
+    double b[9];
+    vals.load(k, b);
+    block_fma(b, x0, x1, x2, y0, y1, y2);
   }
   __shared__ double red[kFullThreads / 64];
   y0 = block_sum<kFullThreads>(y0, red);
@@ -654,14 +646,15 @@ extern "C" int ysp_pcg_solve_full(uint32_t max_iterations, double threshold,
                                   double* solution, const double* initial_guess,
                                   int32_t zero_initial_guess, double* work, double* cg_scalars,
                                   ysp_pcg_stats* stats, void* stream) {
-  if (n % 3 || (nnz && (!row_ptr || !col || !values))) {
+  if (n % 3 || (nnz && (!row_ptr || !col || !values)) ||
+      (values_f32 != YSP_VALUES_F64 && values_f32 != YSP_VALUES_F32 &&
+       values_f32 != YSP_VALUES_UPPER)) {
     if (stats) *stats = ysp_pcg_stats{0, 0, 0, 0, 0, 0};
     ysp_set_error(__FILE__, __LINE__, "ysp_pcg_solve_full: bad arguments");
     return YSP_ERR_ARG;
   }
   hipStream_t s = as_stream(stream);
   const uint32_t nbr = n / 3;
-  const uint32_t stride = (nnz + 63u) & ~63u;
   // rows longer than the threshold, listed once per solve from the row starts
   const uint32_t long_row = long_row_threshold ? long_row_threshold : kLongRow;
   uint32_t num_long_rows = 0;
@@ -684,22 +677,25 @@ extern "C" int ysp_pcg_solve_full(uint32_t max_iterations, double threshold,
     }
   }
   int reverse = 0;                 // direction of the next product's row walk
-  // FP32 values (the mixed-precision inner solve) are widened to FP64 as they are read
-  auto launch = [&](auto vals, const double* x, double* y, double* quad, const double* stop) {
-    using VT = std::remove_const_t<std::remove_pointer_t<decltype(vals)>>;
-    hipLaunchKernelGGL(full3_product_kernel<VT>, dim3(grid_for(nbr, kFullRows)),
-                       dim3(kFullThreads), 0, s, row_ptr, col, vals, stride, nbr, long_row, reverse,
-                       x, y, quad, stop);
-    if (num_long_rows)
-      hipLaunchKernelGGL(full3_long_row_kernel<VT>, dim3(num_long_rows), dim3(kFullThreads), 0, s,
-                         row_ptr, col, vals, stride, long_rows, reverse, x, y, quad, stop);
-  };
+  // value source (values.hpp): FP64 or FP32 copy (FP32 values, the mixed-precision inner
+  // solve, are widened to FP64 as they are read) or the upper storage read in place
   auto product = [&](const double* x, double* y, double* quad, const double* stop) -> int {
     if (nbr == 0) return YSP_OK;
-    if (values_f32)
-      launch(static_cast<const float*>(values), x, y, quad, stop);
-    else
-      launch(static_cast<const double*>(values), x, y, quad, stop);
+    if (int rc = with_values(values, values_f32, nnz, "ysp_pcg_solve_full: bad value source",
+                             [&](auto vals) {
+                               using Src = decltype(vals);
+                               hipLaunchKernelGGL(full3_product_kernel<Src>,
+                                                  dim3(grid_for(nbr, kFullRows)),
+                                                  dim3(kFullThreads), 0, s, row_ptr, col, vals,
+                                                  nbr, long_row, reverse, x, y, quad, stop);
+                               if (num_long_rows)
+                                 hipLaunchKernelGGL(full3_long_row_kernel<Src>,
+                                                    dim3(num_long_rows), dim3(kFullThreads), 0, s,
+                                                    row_ptr, col, vals, long_rows, reverse, x, y,
+                                                    quad, stop);
+                               return YSP_OK;
+                             }))
+      return rc;
     reverse ^= 1;
     YSP_CHECK_LAUNCH();
     return YSP_OK;

@@ -239,6 +239,10 @@ class hessian:
         self.full: Optional[FullBsr] = None
         self.split: Optional[FullBsr] = None     # 3x3 sub-blocks of 9x9 / 9x3 / 3x9 layouts
         self.want_f32 = False
+        # where the solve loop reads FP64 operator values: "auto" (the upper storage when
+        # the mirrored copy would not stay in the Infinity Cache), "mirrored" or "upper"
+        self.value_source = "auto"
+        self._upper = None
         self.use_staging = True
         self._grad_offsets = None
 
@@ -565,6 +569,35 @@ class hessian:
         """The solve loop's mirrored full 3x3 block-CSR: ``full`` for 3x3 layouts, ``split``
         for layouts with affine-body segments (None for other layouts)."""
         return self.full if self.full is not None else self.split
+
+    # mirrored FP64 copies above this many bytes leave too little of the 256 MB Infinity
+    # Cache for the rest of the loop's working set
+    UPPER_VALUE_BYTES = 160 << 20
+
+    def solve_values(self):
+        """(pointer, format) of the FP64 operator values the solve loop reads: the
+        component-major mirrored copy (``YSP_VALUES_F64``) or the upper block storage read
+        through each entry's source word (``YSP_VALUES_UPPER``, a ``ysp_upper_values`` held
+        by this object), chosen by ``value_source``. Both give the same products."""
+        f = self.operator
+        if f is None:
+            raise NotImplementedError("the solve loop's value sources need 3 x 3 blocks or "
+                                      "affine-body segments of multiples of 3")
+        if self.value_source not in ("auto", "mirrored", "upper"):
+            raise ValueError("value_source must be 'auto', 'mirrored' or 'upper'")
+        # "auto": only 3x3 layouts without a dynamic part, whose upper blocks of a block row
+        # lie next to each other (the split layout's categories and the dynamic part scatter
+        # a row's sources, and its reads then cost more than the mirrored copy's)
+        upper = self.value_source == "upper" or (
+            self.value_source == "auto" and self.full is not None and self.dynamic.values is None
+            and 72 * f.nnz > self.UPPER_VALUE_BYTES)
+        if not upper:
+            return dev.ptr(f.v64), _lib.VALUES_F64
+        sv = self.static.values if self.static.values is not None else self.dynamic.values
+        dv = self.dynamic.values
+        self._upper = _lib.UpperValues(dev.ptr(f.src), int(self.full is None), dev.ptr(sv),
+                                       dev.ptr(dv) if dv is not None else None)
+        return C.addressof(self._upper), _lib.VALUES_UPPER
 
     def full_values_f32(self):
         """FP32 copy of the full (or split) block-CSR values (gathered with the FP64 values

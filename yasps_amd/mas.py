@@ -584,9 +584,10 @@ class masSolver:
         stats = _lib.PcgStats()
         if self.precision == "mixed":
             return self._solve_mixed(h, rhs, n, tolerance, maxIterations, rebuilt, t0, t1, t2)
+        vals, fmt = h.solve_values()
         rc = lib.ysp_pcg3_mas_solve(
             int(maxIterations), float(tolerance), _MODES[self.threshold_mode],
-            dev.ptr(f.row_ptr), dev.ptr(f.col), dev.ptr(f.v64), 0, f.nnz, C.byref(m.args),
+            dev.ptr(f.row_ptr), dev.ptr(f.col), vals, fmt, f.nnz, C.byref(m.args),
             dev.ptr(rhs), n, dev.ptr(self._solution), dev.ptr(guess), int(guess is None),
             dev.ptr(self._work), dev.ptr(self._scalars), C.byref(stats), dev.stream())
         _lib.check(rc, "ysp_pcg3_mas_solve")

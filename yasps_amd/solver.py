@@ -99,17 +99,19 @@ class jacobianPCGSolver:
             return self._solve_mixed(h, right_hand_side, n, tolerance, maxIterations, t0, t1, t2)
         if h.full is not None:
             f = h.full
+            vals, fmt = h.solve_values()
             rc = lib.ysp_pcg3_solve(
                 int(maxIterations), float(tolerance), _MODES[self.threshold_mode],
-                dev.ptr(f.row_ptr), dev.ptr(f.col), dev.ptr(f.v64), 0, f.nnz,
+                dev.ptr(f.row_ptr), dev.ptr(f.col), vals, fmt, f.nnz,
                 dev.ptr(h.diagonal_blocks_inverse), 0, dev.ptr(right_hand_side.value), n,
                 dev.ptr(self.__solution), dev.ptr(guess), zero, dev.ptr(self.__work),
                 dev.ptr(self.__scalars), C.byref(stats), s)
         elif h.split is not None:
             f = h.split
+            vals, fmt = h.solve_values()
             rc = lib.ysp_pcg_solve_full(
                 int(maxIterations), float(tolerance), _MODES[self.threshold_mode],
-                dev.ptr(f.row_ptr), dev.ptr(f.col), dev.ptr(f.v64), 0, f.nnz, 0,
+                dev.ptr(f.row_ptr), dev.ptr(f.col), vals, fmt, f.nnz, 0,
                 dev.ptr(h.diagonal_blocks_inverse), _lib.u32_array(starts), _lib.u32_array(counts),
                 _lib.u32_array(sizes), _lib.u32_array(h.gradient_segments_start_cpu[:-1]),
                 len(h.wrt), dev.ptr(right_hand_side.value), n, dev.ptr(self.__solution),
