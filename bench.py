@@ -336,7 +336,8 @@ def adjoint_leg(args):
     """Material inverse on an n^3 cube (bottom fixed, gravity, theta = log E per tet ~
     N(log 1e5, 0.3)): equilibrium, loss against the equilibrium of a second theta, adjoint
     gradient in FP64 and mixed precision, central differences (h = 1e-6) on random
-    entries."""
+    entries; and the gradients w.r.t. E and nu per tet through the generalised mixed
+    Jacobian (diff2), checked against the log-E adjoint and central differences in nu."""
     from yasps_amd.adjoint import MaterialProblem
     prob = MaterialProblem(n=args.adjoint_n)
     rng = np.random.default_rng(args.seed)
@@ -364,7 +365,28 @@ def adjoint_leg(args):
               - prob.loss(prob.equilibrium(tm, x0=x, tol=1e-13), target)) / (2 * h)
         err.append(abs(fd - g[i]))
     scale = float(np.abs(g).max())
+    # dL/dE and dL/dnu per tet through the generalised mixed Jacobian d g / d (E, nu)
+    xbar = prob.loss_grad_x(x, target)
+    prob.set_theta(theta)
+    t5 = time.perf_counter()
+    gE, gnu = prob.vjp_material(x, xbar)
+    t6 = time.perf_counter()
+    E = np.exp(theta)
+    nu0 = np.full(prob.num_tets, prob.poisson)
+    err_nu = []
+    for i in idx[:8]:
+        npl, nmi = nu0.copy(), nu0.copy()
+        npl[i] += h
+        nmi[i] -= h
+        fd = (prob.loss(prob.equilibrium_material(E, npl, x0=x, tol=1e-13), target)
+              - prob.loss(prob.equilibrium_material(E, nmi, x0=x, tol=1e-13), target)) / (2 * h)
+        err_nu.append(abs(fd - gnu[i]))
+    material = {"vjp_E_nu_s": round(t6 - t5, 4),
+                "E_dLdE_vs_log_E_adjoint_rel": float(np.abs(E * gE - g).max() / scale),
+                "fd_nu_entries": len(err_nu),
+                "fd_nu_max_abs_err_rel_to_max_grad": float(max(err_nu) / np.abs(gnu).max())}
     return {"n": args.adjoint_n, "tets": prob.num_tets, "unknowns": 3 * prob.n_free,
+            "material_gradients": material,
             "newton_iterations": prob.newton_iterations,
             "equilibrium_s": round(t1 - t0, 3), "adjoint_fp64_s": round(t2 - t1, 4),
             "adjoint_mixed_s": round(t4 - t3, 4),

@@ -10,6 +10,13 @@ Adjoint: for a loss L(x*(theta)), H(x*) lambda = dL/dx (the same PCG, FP64 or th
 FP32/FP64 refinement) and dL/dtheta_e = -lambda_e . dg_e/dtheta_e, evaluated per tet by
 ``ysp_snh_theta_adjoint``. ``EquilibriumFunction`` exposes theta -> x* as a
 ``torch.autograd.Function`` whose backward is this adjoint.
+
+The tets carry Young's modulus and Poisson's ratio as per-tet constants and the SNH
+coefficients are expressions of them (``lame_parameters`` applied to the attributes), so
+``vjp_material`` gives dL/dE and dL/dnu per tet through the generalised mixed
+Jacobian d g / d (E, nu) of ``yasps_amd.differentiator`` (the material example's
+``young_jacobian`` / ``poisson_jacobian`` products,
+``examples/bunny_material_optimization/bunny_material_optimization.py:616-617,1123-1126``).
 """
 from __future__ import annotations
 
@@ -49,12 +56,13 @@ class MaterialProblem:
         self.x_att.updateValue(cs.rest[nf:])
         m.vertices.addAttribute("position")
         m.vertices.addAttribute("rest_position")
-        self.mu_att = tets.addAttribute("mu", rows=1, cols=1)
-        self.lam_att = tets.addAttribute("lam", rows=1, cols=1)
+        self.young_att = tets.addConstant("young", rows=1, cols=1)
+        self.poisson_att = tets.addConstant("poisson", rows=1, cols=1)
+        self.mu_expr, self.lam_expr = lame_parameters(self.young_att, self.poisson_att)
         tet2v = tets.addConnectivity("tet2v", m.vertices, cs.tets, 4)
         tp = tets.addAttribute("positions", through=tet2v, source=m.vertices["position"])
         trp = tets.addAttribute("rest_positions", through=tet2v, source=m.vertices["rest_position"])
-        self.snh = stable_neo_hookean(trp, tp, self.mu_att, self.lam_att, dt)
+        self.snh = stable_neo_hookean(trp, tp, self.mu_expr, self.lam_expr, dt)
         snh_att = tets.addAttribute("stable_neo_hookean", computed_attribute=self.snh)
         s0.addEnergy(snh_att, projection_method=projection_method)
         s0.addMinimizeTarget([self.x_att])
@@ -76,11 +84,17 @@ class MaterialProblem:
     def num_tets(self) -> int:
         return self.cs.tets.shape[0]
 
+    def set_material(self, young, poisson):
+        """Per-tet Young's modulus and Poisson's ratio (arrays or scalars)."""
+        T = self.num_tets
+        self.young = np.broadcast_to(np.asarray(young, dtype=np.float64), (T,)).copy()
+        self.nu = np.broadcast_to(np.asarray(poisson, dtype=np.float64), (T,)).copy()
+        self.young_att.updateValue(self.young)
+        self.poisson_att.updateValue(self.nu)
+
     def set_theta(self, theta: np.ndarray):
         theta = np.asarray(theta, dtype=np.float64)
-        mu, lam = lame_parameters(np.exp(theta), self.poisson)
-        self.mu_att.updateValue(mu)
-        self.lam_att.updateValue(lam)
+        self.set_material(np.exp(theta), self.poisson)
         self.theta = theta.copy()
 
     def _set_x(self, x):
@@ -108,6 +122,16 @@ class MaterialProblem:
                     pcg_tol: float = 1e-11):
         """Static equilibrium x* (free vertex positions, flat) for theta."""
         self.set_theta(theta)
+        return self._newton(x0, tol, max_newton, pcg_tol)
+
+    def equilibrium_material(self, young, poisson, x0=None, tol: float = 1e-12,
+                             max_newton: int = 60, pcg_tol: float = 1e-11):
+        """Static equilibrium x* for per-tet Young's modulus and Poisson's ratio."""
+        self.set_material(young, poisson)
+        self.theta = None
+        return self._newton(x0, tol, max_newton, pcg_tol)
+
+    def _newton(self, x0, tol, max_newton, pcg_tol):
         x = (self.cs.rest[self.cs.n_fixed:].reshape(-1) if x0 is None else np.asarray(x0)).copy()
         self._set_x(x)
         fnorm = float(np.linalg.norm(self.f_host))
@@ -160,9 +184,9 @@ class MaterialProblem:
         return g.reshape(-1)
 
     # ------------------------------------------------------------ adjoint
-    def vjp(self, x, xbar, pcg_tol: float = 1e-12, projection_method: int = 0):
-        """dL/dtheta for dL/dx = xbar at the equilibrium x (H unprojected by default: the
-        adjoint of the exact equilibrium)."""
+    def adjoint(self, x, xbar, pcg_tol: float = 1e-12, projection_method: int = 0):
+        """lambda with H(x) lambda = xbar at the equilibrium x (H unprojected by default:
+        the adjoint of the exact equilibrium); returns (lambda, the assembled Hessian)."""
         self._set_x(x)
         mn = self.scene.minimizer
         hs = mn._active()
@@ -173,6 +197,27 @@ class MaterialProblem:
             lam = self._solve(hs, dev.to_device(np.asarray(xbar, dtype=np.float64), "float64"), pcg_tol)
         finally:
             hs.terms = saved
+        return lam, hs
+
+    def vjp_material(self, x, xbar, pcg_tol: float = 1e-12, projection_method: int = 0):
+        """(dL/dE, dL/dnu) per tet for dL/dx = xbar at the equilibrium x:
+        -J^T lambda with J = d g / d (E, nu) from ``differentiator().diff2`` (the chain
+        through the SNH coefficients' expressions)."""
+        from .differentiator import differentiator
+        lam, _ = self.adjoint(x, xbar, pcg_tol, projection_method)
+        if getattr(self, "_material_jacobian", None) is None:
+            self._material_jacobian = differentiator().diff2(
+                [self.snh_att], [self.x_att], [self.young_att, self.poisson_att])
+        J = self._material_jacobian.compute()
+        g = -dev.to_host(J.spmv(lam, transpose=True).value)
+        T = self.num_tets
+        return g[:T], g[T:]
+
+    def vjp(self, x, xbar, pcg_tol: float = 1e-12, projection_method: int = 0):
+        """dL/dtheta for dL/dx = xbar at the equilibrium x (H unprojected by default: the
+        adjoint of the exact equilibrium)."""
+        from .expr import values as expr_values
+        lam, hs = self.adjoint(x, xbar, pcg_tol, projection_method)
         out = dev.empty(self.num_tets, "float64")
         ti = hs.static_index[0]
         sa = hs._scatter_args(ti, hs.static)
@@ -180,10 +225,13 @@ class MaterialProblem:
         e = self.snh
         dm_inv, vol = e.rest_data()
         u = union_view(e.pos.source)
+        T = self.num_tets
+        mu = expr_values(self.mu_expr, T, 1).reshape(-1)
+        la = expr_values(self.lam_expr, T, 1).reshape(-1)
         _lib.check(_lib.load().ysp_snh_theta_adjoint(
-            C.byref(u), dev.ptr(e.conn.value), self.num_tets, dev.ptr(dm_inv), dev.ptr(vol),
-            dev.ptr(self.mu_att.value), 1, dev.ptr(self.lam_att.value), 1, 1.0, C.byref(sa),
-            dev.ptr(lam), dev.ptr(out), dev.stream()), "ysp_snh_theta_adjoint")
+            C.byref(u), dev.ptr(e.conn.value), T, dev.ptr(dm_inv), dev.ptr(vol),
+            dev.ptr(mu), 1, dev.ptr(la), 1, 1.0, C.byref(sa), dev.ptr(lam), dev.ptr(out),
+            dev.stream()), "ysp_snh_theta_adjoint")
         _lib.check(_lib.load().ysp_check_deferred(dev.stream()), "ysp_snh_theta_adjoint")
         return dev.to_host(out)
 
