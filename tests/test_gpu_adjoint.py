@@ -45,6 +45,7 @@ def test_material_gradients_through_mixed_jacobians(setup):
     matches central differences of the loss (equilibrium re-solved per perturbed tet)."""
     prob, theta, target, x = setup
     print("material gradients via diff2", flush=True)
+    prob.set_theta(theta)               # earlier tests leave a perturbed theta set
     xbar = prob.loss_grad_x(x, target)
     g_theta = prob.vjp(x, xbar)
     gE, gnu = prob.vjp_material(x, xbar)
