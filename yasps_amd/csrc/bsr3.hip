@@ -342,7 +342,8 @@ __device__ __forceinline__ void pcg3_spmv_body(const uint32_t* __restrict__ row_
                                                uint32_t rows_per_wg, uint32_t blk, uint32_t sb,
                                                bool lead_wg,
                                                const uint32_t* __restrict__ long_slot = nullptr,
-                                               const double* __restrict__ ylong = nullptr) {
+                                               const double* __restrict__ ylong = nullptr,
+                                               const uint32_t* __restrict__ chunk_start = nullptr) {
   if (stopped(st)) return;
   const int cur = k % 3, prev = (k + 2) % 3, prev2 = (k + 1) % 3;
   __shared__ double lds[kB / 64];
@@ -374,14 +375,15 @@ __device__ __forceinline__ void pcg3_spmv_body(const uint32_t* __restrict__ row_
     st[S_FINAL_RR] = rr1;
     st[S_CUR_ITER] = k - 1;
   }
-  // Each workgroup walks a contiguous chunk of rows_per_wg (<= kMaxRowsPerWg) block rows,
-  // 16 at a time, so the grid stays near a few thousand workgroups: fewer partial-sum
+  // Each workgroup walks a contiguous chunk of block rows (rows_per_wg of them, or the
+  // entry-balanced chunk_start[blk] .. chunk_start[blk + 1]; at most kMaxRowsPerWg), 16
+  // at a time, so the grid stays near a few thousand workgroups: fewer partial-sum
   // atomics and fewer slot reads by the consumers than one workgroup per 16 rows. The
   // row products meet in LDS; the chunk's 3 * rows scalars of p, q and z are then
   // updated one per thread, in coalesced order.
   __shared__ double ys[3 * kMaxRowsPerWg];
-  const uint32_t rbase = blk * rows_per_wg;
-  const uint32_t r_end = min(nbr, rbase + rows_per_wg);
+  const uint32_t rbase = chunk_start ? chunk_start[blk] : blk * rows_per_wg;
+  const uint32_t r_end = chunk_start ? chunk_start[blk + 1] : min(nbr, rbase + rows_per_wg);
   const int l = threadIdx.x % kG;
   for (uint32_t r = rbase + threadIdx.x / kG; r < r_end; r += kB / kG) {
     double y0, y1, y2;
@@ -417,23 +419,29 @@ __global__ void __launch_bounds__(kB)
                      Src vals, uint32_t nbr, const double* __restrict__ z, double* __restrict__ p,
                      double* __restrict__ q, double* __restrict__ st, int k,
                      uint32_t rows_per_wg, const uint32_t* __restrict__ long_slot,
-                     const double* __restrict__ ylong) {
+                     const double* __restrict__ ylong, const uint32_t* __restrict__ chunk_start,
+                     int reverse) {
+  const uint32_t blk = xcd_swizzle(blockIdx.x, gridDim.x);
   pcg3_spmv_body(row_ptr, col, vals, nbr, z, p, q, st, k, rows_per_wg,
-                     xcd_swizzle(blockIdx.x, gridDim.x), blockIdx.x, blockIdx.x == 0, long_slot,
-                     ylong);
+                     reverse ? gridDim.x - 1 - blk : blk, blockIdx.x, blockIdx.x == 0, long_slot,
+                     ylong, chunk_start);
 }
 
 // The block rows longer than kLongRow3 entries (the coupling rows of affine bodies, the
 // cloth rows under a ball): one workgroup per row computes (A z)_r into ylong ahead of
 // the product kernel, which reads it instead of walking the row with 16 lanes.
+// `reverse` walks the rows (and the product kernel its chunks) in the opposite order
+// every other iteration: an operator larger than the Infinity Cache then starts each
+// pass on the blocks the previous pass read last, which are still cached.
 template <typename Src>
 __global__ void __launch_bounds__(kB)
     pcg3_long_rows_kernel(const uint32_t* __restrict__ row_ptr, const uint32_t* __restrict__ col,
                           Src vals, const uint32_t* __restrict__ long_rows,
                           const double* __restrict__ z,
-                          double* __restrict__ ylong, const double* __restrict__ st) {
+                          double* __restrict__ ylong, const double* __restrict__ st, int reverse) {
   if (stopped(st)) return;
-  const uint32_t r = long_rows[blockIdx.x];
+  const uint32_t i = reverse ? gridDim.x - 1 - blockIdx.x : blockIdx.x;
+  const uint32_t r = long_rows[i];
   const uint32_t k0 = row_ptr[r], k1 = row_ptr[r + 1];
   double y0 = 0.0, y1 = 0.0, y2 = 0.0;
   for (uint32_t k = k0 + threadIdx.x; k < k1; k += kB) {
@@ -448,9 +456,9 @@ __global__ void __launch_bounds__(kB)
   y1 = block_sum<kB>(y1, red);
   y2 = block_sum<kB>(y2, red);
   if (threadIdx.x == 0) {
-    ylong[3 * static_cast<size_t>(blockIdx.x)] = y0;
-    ylong[3 * static_cast<size_t>(blockIdx.x) + 1] = y1;
-    ylong[3 * static_cast<size_t>(blockIdx.x) + 2] = y2;
+    ylong[3 * static_cast<size_t>(i)] = y0;
+    ylong[3 * static_cast<size_t>(i) + 1] = y1;
+    ylong[3 * static_cast<size_t>(i) + 2] = y2;
   }
 }
 
@@ -920,6 +928,46 @@ __global__ void __launch_bounds__(kB) fill_zero_kernel(double* __restrict__ p, u
 // process-wide: single-system solves issued from several host threads run one at a time.
 std::mutex g_pcg3_solve_mutex;
 
+// Entry-balanced chunks of block rows for the PCG product, or none when the uniform
+// chunks of spmv_rows_per_wg rows are balanced already (no short row longer than twice the
+// mean). Cost of a row: its entries (one for a long row, whose product comes from the
+// long-row kernel) plus 2 for its share of the p / q update. A chunk closes at
+// kMaxRowsPerWg rows or when it holds the work of kChunkMeanRows mean rows; the result is
+// the chunks' first rows and nv. On the contact scene's operator (121,576 block rows of 1
+// to 1,005 entries) chunks of 8 / 16 / 32 / 64 mean rows measured 77.8 / 66.7 / 62.9 /
+// 67.0 us per product, uniform 64-row chunks 77.6 us.
+constexpr uint32_t kChunkMeanRows = 32;
+std::vector<uint32_t> balanced_chunks(const std::vector<uint32_t>& rp, uint32_t nv) {
+  std::vector<uint32_t> out;
+  if (nv == 0) return out;
+  uint64_t total = 0;
+  uint32_t longest = 0;
+  for (uint32_t r = 0; r < nv; ++r) {
+    const uint32_t len = rp[r + 1] - rp[r];
+    const uint32_t cost = (len > kLongRow3 ? 1u : len) + 2u;
+    total += cost;
+    if (len <= kLongRow3) longest = std::max(longest, len);
+  }
+  const double mean = static_cast<double>(total) / nv;
+  if (longest + 2 <= 2.0 * mean) return out;
+  const double target = kChunkMeanRows * mean;
+  out.push_back(0);
+  double acc = 0.0;
+  uint32_t rows = 0;
+  for (uint32_t r = 0; r < nv; ++r) {
+    const uint32_t len = rp[r + 1] - rp[r];
+    acc += (len > kLongRow3 ? 1u : len) + 2u;
+    ++rows;
+    if (rows == kMaxRowsPerWg || acc >= target) {
+      if (r + 1 < nv) out.push_back(r + 1);
+      acc = 0.0;
+      rows = 0;
+    }
+  }
+  out.push_back(nv);
+  return out;
+}
+
 // no_alloc: the variant behind ysp_pcg3_solve_noalloc. It allocates nothing (no pinned
 // status ring, no events, no long-row tables: every row goes through the 16-lane product)
 // and clears the scalar state with a kernel; the host waits for each chunk's status with
@@ -952,23 +1000,35 @@ int pcg3_run(uint32_t max_iterations, double threshold, int32_t mode, const uint
   // (row_ptr, rows, entries) differ from the previous solve's. Any list of rows gives the
   // same product (a listed row is computed whole by its workgroup), so a list kept for an
   // operator rebuilt in place costs speed at most, never correctness.
-  static struct { const uint32_t* row_ptr; uint32_t nv, nnz, num_long; } cache{nullptr, 0, 0, 0};
-  uint32_t num_long = 0;
+  // With row lengths that vary widely (contact couplings), the product's chunks are cut
+  // to about equal work instead (entry-balanced, listed with the long rows): a uniform
+  // chunk of long rows would set the kernel's time while chunks of short rows idle.
+  static struct {
+    const uint32_t* row_ptr;
+    uint32_t nv, nnz, num_long, num_chunks;
+  } cache{nullptr, 0, 0, 0, 0};
+  uint32_t num_long = 0, num_chunks = 0;
   uint32_t* long_rows = nullptr;
   uint32_t* long_slot = nullptr;
   double* ylong = nullptr;
+  uint32_t* chunk_start = nullptr;
   if (no_alloc) {
-    // every row through the product kernel
+    // every row through the product kernel, uniform chunks
   } else if (cache.row_ptr == row_ptr && cache.nv == nv && cache.nnz == nnz) {
     num_long = cache.num_long;
+    num_chunks = cache.num_chunks;
     if (num_long) {
       long_rows = static_cast<uint32_t*>(ysp::scratch(67, sizeof(uint32_t) * num_long));
       long_slot = static_cast<uint32_t*>(ysp::scratch(68, sizeof(uint32_t) * nv));
       ylong = static_cast<double*>(ysp::scratch(69, sizeof(double) * 3 * num_long));
       if (!long_rows || !long_slot || !ylong) return YSP_ERR_HIP;
     }
+    if (num_chunks) {
+      chunk_start = static_cast<uint32_t*>(ysp::scratch(76, sizeof(uint32_t) * (num_chunks + 1)));
+      if (!chunk_start) return YSP_ERR_HIP;
+    }
   } else {
-    cache = {nullptr, 0, 0, 0};
+    cache = {nullptr, 0, 0, 0, 0};
     std::vector<uint32_t> rp(static_cast<size_t>(nv) + 1);
     YSP_CHECK(hipMemcpyAsync(rp.data(), row_ptr, sizeof(uint32_t) * rp.size(),
                              hipMemcpyDeviceToHost, s));
@@ -976,6 +1036,15 @@ int pcg3_run(uint32_t max_iterations, double threshold, int32_t mode, const uint
     std::vector<uint32_t> longs;
     for (uint32_t r = 0; r < nv; ++r)
       if (rp[r + 1] - rp[r] > kLongRow3) longs.push_back(r);
+    const std::vector<uint32_t> chunks = balanced_chunks(rp, nv);
+    if (!chunks.empty()) {
+      num_chunks = static_cast<uint32_t>(chunks.size() - 1);
+      chunk_start = static_cast<uint32_t*>(ysp::scratch(76, sizeof(uint32_t) * chunks.size()));
+      if (!chunk_start) return YSP_ERR_HIP;
+      YSP_CHECK(hipMemcpyAsync(chunk_start, chunks.data(), sizeof(uint32_t) * chunks.size(),
+                               hipMemcpyHostToDevice, s));
+      YSP_CHECK(hipStreamSynchronize(s));
+    }
     if (!longs.empty()) {
       num_long = static_cast<uint32_t>(longs.size());
       std::vector<uint32_t> slot(nv, kNoLongSlot);
@@ -990,10 +1059,11 @@ int pcg3_run(uint32_t max_iterations, double threshold, int32_t mode, const uint
                                hipMemcpyHostToDevice, s));
       YSP_CHECK(hipStreamSynchronize(s));
     }
-    cache = {row_ptr, nv, nnz, num_long};
+    cache = {row_ptr, nv, nnz, num_long, num_chunks};
   }
   static_assert(kStateDoubles <= YSP_PCG3_SCALARS, "cg_scalars too small");
-  const dim3 vg(grid_for(nv, kB)), sg(grid_for(nv, rows_per_wg)), ug(grid_for(n, kUB)), bl(kB);
+  const dim3 vg(grid_for(nv, kB)), sg(num_chunks ? num_chunks : grid_for(nv, rows_per_wg)),
+      ug(grid_for(n, kUB)), bl(kB);
   hipLaunchKernelGGL(pcg3_init_kernel<MT>, vg, bl, 0, s, minv, rhs, x0, ax0, x, r, z, nv, st);
   if (mas) {
     if (x0) {
@@ -1008,11 +1078,13 @@ int pcg3_run(uint32_t max_iterations, double threshold, int32_t mode, const uint
   YSP_CHECK_LAUNCH();
   const uint32_t end = max_iterations + 1;
   auto launch_pass = [&](uint32_t it) -> int {
+    const int reverse = static_cast<int>(it & 1u);
     if (num_long)
       hipLaunchKernelGGL(pcg3_long_rows_kernel<Src>, dim3(num_long), bl, 0, s, row_ptr, col, vals,
-                         long_rows, z, ylong, st);
+                         long_rows, z, ylong, st, reverse);
     hipLaunchKernelGGL(pcg3_spmv_kernel<Src>, sg, bl, 0, s, row_ptr, col, vals, nv, z, pa, q,
-                       st, static_cast<int>(it), rows_per_wg, long_slot, ylong);
+                       st, static_cast<int>(it), rows_per_wg, long_slot, ylong, chunk_start,
+                       reverse);
     if (it <= max_iterations) {
       hipLaunchKernelGGL(pcg3_update_kernel<MT>, ug, dim3(kUB), 0, s, minv, x, r, pa, q, z, n, st,
                          static_cast<int>(it));
@@ -1225,7 +1297,7 @@ extern "C" int ysp_pcg3_product_pass(const uint32_t* row_ptr, const uint32_t* co
                      [&](auto vals) {
                        hipLaunchKernelGGL(pcg3_spmv_kernel<decltype(vals)>, sg, bl, 0, s, row_ptr,
                                           col, vals, nv, z, pa, q, cg_scalars, 2, rows_per_wg,
-                                          nullptr, nullptr);
+                                          nullptr, nullptr, nullptr, 0);
                        YSP_CHECK_LAUNCH();
                        return YSP_OK;
                      });
