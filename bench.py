@@ -461,7 +461,8 @@ def contact_leg(args):
     orthogonality. One Newton linear solve per step: re-index of the dynamic pair terms,
     element evaluation, PSD projection, assembly (9x9 / 9x3 / 3x9 / 3x3 blocks), the split
     3x3 operator, preconditioner setup and PCG to a true relative residual of --tol: with
-    MAS (32-DoF leaf domains, 4 levels, the affine DoFs as 3-DoF nodes; the leg's value)
+    MAS (32-DoF leaf domains, 4 levels, nodes typed by target: 9-DoF affine matrices,
+    3-DoF translations and vertices; the leg's value)
     and with block-Jacobi (9x9 blocks for the affine matrices)."""
     from yasps_amd import device as dev
     from yasps_amd.scenes import contact_scene, build_contact_scene

@@ -317,3 +317,95 @@ def test_duplicate_rule_matches_oracle_and_weights_apply(cube, levels, target):
     ref = omas.apply(h, invs, r, w, scales)
     got = emulate_device_apply(layout, A.data, r, w, duplicate_weight=0.5)
     assert np.abs(got - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+def _typed_operator(nv=60, nb=3, seed=0):
+    """An SPD operator over typed nodes as an affine-body scene has them: nv 3-DoF vertex
+    nodes (type 0), nb 9-DoF affine matrices (type 1) and nb 3-DoF translations (type 2),
+    vertices coupled along a chain and a stride-6 lattice, every body's matrix and
+    translation to each other and to six vertices. Returns dims, types, the block-row
+    pairs and the 3x3 BSR operator."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    dims = np.array([3] * nv + [9] * nb + [3] * nb)
+    types = np.array([0] * nv + [1] * nb + [2] * nb)
+    off = mh.offsets_of(dims)
+    edges = [(i, i + 1) for i in range(nv - 1)] + [(i, i + 6) for i in range(nv - 6)]
+    for b in range(nb):
+        a, t = nv + b, nv + nb + b
+        edges.append((a, t))
+        edges += [(a, int(v)) for v in rng.choice(nv, 6, replace=False)]
+        edges += [(t, int(v)) for v in rng.choice(nv, 6, replace=False)]
+    n = int(dims.sum())
+    A = np.zeros((n, n))
+    for i, j in edges:
+        M = 0.2 * rng.standard_normal((dims[i], dims[j]))
+        A[off[i]:off[i] + dims[i], off[j]:off[j] + dims[j]] += M
+        A[off[j]:off[j] + dims[j], off[i]:off[i] + dims[i]] += M.T
+    for i in range(dims.size):
+        B = rng.standard_normal((dims[i], dims[i]))
+        A[off[i]:off[i] + dims[i], off[i]:off[i] + dims[i]] += 0.1 * (B + B.T)
+    A += np.diag(np.abs(A).sum(1) + 1.0)
+    bsr = sp.bsr_matrix(A, blocksize=(3, 3))
+    bsr.sort_indices()
+    return dims, types, bsr
+
+
+def _typed_hierarchy(dims, types, bsr, **kw):
+    node_of_row = np.repeat(np.arange(dims.size), dims // 3)
+    pairs = node_of_row[static_pairs(bsr.indptr, bsr.indices,
+                                     np.zeros(bsr.indices.size, dtype=np.uint32))]
+    kw = {"max_domain_dofs": 32, "max_levels": 4, "target_nodes_per_partition": 4} | kw
+    return mh.build_hierarchy(dims, pairs, type_ids=types, **kw)
+
+
+def test_typed_nodes_never_mix_types_or_dimensions():
+    """Coarse nodes are unions of nodes of one type and one dimension: every node's
+    parent has its type and its dimension, and every level keeps the scalar count of
+    each type (the reference's rule, solver/mas/hierarchy.py:105-147)."""
+    dims, types, bsr = _typed_operator()
+    h = _typed_hierarchy(dims, types, bsr)
+    assert h.num_levels >= 3
+    for l in range(1, h.num_levels):
+        fine, coarse = h.levels[l - 1], h.levels[l]
+        m = fine.fine_to_parent
+        assert np.array_equal(coarse.types[m], fine.types)
+        assert np.array_equal(coarse.dims[m], fine.dims)
+        for t in range(3):
+            assert coarse.dims[coarse.types == t].sum() <= fine.dims[fine.types == t].sum()
+        assert coarse.num_nodes < fine.num_nodes
+    assert set(h.levels[0].dims[h.levels[0].types == 1].tolist()) == {9}
+
+
+def test_slot_expansion_gives_the_same_preconditioner():
+    """The hierarchy restated over 3-scalar slots applies the same operator as the typed
+    one (oracle apply on both, to rounding), a 9-DoF node's three slots sit in one domain
+    and aggregate slot by slot, and the device tables built from the slots (emulated)
+    match the typed oracle apply."""
+    dims, types, bsr = _typed_operator()
+    h = _typed_hierarchy(dims, types, bsr)
+    hs = mh.expand_to_slots(h)
+    assert hs.num_levels == h.num_levels
+    assert all(np.all(lv.dims == 3) for lv in hs.levels)
+    Ad = bsr.toarray()
+    w = level_weights(h.num_levels, 2.5).astype(np.float64)
+    invs = omas.domain_inverses(h, omas.level_matrices(h, Ad))
+    invs_s = omas.domain_inverses(hs, omas.level_matrices(hs, Ad))
+    r = np.random.default_rng(5).standard_normal(Ad.shape[0])
+    ref = omas.apply(h, invs, r, w)
+    got = omas.apply(hs, invs_s, r, w)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    # the slots of every 9-DoF level-0 node: one domain, consecutive parent slots
+    lv0 = h.levels[0]
+    slot_dom = np.full(hs.levels[0].num_nodes, -1)
+    for k, d in enumerate(hs.levels[0].domains):
+        slot_dom[d] = k
+    for a in np.flatnonzero(lv0.dims == 9):
+        s0 = lv0.offsets[a] // 3
+        assert len(set(slot_dom[s0:s0 + 3].tolist())) == 1
+        if hs.num_levels > 1:
+            p = hs.levels[0].fine_to_parent[s0:s0 + 3]
+            assert np.array_equal(p, p[0] + np.arange(3))
+    layout = build_layout(hs, bsr.indptr, bsr.indices)
+    emu = emulate_device_apply(layout, bsr.data, r, w)
+    assert np.abs(emu - ref).max() <= 1e-10 * np.abs(ref).max()

@@ -8,10 +8,13 @@ scatters the in-domain blocks into dense 32 x 32 domain matrices, inverts them (
 Gauss-Jordan, FP32 storage) and runs PCG with z = sum_l w_l P_l B_l^-1 R_l r
 (``solver/mas/preconditioner.py:128-154``, ``solver/mas/pcg.py:23-102``).
 
-All variable blocks must be 3 x 3 (free vertices) and domains hold at most 32 scalars.
-The node grouping is a breadth-first replacement of METIS (see ``mas_hierarchy``), so
-domains and iteration counts differ from the reference's; the preconditioner is still
-symmetric positive definite and PCG reaches the same stated residual.
+Level-0 nodes are the targets' instances, typed by target, as in the reference: a
+vertex or an affine body's translation is a 3-DoF node, an affine body's matrix a 9-DoF
+node whose three block rows stay in one domain at every level. Target sizes must be
+multiples of 3 and domains hold at most 32 scalars. The node grouping is a breadth-first
+replacement of METIS (see ``mas_hierarchy``), so domains and iteration counts differ from
+the reference's; the preconditioner is still symmetric positive definite and PCG reaches
+the same stated residual.
 """
 from __future__ import annotations
 
@@ -24,7 +27,7 @@ import numpy as np
 
 from . import _lib, device as dev
 from .hessian import vector
-from .mas_hierarchy import Hierarchy, build_hierarchy
+from .mas_hierarchy import Hierarchy, build_hierarchy, expand_to_slots
 from .refine import RefinementBuffers, refine_solve
 
 
@@ -117,6 +120,20 @@ def static_pairs(row_ptr: np.ndarray, col: np.ndarray, src: np.ndarray,
     keep = ((src.astype(np.int64) >> dynamic_bit) & 1) == 0
     keep &= rows != col
     return np.stack([rows[keep], col.astype(np.int64)[keep]], 1)
+
+
+def target_nodes(h, n: int):
+    """Level-0 node dimensions and type ids of a Hessian's targets
+    (``solver/yaspsMatrixView.py:69-78``: one node per instance of each target, typed by
+    the target's position in the list), and the node of every 3-scalar block row."""
+    dims = np.concatenate([np.full(int(w.numInstances), int(w.size), dtype=np.int64)
+                           for w in h.wrt])
+    types = np.concatenate([np.full(int(w.numInstances), t, dtype=np.int64)
+                            for t, w in enumerate(h.wrt)])
+    if np.any(dims % 3) or int(dims.sum()) != n:
+        raise NotImplementedError("masSolver: the device MAS path needs target sizes that are "
+                                  "multiples of 3")
+    return dims, types, np.repeat(np.arange(dims.size, dtype=np.int64), dims // 3)
 
 
 def duplicate_domains(h: Hierarchy) -> List[np.ndarray]:
@@ -497,8 +514,9 @@ class masSolver:
         return self._device
 
     def _ensure(self, h, n):
-        # every block 3x3, or the split 3x3 sub-blocks of an affine-body layout (an affine
-        # body's 12 DoFs are then four 3-DoF nodes of the hierarchy)
+        # every block 3x3, or the split 3x3 sub-blocks of an affine-body layout; the
+        # hierarchy's nodes are the targets' instances (a 9-DoF affine matrix spans three
+        # block rows), and the device tables take it over 3-scalar slots
         f = h.full if h.full is not None else h.split
         if f is None:
             raise NotImplementedError("masSolver: the device MAS path needs block dimensions "
@@ -513,17 +531,18 @@ class masSolver:
                 split = f is h.split
                 src = dev.to_host(f.src[:f.nnz], "int64" if split else "uint32")
                 t0 = time.perf_counter()
+                dims, types, node_of_row = target_nodes(h, n)
+                pairs = node_of_row[static_pairs(row_ptr, col, src, 43 if split else 30)]
                 self._hierarchy = build_hierarchy(
-                    np.full(f.nbr, 3), static_pairs(row_ptr, col, src, 43 if split else 30),
-                    self.max_domain_dofs,
+                    dims, pairs, self.max_domain_dofs,
                     self.max_levels, self.target_nodes_per_partition, self.global_coarse_domain,
                     minimum_domains_for_next_level=self.minimum_domains_for_next_level,
                     domain_dof_schedule=self.domain_dof_schedule,
-                    target_node_schedule=self.target_node_schedule)
+                    target_node_schedule=self.target_node_schedule, type_ids=types)
                 self._hierarchy_key = skey
                 self.hierarchy_build_count += 1
                 self._hierarchy_seconds = time.perf_counter() - t0
-            layout = build_layout(self._hierarchy, row_ptr, col)
+            layout = build_layout(expand_to_slots(self._hierarchy), row_ptr, col)
             w = level_weights(self._hierarchy.num_levels, self.coarsest_level_weight, self.level_weights)
             layout.weights_base = w.copy()
             self._device = _DeviceMas(layout, w, n, self.coarse_bank, self.duplicate_level_weight)
@@ -606,8 +625,8 @@ class masSolver:
             "layout_rebuilt": rebuilt, "numeric_rebuilt": numeric_rebuilt,
             "fine_level_weight": float(m.weights_host[0]),
             "duplicate_domains": int(lay.duplicate.sum()) if lay.duplicate is not None else 0,
-            "nodes_per_level": [lv.num_nodes for lv in lay.hierarchy.levels],
-            "domains_per_level": [len(lv.domains) for lv in lay.hierarchy.levels],
+            "nodes_per_level": [lv.num_nodes for lv in self._hierarchy.levels],
+            "domains_per_level": [len(lv.domains) for lv in self._hierarchy.levels],
             "inverse_status": inverse_status,
             "numeric_update_seconds": t2 - t1, "pcg_seconds": t3 - t2,
             "solve_seconds": t3 - t0, "matrix_size": n, "tolerance": float(tolerance),
@@ -651,8 +670,8 @@ class masSolver:
             "relative_residual": out["relative_residual"], "outer_history": out["outer_history"],
             "result": 0 if out["converged"] else -4,
             "hierarchy_build_count": self.hierarchy_build_count, "layout_rebuilt": rebuilt,
-            "nodes_per_level": [lv.num_nodes for lv in lay.hierarchy.levels],
-            "domains_per_level": [len(lv.domains) for lv in lay.hierarchy.levels],
+            "nodes_per_level": [lv.num_nodes for lv in self._hierarchy.levels],
+            "domains_per_level": [len(lv.domains) for lv in self._hierarchy.levels],
             "inverse_status": inverse_status, "numeric_update_seconds": t2 - t1,
             "pcg_seconds": t3 - t2, "solve_seconds": t3 - t0, "matrix_size": n,
             "tolerance": float(tolerance),
