@@ -405,8 +405,9 @@ int ysp_pcg_solve_full(uint32_t max_iterations, double threshold, int32_t thresh
 /* Multilevel additive Schwarz (solver/mas): the hierarchy (domains, identity
  * aggregation maps) is built on the host once per static structure
  * (solver/mas/hierarchy.py:150-271); these entry points do the per-solve numeric work
- * and the apply on the device. All nodes are 3-vectors; domains are padded to 32
- * scalars. */
+ * and the apply on the device. Nodes are 3-scalar slots (a larger node of the
+ * hierarchy is several slots sharing its domain); domains are padded to 32 scalars (the
+ * configurations' leaf size) or 48 (the reference's default, solver/mas/solver.py:91). */
 
 /* values (level l, AoS 9 doubles per block) = sum of the level-(l-1) blocks listed in
  * items[offsets[e] .. offsets[e+1]) (identity-aggregation Galerkin product); the previous
@@ -415,20 +416,23 @@ int ysp_mas_coarsen(const double* prev_values, int32_t prev_is_full_bsr, uint32_
                     const uint32_t* offsets, const uint32_t* items, uint32_t nnz,
                     double* values, void* stream);
 
-/* Copy every block e with dest[e] != ~0u into the dense domain matrices (FP64, 32 x 32
- * row-major per domain) at offset dest[e] (numeric_assembly.py:47-61). */
+/* Copy every block e with dest[e] != ~0u into the dense domain matrices (FP64, D x D
+ * row-major per domain, D = domain_size: 32 (or 0) or 48) at offset dest[e]
+ * (numeric_assembly.py:47-61). */
 int ysp_mas_domain_scatter(const double* values, int32_t is_full_bsr, uint32_t nnz,
-                           const uint32_t* dest, double* domains, void* stream);
+                           const uint32_t* dest, double* domains, uint32_t domain_size,
+                           void* stream);
 
 /* Explicit inverse of every domain matrix (identity in the padding), FP64 Gauss-Jordan,
  * stored FP32 column-major (solver/mas/cuda/local_inverse.cu:6-308). With inverses_bf16
  * non-NULL, domains d >= bf16_first_domain are stored BF16 (round to nearest even) at
- * inverses_bf16[(d - bf16_first_domain) * 32 * 32] instead (the BF16 coarse banks of the
- * mixed-precision MAS); the others stay in inverses[d * 32 * 32]. *status (device int)
- * gets bit 0 set when a pivot is below 1e-12 * max|diag|. */
+ * inverses_bf16[(d - bf16_first_domain) * D * D] instead (the BF16 coarse banks of the
+ * mixed-precision MAS); the others stay in inverses[d * D * D] (D = domain_size: 32 (or
+ * 0) or 48). *status (device int) gets bit 0 set when a pivot is below
+ * 1e-12 * max|diag|. */
 int ysp_mas_invert_domains(double* domains, const uint32_t* domain_sizes, uint32_t num_domains,
                            float* inverses, uint16_t* inverses_bf16, uint32_t bf16_first_domain,
-                           int32_t* status, void* stream);
+                           int32_t* status, uint32_t domain_size, void* stream);
 
 #define YSP_MAS_MAX_COARSE_LEVELS 8
 
@@ -442,7 +446,7 @@ typedef struct ysp_mas_args {
   uint32_t level_offsets[YSP_MAS_MAX_COARSE_LEVELS + 1];  /* coarse ids of level l + 1:
                                            [level_offsets[l], level_offsets[l + 1]) */
   const uint32_t* prolong_maps;         /* [num_coarse_levels x n/3] coarse ids */
-  const float* inverses;                /* [num_domains x 32 x 32] */
+  const float* inverses;                /* [num_domains x D x D] */
   const uint32_t* domain_nodes;         /* level-0 nodes or coarse ids */
   const uint32_t* domain_node_offsets;  /* [num_domains + 1] */
   uint32_t num_domains;
@@ -457,6 +461,7 @@ typedef struct ysp_mas_args {
                                            duplicate_level_weight for a domain that repeats
                                            the previous level's (preconditioner_specialized.cu
                                            :415-440), 1 otherwise */
+  uint32_t domain_size;                 /* D: 32 (or 0) or 48 scalars per padded domain */
 } ysp_mas_args;
 
 /* z = sum_l w_l P_l S_l B_l^-1 R_l r (solver/mas/preconditioner.py:128-154; S_l the

@@ -27,15 +27,15 @@ def _oracle_matrix(cs):
     return newton.bsr_matrix(st, asm.values), asm.gradient
 
 
-@pytest.mark.parametrize("levels,coarsest,target", [(4, 2.5, 8), (3, 1.0, 8), (6, 2.5, 3)])
-def test_mas_apply_and_inverses_match_oracle(levels, coarsest, target):
+@pytest.mark.parametrize("levels,coarsest,target,dofs", [(4, 2.5, 8, 32), (3, 1.0, 8, 32),
+                                                         (6, 2.5, 3, 32), (4, 2.5, 16, 48)])
+def test_mas_apply_and_inverses_match_oracle(levels, coarsest, target, dofs):
     """(6 levels, 3 nodes per partition: levels 4 and 5 restrict from level 3 in a launch
-    of their own.)"""
+    of their own; 48 scalars per domain: the 48 x 48 inversion and local solve.)"""
     from yasps_amd import device as dev
-    from yasps_amd.mas import DOMAIN_SCALARS
-    print(f"mas apply n=5 levels={levels} target={target}", flush=True)
+    print(f"mas apply n=5 levels={levels} target={target} domain dofs={dofs}", flush=True)
     cs, s0, hs = _mas_scene(5, seed=1, max_levels=levels, coarsest_level_weight=coarsest,
-                            target_nodes_per_partition=target)
+                            target_nodes_per_partition=target, max_domain_dofs=dofs)
     slv = s0.minimizer.linearSolver.implementation
     slv._ensure(hs, hs.rows)
     m = slv.device_tables
@@ -47,7 +47,10 @@ def test_mas_apply_and_inverses_match_oracle(levels, coarsest, target):
     A, _ = _oracle_matrix(cs)
     mats = omas.level_matrices(h, A.toarray())
     invs = omas.domain_inverses(h, mats)
-    D = DOMAIN_SCALARS
+    D = m.layout.domain_scalars
+    assert D == (48 if dofs == 48 else 32)
+    if dofs == 48:
+        assert max(3 * d.size for lv in h.levels for d in lv.domains) > 32
     inv_dev = dev.to_host(m.inverses).reshape(-1, D, D)      # column-major per domain
     k = 0
     for l, lv in enumerate(h.levels):
@@ -70,15 +73,16 @@ def test_mas_apply_and_inverses_match_oracle(levels, coarsest, target):
         assert np.linalg.norm(got - ref) <= APPLY_TOL * np.linalg.norm(ref)
 
 
-def test_mas_bf16_coarse_bank():
+@pytest.mark.parametrize("dofs", [32, 48])
+def test_mas_bf16_coarse_bank(dofs):
     """Coarse-level inverses stored BF16, level 0 FP32: every coarse inverse within BF16
     rounding (2^-8 of the domain's largest entry) of the oracle's, level 0 as before, the
     apply within 1e-2 norm-wise, and the mixed MAS-PCG still reaches the requested true
-    FP64 residual."""
+    FP64 residual (32- and 48-scalar domains)."""
     from yasps_amd import device as dev
-    from yasps_amd.mas import DOMAIN_SCALARS
-    print("mas bf16 coarse bank n=5", flush=True)
-    cs, s0, hs = _mas_scene(5, seed=1, max_levels=4, coarsest_level_weight=2.5, coarse_bank="bf16")
+    print(f"mas bf16 coarse bank n=5 domain dofs={dofs}", flush=True)
+    cs, s0, hs = _mas_scene(5, seed=1, max_levels=4, coarsest_level_weight=2.5, coarse_bank="bf16",
+                            max_domain_dofs=dofs, target_nodes_per_partition=dofs // 4)
     slv = s0.minimizer.linearSolver.implementation
     slv._ensure(hs, hs.rows)
     m = slv.device_tables
@@ -88,7 +92,8 @@ def test_mas_bf16_coarse_bank():
     h = slv.hierarchy
     A, _ = _oracle_matrix(cs)
     invs = omas.domain_inverses(h, omas.level_matrices(h, A.toarray()))
-    D = DOMAIN_SCALARS
+    D = m.layout.domain_scalars
+    assert D == dofs
     n0 = len(h.levels[0].domains)
     assert m.bf16_first == n0 and m.inverses_bf16 is not None
     f32 = dev.to_host(m.inverses).reshape(-1, D, D)
@@ -113,7 +118,8 @@ def test_mas_bf16_coarse_bank():
     assert np.linalg.norm(dev.to_host(z) - ref) <= 1e-2 * np.linalg.norm(ref)
     cs8 = cube_scene(8, seed=0)
     s8, _ = build_yasps_scene(cs8, solver_name="mas", threshold_mode="relative_residual",
-                              max_levels=4, precision="mixed", coarse_bank="bf16")
+                              max_levels=4, precision="mixed", coarse_bank="bf16",
+                              max_domain_dofs=dofs)
     x = dev.to_host(s8.minimizeEnergy(tolerance=1e-8)[0]).copy()
     assert s8.minimizer.linearSolver.statistics["converged"]
     A8, b8 = _oracle_matrix(cs8)

@@ -114,7 +114,7 @@ def test_mas_operator_is_spd_and_inverts_domains(cube):
 
 def emulate_device_apply(layout, block_values, r, weights, duplicate_weight=1.0):
     """FP64 restatement of the device kernels over the layout tables."""
-    D = DOMAIN_SCALARS
+    D = layout.domain_scalars
     b0 = layout.blocks[0]
     if b0.coarsen_offsets is not None:          # repeated (row, col) entries summed first
         block_values = np.stack([block_values[b0.coarsen_items[b0.coarsen_offsets[e]:
@@ -222,7 +222,7 @@ def test_repeated_entries_are_summed(cube):
 def test_layout_destinations_stay_inside_their_domain(cube):
     _, A, _ = cube
     layout = build_layout(_hierarchy(A), A.indptr, A.indices)
-    D = DOMAIN_SCALARS
+    D = layout.domain_scalars
     for b in layout.blocks:
         d = b.dest[b.dest != NO_DEST].astype(np.int64)
         dom = d // (D * D)
@@ -409,3 +409,27 @@ def test_slot_expansion_gives_the_same_preconditioner():
     layout = build_layout(hs, bsr.indptr, bsr.indices)
     emu = emulate_device_apply(layout, bsr.data, r, w)
     assert np.abs(emu - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+def test_48_scalar_domains(cube):
+    """Domains of up to 48 scalars (the reference's default size) padded to 48: the
+    emulated device tables apply the oracle's operator, and a 32-padded layout refuses
+    such a hierarchy."""
+    from yasps_amd.mas import padded_domain_size
+    _, A, _ = cube
+    h = _hierarchy(A, max_domain_dofs=48, target_nodes_per_partition=16)
+    widest = max(3 * d.size for lv in h.levels for d in lv.domains)
+    assert 32 < widest <= 48 and padded_domain_size(widest) == 48
+    assert padded_domain_size(30) == 32
+    with pytest.raises(ValueError):
+        padded_domain_size(51)
+    with pytest.raises(ValueError):
+        build_layout(h, A.indptr, A.indices, 32)
+    layout = build_layout(h, A.indptr, A.indices, 48)
+    assert layout.domain_scalars == 48
+    w = level_weights(h.num_levels, 2.5).astype(np.float64)
+    invs = omas.domain_inverses(h, omas.level_matrices(h, A.toarray()))
+    r = np.random.default_rng(8).standard_normal(A.shape[0])
+    ref = omas.apply(h, invs, r, w)
+    got = emulate_device_apply(layout, A.data, r, w)
+    assert np.abs(got - ref).max() <= 1e-10 * np.abs(ref).max()

@@ -88,7 +88,7 @@ class MasArgs(C.Structure):
                 ("weights", C.c_void_p), ("coarse_residual", C.c_void_p),
                 ("fine_correction", C.c_void_p), ("coarse_correction", C.c_void_p),
                 ("inverses_bf16", C.c_void_p), ("bf16_first_domain", C.c_uint32),
-                ("domain_scale", C.c_void_p)]
+                ("domain_scale", C.c_void_p), ("domain_size", C.c_uint32)]
 
 
 P = C.c_void_p
@@ -164,8 +164,8 @@ SIGNATURES = {
     "ysp_ccd_active_pairs": (C.c_int, [P, P, P, P, U32, D, P, P, P, P, U32, C.POINTER(U32), P]),
     "ysp_ccd_step_bound": (C.c_int, [P, P, P, P, P, U32, D, P, C.POINTER(D), P]),
     "ysp_mas_coarsen": (C.c_int, [P, I32, U32, P, P, U32, P, P]),
-    "ysp_mas_domain_scatter": (C.c_int, [P, I32, U32, P, P, P]),
-    "ysp_mas_invert_domains": (C.c_int, [P, P, U32, P, P, U32, P, P]),
+    "ysp_mas_domain_scatter": (C.c_int, [P, I32, U32, P, P, U32, P]),
+    "ysp_mas_invert_domains": (C.c_int, [P, P, U32, P, P, U32, P, U32, P]),
     "ysp_mas_apply": (C.c_int, [C.POINTER(MasArgs), P, U32, P, P]),
     "ysp_pcg3_mas_solve": (C.c_int, [U32, D, I32, P, P, P, I32, U32, C.POINTER(MasArgs), P, U32,
                                      P, P, I32, P, P, C.POINTER(PcgStats), P]),

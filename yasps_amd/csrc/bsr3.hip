@@ -1523,16 +1523,17 @@ int batch_run(ysp_pcg3_system* systems, uint32_t count, uint32_t passes, hipStre
     mas_batch_geometry(it, static_cast<uint32_t>(items.size()), mt);
     items.push_back(it);
   }
-  const MasBatchWork* dtab[4] = {nullptr, nullptr, nullptr, nullptr};
-  uint32_t ntab[4] = {0, 0, 0, 0};
+  const MasBatchWork* dtab[kMasBatchTables] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  uint32_t ntab[kMasBatchTables] = {0, 0, 0, 0, 0};
   MasBatchItem* ditems = nullptr;
   if (!items.empty()) {
     ditems = static_cast<MasBatchItem*>(ysp::scratch(70, sizeof(MasBatchItem) * items.size()));
     if (!ditems) return YSP_ERR_HIP;
     YSP_CHECK(hipMemcpyAsync(ditems, items.data(), sizeof(MasBatchItem) * items.size(),
                              hipMemcpyHostToDevice, s));
-    const std::vector<MasBatchWork>* src[4] = {&mt.restrict_, &mt.deep, &mt.solve, &mt.prolong};
-    for (int t = 0; t < 4; ++t) {
+    const std::vector<MasBatchWork>* src[kMasBatchTables] = {&mt.restrict_, &mt.deep, &mt.solve,
+                                                             &mt.prolong, &mt.solve48};
+    for (int t = 0; t < kMasBatchTables; ++t) {
       ntab[t] = static_cast<uint32_t>(src[t]->size());
       if (!ntab[t]) continue;
       MasBatchWork* d = static_cast<MasBatchWork*>(ysp::scratch(71 + t, sizeof(MasBatchWork) * ntab[t]));

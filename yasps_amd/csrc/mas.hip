@@ -6,16 +6,16 @@
 //     mirrored full 3x3 block-CSR of the solve loop (component-major values);
 //   * domain matrices: every level block whose two nodes share a domain is copied into
 //     that domain's dense, zero-padded D x D FP64 matrix (one contribution per entry,
-//     no reduction);
-//   * inverses: in-place Gauss-Jordan in registers, two domains per wavefront, FP64, stored
-//     FP32 column-major (the reference's FP64-factor / FP32-bank split,
+//     no reduction; D = 32 or 48 scalars);
+//   * inverses: in-place Gauss-Jordan in registers, two domains per wavefront (one for
+//     D = 48), FP64, stored FP32 column-major (the reference's FP64-factor / FP32-bank split,
 //     solver/mas/cuda/local_inverse.cu:244-308). A pivot below 1e-12 * max|diag| marks the
 //     domain as failed.
 // Per application z = sum_l w_l P_l B_l^-1 R_l r:
 //   restrict (a node of levels 1 to 3 sums the residual of its level-0 descendants, a
 //   node of a deeper level the level-3 residuals of its level-3 descendants: a second
-//   launch), one 32-lane group per domain multiplies its inverse with the
-//   local residual, prolong adds the weighted corrections of every level and accumulates
+//   launch), one 32-lane group per domain (64 lanes for D = 48) multiplies its inverse
+//   with the local residual, prolong adds the weighted corrections of every level and accumulates
 //   r.z.
 #include "mas.hpp"
 
@@ -23,7 +23,15 @@ namespace ysp {
 namespace {
 
 constexpr unsigned kB = 256;
-constexpr int kD = 32;                 // padded domain size (scalars)
+// Padded domain sizes (scalars): 32, the configurations' leaf size, and 48, the
+// reference's default (solver/mas/solver.py:91); lanes per domain in the local solve and
+// the inversion, and the domains one wavefront holds.
+template <int D>
+struct DomainShape {
+  static_assert(D == 32 || D == 48, "domains of 32 or 48 scalars");
+  static constexpr int lanes = D <= 32 ? 32 : 64;
+  static constexpr int per_wave = 64 / lanes;
+};
 
 __host__ __device__ __forceinline__ uint32_t pad64(uint32_t n) { return (n + 63u) & ~63u; }
 
@@ -48,7 +56,7 @@ __global__ void __launch_bounds__(kB)
 }
 
 // copy the in-domain blocks of one level into the dense domain matrices
-template <bool kSoA>
+template <bool kSoA, int D>
 __global__ void __launch_bounds__(kB)
     domain_scatter_kernel(const double* __restrict__ vals, uint32_t nnz,
                           const uint32_t* __restrict__ dest, double* __restrict__ dom) {
@@ -62,21 +70,22 @@ __global__ void __launch_bounds__(kB)
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j)
-      out[i * kD + j] = kSoA ? vals[(3 * i + j) * stride + e] : vals[9 * static_cast<size_t>(e) + 3 * i + j];
+      out[i * D + j] = kSoA ? vals[(3 * i + j) * stride + e] : vals[9 * static_cast<size_t>(e) + 3 * i + j];
 }
 
 __global__ void pad_identity_kernel(double* __restrict__ dom, const uint32_t* __restrict__ sizes,
-                                    uint32_t ndom) {
+                                    uint32_t ndom, uint32_t D) {
   const uint32_t d = blockIdx.x;
   if (d >= ndom) return;
   const uint32_t P = sizes[d];
-  for (uint32_t i = P + threadIdx.x; i < kD; i += blockDim.x)
-    dom[static_cast<size_t>(d) * kD * kD + i * kD + i] = 1.0;
+  for (uint32_t i = P + threadIdx.x; i < D; i += blockDim.x)
+    dom[static_cast<size_t>(d) * D * D + i * D + i] = 1.0;
 }
 
 // In-place Gauss-Jordan inversion (no pivoting: the domain matrices are principal
-// submatrices of an SPD operator), two domains per wavefront: in each 32-lane half, lane j
-// holds column j of its domain in registers. At step k the half's lane k publishes its
+// submatrices of an SPD operator), two 32-scalar domains per wavefront (one 48-scalar
+// domain, lanes 48-63 idle): in each domain's lane group, lane j holds column j of its
+// domain in registers. At step k the half's lane k publishes its
 // column c = A[:, k] through LDS (broadcast reads) and, with p = c_k,
 //   lane k:      A_kk <- 1/p,  A_ik <- -c_i / p      (i != k)
 //   lane j != k: A_kj <- A_kj / p,  A_ij <- A_ij - c_i A_kj   (i != k)
@@ -90,35 +99,38 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
+template <int D>
 __global__ void __launch_bounds__(64 * kInvWaves)
     invert_domains_kernel(const double* __restrict__ dom, float* __restrict__ inv,
                           uint16_t* __restrict__ inv_bf16, uint32_t bf16_first, uint32_t ndom,
                           int* __restrict__ status) {
-  static_assert(2 * kD == 64, "two domains of D columns per wavefront");
-  __shared__ double colk[kInvWaves][2][kD];
-  __shared__ double tr[kInvWaves][2][kD][kD + 1];
-  const int w = threadIdx.x / 64, h = (threadIdx.x % 64) / kD, j = threadIdx.x % kD;
-  const uint32_t d = (blockIdx.x * kInvWaves + w) * 2 + h;
-  const bool live = d < ndom;                  // a half past the end idles along
-  const double* M = dom + static_cast<size_t>(live ? d : 0) * kD * kD;
-  double x[kD];
+  constexpr int G = DomainShape<D>::lanes, H = DomainShape<D>::per_wave;
+  __shared__ double colk[kInvWaves][H][D];
+  __shared__ double tr[kInvWaves][H][D][D + 1];
+  const int w = threadIdx.x / 64, h = (threadIdx.x % 64) / G, j = threadIdx.x % G;
+  const uint32_t d = (blockIdx.x * kInvWaves + w) * H + h;
+  const bool live = d < ndom;                  // a group past the end idles along
+  const bool col = j < D;                      // lanes past the domain's columns idle along
+  const double* M = dom + static_cast<size_t>(live ? d : 0) * D * D;
+  double x[D];
   double scale = 0.0;
-  if (live) {                                  // column j (the matrix is symmetric)
+  if (live && col) {                           // column j (the matrix is symmetric)
 #pragma unroll
-    for (int i = 0; i < kD; ++i) x[i] = M[i * kD + j];
-    scale = fabs(M[j * kD + j]);
+    for (int i = 0; i < D; ++i) x[i] = M[i * D + j];
+    scale = fabs(M[j * D + j]);
   } else {
 #pragma unroll
-    for (int i = 0; i < kD; ++i) x[i] = i == j ? 1.0 : 0.0;
+    for (int i = 0; i < D; ++i) x[i] = i == j ? 1.0 : 0.0;
   }
 #pragma unroll
-  for (int o = kD / 2; o > 0; o >>= 1) scale = fmax(scale, __shfl_xor(scale, o, kD));
+  for (int o = G / 2; o > 0; o >>= 1) scale = fmax(scale, __shfl_xor(scale, o, G));
   bool bad = false;
-#pragma unroll
-  for (int k = 0; k < kD; ++k) {
+  // fully unrolled (x[k] must name a register, never a private array in scratch)
+#pragma clang loop unroll(full)
+  for (int k = 0; k < D; ++k) {
     if (j == k) {
 #pragma unroll
-      for (int i = 0; i < kD; ++i) colk[w][h][i] = x[i];
+      for (int i = 0; i < D; ++i) colk[w][h][i] = x[i];
     }
     wave_lds_sync();
     const double p = colk[w][h][k];
@@ -126,31 +138,33 @@ __global__ void __launch_bounds__(64 * kInvWaves)
     const double ip = 1.0 / p;
     if (j == k) {
 #pragma unroll
-      for (int i = 0; i < kD; ++i) x[i] = i == k ? ip : -x[i] * ip;
-    } else {
+      for (int i = 0; i < D; ++i) x[i] = i == k ? ip : -x[i] * ip;
+    } else if (col) {
       const double xk = x[k] * ip;
 #pragma unroll
-      for (int i = 0; i < kD; ++i)
+      for (int i = 0; i < D; ++i)
         if (i != k) x[i] = fma(-colk[w][h][i], xk, x[i]);
       x[k] = xk;
     }
     wave_lds_sync();
   }
   // stored FP32 column-major, symmetrised: out[c * D + r] = (B[r][c] + B[c][r]) / 2
+  if (col) {
 #pragma unroll
-  for (int i = 0; i < kD; ++i) tr[w][h][i][j] = x[i];
+    for (int i = 0; i < D; ++i) tr[w][h][i][j] = x[i];
+  }
   wave_lds_sync();
-  if (live) {
+  if (live && col) {
     if (inv_bf16 && d >= bf16_first) {
-      uint16_t* out = inv_bf16 + static_cast<size_t>(d - bf16_first) * kD * kD;
+      uint16_t* out = inv_bf16 + static_cast<size_t>(d - bf16_first) * D * D;
 #pragma unroll
-      for (int c = 0; c < kD; ++c)
-        out[c * kD + j] = to_bf16(static_cast<float>(0.5 * (tr[w][h][j][c] + tr[w][h][c][j])));
+      for (int c = 0; c < D; ++c)
+        out[c * D + j] = to_bf16(static_cast<float>(0.5 * (tr[w][h][j][c] + tr[w][h][c][j])));
     } else {
-      float* out = inv + static_cast<size_t>(d) * kD * kD;
+      float* out = inv + static_cast<size_t>(d) * D * D;
 #pragma unroll
-      for (int c = 0; c < kD; ++c)
-        out[c * kD + j] = static_cast<float>(0.5 * (tr[w][h][j][c] + tr[w][h][c][j]));
+      for (int c = 0; c < D; ++c)
+        out[c * D + j] = static_cast<float>(0.5 * (tr[w][h][j][c] + tr[w][h][c][j]));
     }
     if (bad && j == 0) atomicOr(status, 1);
   }
@@ -259,11 +273,12 @@ __global__ void __launch_bounds__(kB)
   restrict_deep_body(offsets, items, n3, nc, rc, blockIdx.x);
 }
 
-// One 32-lane group per domain: c_loc = B^-1 r_loc. Lane i reads row i of the inverse,
-// which is its column i (the stored inverse is exactly symmetric): eight 16-byte loads
-// from the FP32 bank, or four from the BF16 bank; the domain's residual reaches every
-// lane through LDS (the 32 lanes sit in one wavefront).
-template <bool kBF16>
+// One lane group per domain (32 lanes, 64 for D = 48 with lanes 48-63 idle):
+// c_loc = B^-1 r_loc. Lane i reads row i of the inverse, which is its column i (the stored
+// inverse is exactly symmetric): D/4 16-byte loads from the FP32 bank, or D/8 from the
+// BF16 bank; the domain's residual reaches every lane through LDS (the group sits in one
+// wavefront).
+template <bool kBF16, int D>
 __device__ __forceinline__ void local_solve_domain(
     const void* __restrict__ bank, const uint32_t* __restrict__ dom_nodes,
     const uint32_t* __restrict__ dom_node_off, uint32_t d, size_t row,
@@ -279,14 +294,15 @@ __device__ __forceinline__ void local_solve_domain(
     const uint32_t s = 3 * node + lane % 3;
     rl = static_cast<float>(d < level0_domains ? r0[s] : rc[s]);
   }
-  rs[lane] = rl;
+  if (lane < D) rs[lane] = rl;
   wave_lds_sync();
+  if (lane >= D) return;                       // no row of the bank
   const float4* R = reinterpret_cast<const float4*>(rs);
   float acc = 0.0f;
   if (kBF16) {
-    const uint4* Bi = reinterpret_cast<const uint4*>(static_cast<const uint16_t*>(bank) + row * kD);
+    const uint4* Bi = reinterpret_cast<const uint4*>(static_cast<const uint16_t*>(bank) + row * D);
 #pragma unroll
-    for (int q = 0; q < kD / 8; ++q) {
+    for (int q = 0; q < D / 8; ++q) {
       const uint4 b = Bi[q];
       const float4 ra = R[2 * q], rb = R[2 * q + 1];
       acc = fmaf(__uint_as_float(b.x << 16), ra.x, acc);
@@ -299,9 +315,9 @@ __device__ __forceinline__ void local_solve_domain(
       acc = fmaf(__uint_as_float(b.w & 0xffff0000u), rb.w, acc);
     }
   } else {
-    const float4* Bi = reinterpret_cast<const float4*>(static_cast<const float*>(bank) + row * kD);
+    const float4* Bi = reinterpret_cast<const float4*>(static_cast<const float*>(bank) + row * D);
 #pragma unroll
-    for (int q = 0; q < kD / 4; ++q) {
+    for (int q = 0; q < D / 4; ++q) {
       const float4 b = Bi[q], rq = R[q];
       acc = fmaf(b.x, rq.x, acc);
       acc = fmaf(b.y, rq.y, acc);
@@ -319,30 +335,40 @@ __device__ __forceinline__ void local_solve_domain(
 
 // Workgroups [0, nb32) hold the FP32-bank domains [0, bf16_first), the rest the BF16-bank
 // domains [bf16_first, ndom): the bank type is uniform per workgroup.
+template <int D>
 __device__ __forceinline__ void local_solve_body(
     const float* __restrict__ inv, const uint16_t* __restrict__ inv_bf16, uint32_t bf16_first,
     const uint32_t* __restrict__ dom_nodes, const uint32_t* __restrict__ dom_node_off,
     uint32_t ndom, const double* __restrict__ r0, const double* __restrict__ rc,
     uint32_t level0_domains, const float* __restrict__ dscale, float* __restrict__ c0,
     float* __restrict__ cc, uint32_t bid) {
-  constexpr uint32_t kPer = kB / kD;
-  __shared__ __align__(16) float rs[kPer][kD];
-  const int g = threadIdx.x / kD, lane = threadIdx.x % kD;
+  constexpr int G = DomainShape<D>::lanes;
+  constexpr uint32_t kPer = kB / G;
+  __shared__ __align__(16) float rs[kPer][D];
+  const int g = threadIdx.x / G, lane = threadIdx.x % G;
   const uint32_t nb32 = (bf16_first + kPer - 1) / kPer;
   if (bid < nb32) {
     const uint32_t d = bid * kPer + g;
     if (d >= bf16_first) return;
-    local_solve_domain<false>(inv, dom_nodes, dom_node_off, d, static_cast<size_t>(d) * kD + lane,
-                              r0, rc, level0_domains, dscale, c0, cc, rs[g], lane);
+    local_solve_domain<false, D>(inv, dom_nodes, dom_node_off, d,
+                                 static_cast<size_t>(d) * D + lane, r0, rc, level0_domains,
+                                 dscale, c0, cc, rs[g], lane);
   } else {
     const uint32_t d = bf16_first + (bid - nb32) * kPer + g;
     if (d >= ndom) return;
-    local_solve_domain<true>(inv_bf16, dom_nodes, dom_node_off, d,
-                             static_cast<size_t>(d - bf16_first) * kD + lane, r0, rc,
-                             level0_domains, dscale, c0, cc, rs[g], lane);
+    local_solve_domain<true, D>(inv_bf16, dom_nodes, dom_node_off, d,
+                                static_cast<size_t>(d - bf16_first) * D + lane, r0, rc,
+                                level0_domains, dscale, c0, cc, rs[g], lane);
   }
 }
 
+// Workgroups of the local solve for the FP32 bank domains [0, first) and the BF16 ones.
+inline uint32_t local_solve_blocks(int D, uint32_t first, uint32_t ndom) {
+  const uint32_t per = kB / (D <= 32 ? 32 : 64);
+  return grid_for(first, per) + grid_for(ndom - first, per);
+}
+
+template <int D>
 __global__ void __launch_bounds__(kB)
     local_solve_kernel(const float* __restrict__ inv, const uint16_t* __restrict__ inv_bf16,
                        uint32_t bf16_first, const uint32_t* __restrict__ dom_nodes,
@@ -352,8 +378,8 @@ __global__ void __launch_bounds__(kB)
                        float* __restrict__ c0, float* __restrict__ cc,
                        const double* __restrict__ stop) {
   if (mas_stopped(stop)) return;
-  local_solve_body(inv, inv_bf16, bf16_first, dom_nodes, dom_node_off, ndom, r0, rc,
-                   level0_domains, dscale, c0, cc, blockIdx.x);
+  local_solve_body<D>(inv, inv_bf16, bf16_first, dom_nodes, dom_node_off, ndom, r0, rc,
+                      level0_domains, dscale, c0, cc, blockIdx.x);
 }
 
 // z = w0 c0 + sum_l w_l c_l[map_l(node)], and r.z
@@ -429,13 +455,14 @@ __global__ void __launch_bounds__(kB)
                      it.m.coarse_residual, w.blk);
 }
 
+template <int D>
 __global__ void __launch_bounds__(kB)
     batch_local_solve_kernel(const MasBatchItem* __restrict__ I,
                              const MasBatchWork* __restrict__ W, int pass) {
   const MasBatchWork w = W[blockIdx.x];
   const MasBatchItem& it = I[w.item];
   if (batch_idle(it, pass)) return;
-  local_solve_body(it.m.inverses, it.m.inverses_bf16, it.first, it.m.domain_nodes,
+  local_solve_body<D>(it.m.inverses, it.m.inverses_bf16, it.first, it.m.domain_nodes,
                    it.m.domain_node_offsets, it.m.num_domains, it.r, it.m.coarse_residual,
                    it.m.level0_domains, it.m.domain_scale, it.m.fine_correction,
                    it.m.coarse_correction, w.blk);
@@ -473,16 +500,33 @@ extern "C" int ysp_mas_coarsen(const double* prev_values, int32_t prev_is_full_b
   return YSP_OK;
 }
 
+namespace {
+// 0 stands for 32; anything but 32 or 48 is refused.
+int domain_size_of(uint32_t domain_size) {
+  const uint32_t D = domain_size ? domain_size : 32;
+  return D == 32 || D == 48 ? static_cast<int>(D) : 0;
+}
+}  // namespace
+
 extern "C" int ysp_mas_domain_scatter(const double* values, int32_t is_full_bsr, uint32_t nnz,
-                                      const uint32_t* dest, double* domains, void* stream) {
+                                      const uint32_t* dest, double* domains, uint32_t domain_size,
+                                      void* stream) {
+  const int D = domain_size_of(domain_size);
+  if (!D) {
+    ysp_set_error(__FILE__, __LINE__, "ysp_mas_domain_scatter: domain_size must be 32 or 48");
+    return YSP_ERR_ARG;
+  }
   if (nnz == 0) return YSP_OK;
   const dim3 g(grid_for(nnz, kB)), b(kB);
-  if (is_full_bsr)
-    hipLaunchKernelGGL(domain_scatter_kernel<true>, g, b, 0, as_stream(stream), values, nnz, dest,
-                       domains);
+  hipStream_t s = as_stream(stream);
+  if (is_full_bsr && D == 32)
+    hipLaunchKernelGGL((domain_scatter_kernel<true, 32>), g, b, 0, s, values, nnz, dest, domains);
+  else if (is_full_bsr)
+    hipLaunchKernelGGL((domain_scatter_kernel<true, 48>), g, b, 0, s, values, nnz, dest, domains);
+  else if (D == 32)
+    hipLaunchKernelGGL((domain_scatter_kernel<false, 32>), g, b, 0, s, values, nnz, dest, domains);
   else
-    hipLaunchKernelGGL(domain_scatter_kernel<false>, g, b, 0, as_stream(stream), values, nnz, dest,
-                       domains);
+    hipLaunchKernelGGL((domain_scatter_kernel<false, 48>), g, b, 0, s, values, nnz, dest, domains);
   YSP_CHECK_LAUNCH();
   return YSP_OK;
 }
@@ -490,20 +534,29 @@ extern "C" int ysp_mas_domain_scatter(const double* values, int32_t is_full_bsr,
 extern "C" int ysp_mas_invert_domains(double* domains, const uint32_t* domain_sizes,
                                       uint32_t num_domains, float* inverses,
                                       uint16_t* inverses_bf16, uint32_t bf16_first_domain,
-                                      int32_t* status, void* stream) {
-  if (num_domains == 0) return YSP_OK;
+                                      int32_t* status, uint32_t domain_size, void* stream) {
+  const int D = domain_size_of(domain_size);
   const uint32_t fp32_domains = inverses_bf16 ? bf16_first_domain : num_domains;
-  if (!domains || !domain_sizes || !status || (inverses_bf16 && bf16_first_domain > num_domains) ||
-      (fp32_domains && !inverses)) {
+  if (!D || (num_domains && (!domains || !domain_sizes || !status ||
+                             (inverses_bf16 && bf16_first_domain > num_domains) ||
+                             (fp32_domains && !inverses)))) {
     ysp_set_error(__FILE__, __LINE__, "ysp_mas_invert_domains: bad arguments");
     return YSP_ERR_ARG;
   }
+  if (num_domains == 0) return YSP_OK;
   hipStream_t s = as_stream(stream);
   hipLaunchKernelGGL(pad_identity_kernel, dim3(num_domains), dim3(64), 0, s, domains, domain_sizes,
-                     num_domains);
-  hipLaunchKernelGGL(invert_domains_kernel, dim3(grid_for(num_domains, 2 * kInvWaves)),
-                     dim3(64 * kInvWaves), 0, s, domains, inverses, inverses_bf16,
-                     bf16_first_domain, num_domains, status);
+                     num_domains, static_cast<uint32_t>(D));
+  if (D == 32)
+    hipLaunchKernelGGL(invert_domains_kernel<32>,
+                       dim3(grid_for(num_domains, DomainShape<32>::per_wave * kInvWaves)),
+                       dim3(64 * kInvWaves), 0, s, domains, inverses, inverses_bf16,
+                       bf16_first_domain, num_domains, status);
+  else
+    hipLaunchKernelGGL(invert_domains_kernel<48>,
+                       dim3(grid_for(num_domains, DomainShape<48>::per_wave * kInvWaves)),
+                       dim3(64 * kInvWaves), 0, s, domains, inverses, inverses_bf16,
+                       bf16_first_domain, num_domains, status);
   YSP_CHECK_LAUNCH();
   return YSP_OK;
 }
@@ -528,12 +581,18 @@ int launch_mas_apply(const MasApply& m, const double* r, uint32_t n, double* z,
   }
   if (m.num_domains) {
     const uint32_t first = m.inverses_bf16 ? m.bf16_first_domain : m.num_domains;
-    hipLaunchKernelGGL(local_solve_kernel,
-                       dim3(grid_for(first, kB / kD) + grid_for(m.num_domains - first, kB / kD)),
-                       dim3(kB), 0, s, m.inverses, m.inverses_bf16, first, m.domain_nodes,
-                       m.domain_node_offsets, m.num_domains, r, m.coarse_residual,
-                       m.level0_domains, m.domain_scale, m.fine_correction, m.coarse_correction,
-                       stop);
+    const int D = m.domain_size == 48 ? 48 : 32;
+    const dim3 grid(local_solve_blocks(D, first, m.num_domains));
+    if (D == 32)
+      hipLaunchKernelGGL(local_solve_kernel<32>, grid, dim3(kB), 0, s, m.inverses,
+                         m.inverses_bf16, first, m.domain_nodes, m.domain_node_offsets,
+                         m.num_domains, r, m.coarse_residual, m.level0_domains, m.domain_scale,
+                         m.fine_correction, m.coarse_correction, stop);
+    else
+      hipLaunchKernelGGL(local_solve_kernel<48>, grid, dim3(kB), 0, s, m.inverses,
+                         m.inverses_bf16, first, m.domain_nodes, m.domain_node_offsets,
+                         m.num_domains, r, m.coarse_residual, m.level0_domains, m.domain_scale,
+                         m.fine_correction, m.coarse_correction, stop);
   }
   hipLaunchKernelGGL(prolong_kernel, dim3(grid_for(nv, kB)), dim3(kB), 0, s, m.fine_correction,
                      m.coarse_correction, m.prolong_maps, nv, m.num_coarse_levels, m.weights, r, z,
@@ -560,14 +619,16 @@ void mas_batch_geometry(MasBatchItem& it, uint32_t item, MasBatchTables& t) {
   }
   it.first = m.inverses_bf16 ? m.bf16_first_domain : m.num_domains;
   if (m.num_domains)
-    for (uint32_t b = 0, nb = grid_for(it.first, kB / kD) + grid_for(m.num_domains - it.first, kB / kD);
+    for (uint32_t b = 0, nb = local_solve_blocks(m.domain_size == 48 ? 48 : 32, it.first,
+                                                 m.num_domains);
          b < nb; ++b)
-      t.solve.push_back({item, b});
+      (m.domain_size == 48 ? t.solve48 : t.solve).push_back({item, b});
   for (uint32_t b = 0, nb = grid_for(it.nv, kB); b < nb; ++b) t.prolong.push_back({item, b});
 }
 
-int launch_mas_apply_batch(const MasBatchItem* items, const MasBatchWork* const tables[4],
-                           const uint32_t counts[4], int pass, hipStream_t s) {
+int launch_mas_apply_batch(const MasBatchItem* items,
+                           const MasBatchWork* const tables[kMasBatchTables],
+                           const uint32_t counts[kMasBatchTables], int pass, hipStream_t s) {
   if (counts[0])
     hipLaunchKernelGGL(batch_restrict_kernel, dim3(counts[0]), dim3(kB), 0, s, items, tables[0],
                        pass);
@@ -575,8 +636,11 @@ int launch_mas_apply_batch(const MasBatchItem* items, const MasBatchWork* const 
     hipLaunchKernelGGL(batch_restrict_deep_kernel, dim3(counts[1]), dim3(kB), 0, s, items,
                        tables[1], pass);
   if (counts[2])
-    hipLaunchKernelGGL(batch_local_solve_kernel, dim3(counts[2]), dim3(kB), 0, s, items,
+    hipLaunchKernelGGL(batch_local_solve_kernel<32>, dim3(counts[2]), dim3(kB), 0, s, items,
                        tables[2], pass);
+  if (counts[4])
+    hipLaunchKernelGGL(batch_local_solve_kernel<48>, dim3(counts[4]), dim3(kB), 0, s, items,
+                       tables[4], pass);
   if (counts[3])
     hipLaunchKernelGGL(batch_prolong_kernel, dim3(counts[3]), dim3(kB), 0, s, items, tables[3],
                        pass);
@@ -590,7 +654,8 @@ int check_mas_args(const MasApply& m, uint32_t n) {
       (m.num_domains && (!m.domain_nodes || !m.domain_node_offsets)) ||
       (m.inverses_bf16 && m.bf16_first_domain > m.num_domains) ||
       ((m.inverses_bf16 ? m.bf16_first_domain : m.num_domains) && !m.inverses) ||
-      m.level0_domains > m.num_domains ||
+      m.level0_domains > m.num_domains || (m.domain_size && m.domain_size != 32 &&
+                                           m.domain_size != 48) ||
       (m.num_coarse_levels &&
        (!m.prolong_maps || !m.restrict_offsets || !m.restrict_items || !m.coarse_residual ||
         !m.coarse_correction || m.level_offsets[0] != 0 ||

@@ -29,10 +29,12 @@ struct MasApply {
   const uint16_t* inverses_bf16;      // nullptr, or BF16 inverses of domains >= bf16_first
   uint32_t bf16_first_domain;
   const float* domain_scale;          // nullptr, or a factor per domain on its correction
+  uint32_t domain_size;               // padded domain size: 32 (or 0) or 48 scalars
 };
 static_assert(sizeof(MasApply) == sizeof(ysp_mas_args) &&
                   offsetof(MasApply, bf16_first_domain) == offsetof(ysp_mas_args, bf16_first_domain) &&
-                  offsetof(MasApply, domain_scale) == offsetof(ysp_mas_args, domain_scale),
+                  offsetof(MasApply, domain_scale) == offsetof(ysp_mas_args, domain_scale) &&
+                  offsetof(MasApply, domain_size) == offsetof(ysp_mas_args, domain_size),
               "MasApply mirrors ysp_mas_args");
 
 // BF16 (round to nearest even) and back.
@@ -67,16 +69,19 @@ struct MasBatchWork {
   uint32_t blk;    // workgroup index within that system's launch
 };
 struct MasBatchTables {
-  std::vector<MasBatchWork> restrict_, deep, solve, prolong;
+  std::vector<MasBatchWork> restrict_, deep, solve, prolong, solve48;  // solve48: D = 48
 };
 
 // Fills the geometry of item `item` and appends its workgroups to the four work tables.
 void mas_batch_geometry(MasBatchItem& it, uint32_t item, MasBatchTables& t);
 
-// The four launches of one batched apply (tables: restrict, deep restrict, local solve,
-// prolong, device arrays of counts[i] entries; empty tables launch nothing).
-int launch_mas_apply_batch(const MasBatchItem* items, const MasBatchWork* const tables[4],
-                           const uint32_t counts[4], int pass, hipStream_t s);
+// The launches of one batched apply (tables: restrict, deep restrict, local solve of the
+// 32-scalar domains, prolong, local solve of the 48-scalar domains; device arrays of
+// counts[i] entries; empty tables launch nothing).
+constexpr int kMasBatchTables = 5;
+int launch_mas_apply_batch(const MasBatchItem* items,
+                           const MasBatchWork* const tables[kMasBatchTables],
+                           const uint32_t counts[kMasBatchTables], int pass, hipStream_t s);
 
 // YSP_OK when the argument block is consistent (sizes, level ranges, pointers).
 int check_mas_args(const MasApply& m, uint32_t n);

@@ -4,14 +4,15 @@
 sparsity structure (``solver/mas/solver.py:266-295``) and numeric work on every solve
 (``solver/mas/numeric_assembly.py``, ``local_inverse.py``): on the device, each solve
 coarsens the current Hessian level by level (identity-aggregation Galerkin sums),
-scatters the in-domain blocks into dense 32 x 32 domain matrices, inverts them (FP64
+scatters the in-domain blocks into dense 32 x 32 (or 48 x 48) domain matrices, inverts them (FP64
 Gauss-Jordan, FP32 storage) and runs PCG with z = sum_l w_l P_l B_l^-1 R_l r
 (``solver/mas/preconditioner.py:128-154``, ``solver/mas/pcg.py:23-102``).
 
 Level-0 nodes are the targets' instances, typed by target, as in the reference: a
 vertex or an affine body's translation is a 3-DoF node, an affine body's matrix a 9-DoF
 node whose three block rows stay in one domain at every level. Target sizes must be
-multiples of 3 and domains hold at most 32 scalars. The node grouping is a breadth-first
+multiples of 3 and domains hold at most 48 scalars (padded to 32 when every domain fits
+in 32). The node grouping is a breadth-first
 replacement of METIS (see ``mas_hierarchy``), so domains and iteration counts differ from
 the reference's; the preconditioner is still symmetric positive definite and PCG reaches
 the same stated residual.
@@ -38,7 +39,15 @@ class LocalInverseError(np.linalg.LinAlgError):
     numeric update (``solver/mas/cuda_runtime.py:2904-2920``)."""
 
 _MODES = {"reference": 0, "relative_residual": 1}
-DOMAIN_SCALARS = 32          # padded domain size of the device kernels
+DOMAIN_SCALARS = 32          # padded domain size of the device kernels (the default)
+MAX_DOMAIN_SCALARS = 48      # the larger padded size (the reference's default domain)
+
+
+def padded_domain_size(scalars: int) -> int:
+    """Padded domain size of the device kernels for domains of up to ``scalars``."""
+    if scalars > MAX_DOMAIN_SCALARS:
+        raise ValueError(f"MAS domains hold at most {MAX_DOMAIN_SCALARS} scalars on this device path")
+    return DOMAIN_SCALARS if scalars <= DOMAIN_SCALARS else MAX_DOMAIN_SCALARS
 NO_DEST = np.uint32(0xFFFFFFFF)
 
 
@@ -73,6 +82,7 @@ class MasLayout:
     weights: np.ndarray = field(default=None)
     duplicate: np.ndarray = field(default=None)   # per domain: repeats a previous-level domain
     weights_base: np.ndarray = field(default=None)   # configured level weights
+    domain_scalars: int = DOMAIN_SCALARS          # padded domain size D (32 or 48)
 
     @property
     def num_domains(self) -> int:
@@ -164,9 +174,14 @@ def duplicate_domains(h: Hierarchy) -> List[np.ndarray]:
     return out
 
 
-def build_layout(h: Hierarchy, row_ptr: np.ndarray, col: np.ndarray) -> MasLayout:
+def build_layout(h: Hierarchy, row_ptr: np.ndarray, col: np.ndarray,
+                 domain_scalars: int = DOMAIN_SCALARS) -> MasLayout:
     """Device tables of the MAS apply and numeric update for the full block-CSR
-    (row_ptr, col) of the current operator and a hierarchy over its block rows."""
+    (row_ptr, col) of the current operator and a hierarchy over its block rows, with
+    domains padded to ``domain_scalars`` (32 or 48)."""
+    if domain_scalars not in (DOMAIN_SCALARS, MAX_DOMAIN_SCALARS):
+        raise ValueError("MAS layout: domains are padded to 32 or 48 scalars")
+    D = domain_scalars
     lv0 = h.levels[0]
     nv = lv0.num_nodes
     if np.any(lv0.dims != 3) or row_ptr.size != nv + 1:
@@ -207,8 +222,8 @@ def build_layout(h: Hierarchy, row_ptr: np.ndarray, col: np.ndarray) -> MasLayou
         npos = np.zeros(lv.num_nodes, dtype=np.int64)
         for d in lv.domains:
             g = len(dom_sizes)
-            if 3 * d.size > DOMAIN_SCALARS:
-                raise ValueError(f"MAS layout: a domain holds {3 * d.size} scalars (max {DOMAIN_SCALARS})")
+            if 3 * d.size > D:
+                raise ValueError(f"MAS layout: a domain holds {3 * d.size} scalars (max {D})")
             nd[d] = g
             npos[d] = np.arange(d.size)
             dom_nodes.append(d + (bases[l] if l else 0))
@@ -240,7 +255,6 @@ def build_layout(h: Hierarchy, row_ptr: np.ndarray, col: np.ndarray) -> MasLayou
         uniq, inv = np.unique(key, return_inverse=True)
         off, items = _csr_by_key(inv.reshape(-1), uniq.size)
         blocks.append(LevelBlocks(uniq // n_l, uniq % n_l, off, items))
-    D = DOMAIN_SCALARS
     for l, b in enumerate(blocks):
         same = node_dom[l][b.rows] == node_dom[l][b.cols]
         dest = (node_dom[l][b.rows] * (D * D) + 3 * node_pos[l][b.rows] * D
@@ -249,7 +263,7 @@ def build_layout(h: Hierarchy, row_ptr: np.ndarray, col: np.ndarray) -> MasLayou
     return MasLayout(h, blocks, level_offsets, restrict_offsets, restrict_items, prolong,
                      np.concatenate(dom_nodes), np.asarray(dom_off, dtype=np.int64),
                      np.asarray(dom_sizes, dtype=np.int64), len(h.levels[0].domains),
-                     duplicate=np.concatenate(duplicate_domains(h)))
+                     duplicate=np.concatenate(duplicate_domains(h)), domain_scalars=D)
 
 
 class _DeviceMas:
@@ -272,7 +286,7 @@ class _DeviceMas:
         self.domain_nodes = u32(layout.domain_nodes)
         self.domain_node_offsets = u32(layout.domain_node_offsets)
         self.domain_sizes = u32(layout.domain_sizes)
-        D = DOMAIN_SCALARS
+        D = self.D = layout.domain_scalars
         self.domains = dev.empty(layout.num_domains * D * D, "float64")
         # domains >= bf16_first keep BF16 inverses when the coarse bank is BF16
         self.bf16_first = layout.level0_domains if coarse_bank == "bf16" else layout.num_domains
@@ -309,6 +323,7 @@ class _DeviceMas:
         a.inverses_bf16 = dev.ptr(self.inverses_bf16)
         a.bf16_first_domain = self.bf16_first
         a.domain_scale = dev.ptr(self.domain_scale)
+        a.domain_size = D
         self.args = a
 
     def set_fine_weight(self, w0: float):
@@ -331,11 +346,13 @@ class _DeviceMas:
             _lib.check(lib.ysp_mas_coarsen(dev.ptr(full.v64), 1, full.nnz, dev.ptr(off), dev.ptr(items),
                                            blocks[0].nnz, dev.ptr(self.values[0]), s), "ysp_mas_coarsen")
             _lib.check(lib.ysp_mas_domain_scatter(dev.ptr(self.values[0]), 0, blocks[0].nnz,
-                                                  dev.ptr(self.dest[0]), dev.ptr(self.domains), s),
+                                                  dev.ptr(self.dest[0]), dev.ptr(self.domains),
+                                                  self.D, s),
                        "ysp_mas_domain_scatter")
         else:
             _lib.check(lib.ysp_mas_domain_scatter(dev.ptr(full.v64), 1, full.nnz, dev.ptr(self.dest[0]),
-                                                  dev.ptr(self.domains), s), "ysp_mas_domain_scatter")
+                                                  dev.ptr(self.domains), self.D, s),
+                       "ysp_mas_domain_scatter")
         for l in range(1, len(blocks)):
             from_full = l == 1 and not self.merged0
             prev_vals = full.v64 if from_full else self.values[l - 1]
@@ -345,12 +362,14 @@ class _DeviceMas:
                                            dev.ptr(items), blocks[l].nnz, dev.ptr(self.values[l]), s),
                        "ysp_mas_coarsen")
             _lib.check(lib.ysp_mas_domain_scatter(dev.ptr(self.values[l]), 0, blocks[l].nnz,
-                                                  dev.ptr(self.dest[l]), dev.ptr(self.domains), s),
+                                                  dev.ptr(self.dest[l]), dev.ptr(self.domains),
+                                                  self.D, s),
                        "ysp_mas_domain_scatter")
         _lib.check(lib.ysp_mas_invert_domains(dev.ptr(self.domains), dev.ptr(self.domain_sizes),
                                               self.layout.num_domains, dev.ptr(self.inverses),
                                               dev.ptr(self.inverses_bf16), self.bf16_first,
-                                              dev.ptr(self.status), s), "ysp_mas_invert_domains")
+                                              dev.ptr(self.status), self.D, s),
+                   "ysp_mas_invert_domains")
 
     def apply(self, r, z):
         _lib.check(_lib.load().ysp_mas_apply(C.byref(self.args), dev.ptr(r), r.numel(), dev.ptr(z),
@@ -363,7 +382,8 @@ class masSolver:
     Takes the options of the reference's ``MASSolver`` (``solver/mas/solver.py:89-120``)
     by name, with its validation:
 
-    * hierarchy: ``max_domain_dofs`` (at most 32 scalars on this device path),
+    * hierarchy: ``max_domain_dofs`` (at most 48 scalars on this device path; 32, the
+      configurations' leaf size, by default, where the reference defaults to 48),
       ``max_levels``, ``target_nodes_per_partition``, ``minimum_domains_for_next_level``,
       ``global_coarse_domain``, ``domain_dof_schedule``, ``target_node_schedule``;
     * weights: ``coarsest_level_weight``, ``level_weights``, ``duplicate_level_weight`` (a
@@ -444,10 +464,10 @@ class masSolver:
         if coarse_bank not in ("fp32", "bf16"):
             raise ValueError("coarse_bank must be 'fp32' or 'bf16'")
         # device-path limits, named
-        if max_domain_dofs is not None and max_domain_dofs > DOMAIN_SCALARS:
-            raise ValueError(f"max_domain_dofs is limited to {DOMAIN_SCALARS} scalars on this device path")
-        if domain_dof_schedule is not None and max(int(v) for v in domain_dof_schedule) > DOMAIN_SCALARS:
-            raise ValueError(f"domain_dof_schedule entries are limited to {DOMAIN_SCALARS} scalars")
+        if max_domain_dofs is not None and max_domain_dofs > MAX_DOMAIN_SCALARS:
+            raise ValueError(f"max_domain_dofs is limited to {MAX_DOMAIN_SCALARS} scalars on this device path")
+        if domain_dof_schedule is not None and max(int(v) for v in domain_dof_schedule) > MAX_DOMAIN_SCALARS:
+            raise ValueError(f"domain_dof_schedule entries are limited to {MAX_DOMAIN_SCALARS} scalars")
         if max_levels is not None and max_levels > _lib.MAS_MAX_COARSE_LEVELS + 1:
             raise ValueError(f"max_levels is limited to {_lib.MAS_MAX_COARSE_LEVELS + 1}")
         unsupported = [name for name, bad in (
@@ -542,7 +562,9 @@ class masSolver:
                 self._hierarchy_key = skey
                 self.hierarchy_build_count += 1
                 self._hierarchy_seconds = time.perf_counter() - t0
-            layout = build_layout(expand_to_slots(self._hierarchy), row_ptr, col)
+            slots = expand_to_slots(self._hierarchy)
+            widest = max(3 * len(d) for lv in slots.levels for d in lv.domains)
+            layout = build_layout(slots, row_ptr, col, padded_domain_size(widest))
             w = level_weights(self._hierarchy.num_levels, self.coarsest_level_weight, self.level_weights)
             layout.weights_base = w.copy()
             self._device = _DeviceMas(layout, w, n, self.coarse_bank, self.duplicate_level_weight)
