@@ -54,6 +54,34 @@ def to_device(arr, dtype: str):
     return torch().from_numpy(a.copy()).to(device())
 
 
+_CONSTANTS = {}
+
+
+def constant(values, dtype: str):
+    """Device copy of a small host array that no one writes (a kernel's parameters, a
+    child count), uploaded once per distinct content. An upload from pageable host memory
+    synchronises the stream, so one per launch would stall the device between launches."""
+    a = np.ascontiguousarray(np.asarray(values).astype(dtype, copy=False))
+    key = (dtype, a.shape, a.tobytes())
+    t = _CONSTANTS.get(key)
+    if t is None:
+        if len(_CONSTANTS) >= 4096:
+            _CONSTANTS.clear()
+        t = _CONSTANTS[key] = to_device(a, dtype)
+    return t
+
+
+def constant_full(n: int, value: float, dtype: str = "float64"):
+    """``constant`` of n copies of one value (keyed by n and the value)."""
+    key = (dtype, "full", int(n), float(value))
+    t = _CONSTANTS.get(key)
+    if t is None:
+        if len(_CONSTANTS) >= 4096:
+            _CONSTANTS.clear()
+        t = _CONSTANTS[key] = to_device(np.full(int(n), float(value)), dtype)
+    return t
+
+
 def empty(n, dtype: str):
     return torch().empty(int(n), dtype=_tdtype(dtype), device=device())
 

@@ -75,14 +75,14 @@ def union_view(src: attribute) -> _lib.UnionF64:
         u._keep = [src.correspondance.counts_device] + vals
     elif is_affine(src):
         u.num_children = 1
-        counts = dev.to_device(np.array([src.numInstances], np.uint32), "uint32")
+        counts = dev.constant([src.numInstances], "uint32")
         v = src.expression.values()
         u.counts = dev.ptr(counts)
         u.data[0] = dev.ptr(v)
         u._keep = [counts, v]
     else:
         u.num_children = 1
-        counts = dev.to_device(np.array([src.numInstances], np.uint32), "uint32")
+        counts = dev.constant([src.numInstances], "uint32")
         u.counts = dev.ptr(counts)
         u.data[0] = dev.ptr(src.value)
         u._keep = [counts, src.value]
@@ -113,7 +113,7 @@ def _scalar_or_array(att, n_instances: int):
     constant is broadcast (stride 0), a per-instance attribute is read at stride 1; an
     expression (yasps_amd.expr) is evaluated per instance."""
     if isinstance(att, (int, float)):
-        return dev.to_device(np.array([float(att)]), "float64"), 0
+        return dev.constant([float(att)], "float64"), 0
     if is_expr(att):
         v = expr_values(att, n_instances, 1)
         return v.reshape(-1), (1 if v.shape[0] == n_instances and n_instances != 1 else 0)
@@ -253,8 +253,8 @@ class StableNeoHookean(EnergyExpr):
         dm_inv, vol = self.rest_data()
         t = dev.torch()
         ep = t.cat([dm_inv.reshape(n, 9), vol.reshape(n, 1)], dim=1).contiguous()
-        sp = dev.to_device(np.array([_host_scalar(self.mu), _host_scalar(self.lam),
-                                     _host_scalar(self.dt)]), "float64")
+        sp = dev.constant([_host_scalar(self.mu), _host_scalar(self.lam), _host_scalar(self.dt)],
+                          "float64")
         return ep, sp
 
     def evaluate(self, scatter: _lib.ScatterArgs, projection_method: int):
@@ -351,7 +351,7 @@ class Inertia(EnergyExpr):
         t = dev.torch()
         ep = t.cat([self._vec(self.xp), self._vec(self.v), self._mass().reshape(n, 1)],
                    dim=1).contiguous()
-        sp = dev.to_device(np.array([_host_scalar(self.dt)]), "float64")
+        sp = dev.constant([_host_scalar(self.dt)], "float64")
         if self._iota is None or self._iota.numel() != n:
             self._iota = dev.to_device(np.arange(n, dtype=np.uint32), "uint32")
         return ep, sp, union_view(self.x), affine_children(self.x)
@@ -359,7 +359,7 @@ class Inertia(EnergyExpr):
     def _mass(self):
         n = self.num_instances
         if isinstance(self.mass, (int, float)):
-            return dev.to_device(np.full(n, float(self.mass)), "float64")
+            return dev.constant_full(n, float(self.mass))
         if is_expr(self.mass):
             return expr_values(self.mass, n, 1).expand(n, 1).reshape(n).contiguous()
         if self.mass.numInstances == 1 and n != 1:
@@ -487,7 +487,7 @@ class ElementEnergy(EnergyExpr):
 
     def _scene(self):
         vals = [_host_scalar(v) for v in self.scene_params]
-        return dev.to_device(np.array(vals, dtype=np.float64), "float64")
+        return dev.constant(vals, "float64")
 
     def elem2v(self):
         return self.conn.value
@@ -584,7 +584,7 @@ class AffineOrthogonality(ElementEnergy):
     def positions(self):
         u = _lib.UnionF64()
         u.num_children = 1
-        counts = dev.to_device(np.array([self.A.numInstances], np.uint32), "uint32")
+        counts = dev.constant([self.A.numInstances], "uint32")
         u.counts = dev.ptr(counts)
         u.data[0] = dev.ptr(self.A.value)
         u.width = 9
