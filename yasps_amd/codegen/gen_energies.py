@@ -167,10 +167,23 @@ ENERGIES = {
 
 def generate(name):
     fn, n, npar, nsc = ENERGIES[name]
-    x = sp.symbols(f"x0:{n}", real=True)
-    p = sp.symbols(f"p0:{max(npar, 1)}", real=True)
-    s = sp.symbols(f"s0:{nsc}", real=True)
-    E = fn(x, p, s)
+    return emit_energy(name, fn, n, npar, nsc)
+
+
+def energy_symbols(n, npar, nsc):
+    """The SymPy symbols x, p, s an energy of n DoF, npar element and nsc scene
+    parameters is written in."""
+    return (sp.symbols(f"x0:{n}", real=True), sp.symbols(f"p0:{max(npar, 1)}", real=True),
+            sp.symbols(f"s0:{max(nsc, 1)}", real=True))
+
+
+def emit_energy(name, fn, n, npar, nsc, qualifier="__device__ __forceinline__", energy=None):
+    """C++ of gen_<name>(x, p, s, E, g, H) for the energy fn(x, p, s) of n DoF, npar
+    element and nsc scene parameters (SymPy symbols of ``energy_symbols``; ``energy``: the
+    expression already built from them), declared with ``qualifier``; returns (code,
+    number of shared subexpressions)."""
+    x, p, s = energy_symbols(n, npar, nsc)
+    E = sp.sympify(fn(x, p, s) if energy is None else energy)
     g = [sp.diff(E, xi) for xi in x]
     H = [sp.diff(g[i], x[j]) for i in range(n) for j in range(i, n)]
     repl, red = sp.cse([E] + g + H, symbols=sp.numbered_symbols("t"), optimizations="basic")
@@ -179,7 +192,7 @@ def generate(name):
     subs.update({si: sp.Symbol(f"s[{i}]") for i, si in enumerate(s)})
     lines = [f"// {name}: {n} DoF, {npar} element parameters, {nsc} scene parameters",
              f"constexpr int kGen_{name}_N = {n};",
-             f"__device__ __forceinline__ void gen_{name}(const double* __restrict__ x, "
+             f"{qualifier} void gen_{name}(const double* __restrict__ x, "
              "const double* __restrict__ p, const double* __restrict__ s, double& E, "
              "double* __restrict__ g, double* __restrict__ H) {"]
     for sym, expr in repl:
