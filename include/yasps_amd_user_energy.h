@@ -31,6 +31,17 @@ int ysp_user_energy_eval(const ysp_union_f64* positions, const ysp_affine_childr
                          const double* scene_params, uint32_t projection_method,
                          const ysp_scatter_args* out, void* stream);
 
+/* Element blocks of d g / d theta from the caller's d p / d theta and d s / d theta, as
+ * ysp_element_param_jacobian for a library energy (same arguments without the kind):
+ * every element parameter and scene slot of the energy has its generated derivative. */
+int ysp_user_energy_param_jacobian(const ysp_union_f64* positions,
+                                   const uint32_t* element_vertices, uint32_t n,
+                                   const double* element_params, const double* scene_params,
+                                   uint32_t scene_stride, const double* dparams,
+                                   uint32_t dparams_stride, const double* dscene,
+                                   uint32_t dscene_stride, uint32_t w, double* blocks,
+                                   void* stream);
+
 /* energy[e] of every element (line search), as ysp_element_energy_values. */
 int ysp_user_energy_values(const ysp_union_f64* positions, const uint32_t* elem2v, uint32_t n,
                            const double* element_params, const double* scene_params,

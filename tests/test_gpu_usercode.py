@@ -152,3 +152,105 @@ def test_newton_solve_with_user_springs():
     r = hs.spmv(vector(hs.rows, x.clone())).value - hs.gradient.value
     rel = float(dev.torch().linalg.norm(r) / dev.torch().linalg.norm(hs.gradient.value))
     assert s0.minimizer.linearSolver.statistics["converged"] and rel < 1.5e-8
+
+
+PARAMS = {"point_triangle": ["dhat", "kappa"], "edge_edge": ["dhat", "kappa"], "bending": ["k"],
+          "stvk_membrane": ["mu", "lam", "dt"]}
+
+
+@pytest.mark.parametrize("name", list(SHAPES))
+def test_user_energy_scene_jacobians_match_oracle(name):
+    """d g / d s of the restated library energies through diff2 and the module's mixed
+    kernel, with every scene parameter a one-instance constant (a column target), against
+    the oracle's scene gradients of the golden elements (as the library kinds' test)."""
+    from yasps_amd import device as dev, scene
+    from yasps_amd.differentiator import differentiator
+    from oracle import jacobian as oj
+    from test_gpu_energies import _golden
+    from test_gpu_jacobian import _products
+    print(f"user energy jacobian {name}", flush=True)
+    G = _golden(name)
+    arity, npar, nsc = SHAPES[name]
+    X = G["X"].reshape(-1, arity, 3)
+    n = X.shape[0]
+    fixed = np.zeros((n, arity), dtype=bool)
+    fixed[::4, 0] = True
+    flat, fm = X.reshape(-1, 3), fixed.reshape(-1)
+    fids, mids = np.flatnonzero(fm), np.flatnonzero(~fm)
+    uid = np.empty(flat.shape[0], dtype=np.int64)
+    uid[fids] = np.arange(fids.size)
+    uid[mids] = fids.size + np.arange(mids.size)
+    s0 = scene("userjac")
+    sc = [s0.addConstant(nm).updateValue([float(G["s"][i])]) for i, nm in enumerate(PARAMS[name])]
+    m = s0.addMesh("m")
+    fv = m.addPrimitive("fixed", numInstances=fids.size)
+    mv = m.addPrimitive("moving", numInstances=mids.size)
+    m.addPrimitiveUnion("vertices", [fv, mv])
+    fv.addAttribute("position", rows=3, cols=1).updateValue(flat[fids])
+    pos = mv.addAttribute("position", rows=3, cols=1).updateValue(flat[mids])
+    m.vertices.addAttribute("position")
+    el = m.addPrimitive("elems", numInstances=n)
+    e2v = el.addConnectivity("e2v", m.vertices, uid.reshape(n, arity), arity)
+    p = el.addAttribute("positions", through=e2v, source=m.vertices["position"])
+    att = el.addAttribute(name, computed_attribute=_kind(name)(p, G["P"] if npar else None, sc))
+    s0.addEnergy(att, projection_method=2)
+    s0.addMinimizeTarget([pos])
+    J = differentiator().diff2([att], [pos], sc).compute()
+    Gs = oj.element_scene_gradients(name, G["X"], G.get("P"), np.asarray(G["s"], np.float64))
+    rows = np.where(fixed[:, :, None], -1, 3 * (uid.reshape(n, arity)[:, :, None] - fids.size)
+                    + np.arange(3)).reshape(n, 3 * arity)
+    R = 3 * mids.size
+    dense = np.concatenate([oj.dense_jacobian(rows, Gs[:, :, j:j + 1], np.zeros(n, int), R, 1)
+                            for j in range(len(sc))], axis=1)
+    et, eb = _products(J, dense, np.random.default_rng(11))
+    print(f"  J^T a {et:.2e}, J b {eb:.2e}", flush=True)
+    assert et < REL and eb < REL
+
+
+def test_user_spring_jacobian_in_rest_lengths_and_stiffness():
+    """A user energy's element parameters as a per-element column target: for the spring
+    k/2 (l - L)^2, d g_b / d L = -k d / l on the second vertex (+k d / l on the first), and
+    the stiffness as a one-instance target: d g / d k = g / k."""
+    from yasps_amd import device as dev, scene, usercode
+    from yasps_amd.differentiator import differentiator
+    from yasps_amd.scenes import cube_scene
+    print("user spring jacobian", flush=True)
+    cs = cube_scene(2, seed=3)
+    s0 = scene("springjac")
+    m = s0.addMesh("cube")
+    fv = m.addPrimitive("fixed_vertices", numInstances=cs.n_fixed)
+    mv = m.addPrimitive("moving_vertices", numInstances=cs.n_moving)
+    m.addPrimitiveUnion("vertices", [fv, mv])
+    fv.addAttribute("position", rows=3, cols=1).updateValue(cs.position[:cs.n_fixed])
+    pos = mv.addAttribute("position", rows=3, cols=1).updateValue(cs.position[cs.n_fixed:])
+    m.vertices.addAttribute("position")
+    edges = np.unique(np.sort(np.concatenate([cs.tets[:, [a, b]] for a in range(4)
+                                              for b in range(a + 1, 4)]), axis=1), axis=0)
+    ne = edges.shape[0]
+    L = 0.9 * np.linalg.norm(cs.rest[edges[:, 1]] - cs.rest[edges[:, 0]], axis=1)
+    sp_prim = m.addPrimitive("springs", numInstances=ne)
+    s2v = sp_prim.addConnectivity("s2v", m.vertices, edges.astype(np.uint32), 2)
+    spos = sp_prim.addAttribute("spring_positions", through=s2v, source=m.vertices["position"])
+    rest = sp_prim.addAttribute("rest_length", rows=1, cols=1).updateValue(L)
+    k = s0.addConstant("k").updateValue([50.0])
+    att = sp_prim.addAttribute("spring", computed_attribute=usercode.example("spring")(spos, rest, [k]))
+    s0.addEnergy(att, projection_method=2)
+    s0.addMinimizeTarget([pos])
+    J = differentiator().diff2([att], [pos], [rest, k]).compute()
+    x = cs.position
+    d = x[edges[:, 1]] - x[edges[:, 0]]
+    l = np.linalg.norm(d, axis=1)
+    nf = cs.n_fixed
+    R = 3 * (x.shape[0] - nf)
+    dense = np.zeros((R, ne + 1))
+    for e, (a, b) in enumerate(edges):
+        g_b = 50.0 * (l[e] - L[e]) / l[e] * d[e]
+        for v, sign in ((a, -1.0), (b, 1.0)):
+            if v < nf:
+                continue
+            r = 3 * (v - nf)
+            dense[r:r + 3, e] += sign * (-50.0 * d[e] / l[e])
+            dense[r:r + 3, ne] += sign * g_b / 50.0
+    from test_gpu_jacobian import _products
+    et, eb = _products(J, dense, np.random.default_rng(5))
+    assert et < REL and eb < REL

@@ -94,7 +94,7 @@ def test_device_module_builds_and_exports_its_entry_points():
     header = Path(__file__).resolve().parents[1] / "include" / "yasps_amd_user_energy.h"
     names = re.findall(r"^int\s+(ysp_\w+)\(", header.read_text(), re.M)
     assert sorted(names) == ["ysp_user_energy_arity", "ysp_user_energy_eval",
-                             "ysp_user_energy_values"]
+                             "ysp_user_energy_param_jacobian", "ysp_user_energy_values"]
     kind = usercode.example("spring")
     lib = kind.library()
     for name in names:

@@ -218,12 +218,18 @@ def generate_mixed(name):
     the element blocks of a rectangular Jacobian d g / d theta once multiplied by the
     parameters' own derivatives d (p, s) / d theta."""
     fn, n, npar, nsc = ENERGIES[name]
-    x = sp.symbols(f"x0:{n}", real=True)
-    p = sp.symbols(f"p0:{max(npar, 1)}", real=True)
-    s = sp.symbols(f"s0:{nsc}", real=True)
-    E = fn(x, p, s)
+    return emit_mixed(name, fn, n, npar, nsc, npar if name in MIXED_P else 0)
+
+
+def emit_mixed(name, fn, n, npar, nsc, wp, qualifier="__device__ __forceinline__",
+               energy=None):
+    """C++ of mixed_<name>(x, p, s, Gp, Gs): d g / d p for the first ``wp`` element
+    parameters (n x wp, row-major) and d g / d s (n x nsc) of the energy fn(x, p, s)
+    (``energy``: the expression already built from ``energy_symbols``)."""
+    x, p, s = energy_symbols(n, npar, nsc)
+    E = sp.sympify(fn(x, p, s) if energy is None else energy)
+    s = s[:nsc]
     g = [sp.diff(E, xi) for xi in x]
-    wp = npar if name in MIXED_P else 0
     Gp = [sp.diff(g[i], p[j]) for i in range(n) for j in range(wp)]
     Gs = [sp.diff(g[i], s[j]) for i in range(n) for j in range(nsc)]
     repl, red = sp.cse(Gp + Gs, symbols=sp.numbered_symbols("t"), optimizations="basic")
@@ -232,7 +238,7 @@ def generate_mixed(name):
     subs.update({si: sp.Symbol(f"s[{i}]") for i, si in enumerate(s)})
     lines = [f"// {name}: d g / d p ({n} x {wp}), d g / d s ({n} x {nsc})",
              f"constexpr int kMixed_{name}_P = {wp};",
-             f"__device__ __forceinline__ void mixed_{name}(const double* __restrict__ x, "
+             f"{qualifier} void mixed_{name}(const double* __restrict__ x, "
              "const double* __restrict__ p, const double* __restrict__ s, "
              "double* __restrict__ Gp, double* __restrict__ Gs) {"]
     for sym, expr in repl:

@@ -17,18 +17,11 @@
 // every element). Products: y += B x over rows (FP64 atomics: elements share vertices)
 // and y += B^T x over columns (per-element columns are written by their element alone;
 // shared columns are reduced per wavefront, then one atomic per wavefront and column).
-#include "common.hpp"
-#include "element_common.hpp"
+#include "jacobian_kernels.hpp"
 #include "generated_mixed.hpp"
 
 namespace ysp {
 namespace {
-
-constexpr unsigned kJB = 64;
-constexpr uint32_t kMaxCols = 9;
-
-template <int ID>
-struct MixedKind;
 
 #define YSP_MIXED_KIND(ID, NAME, ARITY, WIDTH, NPAR, NSCENE)                                 \
   template <>                                                                              \
@@ -52,66 +45,6 @@ YSP_MIXED_KIND(YSP_ENERGY_AFFINE_ORTHOGONALITY, affine_orthogonality, 1, 9, 0, 1
 YSP_MIXED_KIND(YSP_ENERGY_SNH, snh, 4, 3, 10, 3)
 YSP_MIXED_KIND(YSP_ENERGY_INERTIA, inertia, 1, 3, 7, 1)
 #undef YSP_MIXED_KIND
-
-struct JacArgs {
-  UnionView U;
-  const uint32_t* elem2v;
-  uint32_t n;
-  const double* ep;             // [n, P] element parameters (P > 0)
-  const double* se;             // scene parameters, se_stride apart per element
-  uint32_t se_stride;           // 0: shared by every element
-  const double* cp;             // [n, PM, w] d p / d theta, cp_stride apart (null: zero)
-  uint32_t cp_stride;
-  const double* cs;             // [n, S, w] d s / d theta, cs_stride apart (null: zero)
-  uint32_t cs_stride;
-  uint32_t w;
-  double* blocks;               // [n, N, w]
-};
-
-template <int ID>
-__global__ void __launch_bounds__(kJB) param_jacobian_kernel(JacArgs a) {
-  using K = MixedKind<ID>;
-  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= a.n) return;
-  double x[K::N];
-#pragma unroll
-  for (int v = 0; v < K::A; ++v) {
-    const double* row = union_row(a.U, a.elem2v[static_cast<size_t>(e) * K::A + v]);
-#pragma unroll
-    for (int i = 0; i < K::W; ++i) x[K::W * v + i] = row ? row[i] : 0.0;
-  }
-  double p[K::P > 0 ? K::P : 1];
-#pragma unroll
-  for (int j = 0; j < K::P; ++j) p[j] = a.ep[static_cast<size_t>(e) * K::P + j];
-  if constexpr (K::P == 0) p[0] = 0.0;
-  double s[K::S];
-  const double* sr = a.se + static_cast<size_t>(e) * a.se_stride;
-#pragma unroll
-  for (int j = 0; j < K::S; ++j) s[j] = sr[j];
-  if constexpr (ID == YSP_ENERGY_BARAFF_WITKIN) bw_stretch_coefficients(x, p, s[3], s[4]);
-  double Gp[K::N * (K::PM > 0 ? K::PM : 1)];
-  double Gs[K::N * K::S];
-  K::mixed(x, p, s, Gp, Gs);
-  const double* cp = a.cp ? a.cp + static_cast<size_t>(e) * a.cp_stride : nullptr;
-  const double* cs = a.cs ? a.cs + static_cast<size_t>(e) * a.cs_stride : nullptr;
-  double* out = a.blocks + static_cast<size_t>(e) * K::N * a.w;
-  for (uint32_t c = 0; c < a.w; ++c) {
-    double dp[K::PM > 0 ? K::PM : 1], ds[K::S];
-#pragma unroll
-    for (int j = 0; j < K::PM; ++j) dp[j] = cp ? cp[j * a.w + c] : 0.0;
-#pragma unroll
-    for (int j = 0; j < K::S; ++j) ds[j] = cs ? cs[j * a.w + c] : 0.0;
-#pragma unroll
-    for (int r = 0; r < K::N; ++r) {
-      double acc = 0.0;
-#pragma unroll
-      for (int j = 0; j < K::PM; ++j) acc += Gp[r * K::PM + j] * dp[j];
-#pragma unroll
-      for (int j = 0; j < K::S; ++j) acc += Gs[r * K::S + j] * ds[j];
-      out[r * a.w + c] = acc;
-    }
-  }
-}
 
 // Global row of every DoF of every element: group k of element e (K groups, sizes from
 // get_indices) covers the next size DoF; index 0 / 1 (union padding / not a row target)
@@ -195,11 +128,6 @@ __global__ void __launch_bounds__(kJB)
   }
 }
 
-template <int ID>
-void launch_kind(const JacArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(param_jacobian_kernel<ID>, dim3(grid_for(a.n, kJB)), dim3(kJB), 0, s, a);
-}
-
 }  // namespace
 }  // namespace ysp
 
@@ -212,26 +140,14 @@ extern "C" int ysp_element_param_jacobian(int32_t kind, const ysp_union_f64* pos
                                           uint32_t dparams_stride, const double* dscene,
                                           uint32_t dscene_stride, uint32_t w, double* blocks,
                                           void* stream) {
-  if (n == 0) return YSP_OK;
-  if (!positions || !element_vertices || !scene_params || !blocks || w == 0 || w > kMaxCols ||
-      positions->num_children == 0 || positions->num_children > YSP_MAX_UNION_CHILDREN) {
-    ysp_set_error(__FILE__, __LINE__, "ysp_element_param_jacobian: bad arguments");
-    return YSP_ERR_ARG;
-  }
-  JacArgs a{make_view(positions), element_vertices, n, element_params, scene_params, scene_stride,
-            dparams, dparams_stride, dscene, dscene_stride, w, blocks};
-  hipStream_t s = as_stream(stream);
-  auto need = [&](int width, int P) -> bool {
-    if (static_cast<int>(positions->width) != width) return false;
-    return P == 0 || element_params != nullptr;
-  };
-  bool ok = true;
+  const char* who = "ysp_element_param_jacobian: bad arguments (vertex width, element "
+                    "parameters or column count do not match the energy kind)";
   switch (kind) {
-#define YSP_CASE(ID)                                                    \
-  case ID:                                                              \
-    ok = need(MixedKind<ID>::W, MixedKind<ID>::P);                      \
-    if (ok) launch_kind<ID>(a, s);                                      \
-    break;
+#define YSP_CASE(ID)                                                                         \
+  case ID:                                                                                   \
+    return param_jacobian<ID>(positions, element_vertices, n, element_params, scene_params,  \
+                              scene_stride, dparams, dparams_stride, dscene, dscene_stride, w, \
+                              blocks, stream, who);
     YSP_CASE(YSP_ENERGY_POINT_POINT)
     YSP_CASE(YSP_ENERGY_POINT_EDGE)
     YSP_CASE(YSP_ENERGY_POINT_TRIANGLE)
@@ -248,14 +164,6 @@ extern "C" int ysp_element_param_jacobian(int32_t kind, const ysp_union_f64* pos
                     "ysp_element_param_jacobian: no mixed derivatives for this energy kind");
       return YSP_ERR_ARG;
   }
-  if (!ok) {
-    ysp_set_error(__FILE__, __LINE__,
-                  "ysp_element_param_jacobian: vertex width or element parameters do not match "
-                  "the energy kind");
-    return YSP_ERR_ARG;
-  }
-  YSP_CHECK_LAUNCH();
-  return YSP_OK;
 }
 
 extern "C" int ysp_jacobian_rows(const uint32_t* segment_indices, const uint16_t* segment_sizes,

@@ -22,12 +22,14 @@ A column target has one instance (shared by every element) or one per element of
 energy. The block of an element is (d g / d p) (d p / d theta) + (d g / d s)
 (d s / d theta): generated derivatives of the element gradient w.r.t. the energy's
 parameters (``csrc/generated_mixed.hpp``) times the expressions' derivatives, formed by
-the HIP kernel ``ysp_element_param_jacobian``. Not supported (NotImplementedError): energies
+the HIP kernel ``ysp_element_param_jacobian`` (for a run-time generated user energy, the
+same kernel in its module, ``yasps_amd/usercode.py``). Not supported (NotImplementedError): energies
 on affine-body vertices, ``quadratic_form``, columns that are DoF of the energy or
 rest-shape inputs, and ``local_targets``."""
 from __future__ import annotations
 
 import ctypes as C
+import functools
 from typing import List, Optional
 
 import numpy as np
@@ -177,8 +179,10 @@ class _Term:
             cs = stack(ds, [1] * len(info.s_args))
             blocks = dev.empty(n * info.dofs * w, "float64")
             ep = info.ep
-            _lib.check(lib.ysp_element_param_jacobian(
-                info.kind, C.byref(info.positions), dev.ptr(info.elem2v), n,
+            kernel = (info.param_jacobian if info.param_jacobian is not None
+                      else functools.partial(lib.ysp_element_param_jacobian, info.kind))
+            _lib.check(kernel(
+                C.byref(info.positions), dev.ptr(info.elem2v), n,
                 dev.ptr(ep) if ep is not None else None, dev.ptr(info.se), info.se_stride,
                 dev.ptr(cp) if cp is not None else None, cp.shape[1] * w if cp is not None else 0,
                 dev.ptr(cs) if cs is not None else None, cs.shape[1] * w if cs is not None else 0,

@@ -162,6 +162,8 @@ class MixedInfo:
     s_args: list
     dof_leaves: list
     rest_leaves: list = field(default_factory=list)
+    param_jacobian: object = None    # another entry point with the ABI of
+                                     # ysp_element_param_jacobian minus the kind (user energies)
 
 
 def attribute_leaves(src) -> list:
