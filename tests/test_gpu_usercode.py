@@ -254,3 +254,28 @@ def test_user_spring_jacobian_in_rest_lengths_and_stiffness():
     from test_gpu_jacobian import _products
     et, eb = _products(J, dense, np.random.default_rng(5))
     assert et < REL and eb < REL
+
+
+def test_user_barriers_on_affine_bodies_match_library():
+    """The contact scene (affine and free balls on a cloth) with its point-triangle and
+    edge-edge barriers replaced by the user restatements: pairs whose points follow an
+    affine body go through the module's general kernel and chain rule, and the assembled
+    gradient and projected Hessian equal the library's to 1e-12."""
+    from yasps_amd import device as dev, usercode
+    from yasps_amd.scenes import contact_scene, build_contact_scene
+    print("user barriers on the contact scene", flush=True)
+    cs = contact_scene(bodies_x=2, bodies_z=1, cells=4, radius=0.12, cloth_m=9,
+                       max_pairs=10 ** 9, seed=0)
+    pt, ee = usercode.example("user_point_triangle"), usercode.example("user_edge_edge")
+    user = {"pt": lambda pos, dhat, kappa: pt(pos, None, [dhat, kappa]),
+            "ee": lambda pos, dhat, kappa: ee(pos, None, [dhat, kappa])}
+    out = []
+    for barriers in (None, user):
+        s0, handles = build_contact_scene(cs, threshold_mode="relative_residual", barriers=barriers)
+        hs = s0.minimizer.computeNumericValue()
+        out.append((dev.to_host(hs.gradient.value), hs.dense(),
+                    dev.to_host(handles["pt"].compute().value)))
+    (g0, D0, e0), (g1, D1, e1) = out
+    assert cs.n_bodies > 0 and cs.pt.shape[0] > 0
+    rel = lambda a, b: np.abs(a - b).max() / np.abs(b).max()      # noqa: E731
+    assert rel(e1, e0) < REL and rel(g1, g0) < REL and rel(D1, D0) < REL

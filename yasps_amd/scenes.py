@@ -630,15 +630,18 @@ def contact_scene(bodies_x: int = 5, bodies_z: int = 8, cells: int = 16, radius:
 
 
 def build_contact_scene(cs: ContactScene, projection_method: int = 2, solver_name: str = "jacobian",
-                        **solver_options):
+                        barriers=None, **solver_options):
     """The contact scene through the drop-in API: union [fixed, abd, moving] vertices
     (``one_bunny_partial_abd.py:102-258`` for the affine part), SNH on the body tets, StVK
     membrane and dihedral bending on the cloth, IPC point-triangle and edge-edge barriers on
     dynamic pair primitives (``dynamic_instances=True``, re-indexed on every assembly as
     ``hessian.pyx:846-880``), inertia on abd and moving vertices, affine orthogonality on
-    the bodies; targets [moving position, affine_matrix, translation]."""
+    the bodies; targets [moving position, affine_matrix, translation]. ``barriers`` may
+    replace the point-triangle / edge-edge energy constructors ({"pt": fn, "ee": fn},
+    fn(positions, dhat, kappa)), e.g. by user energies."""
     from . import scene, stable_neo_hookean, inertia
     from . import energies as en
+    barriers = dict(barriers or {})
     s0 = scene("contact")
     dt = s0.addConstant("dt", rows=1, cols=1)
     dt.updateValue([cs.dt])
@@ -689,7 +692,8 @@ def build_contact_scene(cs: ContactScene, projection_method: int = 2, solver_nam
         else:
             handles["bending"] = el.addAttribute("bending",
                                                  computed_attribute=en.bending(pos, rp, cs.bend))
-    for name, pairs, fn in (("pt", cs.pt, en.point_triangle), ("ee", cs.ee, en.edge_edge)):
+    for name, pairs, fn in (("pt", cs.pt, barriers.get("pt", en.point_triangle)),
+                            ("ee", cs.ee, barriers.get("ee", en.edge_edge))):
         prim = m.addPrimitive(name, numInstances=pairs.shape[0], isDynamic=True)
         conn = prim.addConnectivity(name + "2v", m.vertices, pairs, 4)
         pos = prim.addAttribute("positions", through=conn, source=m.vertices["position"])
